@@ -1,0 +1,564 @@
+// Grouped block GEMM on CDNA4 matrix cores (gfx950).
+//
+// One launch executes every task of a get_map task list (or of a block-pair
+// contraction).  A work item is one 64x64 output tile together with a K-chunk
+// of the concatenated segments of all tasks that write that output block, so
+// the sum over contracted sectors happens in registers, uneven sector shapes
+// share a launch, and long K is split across work items (atomic accumulate)
+// to fill all CUs.  Three arithmetic modes:
+//   PREC_FP64   v_mfma_f64_16x16x4f64, FP64 storage
+//   PREC_FP32   v_mfma_f32_16x16x4f32, FP32 storage
+//   PREC_BF16X3 v_mfma_f32_16x16x32_bf16 with each FP32 operand split into
+//               hi+lo bfloat16 parts; hi*hi + hi*lo + lo*hi, FP32 accumulate
+// Complex operands (interleaved re/im) use four real products per tile.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace dmrgx {
+
+GemmPlan::~GemmPlan() {
+  delete d_items;
+  delete d_segs;
+}
+
+namespace {
+
+constexpr int BM = 64, BN = 64, NT = 256;
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8v __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <typename S>
+struct Mfma4;
+template <>
+struct Mfma4<double> {
+  typedef d4v acc;
+  static __device__ __forceinline__ acc op(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct Mfma4<float> {
+  typedef f4v acc;
+  static __device__ __forceinline__ acc op(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
+__device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+
+// Walks the (segment, k-tile) sequence of one work item.
+struct Cursor {
+  int seg, seg_end, k;
+  GemmSeg s;
+  __device__ void start(const GemmSeg* segs, int b, int e) {
+    seg = b;
+    seg_end = e;
+    if (seg < seg_end) {
+      s = segs[seg];
+      k = s.k0;
+    }
+  }
+  __device__ bool valid() const { return seg < seg_end; }
+  template <int BK>
+  __device__ void next(const GemmSeg* segs) {
+    k += BK;
+    if (k >= s.k1) {
+      ++seg;
+      if (seg < seg_end) {
+        s = segs[seg];
+        k = s.k0;
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32 kernel (MFMA 16x16x4), LDS tiles row-major [row][k] padded.
+// AKC: A is k-contiguous (transA 'N'); BKC: B is k-contiguous (transB 'T').
+template <typename S, bool CPLX, bool AKC, bool BKC>
+__global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items, int nitems,
+                                              int per_xcd, const GemmSeg* __restrict__ segs,
+                                              const S* __restrict__ A, const S* __restrict__ B,
+                                              S* __restrict__ C) {
+  constexpr int BK = 16;
+  constexpr int LDK = BK + 1;
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int EPT = BM * BK / NT;  // 4
+  const int bid = blockIdx.x;
+  const int item_id = (bid & 7) * per_xcd + (bid >> 3);
+  if (item_id >= nitems) return;
+  const GemmItem it = items[item_id];
+
+  __shared__ S sA[NP][BM * LDK];
+  __shared__ S sB[NP][BN * LDK];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  using acc_t = typename Mfma4<S>::acc;
+  acc_t acc[NP][2][2];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[p][i][j] = acc_t{0, 0, 0, 0};
+
+  S ra[NP][EPT], rb[NP][EPT];
+  int lr[EPT], lk[EPT];  // LDS coordinates of A loads
+  int mr[EPT], mk[EPT];  // LDS coordinates of B loads
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    if (AKC) { lr[j] = t / BK + (NT / BK) * j; lk[j] = t % BK; }
+    else     { lr[j] = t % BM; lk[j] = t / BM + (NT / BM) * j; }
+    if (BKC) { mr[j] = t / BK + (NT / BK) * j; mk[j] = t % BK; }
+    else     { mr[j] = t % BN; mk[j] = t / BN + (NT / BN) * j; }
+  }
+
+  Cursor cur;
+  cur.start(segs, it.seg_begin, it.seg_end);
+
+  auto gload = [&](const Cursor& c) {
+    const GemmSeg& s = c.s;
+    const S al = (S)s.alpha;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      int gr = it.m0 + lr[j], gk = c.k + lk[j];
+      bool ok = gr < it.M && gk < s.k1;
+      int64_t e = AKC ? s.a_off + (int64_t)gr * s.K + gk : s.a_off + (int64_t)gk * it.M + gr;
+      if (CPLX) {
+        ra[0][j] = ok ? A[2 * e] * al : (S)0;
+        ra[NP - 1][j] = ok ? A[2 * e + 1] * al : (S)0;
+      } else {
+        ra[0][j] = ok ? A[e] * al : (S)0;
+      }
+      int gn = it.n0 + mr[j], gk2 = c.k + mk[j];
+      bool ok2 = gn < it.N && gk2 < s.k1;
+      int64_t f = BKC ? s.b_off + (int64_t)gn * s.K + gk2 : s.b_off + (int64_t)gk2 * it.N + gn;
+      if (CPLX) {
+        rb[0][j] = ok2 ? B[2 * f] : (S)0;
+        rb[NP - 1][j] = ok2 ? B[2 * f + 1] : (S)0;
+      } else {
+        rb[0][j] = ok2 ? B[f] : (S)0;
+      }
+    }
+  };
+
+  if (cur.valid()) gload(cur);
+  while (cur.valid()) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        sA[p][lr[j] * LDK + lk[j]] = ra[p][j];
+        sB[p][mr[j] * LDK + mk[j]] = rb[p][j];
+      }
+    __syncthreads();
+    cur.next<BK>(segs);
+    if (cur.valid()) gload(cur);
+#pragma unroll
+    for (int ks = 0; ks < BK / 4; ++ks) {
+      const int kx = ks * 4 + (lane >> 4);
+      S a[NP][2], b[NP][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          a[p][i] = sA[p][(wm + i * 16 + (lane & 15)) * LDK + kx];
+          b[p][i] = sB[p][(wn + i * 16 + (lane & 15)) * LDK + kx];
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if (CPLX) {
+            acc[0][i][j] = Mfma4<S>::op(a[0][i], b[0][j], acc[0][i][j]);
+            acc[0][i][j] = Mfma4<S>::op(-a[NP - 1][i], b[NP - 1][j], acc[0][i][j]);
+            acc[NP - 1][i][j] = Mfma4<S>::op(a[0][i], b[NP - 1][j], acc[NP - 1][i][j]);
+            acc[NP - 1][i][j] = Mfma4<S>::op(a[NP - 1][i], b[0][j], acc[NP - 1][i][j]);
+          } else {
+            acc[0][i][j] = Mfma4<S>::op(a[0][i], b[0][j], acc[0][i][j]);
+          }
+        }
+    }
+  }
+
+  // epilogue: lane holds column lane%16 of each 16x16 tile; the row of
+  // accumulator r is 4*(lane/16)+r for f32 and r*4+lane/16 for f64 MFMA
+  const bool atomic = it.flags & ITEM_ATOMIC;
+  const bool beta = it.flags & ITEM_BETA;
+  constexpr bool F64 = sizeof(S) == 8;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = it.m0 + wm + i * 16 + (F64 ? r * 4 + (lane >> 4) : (lane >> 4) * 4 + r);
+        int col = it.n0 + wn + j * 16 + (lane & 15);
+        if (row < it.M && col < it.N) {
+          int64_t e = it.c_off + (int64_t)row * it.N + col;
+          if (CPLX) {
+            S vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
+            if (atomic) { atomic_add(&C[2 * e], vr); atomic_add(&C[2 * e + 1], vi); }
+            else if (beta) { C[2 * e] += vr; C[2 * e + 1] += vi; }
+            else { C[2 * e] = vr; C[2 * e + 1] = vi; }
+          } else {
+            S v = acc[0][i][j][r];
+            if (atomic) atomic_add(&C[e], v);
+            else if (beta) C[e] += v;
+            else C[e] = v;
+          }
+        }
+      }
+}
+
+// ---------------------------------------------------------------------------
+// BF16x3 kernel: FP32 storage, operands split on the way into LDS.
+__device__ __forceinline__ unsigned short f2bf_rne(float x) {
+  unsigned u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <bool CPLX, bool AKC, bool BKC>
+__global__ __launch_bounds__(NT) void k_gemm_bf16x3(const GemmItem* __restrict__ items, int nitems,
+                                                    int per_xcd, const GemmSeg* __restrict__ segs,
+                                                    const float* __restrict__ A,
+                                                    const float* __restrict__ B,
+                                                    float* __restrict__ C) {
+  constexpr int BK = 32;
+  constexpr int LDK = BK + 8;  // 80-byte rows keep 16-byte fragment reads aligned
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int EPT = BM * BK / NT;  // 8
+  const int bid = blockIdx.x;
+  const int item_id = (bid & 7) * per_xcd + (bid >> 3);
+  if (item_id >= nitems) return;
+  const GemmItem it = items[item_id];
+
+  // planes: [part][hi/lo][rows*LDK]
+  __shared__ __attribute__((aligned(16))) unsigned short sA[NP][2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[NP][2][BN * LDK];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f4v acc[NP][2][2];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) acc[p][i][j] = f4v{0, 0, 0, 0};
+
+  float ra[NP][EPT], rb[NP][EPT];
+  int lr[EPT], lk[EPT], mr[EPT], mk[EPT];
+#pragma unroll
+  for (int j = 0; j < EPT; ++j) {
+    if (AKC) { lr[j] = t / BK + (NT / BK) * j; lk[j] = t % BK; }
+    else     { lr[j] = t % BM; lk[j] = t / BM + (NT / BM) * j; }
+    if (BKC) { mr[j] = t / BK + (NT / BK) * j; mk[j] = t % BK; }
+    else     { mr[j] = t % BN; mk[j] = t / BN + (NT / BN) * j; }
+  }
+  Cursor cur;
+  cur.start(segs, it.seg_begin, it.seg_end);
+  auto gload = [&](const Cursor& c) {
+    const GemmSeg& s = c.s;
+    const float al = (float)s.alpha;
+#pragma unroll
+    for (int j = 0; j < EPT; ++j) {
+      int gr = it.m0 + lr[j], gk = c.k + lk[j];
+      bool ok = gr < it.M && gk < s.k1;
+      int64_t e = AKC ? s.a_off + (int64_t)gr * s.K + gk : s.a_off + (int64_t)gk * it.M + gr;
+      if (CPLX) {
+        ra[0][j] = ok ? A[2 * e] * al : 0.f;
+        ra[NP - 1][j] = ok ? A[2 * e + 1] * al : 0.f;
+      } else {
+        ra[0][j] = ok ? A[e] * al : 0.f;
+      }
+      int gn = it.n0 + mr[j], gk2 = c.k + mk[j];
+      bool ok2 = gn < it.N && gk2 < s.k1;
+      int64_t f = BKC ? s.b_off + (int64_t)gn * s.K + gk2 : s.b_off + (int64_t)gk2 * it.N + gn;
+      if (CPLX) {
+        rb[0][j] = ok2 ? B[2 * f] : 0.f;
+        rb[NP - 1][j] = ok2 ? B[2 * f + 1] : 0.f;
+      } else {
+        rb[0][j] = ok2 ? B[f] : 0.f;
+      }
+    }
+  };
+  if (cur.valid()) gload(cur);
+  while (cur.valid()) {
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int j = 0; j < EPT; ++j) {
+        unsigned short h = f2bf_rne(ra[p][j]);
+        sA[p][0][lr[j] * LDK + lk[j]] = h;
+        sA[p][1][lr[j] * LDK + lk[j]] = f2bf_rne(ra[p][j] - bf2f(h));
+        unsigned short g = f2bf_rne(rb[p][j]);
+        sB[p][0][mr[j] * LDK + mk[j]] = g;
+        sB[p][1][mr[j] * LDK + mk[j]] = f2bf_rne(rb[p][j] - bf2f(g));
+      }
+    __syncthreads();
+    cur.next<BK>(segs);
+    if (cur.valid()) gload(cur);
+    const int kx = (lane >> 4) * 8;
+    bf8v a[NP][2][2], b[NP][2][2];  // [part][hi/lo][tile]
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          u16x8 va = *reinterpret_cast<const u16x8*>(&sA[p][h][(wm + i * 16 + (lane & 15)) * LDK + kx]);
+          u16x8 vb = *reinterpret_cast<const u16x8*>(&sB[p][h][(wn + i * 16 + (lane & 15)) * LDK + kx]);
+          a[p][h][i] = __builtin_bit_cast(bf8v, va);
+          b[p][h][i] = __builtin_bit_cast(bf8v, vb);
+        }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        // real part (and the whole product when real): Ar*Br (- Ai*Bi)
+        acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][1][i], b[0][0][j], acc[0][i][j], 0, 0, 0);
+        acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][0][i], b[0][1][j], acc[0][i][j], 0, 0, 0);
+        acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][0][i], b[0][0][j], acc[0][i][j], 0, 0, 0);
+        if (CPLX) {
+          u16x8 m0 = __builtin_bit_cast(u16x8, a[NP - 1][0][i]) ^ (unsigned short)0x8000;
+          u16x8 m1 = __builtin_bit_cast(u16x8, a[NP - 1][1][i]) ^ (unsigned short)0x8000;
+          bf8v nih = __builtin_bit_cast(bf8v, m0), nil = __builtin_bit_cast(bf8v, m1);
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nil, b[NP - 1][0][j], acc[0][i][j], 0, 0, 0);
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, b[NP - 1][1][j], acc[0][i][j], 0, 0, 0);
+          acc[0][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(nih, b[NP - 1][0][j], acc[0][i][j], 0, 0, 0);
+          // imaginary part: Ar*Bi + Ai*Br
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][1][i], b[1][0][j], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][0][i], b[1][1][j], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][0][i], b[1][0][j], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][1][i], b[0][0][j], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][0][i], b[0][1][j], acc[1][i][j], 0, 0, 0);
+          acc[1][i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][0][i], b[0][0][j], acc[1][i][j], 0, 0, 0);
+        }
+      }
+  }
+
+  const bool atomic = it.flags & ITEM_ATOMIC;
+  const bool beta = it.flags & ITEM_BETA;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = it.m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        int col = it.n0 + wn + j * 16 + (lane & 15);
+        if (row < it.M && col < it.N) {
+          int64_t e = it.c_off + (int64_t)row * it.N + col;
+          if (CPLX) {
+            float vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
+            if (atomic) { atomic_add(&C[2 * e], vr); atomic_add(&C[2 * e + 1], vi); }
+            else if (beta) { C[2 * e] += vr; C[2 * e + 1] += vi; }
+            else { C[2 * e] = vr; C[2 * e + 1] = vi; }
+          } else {
+            float v = acc[0][i][j][r];
+            if (atomic) atomic_add(&C[e], v);
+            else if (beta) C[e] += v;
+            else C[e] = v;
+          }
+        }
+      }
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] = (float)s[i];
+}
+__global__ void k_f32_to_f64(const float* __restrict__ s, double* __restrict__ d, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) d[i] = (double)s[i];
+}
+
+template <typename S, bool CPLX>
+void launch4(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, int per) {
+  auto it = (const GemmItem*)p.d_items->ptr;
+  auto sg = (const GemmSeg*)p.d_segs->ptr;
+  const bool akc = (p.ta == 'N' || p.ta == 'n'), bkc = (p.tb == 'T' || p.tb == 't');
+  dim3 g(p.grid), b(NT);
+#define L4(AK, BKc) hipLaunchKernelGGL((k_gemm4<S, CPLX, AK, BKc>), g, b, 0, s, it, p.n_items, per, sg, \
+                                       (const S*)A, (const S*)B, (S*)C)
+  if (akc && bkc) L4(true, true);
+  else if (akc) L4(true, false);
+  else if (bkc) L4(false, true);
+  else L4(false, false);
+#undef L4
+}
+
+template <bool CPLX>
+void launch_bf(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, int per) {
+  auto it = (const GemmItem*)p.d_items->ptr;
+  auto sg = (const GemmSeg*)p.d_segs->ptr;
+  const bool akc = (p.ta == 'N' || p.ta == 'n'), bkc = (p.tb == 'T' || p.tb == 't');
+  dim3 g(p.grid), b(NT);
+#define LB(AK, BKc) hipLaunchKernelGGL((k_gemm_bf16x3<CPLX, AK, BKc>), g, b, 0, s, it, p.n_items, per, sg, \
+                                       (const float*)A, (const float*)B, (float*)C)
+  if (akc && bkc) LB(true, true);
+  else if (akc) LB(true, false);
+  else if (bkc) LB(false, true);
+  else LB(false, false);
+#undef LB
+}
+
+}  // namespace
+
+std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, int dtype, int prec,
+                                         char ta, char tb,
+                                         const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
+                                         int split_target) {
+  ensure_init();
+  auto plan = std::make_unique<GemmPlan>();
+  plan->dtype = dtype;
+  plan->prec = prec;
+  plan->ta = ta;
+  plan->tb = tb;
+  const int BKq = prec == PREC_BF16X3 ? 32 : 16;
+  const double cf = dtype == DT_C64 ? 8.0 : 2.0;
+
+  // group valid tasks by output block, first-appearance order
+  struct Group { int64_t c_off; int M, N; int beta; std::vector<int> t; int64_t ksum = 0; };
+  std::vector<Group> groups;
+  std::map<int64_t, int> gidx;
+  for (int i = 0; i < (int)tasks_in.size(); ++i) {
+    const GemmTask& k = tasks_in[i];
+    if (k.M <= 0 || k.N <= 0 || k.K <= 0) continue;  // BLAS wrapper skips empty GEMMs
+    auto f = gidx.find(k.c_off);
+    if (f == gidx.end()) {
+      gidx[k.c_off] = (int)groups.size();
+      groups.push_back(Group{k.c_off, k.M, k.N, k.beta, {}, 0});
+      f = gidx.find(k.c_off);
+    }
+    Group& g = groups[f->second];
+    DMRGX_CHECK(g.M == k.M && g.N == k.N, "gemm plan: tasks into one output block disagree on shape");
+    g.t.push_back(i);
+    g.ksum += k.K;
+    plan->flops += cf * (double)k.M * k.N * k.K;
+  }
+  // split-K so the launch has enough work items for every CU
+  double work = 0;
+  int64_t tiles_total = 0;
+  for (auto& g : groups) {
+    int64_t tl = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+    tiles_total += tl;
+    work += (double)tl * g.ksum;
+  }
+  int target = split_target > 0 ? split_target : 4 * std::max(rt().num_cu, 1);
+  int64_t kchunk = (int64_t)std::max(8.0 * BKq, work / std::max<double>(target, 1.0));
+  kchunk = (kchunk + BKq - 1) / BKq * BKq;
+
+  std::vector<GemmItem> items;
+  std::vector<GemmSeg> segs;
+  for (auto& g : groups) {
+    int nch = (int)std::max<int64_t>(1, (g.ksum + kchunk - 1) / kchunk);
+    int64_t ch = (g.ksum + nch - 1) / nch;
+    ch = (ch + BKq - 1) / BKq * BKq;
+    nch = (int)((g.ksum + ch - 1) / ch);
+    const bool shared = nch > 1;
+    if (shared && g.beta == 0) plan->zero_ranges.push_back({g.c_off, (int64_t)g.M * g.N});
+    // cut the concatenated K of the group's tasks into chunks of length ch
+    std::vector<std::pair<int, int>> chunk_rng;  // seg range per chunk
+    size_t ti = 0;
+    int64_t kin = 0;  // position inside current task
+    for (int c = 0; c < nch; ++c) {
+      int sb = (int)segs.size();
+      int64_t need = ch;
+      while (need > 0 && ti < g.t.size()) {
+        const GemmTask& k = tasks_in[g.t[ti]];
+        int64_t take = std::min<int64_t>(need, k.K - kin);
+        GemmSeg s{};
+        s.a_off = k.a_off;
+        s.b_off = k.b_off;
+        s.K = k.K;
+        s.k0 = (int)kin;
+        s.k1 = (int)(kin + take);
+        s.alpha = k.alpha;
+        segs.push_back(s);
+        need -= take;
+        kin += take;
+        if (kin >= k.K) { ++ti; kin = 0; }
+      }
+      chunk_rng.push_back({sb, (int)segs.size()});
+    }
+    for (int c = 0; c < nch; ++c)
+      for (int n0 = 0; n0 < g.N; n0 += BN)
+        for (int m0 = 0; m0 < g.M; m0 += BM) {
+          GemmItem itm{};
+          itm.c_off = g.c_off;
+          itm.M = g.M;
+          itm.N = g.N;
+          itm.m0 = m0;
+          itm.n0 = n0;
+          itm.seg_begin = chunk_rng[c].first;
+          itm.seg_end = chunk_rng[c].second;
+          itm.flags = shared ? ITEM_ATOMIC : (g.beta ? ITEM_BETA : 0);
+          items.push_back(itm);
+        }
+  }
+  for (auto& z : extra_zero) plan->zero_ranges.push_back(z);
+  plan->n_items = (int)items.size();
+  plan->n_segs = (int)segs.size();
+  int per = (plan->n_items + 7) / 8;
+  plan->grid = per * 8;
+  if (plan->n_items) {
+    plan->d_items = new DevBuf(items.size() * sizeof(GemmItem));
+    plan->d_segs = new DevBuf(std::max<size_t>(segs.size(), 1) * sizeof(GemmSeg));
+    DMRGX_HIP(hipMemcpyAsync(plan->d_items->ptr, items.data(), items.size() * sizeof(GemmItem),
+                             hipMemcpyHostToDevice, rt().stream));
+    if (!segs.empty())
+      DMRGX_HIP(hipMemcpyAsync(plan->d_segs->ptr, segs.data(), segs.size() * sizeof(GemmSeg),
+                               hipMemcpyHostToDevice, rt().stream));
+    DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  }
+  return plan;
+}
+
+void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s) {
+  const size_t eb = (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);
+  for (auto& z : p.zero_ranges) fill_zero((char*)C + z.first * eb, (size_t)z.second * eb, s);
+  if (!p.n_items) return;
+  const int per = p.grid / 8;
+  if (p.prec == PREC_FP64) {
+    if (p.dtype == DT_C64) launch4<double, true>(p, A, B, C, s, per);
+    else launch4<double, false>(p, A, B, C, s, per);
+  } else if (p.prec == PREC_FP32) {
+    if (p.dtype == DT_C64) launch4<float, true>(p, A, B, C, s, per);
+    else launch4<float, false>(p, A, B, C, s, per);
+  } else {
+    if (p.dtype == DT_C64) launch_bf<true>(p, A, B, C, s, per);
+    else launch_bf<false>(p, A, B, C, s, per);
+  }
+  DMRGX_HIP(hipGetLastError());
+}
+
+void convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_f64_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+  DMRGX_HIP(hipGetLastError());
+}
+void convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_f32_to_f64, dim3((unsigned)blocks), dim3(256), 0, s, src, dst, n);
+  DMRGX_HIP(hipGetLastError());
+}
+
+}  // namespace dmrgx

@@ -1,0 +1,262 @@
+// H_eff plans and the C entry points of the task GEMM, the matvec and the
+// eigensolver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "../../include/dmrgx.h"
+#include "common.hpp"
+#include "gemm.hpp"
+#include "heff.hpp"
+#include "lanczos.hpp"
+
+namespace dmrgx {
+
+std::vector<GemmTask> tasks_from_map(const int32_t* map, int n_task, const int64_t* blk_off,
+                                     const int64_t* blk_size) {
+  std::vector<GemmTask> t;
+  t.reserve(n_task);
+  for (int i = 0; i < n_task; ++i) {
+    const int32_t* m = map + 7 * i;
+    GemmTask g{};
+    int col = m[4], row = m[3];
+    DMRGX_CHECK(col > 0 && row > 0, "task with empty row/col");
+    DMRGX_CHECK(blk_size[m[0]] % col == 0, "task col does not divide the block size");
+    g.a_off = blk_off[m[0]];
+    g.b_off = m[1];
+    g.c_off = m[2];
+    g.M = (int)(blk_size[m[0]] / col);
+    g.N = row;
+    g.K = col;
+    g.beta = m[5];
+    g.alpha = m[6];
+    t.push_back(g);
+  }
+  return t;
+}
+
+std::vector<std::pair<int64_t, int64_t>> uncovered(const std::vector<GemmTask>& t, int64_t n_out) {
+  std::vector<std::pair<int64_t, int64_t>> iv;
+  for (auto& g : t) iv.push_back({g.c_off, g.c_off + (int64_t)g.M * g.N});
+  std::sort(iv.begin(), iv.end());
+  std::vector<std::pair<int64_t, int64_t>> gaps;
+  int64_t pos = 0;
+  for (auto& r : iv) {
+    if (r.first > pos) gaps.push_back({pos, r.first - pos});
+    pos = std::max(pos, r.second);
+  }
+  if (pos < n_out) gaps.push_back({pos, n_out - pos});
+  return gaps;
+}
+
+static std::unique_ptr<DevBuf> to_f32(const void* src, int64_t nreal, hipStream_t s) {
+  auto b = std::make_unique<DevBuf>(std::max<int64_t>(nreal, 1) * 4);
+  convert_f64_to_f32((const double*)src, (float*)b->ptr, nreal, s);
+  return b;
+}
+
+std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const int64_t* rblk_off,
+                                    const int64_t* rblk_size, int n_rblk, int64_t renv_len,
+                                    const int32_t* rmap, int n_rtask, const void* lenv,
+                                    const int64_t* lblk_off, const int64_t* lblk_size, int n_lblk,
+                                    int64_t lenv_len, const int32_t* lmap, int n_ltask, int64_t n_psi,
+                                    int64_t n_store) {
+  ensure_init();
+  DMRGX_CHECK(dtype == DT_R64 || dtype == DT_C64, "heff: dtype must be R64 or C64");
+  DMRGX_CHECK(prec >= PREC_FP64 && prec <= PREC_BF16X3, "heff: unknown precision");
+  for (int i = 0; i < n_rtask; ++i)
+    DMRGX_CHECK(rmap[7 * i] >= 0 && rmap[7 * i] < n_rblk, "heff: rmap block index out of range");
+  for (int i = 0; i < n_ltask; ++i)
+    DMRGX_CHECK(lmap[7 * i] >= 0 && lmap[7 * i] < n_lblk, "heff: lmap block index out of range");
+  auto h = std::make_unique<HeffPlan>();
+  h->dtype = dtype;
+  h->prec = prec;
+  h->n_psi = n_psi;
+  h->n_store = n_store;
+  h->renv = renv;
+  h->lenv = lenv;
+  auto rt_ = tasks_from_map(rmap, n_rtask, rblk_off, rblk_size);
+  auto lt_ = tasks_from_map(lmap, n_ltask, lblk_off, lblk_size);
+  for (auto& g : rt_) {
+    DMRGX_CHECK(g.b_off + (int64_t)g.N * g.K <= n_psi, "heff: rmap reads past the end of psi");
+    DMRGX_CHECK(g.c_off + (int64_t)g.M * g.N <= n_store, "heff: rmap writes past the end of store");
+    DMRGX_CHECK(g.a_off + (int64_t)g.M * g.K <= renv_len, "heff: rmap block past the end of renv");
+  }
+  for (auto& g : lt_) {
+    DMRGX_CHECK(g.b_off + (int64_t)g.N * g.K <= n_store, "heff: lmap reads past the end of store");
+    DMRGX_CHECK(g.c_off + (int64_t)g.M * g.N <= n_psi, "heff: lmap writes past the end of psi");
+    DMRGX_CHECK(g.a_off + (int64_t)g.M * g.K <= lenv_len, "heff: lmap block past the end of lenv");
+  }
+  h->rplan = make_gemm_plan(rt_, dtype, prec, 'N', 'T', uncovered(rt_, n_store));
+  h->lplan = make_gemm_plan(lt_, dtype, prec, 'N', 'T', uncovered(lt_, n_psi));
+  h->flops = h->rplan->flops + h->lplan->flops;
+  const int c = dtype == DT_C64 ? 2 : 1;
+  hipStream_t s = rt().stream;
+  if (prec == PREC_FP64) {
+    h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 8 * c);
+  } else {
+    h->renv_lo = to_f32(renv, renv_len * c, s);
+    h->lenv_lo = to_f32(lenv, lenv_len * c, s);
+    h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 4 * c);
+    h->in_lo = std::make_unique<DevBuf>(std::max<int64_t>(n_psi, 1) * 4 * c);
+    h->out_lo = std::make_unique<DevBuf>(std::max<int64_t>(n_psi, 1) * 4 * c);
+  }
+  DMRGX_HIP(hipStreamSynchronize(s));
+  return h;
+}
+
+void HeffPlan::apply(const double* in, double* out, hipStream_t s) {
+  const int c = dtype == DT_C64 ? 2 : 1;
+  if (prec == PREC_FP64) {
+    run_gemm_plan(*rplan, renv, in, store->ptr, s);
+    run_gemm_plan(*lplan, lenv, store->ptr, out, s);
+  } else {
+    convert_f64_to_f32(in, (float*)in_lo->ptr, n_psi * c, s);
+    run_gemm_plan(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s);
+    run_gemm_plan(*lplan, lenv_lo->ptr, store->ptr, out_lo->ptr, s);
+    convert_f32_to_f64((const float*)out_lo->ptr, out, n_psi * c, s);
+  }
+}
+
+}  // namespace dmrgx
+
+using namespace dmrgx;
+
+struct dmrgx_heff {
+  std::unique_ptr<HeffPlan> p;
+};
+
+#define CAPI_BEGIN try {
+#define CAPI_END                      \
+  return 0;                           \
+  }                                   \
+  catch (const dmrgx::Error& e) {     \
+    set_last_error(e.what());         \
+    return e.code;                    \
+  }                                   \
+  catch (const std::exception& e) {   \
+    set_last_error(e.what());         \
+    return -2;                        \
+  }
+
+static int64_t blocks_len(const int64_t* off, const int64_t* size, int n) {
+  int64_t e = 0;
+  for (int i = 0; i < n; ++i) e = std::max(e, off[i] + size[i]);
+  return e;
+}
+
+extern "C" {
+
+int dmrgx_task_gemm(int dtype, int prec, const void* A, const int64_t* blk_off, const int64_t* blk_size,
+                    int n_blk, const int32_t* map, int n_task, const void* in, void* out, int64_t n_out) {
+  CAPI_BEGIN
+  ensure_init();
+  hipStream_t s = rt().stream;
+  for (int i = 0; i < n_task; ++i)
+    DMRGX_CHECK(map[7 * i] >= 0 && map[7 * i] < n_blk, "task_gemm: block index out of range");
+  auto t = tasks_from_map(map, n_task, blk_off, blk_size);
+  int64_t n_in = 0;
+  for (auto& g : t) {
+    n_in = std::max(n_in, g.b_off + (int64_t)g.N * g.K);
+    DMRGX_CHECK(g.c_off + (int64_t)g.M * g.N <= n_out, "task_gemm: output past n_out");
+  }
+  int64_t alen = blocks_len(blk_off, blk_size, n_blk);
+  // outputs without any task are left untouched, as the reference's loop does
+  auto plan = make_gemm_plan(t, dtype, prec, 'N', 'T', {});
+  const int c = dtype == DT_C64 ? 2 : 1;
+  if (prec == PREC_FP64) {
+    run_gemm_plan(*plan, A, in, out, s);
+  } else {
+    DevBuf a32(std::max<int64_t>(alen, 1) * 4 * c), i32(std::max<int64_t>(n_in, 1) * 4 * c),
+        o32(std::max<int64_t>(n_out, 1) * 4 * c);
+    convert_f64_to_f32((const double*)A, (float*)a32.ptr, alen * c, s);
+    convert_f64_to_f32((const double*)in, (float*)i32.ptr, n_in * c, s);
+    convert_f64_to_f32((const double*)out, (float*)o32.ptr, n_out * c, s);
+    run_gemm_plan(*plan, a32.ptr, i32.ptr, o32.ptr, s);
+    convert_f32_to_f64((const float*)o32.ptr, (double*)out, n_out * c, s);
+    DMRGX_HIP(hipStreamSynchronize(s));
+  }
+  DMRGX_HIP(hipStreamSynchronize(s));
+  CAPI_END
+}
+
+int dmrgx_heff_create(dmrgx_heff** plan, int dtype, int prec, const void* renv, const int64_t* rblk_off,
+                      const int64_t* rblk_size, int n_rblk, const int32_t* rmap, int n_rtask,
+                      const void* lenv, const int64_t* lblk_off, const int64_t* lblk_size, int n_lblk,
+                      const int32_t* lmap, int n_ltask, int64_t n_psi, int64_t n_store) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan != nullptr, "heff_create: null plan pointer");
+  auto h = new dmrgx_heff;
+  try {
+    h->p = make_heff(dtype, prec, renv, rblk_off, rblk_size, n_rblk, blocks_len(rblk_off, rblk_size, n_rblk),
+                     rmap, n_rtask, lenv, lblk_off, lblk_size, n_lblk, blocks_len(lblk_off, lblk_size, n_lblk),
+                     lmap, n_ltask, n_psi, n_store);
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *plan = h;
+  CAPI_END
+}
+
+int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p, "heff_apply: null plan");
+  plan->p->apply((const double*)in, (double*)out, rt().stream);
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  CAPI_END
+}
+
+double dmrgx_heff_flops(const dmrgx_heff* plan) { return plan && plan->p ? plan->p->flops : 0.0; }
+
+int dmrgx_heff_destroy(dmrgx_heff* plan) {
+  CAPI_BEGIN
+  if (plan) {
+    (void)hipStreamSynchronize(rt().stream);
+    delete plan;
+  }
+  CAPI_END
+}
+
+int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart, uint64_t seed,
+                const void* v0, double* evals, void* evec, dmrgx_heff* refine, int refine_k, int* info) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p, "eigsh: null plan");
+  HeffPlan* h = plan->p.get();
+  hipStream_t s = rt().stream;
+  MatVec op = [h, s](const double* x, double* y) { h->apply(x, y, s); };
+  EigshResult r = eigsh(op, h->n_psi, h->dtype == DT_C64, nev, ncv, tol, max_restart, seed,
+                        (const double*)v0, (double*)evec);
+  for (int i = 0; i < (int)r.evals.size(); ++i) evals[i] = r.evals[i];
+  int polish_mv = 0;
+  if (refine && refine->p && h->prec != PREC_FP64) {
+    HeffPlan* f = refine->p.get();
+    MatVec op64 = [f, s](const double* x, double* y) { f->apply(x, y, s); };
+    evals[0] = krylov_polish(op64, f->n_psi, f->dtype == DT_C64, std::max(refine_k, 1), (double*)evec,
+                             &polish_mv);
+  }
+  if (info) {
+    info[0] = r.n_matvec;
+    info[1] = r.n_restart;
+    info[2] = r.converged ? 1 : 0;
+    info[3] = polish_mv;
+  }
+  CAPI_END
+}
+
+int dmrgx_block_gemm(int dtype, char ta, char tb, const void* A, const void* B, void* C, const int64_t* tasks,
+                     int n_task) {
+  CAPI_BEGIN
+  ensure_init();
+  std::vector<GemmTask> t(n_task);
+  for (int i = 0; i < n_task; ++i) {
+    const int64_t* q = tasks + 7 * i;
+    t[i] = GemmTask{q[0], q[1], q[2], (int)q[3], (int)q[4], (int)q[5], 1, (int)q[6]};
+  }
+  auto plan = make_gemm_plan(t, dtype, PREC_FP64, ta, tb, {});
+  run_gemm_plan(*plan, A, B, C, rt().stream);
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  CAPI_END
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// Effective-Hamiltonian plan: the two task-list passes of av().
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+#include "gemm.hpp"
+
+namespace dmrgx {
+
+struct HeffPlan {
+  int dtype = DT_R64;
+  int prec = PREC_FP64;
+  int64_t n_psi = 0, n_store = 0;
+  const void* renv = nullptr;  // FP64 env storage (caller-owned)
+  const void* lenv = nullptr;
+  std::unique_ptr<DevBuf> renv_lo, lenv_lo;  // FP32 copies for reduced precision
+  std::unique_ptr<DevBuf> store, in_lo, out_lo;
+  std::unique_ptr<GemmPlan> rplan, lplan;
+  double flops = 0;
+  void apply(const double* in, double* out, hipStream_t s);
+};
+
+// get_map tuples -> GEMM tasks (M = blk_size / col, N = row, K = col).
+std::vector<GemmTask> tasks_from_map(const int32_t* map, int n_task, const int64_t* blk_off,
+                                     const int64_t* blk_size);
+// C ranges in [0, n_out) that no task writes (they must read as zero).
+std::vector<std::pair<int64_t, int64_t>> uncovered(const std::vector<GemmTask>& t, int64_t n_out);
+
+std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const int64_t* rblk_off,
+                                    const int64_t* rblk_size, int n_rblk, int64_t renv_len,
+                                    const int32_t* rmap, int n_rtask, const void* lenv,
+                                    const int64_t* lblk_off, const int64_t* lblk_size, int n_lblk,
+                                    int64_t lenv_len, const int32_t* lmap, int n_ltask, int64_t n_psi,
+                                    int64_t n_store);
+
+}  // namespace dmrgx

@@ -1,0 +1,491 @@
+// FP64 thick-restart Lanczos (Wu & Simon) with full reorthogonalisation.
+// Krylov basis, reorthogonalisation and restarts stay on the device; only the
+// small projected matrix (<= ncv x ncv, real symmetric) is diagonalised on the
+// host.  Replaces the ARPACK dsaupd/znaupd reverse-communication loop of the
+// reference (src/dsaupd.cpp:65-155, src/znaupd.cpp:78-197) with the same
+// wanted set (nev lowest), ncv and tolerance semantics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+#include "common.hpp"
+#include "lanczos.hpp"
+
+namespace dmrgx {
+namespace {
+
+constexpr int G = 8;  // Krylov columns handled per pass of the dot / combine kernels
+constexpr int DOT_BLOCKS_MAX = 1024;
+
+// Partial <V_j, w> (conjugate-linear in V) for columns j0..j0+g-1.
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int64_t ld, int j0, int g,
+                                               const double* __restrict__ w, int64_t n,
+                                               double* __restrict__ partial) {
+  double re[G], im[G];
+#pragma unroll
+  for (int c = 0; c < G; ++c) re[c] = im[c] = 0;
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    if (CPLX) {
+      double wr = w[2 * i], wi = w[2 * i + 1];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) {
+          const double* v = V + (int64_t)(j0 + c) * ld;
+          double vr = v[2 * i], vi = v[2 * i + 1];
+          re[c] += vr * wr + vi * wi;
+          im[c] += vr * wi - vi * wr;
+        }
+    } else {
+      double wr = w[i];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) re[c] += V[(int64_t)(j0 + c) * ld + i] * wr;
+    }
+  }
+  __shared__ double red[2 * G][256];
+#pragma unroll
+  for (int c = 0; c < G; ++c) {
+    red[2 * c][threadIdx.x] = re[c];
+    red[2 * c + 1][threadIdx.x] = im[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < 2 * G; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 2 * G) partial[blockIdx.x * 2 * G + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_reduce_partials(const double* __restrict__ partial, int nblk, int j0, int g,
+                                  double* __restrict__ out) {
+  // one wave per value (2*g values)
+  const int q = threadIdx.x / 64, lane = threadIdx.x & 63;
+  if (q >= 2 * g) return;
+  double s = 0;
+  for (int b = lane; b < nblk; b += 64) s += partial[b * 2 * G + q];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) out[2 * j0 + q] = s;  // out[2j] = re, out[2j+1] = im
+}
+
+// w -= sum_j V_j h_j
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_gemv_sub(const double* __restrict__ V, int64_t ld, int k,
+                                                  const double* __restrict__ h, double* __restrict__ w,
+                                                  int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    if (CPLX) {
+      double ar = 0, ai = 0;
+      for (int j = 0; j < k; ++j) {
+        const double* v = V + (int64_t)j * ld;
+        double vr = v[2 * i], vi = v[2 * i + 1], hr = h[2 * j], hi = h[2 * j + 1];
+        ar += vr * hr - vi * hi;
+        ai += vr * hi + vi * hr;
+      }
+      w[2 * i] -= ar;
+      w[2 * i + 1] -= ai;
+    } else {
+      double a = 0;
+      for (int j = 0; j < k; ++j) a += V[(int64_t)j * ld + i] * h[2 * j];
+      w[i] -= a;
+    }
+  }
+}
+
+// out_c = sum_j V_j Y[j][c] for c in [c0, c0+g), real coefficients Y (m x ky row-major)
+__global__ __launch_bounds__(256) void k_combine(const double* __restrict__ V, int64_t ld, int m,
+                                                 const double* __restrict__ Y, int ky, int c0, int g,
+                                                 double* __restrict__ out, int64_t nr) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nr; i += stride) {
+    double acc[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) acc[c] = 0;
+    for (int j = 0; j < m; ++j) {
+      double v = V[(int64_t)j * ld + i];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) acc[c] += v * Y[j * ky + c0 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < G; ++c)
+      if (c < g) out[(int64_t)(c0 + c) * ld + i] = acc[c];
+  }
+}
+
+__global__ void k_scale(double* __restrict__ v, int64_t nr, double a) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nr; i += stride) v[i] *= a;
+}
+
+__global__ void k_scale_copy(const double* __restrict__ s, double* __restrict__ d, int64_t nr, double a) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nr; i += stride) d[i] = s[i] * a;
+}
+
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+__global__ void k_random(double* __restrict__ v, int64_t nr, uint64_t seed) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < nr; i += stride) {
+    uint64_t x = splitmix64(seed ^ splitmix64((uint64_t)i));
+    v[i] = (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+  }
+}
+
+inline unsigned grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(b, 4096));
+}
+
+struct Work {
+  DevBuf* partial = nullptr;
+  DevBuf* h = nullptr;   // 2*(ncv+2) doubles
+  ~Work() { delete partial; delete h; }
+};
+
+// h[0..k) = V[0..k)^H w   (device result in h, complex pairs)
+void vdots(const double* V, int64_t ld, int k, const double* w, int64_t n, bool cplx, double* h,
+           double* partial, hipStream_t s) {
+  int nblk = (int)std::min<int64_t>(DOT_BLOCKS_MAX, std::max<int64_t>(1, (n + 1023) / 1024));
+  for (int j0 = 0; j0 < k; j0 += G) {
+    int g = std::min(G, k - j0);
+    if (cplx) hipLaunchKernelGGL(k_vdots<true>, dim3(nblk), dim3(256), 0, s, V, ld, j0, g, w, n, partial);
+    else hipLaunchKernelGGL(k_vdots<false>, dim3(nblk), dim3(256), 0, s, V, ld, j0, g, w, n, partial);
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64 * 2 * G), 0, s, partial, nblk, j0, g, h);
+  }
+}
+
+void gemv_sub(const double* V, int64_t ld, int k, const double* h, double* w, int64_t n, bool cplx,
+              hipStream_t s) {
+  if (k <= 0) return;
+  if (cplx) hipLaunchKernelGGL(k_gemv_sub<true>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, k, h, w, n);
+  else hipLaunchKernelGGL(k_gemv_sub<false>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, k, h, w, n);
+}
+
+}  // namespace
+
+void dev_random(double* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s) {
+  int64_t nr = n * (cplx ? 2 : 1);
+  hipLaunchKernelGGL(k_random, dim3(grid_for(nr)), dim3(256), 0, s, v, nr, seed);
+}
+
+void dev_scale(double* v, int64_t n, bool cplx, double a, hipStream_t s) {
+  int64_t nr = n * (cplx ? 2 : 1);
+  hipLaunchKernelGGL(k_scale, dim3(grid_for(nr)), dim3(256), 0, s, v, nr, a);
+}
+
+void dev_dot(const double* a, const double* b, int64_t n, bool cplx, double out[2], hipStream_t s) {
+  DevBuf partial(DOT_BLOCKS_MAX * 2 * G * sizeof(double));
+  DevBuf h(2 * G * sizeof(double));
+  vdots(a, 0, 1, b, n, cplx, (double*)h.ptr, (double*)partial.ptr, s);
+  DMRGX_HIP(hipMemcpyAsync(out, h.ptr, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  DMRGX_HIP(hipStreamSynchronize(s));
+}
+
+double dev_norm2(const double* v, int64_t n, bool cplx, hipStream_t s) {
+  double o[2];
+  dev_dot(v, v, n, cplx, o, s);
+  return std::sqrt(std::max(o[0], 0.0));
+}
+
+void host_symeig(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& v) {
+  v.assign((size_t)n * n, 0.0);
+  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+  for (int sweep = 0; sweep < 100; ++sweep) {
+    double off = 0, diag = 0;
+    for (int p = 0; p < n; ++p) {
+      diag += a[(size_t)p * n + p] * a[(size_t)p * n + p];
+      for (int q = p + 1; q < n; ++q) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+    }
+    if (off <= 1e-32 * std::max(diag, 1e-300)) break;
+    for (int p = 0; p < n; ++p)
+      for (int q = p + 1; q < n; ++q) {
+        double apq = a[(size_t)p * n + q];
+        if (std::fabs(apq) < 1e-300) continue;
+        double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
+        double tau = (aqq - app) / (2 * apq);
+        double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1 + tau * tau));
+        double c = 1 / std::sqrt(1 + t * t), s = t * c;
+        for (int k = 0; k < n; ++k) {  // columns p,q
+          double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+          a[(size_t)k * n + p] = c * akp - s * akq;
+          a[(size_t)k * n + q] = s * akp + c * akq;
+        }
+        for (int k = 0; k < n; ++k) {  // rows p,q
+          double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+          a[(size_t)p * n + k] = c * apk - s * aqk;
+          a[(size_t)q * n + k] = s * apk + c * aqk;
+        }
+        for (int k = 0; k < n; ++k) {
+          double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
+          v[(size_t)k * n + p] = c * vkp - s * vkq;
+          v[(size_t)k * n + q] = s * vkp + c * vkq;
+        }
+      }
+  }
+  std::vector<int> idx(n);
+  for (int i = 0; i < n; ++i) idx[i] = i;
+  std::sort(idx.begin(), idx.end(), [&](int x, int y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
+  w.resize(n);
+  std::vector<double> vs((size_t)n * n);
+  for (int c = 0; c < n; ++c) {
+    w[c] = a[(size_t)idx[c] * n + idx[c]];
+    for (int k = 0; k < n; ++k) vs[(size_t)k * n + c] = v[(size_t)k * n + idx[c]];
+  }
+  v.swap(vs);
+}
+
+void host_heeig(int n, std::vector<cplx>& a, std::vector<double>& w, std::vector<cplx>& v) {
+  // embed the Hermitian matrix H = X + iY as the real symmetric [[X, -Y], [Y, X]]
+  int m = 2 * n;
+  std::vector<double> r((size_t)m * m);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      cplx h = 0.5 * (a[(size_t)i * n + j] + std::conj(a[(size_t)j * n + i]));
+      r[(size_t)i * m + j] = h.real();
+      r[(size_t)(i + n) * m + j + n] = h.real();
+      r[(size_t)i * m + j + n] = -h.imag();
+      r[(size_t)(i + n) * m + j] = h.imag();
+    }
+  std::vector<double> wr, vr;
+  host_symeig(m, r, wr, vr);
+  // eigenvalues come in pairs; take every other one
+  w.resize(n);
+  v.assign((size_t)n * n, 0.0);
+  for (int c = 0; c < n; ++c) {
+    w[c] = wr[2 * c];
+    double nrm = 0;
+    for (int k = 0; k < n; ++k) {
+      cplx z(vr[(size_t)k * m + 2 * c], vr[(size_t)(k + n) * m + 2 * c]);
+      v[(size_t)k * n + c] = z;
+      nrm += std::norm(z);
+    }
+    nrm = std::sqrt(nrm);
+    for (int k = 0; k < n; ++k) v[(size_t)k * n + c] /= nrm;
+  }
+}
+
+EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
+                  int max_restart, uint64_t seed, const double* v0, double* evec) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  EigshResult res;
+  DMRGX_CHECK(n > 0, "eigsh: empty operator");
+  const int64_t nr = n * (cplx ? 2 : 1);
+  const int64_t ld = nr;
+  nev = (int)std::min<int64_t>(nev, n);
+  ncv = (int)std::min<int64_t>(std::max(ncv, nev + 2), n);
+  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+
+  if (n <= 64 || ncv <= nev + 1) {
+    // materialise the operator column by column and diagonalise on the host
+    DevBuf e(nr * 8), y(nr * 8);
+    std::vector<double> ycol(nr);
+    std::vector<dmrgx::cplx> H((size_t)n * n);
+    for (int64_t j = 0; j < n; ++j) {
+      fill_zero(e.ptr, nr * 8, s);
+      double one = 1.0;
+      DMRGX_HIP(hipMemcpyAsync((double*)e.ptr + (cplx ? 2 * j : j), &one, 8, hipMemcpyHostToDevice, s));
+      op((const double*)e.ptr, (double*)y.ptr);
+      res.n_matvec++;
+      DMRGX_HIP(hipMemcpyAsync(ycol.data(), y.ptr, nr * 8, hipMemcpyDeviceToHost, s));
+      DMRGX_HIP(hipStreamSynchronize(s));
+      for (int64_t i = 0; i < n; ++i)
+        H[(size_t)i * n + j] = cplx ? dmrgx::cplx(ycol[2 * i], ycol[2 * i + 1]) : dmrgx::cplx(ycol[i], 0);
+    }
+    std::vector<double> w;
+    std::vector<dmrgx::cplx> V;
+    host_heeig((int)n, H, w, V);
+    res.evals.assign(w.begin(), w.begin() + nev);
+    std::vector<double> x(nr);
+    for (int64_t i = 0; i < n; ++i) {
+      if (cplx) { x[2 * i] = V[(size_t)i * n].real(); x[2 * i + 1] = V[(size_t)i * n].imag(); }
+      else x[i] = V[(size_t)i * n].real();
+    }
+    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), nr * 8, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipStreamSynchronize(s));
+    res.converged = true;
+    return res;
+  }
+
+  DevBuf Vb((size_t)(ncv + 1) * nr * 8), Tmp((size_t)ncv * nr * 8);
+  DevBuf partial(DOT_BLOCKS_MAX * 2 * G * sizeof(double));
+  DevBuf hbuf(2 * (ncv + 2) * sizeof(double)), h2buf(2 * (ncv + 2) * sizeof(double));
+  DevBuf Yb((size_t)ncv * ncv * sizeof(double));
+  double* V = (double*)Vb.ptr;
+  double* w = (double*)Tmp.ptr;  // scratch vector (first column of Tmp) between restarts
+  double* h = (double*)hbuf.ptr;
+  double* h2 = (double*)h2buf.ptr;
+
+  // start vector
+  if (v0) DMRGX_HIP(hipMemcpyAsync(V, v0, nr * 8, hipMemcpyDeviceToDevice, s));
+  else dev_random(V, n, cplx, seed, s);
+  double nv = dev_norm2(V, n, cplx, s);
+  DMRGX_CHECK(nv > 0, "eigsh: zero start vector");
+  dev_scale(V, n, cplx, 1.0 / nv, s);
+
+  const int m = ncv;
+  std::vector<double> T((size_t)m * m, 0.0);
+  int j0 = 0;
+  std::vector<double> theta, Y;
+  double beta_last = 0;
+  std::vector<double> hh(2 * (m + 2)), hh2(2 * (m + 2));
+  const int keep = std::min(m - 2, nev + std::max(2, (m - nev) / 2));
+
+  for (int restart = 0; restart <= max_restart; ++restart) {
+    res.n_restart = restart;
+    for (int j = j0; j < m; ++j) {
+      double* vj = V + (int64_t)j * ld;
+      op(vj, w);
+      res.n_matvec++;
+      // classical Gram-Schmidt against v_0..v_j, twice
+      vdots(V, ld, j + 1, w, n, cplx, h, (double*)partial.ptr, s);
+      gemv_sub(V, ld, j + 1, h, w, n, cplx, s);
+      vdots(V, ld, j + 1, w, n, cplx, h2, (double*)partial.ptr, s);
+      gemv_sub(V, ld, j + 1, h2, w, n, cplx, s);
+      double nw[2];
+      {
+        vdots(w, 0, 1, w, n, cplx, h2 + 2 * (j + 1), (double*)partial.ptr, s);
+      }
+      DMRGX_HIP(hipMemcpyAsync(hh.data(), h, 2 * (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+      DMRGX_HIP(hipMemcpyAsync(hh2.data(), h2, 2 * (j + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
+      DMRGX_HIP(hipStreamSynchronize(s));
+      nw[0] = hh2[2 * (j + 1)];
+      double alpha = hh[2 * j] + hh2[2 * j];
+      double beta = std::sqrt(std::max(nw[0], 0.0));
+      T[(size_t)j * m + j] = alpha;
+      if (j + 1 < m) {
+        T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = beta;
+      }
+      beta_last = beta;
+      double* vn = V + (int64_t)(j + 1) * ld;
+      if (beta > 1e-300) {
+        hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, w, vn, nr, 1.0 / beta);
+      } else {
+        // invariant subspace: continue with a fresh random direction
+        dev_random(vn, n, cplx, seed + 7919ull * (j + 1) + restart, s);
+        for (int pass = 0; pass < 2; ++pass) {
+          vdots(V, ld, j + 1, vn, n, cplx, h, (double*)partial.ptr, s);
+          gemv_sub(V, ld, j + 1, h, vn, n, cplx, s);
+        }
+        double z = dev_norm2(vn, n, cplx, s);
+        dev_scale(vn, n, cplx, 1.0 / z, s);
+        if (j + 1 < m) T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = 0;
+        beta_last = 0;
+      }
+    }
+    // Rayleigh-Ritz on the projected matrix
+    std::vector<double> A = T;
+    host_symeig(m, A, theta, Y);
+    bool all = true;
+    res.max_resid = 0;
+    for (int i = 0; i < nev; ++i) {
+      double r = std::fabs(beta_last * Y[(size_t)(m - 1) * m + i]);
+      res.max_resid = std::max(res.max_resid, r);
+      if (r > tol * std::max(std::fabs(theta[i]), eps23)) all = false;
+    }
+    if (all || restart == max_restart) {
+      res.converged = all;
+      break;
+    }
+    // thick restart: keep the `keep` lowest Ritz vectors and the residual direction
+    std::vector<double> Yk((size_t)m * keep);
+    for (int r = 0; r < m; ++r)
+      for (int c = 0; c < keep; ++c) Yk[(size_t)r * keep + c] = Y[(size_t)r * m + c];
+    DMRGX_HIP(hipMemcpyAsync(Yb.ptr, Yk.data(), Yk.size() * 8, hipMemcpyHostToDevice, s));
+    for (int c0 = 0; c0 < keep; c0 += G)
+      hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, m, (const double*)Yb.ptr,
+                         keep, c0, std::min(G, keep - c0), (double*)Tmp.ptr, nr);
+    DMRGX_HIP(hipMemcpyAsync(V, Tmp.ptr, (size_t)keep * nr * 8, hipMemcpyDeviceToDevice, s));
+    DMRGX_HIP(hipMemcpyAsync(V + (int64_t)keep * ld, V + (int64_t)m * ld, nr * 8, hipMemcpyDeviceToDevice, s));
+    std::fill(T.begin(), T.end(), 0.0);
+    for (int i = 0; i < keep; ++i) {
+      T[(size_t)i * m + i] = theta[i];
+      double sgl = beta_last * Y[(size_t)(m - 1) * m + i];
+      T[(size_t)i * m + keep] = T[(size_t)keep * m + i] = sgl;
+    }
+    j0 = keep;
+    // the arrow row is consumed by the first step after the restart: its
+    // alpha comes from the Gram-Schmidt coefficients like any other step
+  }
+  res.evals.assign(theta.begin(), theta.begin() + nev);
+  // lowest Ritz vector
+  std::vector<double> y0(m);
+  for (int r = 0; r < m; ++r) y0[r] = Y[(size_t)r * m + 0];
+  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), m * 8, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, m, (const double*)Yb.ptr, 1, 0, 1,
+                     (double*)Tmp.ptr, nr);
+  double z = dev_norm2((double*)Tmp.ptr, n, cplx, s);
+  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, (const double*)Tmp.ptr, evec, nr, 1.0 / z);
+  DMRGX_HIP(hipStreamSynchronize(s));
+  DMRGX_HIP(hipGetLastError());
+  return res;
+}
+
+double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  const int64_t nr = n * (cplx ? 2 : 1);
+  k = (int)std::max<int64_t>(1, std::min<int64_t>(k, n));
+  DevBuf Vb((size_t)(k + 1) * nr * 8), wb(nr * 8);
+  DevBuf partial(DOT_BLOCKS_MAX * 2 * G * sizeof(double));
+  DevBuf hbuf(2 * (k + 2) * 8), h2buf(2 * (k + 2) * 8), Yb((size_t)k * 8);
+  double* V = (double*)Vb.ptr;
+  double* w = (double*)wb.ptr;
+  double nx = dev_norm2(x, n, cplx, s);
+  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, x, V, nr, 1.0 / nx);
+  std::vector<double> T((size_t)k * k, 0.0), hh(2 * (k + 2)), hh2(2 * (k + 2));
+  int kk = k;
+  for (int j = 0; j < k; ++j) {
+    op(V + (int64_t)j * nr, w);
+    if (n_matvec) (*n_matvec)++;
+    vdots(V, nr, j + 1, w, n, cplx, (double*)hbuf.ptr, (double*)partial.ptr, s);
+    gemv_sub(V, nr, j + 1, (double*)hbuf.ptr, w, n, cplx, s);
+    vdots(V, nr, j + 1, w, n, cplx, (double*)h2buf.ptr, (double*)partial.ptr, s);
+    gemv_sub(V, nr, j + 1, (double*)h2buf.ptr, w, n, cplx, s);
+    vdots(w, 0, 1, w, n, cplx, (double*)h2buf.ptr + 2 * (j + 1), (double*)partial.ptr, s);
+    DMRGX_HIP(hipMemcpyAsync(hh.data(), hbuf.ptr, 2 * (j + 1) * 8, hipMemcpyDeviceToHost, s));
+    DMRGX_HIP(hipMemcpyAsync(hh2.data(), h2buf.ptr, 2 * (j + 2) * 8, hipMemcpyDeviceToHost, s));
+    DMRGX_HIP(hipStreamSynchronize(s));
+    T[(size_t)j * k + j] = hh[2 * j] + hh2[2 * j];
+    double beta = std::sqrt(std::max(hh2[2 * (j + 1)], 0.0));
+    if (j + 1 < k) {
+      if (beta < 1e-14 * std::fabs(T[(size_t)j * k + j]) || beta < 1e-300) { kk = j + 1; break; }
+      T[(size_t)j * k + j + 1] = T[(size_t)(j + 1) * k + j] = beta;
+      hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, w, V + (int64_t)(j + 1) * nr, nr,
+                         1.0 / beta);
+    }
+  }
+  std::vector<double> A((size_t)kk * kk), th, Y;
+  for (int i = 0; i < kk; ++i)
+    for (int j = 0; j < kk; ++j) A[(size_t)i * kk + j] = T[(size_t)i * k + j];
+  host_symeig(kk, A, th, Y);
+  std::vector<double> y0(kk);
+  for (int r = 0; r < kk; ++r) y0[r] = Y[(size_t)r * kk];
+  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), kk * 8, hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, nr, kk, (const double*)Yb.ptr, 1, 0, 1, w,
+                     nr);
+  double z = dev_norm2(w, n, cplx, s);
+  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, w, x, nr, 1.0 / z);
+  DMRGX_HIP(hipStreamSynchronize(s));
+  return th[0];
+}
+
+}  // namespace dmrgx

@@ -1,0 +1,138 @@
+/*
+ * dmrgx -- MI355X (gfx950) implementation of the two-site DMRG inner loop of
+ * clover199/DMRG: the effective-Hamiltonian matvec over get_map task lists,
+ * its eigensolver, the blockwise SVD truncation and the environment updates.
+ *
+ * C ABI.  Conventions (as in the reference's numerics wrappers, include/basic.h):
+ *   - element types: DMRGX_R64 = double, DMRGX_C64 = complex double stored as
+ *     interleaved (re, im) pairs, i.e. the layout of std::complex<double>;
+ *   - all matrices are row-major; block data are contiguous in block order;
+ *   - "device" pointers come from dmrgx_malloc (or any hipMalloc'd memory of
+ *     the device passed to dmrgx_init);
+ *   - every function returns 0 on success and a negative code on failure;
+ *     dmrgx_last_error() describes the failure (per host thread);
+ *   - one device per process, one in-order stream; calls are not re-entrant
+ *     on one plan.
+ */
+#ifndef DMRGX_H
+#define DMRGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { DMRGX_R64 = 0, DMRGX_C64 = 1 };
+enum { DMRGX_FP64 = 0, DMRGX_FP32 = 1, DMRGX_BF16X3 = 2 };
+
+/* ---- runtime ------------------------------------------------------------ */
+int dmrgx_init(int device);
+int dmrgx_finalize(void);
+const char* dmrgx_last_error(void);
+int dmrgx_device_info(int* num_cu, char* arch_name, int name_len);
+int dmrgx_malloc(size_t bytes, void** ptr);
+int dmrgx_free(void* ptr);
+int dmrgx_memcpy_h2d(void* dst_dev, const void* src_host, size_t bytes);
+int dmrgx_memcpy_d2h(void* dst_host, const void* src_dev, size_t bytes);
+int dmrgx_memset_zero(void* dst_dev, size_t bytes);
+int dmrgx_synchronize(void);
+/* the library's HIP stream (hipStream_t), for callers that order their own work */
+void* dmrgx_stream(void);
+
+/* ---- task-list GEMM ------------------------------------------------------
+ * Replaces qtensor<T>::contract(T* out, T* in, map, 'N', 'T', num)
+ * (src/qtensor.cpp:986-994 -> src/tensor.cpp:698-718 -> src/dgemm.cpp:29-47,
+ * src/zgemm.cpp:32-51).  `map` holds n_task get_map tuples of 7 int32
+ * (block, in_beg, out_beg, row, col, beta, alpha) (src/qtensor.cpp:1046).
+ * For task t: out[out_beg..] (m x row) = alpha * A_blk (m x col) *
+ * in[in_beg..] (row x col)^T + beta * out, m = blk_size[blk] / col.
+ * blk_off / blk_size (host) give each block of A in elements.  A, in, out are
+ * device pointers of element type `dtype`; `prec` selects the arithmetic
+ * (FP64 operands are converted on the fly for FP32 / BF16X3). */
+int dmrgx_task_gemm(int dtype, int prec, const void* A, const int64_t* blk_off,
+                    const int64_t* blk_size, int n_blk, const int32_t* map, int n_task,
+                    const void* in, void* out, int64_t n_out);
+
+/* ---- effective Hamiltonian ------------------------------------------------
+ * Replaces av(in, out, lanczos<T>&) (src/core.cpp:22-31): store = R * in over
+ * rmap (r_num contracted legs), out = L * store over lmap.  renv / lenv are
+ * device pointers (FP64 storage, kept by reference: they must outlive the
+ * plan).  n_psi, n_store in elements.  Output blocks of `out` that no lmap
+ * task writes read as zero. */
+typedef struct dmrgx_heff dmrgx_heff;
+int dmrgx_heff_create(dmrgx_heff** plan, int dtype, int prec,
+                      const void* renv, const int64_t* rblk_off, const int64_t* rblk_size,
+                      int n_rblk, const int32_t* rmap, int n_rtask,
+                      const void* lenv, const int64_t* lblk_off, const int64_t* lblk_size,
+                      int n_lblk, const int32_t* lmap, int n_ltask,
+                      int64_t n_psi, int64_t n_store);
+int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out);
+/* algorithmic flops of one apply: sum over tasks of c*M*N*K (c = 2 real, 8 complex) */
+double dmrgx_heff_flops(const dmrgx_heff* plan);
+int dmrgx_heff_destroy(dmrgx_heff* plan);
+
+/* ---- eigensolver -------------------------------------------------------------
+ * Replaces znaupd(n, nev, Evals, Evecs, lanczos<T>&) (src/dsaupd.cpp:65-155,
+ * src/znaupd.cpp:78-197): the nev lowest eigenvalues (ascending) of the
+ * Hermitian operator of `plan` and the eigenvector of the lowest.  FP64
+ * thick-restart Lanczos, full reorthogonalisation, ARPACK convergence test
+ * |beta*y_last| <= tol*max(|theta|, eps^(2/3)).  evec (device, n_psi elements)
+ * receives the vector; v0 (device, may be NULL) is the start vector, else a
+ * vector seeded by `seed`.  When `refine` is non-NULL (an FP64 plan of the same
+ * operator) and `plan` is a reduced-precision plan, the lowest pair is polished
+ * with `refine_k` FP64 Krylov steps (Rayleigh-Ritz) and evals[0] comes from it.
+ * info[0] = matvecs, info[1] = restarts, info[2] = converged (0/1),
+ * info[3] = FP64 polish matvecs. */
+int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
+                uint64_t seed, const void* v0, double* evals, void* evec,
+                dmrgx_heff* refine, int refine_k, int* info);
+
+/* ---- blockwise SVD -----------------------------------------------------------
+ * Replaces the per-sector tensor<T>::svd -> ?gesvd loop of qtensor<T>::svd
+ * (src/qtensor.cpp:1167-1176, src/tensor.cpp:736-785, src/dgesvd.cpp:32-97,
+ * src/zgesvd.cpp:36-104).  For each block b: A_b (rows x cols, row-major at
+ * a_off[b]) = U_b diag(S_b) V_b with U_b full (rows x rows, at u_off[b]),
+ * S_b descending (min(rows,cols), at s_off[b] in S), V_b full (cols x cols,
+ * at v_off[b]).  All-zero blocks (every |x| <= 1e-10) give identity U, V and
+ * S = 0; 1x1 blocks give U = 1, S = |a|, V = a/|a| (src/tensor.cpp:764-782).
+ * FP64 batched one-sided Jacobi; A is not modified.  All pointers device. */
+int dmrgx_block_svd(int dtype, int n_blk, const int32_t* rows, const int32_t* cols,
+                    const void* A, const int64_t* a_off, void* U, const int64_t* u_off,
+                    double* S, const int64_t* s_off, void* V, const int64_t* v_off);
+
+/* ---- block contraction kernels used by the environment updates --------------
+ * dmrgx_block_gemm: the batched form of qtensor<T>::contract(A, B, ta, tb, num)
+ * (src/qtensor.cpp:824-899 -> src/tensor.cpp:603-649): one task per block
+ * pair, tasks[t] = {a_off, b_off, c_off, M, N, K, beta} (7 int64); C(MxN)
+ * = op(A) op(B) (+ C if beta); op(A) = A (MxK) for ta 'N', A^T (A is KxM) for
+ * 'T'; op(B) = B (KxN) for tb 'N', B^T (B is NxK) for 'T'.  All FP64. */
+int dmrgx_block_gemm(int dtype, char ta, char tb, const void* A, const void* B, void* C,
+                     const int64_t* tasks, int n_task);
+
+/* dmrgx_leg_contract: the block kernel of qtensor<T>::contract(A, a, B, b,
+ * change, fermion) (src/qtensor.cpp:902-983 -> src/tensor.cpp:652-695):
+ * out[i1,j1,j2,i2] (+)= sign * sum_k A[i1,k,i2] * B[j1,k,j2] for each task
+ * tasks[t] = {a_off, b_off, c_off, al, mid, ar, bl, br, sign, accumulate}
+ * (10 int64). */
+int dmrgx_leg_contract(int dtype, const void* A, const void* B, void* C,
+                       const int64_t* tasks, int n_task);
+
+/* dmrgx_permute: out block = transpose of an in block with up to 8 legs:
+ * tasks[t] = {in_off, out_off, nleg, dims[8] (input order), perm[8]} (19 int64);
+ * output leg q is input leg perm[q] (tensor<T>::exchange, src/tensor.cpp:335-366). */
+int dmrgx_permute(int dtype, const void* in, void* out, const int64_t* tasks, int n_task);
+
+/* dmrgx_copy_boxes: strided 3-D box copies used by combine/split/resize
+ * (src/qtensor.cpp:466-656, src/tensor.cpp:482-513): tasks[t] = {src_off,
+ * src_s0, src_s1, dst_off, dst_s0, dst_s1, n0, n1, n2, scale_sign} (10 int64):
+ * dst[dst_off + i*dst_s0 + j*dst_s1 + k] = src[src_off + i*src_s0 + j*src_s1 + k]
+ * (times -1 when scale_sign < 0; conjugated when scale_sign == 2). */
+int dmrgx_copy_boxes(int dtype, const void* src, void* dst, const int64_t* tasks, int n_task);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMRGX_H */

@@ -1,0 +1,332 @@
+"""numpy/scipy restatement of the reference's DMRG steps and sweep schedules.
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+The eigensolver is scipy's ARPACK (the same dsaupd/znaupd the reference links,
+src/dsaupd.cpp:65-155, src/znaupd.cpp:78-197) with the reference's
+parameters; tiny problems are diagonalised densely.  The SVD is LAPACK
+?gesvd through scipy (oracle.qtensor.tensor_svd).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.linalg
+import scipy.sparse.linalg as spla
+
+from . import qtensor as qt
+from .qtensor import NEV, NSI, TOL, QTensor, generate_dim_sym, get_dimension
+
+
+class MPS:
+    """mps<T>, include/mps.h:9-75: state_ (site tensors), store_ (environments)
+    and the centre singular values."""
+
+    def __init__(self, L):
+        self.state = [None] * L
+        self.store = [None] * L
+        self.singular = None
+        self.cut = 0
+
+    def size(self):
+        return len(self.state)
+
+
+@dataclass
+class StepLog:
+    l: int
+    r: int
+    energies: list
+    singular: list
+    entropy: float
+    trunc_err: float
+    n_matvec: int
+    dim: int
+
+
+@dataclass
+class Lanczos:
+    """lanczos<T>, include/val_for_lanczos.h:7-24."""
+    lenv: QTensor = None
+    renv: QTensor = None
+    lmap: list = field(default_factory=list)
+    rmap: list = field(default_factory=list)
+    r_num: int = 0
+    l_num: int = 0
+    d_store: int = 0
+    n_matvec: int = 0
+
+
+def av(inp, ctx: Lanczos):
+    """src/core.cpp:22-31: store = R * in (rmap), out = L * store (lmap)."""
+    store = np.zeros(ctx.d_store, dtype=inp.dtype)
+    out = np.zeros(inp.shape[0], dtype=inp.dtype)
+    ctx.renv.contract_map(store, inp, ctx.rmap, "N", "T", ctx.r_num)
+    ctx.lenv.contract_map(out, store, ctx.lmap, "N", "T", ctx.l_num)
+    ctx.n_matvec += 1
+    return out
+
+
+def dense_heff(ctx: Lanczos, n, dtype):
+    """Materialise H_eff column by column (as check_hermitian does, src/core.cpp:40-83)."""
+    H = np.zeros((n, n), dtype=dtype)
+    e = np.zeros(n, dtype=dtype)
+    for j in range(n):
+        e[:] = 0
+        e[j] = 1
+        H[:, j] = av(e, ctx)
+    return H
+
+
+def znaupd(n, ctx: Lanczos, dtype, seed=1234, nev=NEV, hermitian=True):
+    """Lowest eigenpairs with the reference's ARPACK settings:
+    real: dsaupd which=SA, ncv=4*nev, tol=1e-10, maxiter=3n (src/dsaupd.cpp:65-91);
+    complex: znaupd which=SR, ncv=6*nev (src/znaupd.cpp:78-111), sorted by Re (:170-183).
+    Returns (nev eigenvalues ascending, lowest eigenvector)."""
+    complex_ = np.dtype(dtype).kind == "c"
+    ncv = (6 if complex_ else 4) * nev
+    ncv = min(ncv, n)
+    if n <= max(2 * ncv, nev + 2):
+        H = dense_heff(ctx, n, dtype)
+        if hermitian:
+            w, v = scipy.linalg.eigh(0.5 * (H + H.conj().T))
+        else:
+            w, v = scipy.linalg.eig(H)
+            order = np.argsort(w.real)
+            w, v = w[order].real, v[:, order]
+        vals = np.zeros(nev)
+        k = min(nev, n)
+        vals[:k] = w[:k]
+        vec = v[:, 0] / np.linalg.norm(v[:, 0])
+        return vals, vec.astype(dtype)
+    rng = np.random.default_rng(seed)
+    v0 = rng.standard_normal(n)
+    if complex_:
+        v0 = v0 + 1j * rng.standard_normal(n)
+    op = spla.LinearOperator((n, n), matvec=lambda x: av(np.asarray(x, dtype=dtype).ravel(), ctx),
+                             dtype=dtype)
+    if not complex_:
+        w, v = spla.eigsh(op, k=nev, which="SA", ncv=ncv, tol=TOL, maxiter=3 * n, v0=v0)
+        order = np.argsort(w)
+    else:
+        w, v = spla.eigs(op, k=nev, which="SR", ncv=ncv, tol=TOL, maxiter=3 * n, v0=v0)
+        order = np.argsort(w.real)
+        w = w.real
+    w, v = w[order], v[:, order]
+    return np.array(w), v[:, 0].astype(dtype)
+
+
+def _log_singular(s, cutoff):
+    """print_singular, src/useful.cpp:75-102: entropy and truncation error."""
+    ent = 0.0
+    for x in s:
+        ent += -x * x * math.log(x * x + TOL)
+    err = 1.0
+    c = min(len(s), cutoff)
+    for i in range(c):
+        err -= s[i] * s[i]
+    return ent, err
+
+
+class Runner:
+    """Step functions of src/core.cpp over an MPS/MPO pair.
+
+    ``sign_rule``: "reference" applies get_map's alpha unconditionally (the
+    reference's behaviour); "physical" applies it only when the model is
+    fermionic (alpha=1 for bosonic models).
+    """
+
+    def __init__(self, mpo, sym_n, fermion, sector=0, sign_rule="physical", seed=1234):
+        self.mpo = mpo
+        self.sym_n = sym_n
+        self.fermion = fermion
+        self.sector = sector
+        self.sign_rule = sign_rule
+        self.seed = seed
+        self.log: list[StepLog] = []
+        qt.set_symmetry(sym_n)
+        self.mps = MPS(mpo.size())
+
+    @property
+    def alpha_on(self):
+        return self.sign_rule == "reference" or self.fermion
+
+    def _solve(self, ctx, lnum, rnum, allow_ed):
+        dim, sym = generate_dim_sym(ctx.lenv.dimension_all(), lnum, ctx.renv.dimension_all(), rnum,
+                                    self.sector)
+        d = get_dimension(dim, sym)
+        dtype = np.result_type(ctx.lenv.dtype, ctx.renv.dtype)
+        return dim, sym, d, dtype
+
+    def _lanczos(self, ctx, dim, sym, d, dtype, r_num, l_num):
+        ctx.r_num, ctx.l_num = r_num, l_num
+        ctx.rmap, dim_ret, sym_ret = ctx.renv.get_map(dim, sym, "N", "T", r_num, 0, False,
+                                                      fermion=self.alpha_on)
+        ctx.d_store = get_dimension(dim_ret, sym_ret)
+        ctx.lmap, _, _ = ctx.lenv.get_map(dim_ret, sym_ret, "N", "T", l_num, 0, True, sym_ret=sym,
+                                          fermion=self.alpha_on)
+        herm = not (self.sign_rule == "reference" and not self.fermion)
+        vals, vec = znaupd(d, ctx, dtype, seed=self.seed + len(self.log), hermitian=herm)
+        return vals, vec
+
+    def _ed(self, whole, d):
+        whole = whole.exchange(2, 3).combine(2, 3).combine(0, 1)
+        w, v = whole.eig(self.sector)
+        vals = np.zeros(NEV)
+        k = min(NEV, len(w))
+        vals[:k] = w[:k]
+        return vals, v
+
+    def _finish(self, l, r, vals, vec, dim, sym, num, cutoff, d, nmv):
+        vecq = QTensor.from_flat(vec, dim, sym)
+        U, S, V, s = vecq.svd(num, cutoff)
+        ent, err = _log_singular(s, cutoff)
+        self.log.append(StepLog(l, r, list(vals), list(s[:NSI]), ent, err, nmv, d))
+        return U, S, V, s
+
+    def two_sites(self, l, r, cutoff):
+        """src/core.cpp:88-172."""
+        m, mps = self.mpo, self.mps
+        ctx = Lanczos(lenv=m[l].copy(), renv=m[r].copy())
+        dim, sym, d, dtype = self._solve(ctx, 1, 1, True)
+        if d < NEV:
+            whole = QTensor(dtype=dtype).contract_index(ctx.lenv, 1, ctx.renv, 1).simplify()
+            vals, vec = self._ed(whole, d)
+        else:
+            vals, vec = self._lanczos(ctx, dim, sym, d, dtype, 1, 2)
+        U, S, V, s = self._finish(l, r, vals, vec, dim, sym, 1, cutoff, d, ctx.n_matvec)
+        mps.state[l] = U
+        mps.singular = S
+        mps.cut = l + 1
+        mps.state[r] = V
+        vecq = QTensor().contract_gemm(ctx.lenv, U, "N", "N")
+        U.conjugate()
+        mps.store[l] = QTensor().contract_gemm(U, vecq, "T", "N")
+        vecq = QTensor().contract_gemm(ctx.renv, V, "N", "T")
+        V.conjugate()
+        mps.store[r] = QTensor().contract_gemm(V, vecq, "N", "N")
+        return vals[0]
+
+    def _lenv(self, l):
+        le = QTensor().contract_index(self.mps.store[l - 1], 1, self.mpo[l], 0)
+        return le.exchange(3, 4)
+
+    def _renv(self, r):
+        ope = self.mpo[r].exchange(0, 2)
+        re = QTensor().contract_index(self.mps.store[r + 1], 1, ope, 0, True)
+        return re.exchange(0, 1)
+
+    def update_two(self, l, r, cutoff):
+        """src/core.cpp:182-283 (two-site L.W.W.R step)."""
+        mps = self.mps
+        ctx = Lanczos(lenv=self._lenv(l), renv=self._renv(r))
+        dim, sym, d, dtype = self._solve(ctx, 2, 2, False)
+        vals, vec = self._lanczos(ctx, dim, sym, d, dtype, 2, 3)
+        U, S, V, s = self._finish(l, r, vals, vec, dim, sym, 2, cutoff, d, ctx.n_matvec)
+        mps.state[l] = U
+        mps.singular = S
+        mps.cut = l + 1
+        mps.state[r] = V
+        vecq = QTensor().contract_gemm(ctx.lenv, U, "N", "N", 2)
+        U.conjugate()
+        mps.store[l] = QTensor().contract_gemm(U, vecq, "T", "N", 2)
+        vecq = QTensor().contract_gemm(ctx.renv, V, "N", "T", 2)
+        V.conjugate()
+        mps.store[r] = QTensor().contract_gemm(V, vecq, "N", "N", 2)
+        return vals[0]
+
+    def move2right(self, l, r, cutoff):
+        """src/core.cpp:293-378 (single explicit site l, renv = store(r))."""
+        mps = self.mps
+        ctx = Lanczos(lenv=self._lenv(l), renv=mps.store[r].copy())
+        dim, sym, d, dtype = self._solve(ctx, 2, 1, True)
+        if d < NEV:
+            tl = ctx.lenv.combine(3, 4).combine(0, 1)
+            whole = QTensor(dtype=dtype).contract_index(tl, 1, ctx.renv, 1).simplify()
+            vals, vec = self._ed(whole, d)
+        else:
+            vals, vec = self._lanczos(ctx, dim, sym, d, dtype, 1, 3)
+        U, S, V, s = self._finish(l, r, vals, vec, dim, sym, 2, cutoff, d, ctx.n_matvec)
+        mps.state[l] = U
+        mps.singular = S
+        mps.cut = l + 1
+        vecq = QTensor().contract_gemm(ctx.lenv, U, "N", "N", 2)
+        U.conjugate()
+        mps.store[l] = QTensor().contract_gemm(U, vecq, "T", "N", 2)
+        return vals[0]
+
+    def move2left(self, l, r, cutoff):
+        """src/core.cpp:388-474 (single explicit site r, lenv = store(l))."""
+        mps = self.mps
+        ctx = Lanczos(lenv=mps.store[l].copy(), renv=self._renv(r))
+        dim, sym, d, dtype = self._solve(ctx, 1, 2, True)
+        if d < NEV:
+            tr = ctx.renv.combine(3, 4).combine(0, 1)
+            whole = QTensor(dtype=dtype).contract_index(ctx.lenv, 1, tr, 1).simplify()
+            vals, vec = self._ed(whole, d)
+        else:
+            vals, vec = self._lanczos(ctx, dim, sym, d, dtype, 2, 2)
+        U, S, V, s = self._finish(l, r, vals, vec, dim, sym, 1, cutoff, d, ctx.n_matvec)
+        mps.singular = S
+        mps.cut = l + 1
+        mps.state[r] = V
+        vecq = QTensor().contract_gemm(ctx.renv, V, "N", "T", 2)
+        V.conjugate()
+        mps.store[r] = QTensor().contract_gemm(V, vecq, "N", "N", 2)
+        return vals[0]
+
+    # ---------------------------------------------------------------- sweeps
+    def dmrg(self, cutoff, sweep):
+        """Single-site schedule, src/functions.cpp:157-227."""
+        L = self.mps.size()
+        pre_cutoff = cutoff if sweep == -1 else 10
+        pre_sweep = cutoff // 100
+        e = self.two_sites(0, L - 1, cutoff * 10)
+        for i in range(1, L // 2):
+            e = self.update_two(i, L - i - 1, pre_cutoff)
+        if sweep != -1:
+            for i in range(L // 2, L - 1):
+                e = self.move2right(i, i + 1, pre_cutoff)
+            for s in range(pre_sweep - 1):
+                for i in range(L - 2, 0, -1):
+                    e = self.move2left(i - 1, i, 100 * (s + 1))
+                for i in range(1, L - 1):
+                    e = self.move2right(i, i + 1, 100 * (s + 1))
+        for s in range(max(sweep, 0)):
+            for i in range(L - 2, 0, -1):
+                e = self.move2left(i - 1, i, cutoff)
+            for i in range(1, L - 1):
+                e = self.move2right(i, i + 1, cutoff)
+        if sweep != -1:
+            for i in range(L - 2, L // 2 - 1, -1):
+                e = self.move2left(i - 1, i, cutoff)
+        return e
+
+    def dmrg2(self, cutoff, sweep, ladder=True):
+        """Two-site schedule, src/functions.cpp:234-299.  ``ladder=False``
+        skips the pre-sweep ladder (cutoff 100, 200, ...)."""
+        L = self.mps.size()
+        pre_cutoff = cutoff if sweep == -1 else 10
+        pre_sweep = cutoff // 100 if ladder else 0
+        e = self.two_sites(0, L - 1, 0)
+        for i in range(1, L // 2):
+            e = self.update_two(i, L - i - 1, pre_cutoff)
+        if sweep != -1:
+            for i in range(L // 2, L - 2):
+                e = self.update_two(i, i + 1, pre_cutoff)
+            for s in range(pre_sweep - 1):
+                for i in range(L - 3, 1, -1):
+                    e = self.update_two(i - 1, i, 100 * (s + 1))
+                for i in range(2, L - 2):
+                    e = self.update_two(i, i + 1, 100 * (s + 1))
+        for s in range(max(sweep, 0)):
+            for i in range(L - 3, 1, -1):
+                e = self.update_two(i - 1, i, cutoff)
+            for i in range(2, L - 2):
+                e = self.update_two(i, i + 1, cutoff)
+        if sweep != -1:
+            for i in range(L - 3, L // 2 - 1, -1):
+                e = self.update_two(i - 1, i, cutoff)
+        return e

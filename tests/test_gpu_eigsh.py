@@ -1,0 +1,53 @@
+"""GPU FP64 Lanczos (dmrgx_eigsh) against dense diagonalisation of the oracle's
+H_eff and against the oracle's ARPACK call (src/dsaupd.cpp / src/znaupd.cpp)."""
+import numpy as np
+import pytest
+import scipy.linalg
+
+from oracle.dmrg import dense_heff
+from tests.cases import dmrg_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _plan(gpu, case, prec):
+    c = gpu.C64 if np.dtype(case.dtype).kind == "c" else gpu.R64
+    R = gpu.DeviceArray.from_numpy(case.rflat)
+    Lb = gpu.DeviceArray.from_numpy(case.lflat)
+    return gpu.Heff(c, gpu.PREC_NAMES[prec], R, case.roff, case.rsize, case.rmap, Lb, case.loff, case.lsize,
+                    case.lmap, case.n_psi, case.n_store)
+
+
+@pytest.mark.parametrize("model,kw", [
+    ("xyz", dict(jx=1.0, jy=0.8, jz=0.5)),
+    ("kitaev", dict(t=-1.0, p=-1.0, u=0.25)),
+])
+def test_eigsh_matches_dense(gpu, model, kw):
+    print(f"case eigsh {model}", flush=True)
+    case, _ = dmrg_case(model, 10, 24, 1, **kw)
+    H = dense_heff(case.ctx, case.n_psi, case.dtype)
+    assert np.max(np.abs(H - H.conj().T)) < 1e-12 * np.max(np.abs(H))
+    w, v = scipy.linalg.eigh(H)
+    h = _plan(gpu, case, "fp64")
+    ev, vec, info = h.eigsh(nev=6, tol=1e-10)
+    assert info[2] == 1
+    assert np.max(np.abs(ev - w[:6])) < 1e-9 * max(1.0, abs(w[0]))
+    x = vec.numpy()
+    ov = abs(np.vdot(v[:, 0], x))
+    assert abs(ov - 1) < 1e-8
+    # residual of the returned pair
+    r = H @ x - ev[0] * x
+    assert np.linalg.norm(r) < 1e-7 * max(1.0, abs(ev[0]))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_eigsh_mixed_with_fp64_polish(gpu, prec):
+    print(f"case eigsh mixed {prec}", flush=True)
+    case, _ = dmrg_case("xyz", 10, 24, 1, jx=1.0, jy=0.8, jz=0.5)
+    H = dense_heff(case.ctx, case.n_psi, case.dtype)
+    w = scipy.linalg.eigvalsh(H)
+    lo = _plan(gpu, case, prec)
+    hi = _plan(gpu, case, "fp64")
+    ev, vec, info = lo.eigsh(nev=6, tol=1e-6 if prec == "fp32" else 1e-4, refine=hi, refine_k=8)
+    assert abs(ev[0] - w[0]) < 1e-8 * max(1.0, abs(w[0])), (ev[0], w[0])
+    assert info[3] > 0

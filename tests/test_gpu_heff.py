@@ -1,0 +1,93 @@
+"""GPU parity of the task-list GEMM and the H_eff matvec against the oracle's
+av() (src/core.cpp:22-31 restated in oracle/dmrg.py)."""
+import numpy as np
+import pytest
+
+from tests.cases import dmrg_case, synthetic_case
+
+pytestmark = pytest.mark.gpu
+
+# Tolerances (max |y - y_oracle| / max |y_oracle|):
+TOL = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 5e-4}
+
+
+def _run(gpu, case, prec):
+    c = gpu.C64 if np.dtype(case.dtype).kind == "c" else gpu.R64
+    R = gpu.DeviceArray.from_numpy(case.rflat)
+    Lb = gpu.DeviceArray.from_numpy(case.lflat)
+    h = gpu.Heff(c, gpu.PREC_NAMES[prec], R, case.roff, case.rsize, case.rmap, Lb, case.loff, case.lsize,
+                 case.lmap, case.n_psi, case.n_store)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(case.n_psi)
+    if c == gpu.C64:
+        x = x + 1j * rng.standard_normal(case.n_psi)
+    xd = gpu.DeviceArray.from_numpy(x.astype(case.dtype))
+    yd = gpu.DeviceArray(case.n_psi, case.dtype)
+    h.apply(xd, yd)
+    y = yd.numpy()
+    ref = case.oracle_apply(x)
+    err = np.max(np.abs(y - ref)) / np.max(np.abs(ref))
+    assert abs(h.flops - case.flops()) < 1e-6 * case.flops()
+    return err, h
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+@pytest.mark.parametrize("shape", [
+    ("z2_small_real", 2, 4, 16, np.float64, None),
+    ("z2_m64_complex", 2, 5, 64, np.complex128, None),
+    ("z3_uneven_complex", 3, 4, 60, np.complex128, [27, 20, 13]),
+    ("z2_ragged_real", 2, 5, 70, np.float64, [37, 33]),
+])
+def test_heff_synthetic(gpu, prec, shape):
+    name, S, D, m, dt, uneven = shape
+    print(f"case {name} prec {prec}", flush=True)
+    case = synthetic_case(S, D, m, dtype=dt, uneven=uneven)
+    err, _ = _run(gpu, case, prec)
+    assert err < TOL[prec], f"{name} {prec}: rel err {err:.3e}"
+
+
+@pytest.mark.parametrize("sign_rule", ["physical", "reference"])
+def test_heff_real_envs_xyz(gpu, sign_rule):
+    print(f"case xyz L=8 m=16 sign={sign_rule}", flush=True)
+    case, _ = dmrg_case("xyz", 8, 16, 1, sign_rule=sign_rule, jx=1.0, jy=0.8, jz=0.5)
+    err, _ = _run(gpu, case, "fp64")
+    assert err < TOL["fp64"]
+
+
+def test_heff_real_envs_kitaev(gpu):
+    print("case kitaev L=10 m=16", flush=True)
+    case, _ = dmrg_case("kitaev", 10, 16, 1, t=-1.0, p=-1.0, u=0.25)
+    assert any(t[6] == -1 for t in case.rmap), "fermion signs must appear in the Kitaev rmap"
+    err, _ = _run(gpu, case, "fp64")
+    assert err < TOL["fp64"]
+
+
+def test_heff_config2_shape(gpu):
+    """XYZ m=512 two-site shapes: 32+32 tasks of (1280,256,256)/(256,256,1280)."""
+    print("case z2 m=512 D=5 complex", flush=True)
+    case = synthetic_case(2, 5, 512, dtype=np.complex128)
+    assert len(case.rmap) == 32 and len(case.lmap) == 32
+    err, h = _run(gpu, case, "fp64")
+    assert err < TOL["fp64"]
+    assert abs(h.flops - 4.294967296e10) < 1e3
+
+
+def test_task_gemm_direct(gpu):
+    """dmrgx_task_gemm on one pass equals the oracle's contract_map."""
+    import ctypes as C
+    case = synthetic_case(2, 4, 32, dtype=np.complex128, seed=3)
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal(case.n_psi) + 1j * rng.standard_normal(case.n_psi))
+    ref = np.zeros(case.n_store, dtype=np.complex128)
+    case.renv.contract_map(ref, x, case.rmap, "N", "T", 2)
+    A = gpu.DeviceArray.from_numpy(case.rflat)
+    xd = gpu.DeviceArray.from_numpy(x)
+    yd = gpu.DeviceArray(case.n_store, np.complex128)
+    yd.zero()
+    m = np.ascontiguousarray(np.asarray(case.rmap, dtype=np.int32))
+    lib = gpu.lib()
+    gpu.check(lib.dmrgx_task_gemm(gpu.C64, gpu.FP64, A.ptr, case.roff.ctypes.data_as(C.POINTER(C.c_int64)),
+                                  case.rsize.ctypes.data_as(C.POINTER(C.c_int64)), len(case.roff),
+                                  m.ctypes.data_as(C.POINTER(C.c_int32)), len(m), xd.ptr, yd.ptr, case.n_store))
+    y = yd.numpy()
+    assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) < 1e-12
