@@ -27,7 +27,7 @@ EXPORTS = [
     "dmrgx_free", "dmrgx_memcpy_h2d", "dmrgx_memcpy_d2h", "dmrgx_memset_zero", "dmrgx_synchronize",
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
     "dmrgx_heff_destroy", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_gemm", "dmrgx_leg_contract",
-    "dmrgx_permute", "dmrgx_copy_boxes",
+    "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time",
 ]
 
 _lib = None
@@ -72,6 +72,7 @@ def _declare(lib):
                                vp, i64p, i64p, C.c_int, i32p, C.c_int, C.c_int64, C.c_int64], C.c_int),
         "dmrgx_heff_apply": ([vp, vp, vp], C.c_int),
         "dmrgx_heff_flops": ([vp], C.c_double),
+        "dmrgx_heff_time": ([vp, vp, vp, C.c_int, dp], C.c_int),
         "dmrgx_heff_destroy": ([vp], C.c_int),
         "dmrgx_eigsh": ([vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, vp, dp, vp, vp, C.c_int,
                          C.POINTER(C.c_int)], C.c_int),
@@ -187,6 +188,13 @@ class Heff:
 
     def apply(self, x: DeviceArray, y: DeviceArray):
         check(lib().dmrgx_heff_apply(self.h, x.ptr, y.ptr), "dmrgx_heff_apply")
+
+    def time(self, x: DeviceArray, y: DeviceArray, iters: int):
+        """(ms per apply, ms R-pass kernel, ms L-pass kernel, R flops, L flops)"""
+        ms = np.zeros(5)
+        check(lib().dmrgx_heff_time(self.h, x.ptr, y.ptr, int(iters), ms.ctypes.data_as(C.POINTER(C.c_double))),
+              "dmrgx_heff_time")
+        return ms
 
     def eigsh(self, nev=6, ncv=None, tol=1e-10, max_restart=1000, seed=1234, v0=None, refine=None,
               refine_k=6):

@@ -1,0 +1,322 @@
+// Step functions (src/core.cpp:88-474) and sweep schedules
+// (src/functions.cpp:157-299) over device-resident qtensors.  The matvec,
+// eigensolver, SVD and environment contractions are the HIP kernels; the
+// control flow, sector bookkeeping and step order follow the reference.
+#include "core.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+
+#include "lanczos.hpp"
+
+namespace dmrgx {
+
+using clk = std::chrono::steady_clock;
+static double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+template <typename T>
+Engine<T>::Engine(const mpo<T>& m, const RunConfig& c) : mpo_(m), mps_(m), cfg(c) {}
+
+template <typename T>
+qtensor<T> Engine<T>::make_lenv(int l) {
+  // src/core.cpp:188-189
+  qtensor<T> t;
+  t.contract(mps_(l - 1), 1, mpo_[l], 0);
+  return t.exchange(3, 4);
+}
+
+template <typename T>
+qtensor<T> Engine<T>::make_renv(int r) {
+  // src/core.cpp:190-192
+  qtensor<T> ope = mpo_[r].exchange(0, 2);
+  qtensor<T> t;
+  t.contract(mps_(r + 1), 1, ope, 0, true);
+  return t.exchange(0, 1);
+}
+
+static void flatten_map(const vector<vector<int>>& m, vector<int32_t>& out) {
+  out.clear();
+  for (auto& t : m) out.insert(out.end(), t.begin(), t.end());
+}
+
+template <typename T>
+void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, int rnum, int r_num, int l_num,
+                      vector<vector<int>>& dim, vector<vector<int>>& sym, double* vals, qtensor<T>& psi,
+                      StepLog& lg, bool allow_ed, int ed_which) {
+  generate_dim_sym(dim, sym, lenv, lnum, renv, rnum, cfg.sector);
+  const int64_t d = get_dimension(dim, sym);
+  lg.d = d;
+  for (int i = 0; i < NEV; ++i) vals[i] = 0;
+  if (allow_ed && d < NEV) {
+    // exact diagonalisation of the tiny superblock (src/core.cpp:106-124, 314-336, 410-433)
+    qtensor<T> whole;
+    if (ed_which == 0) whole.contract(lenv, 1, renv, 1);
+    else if (ed_which == 1) whole.contract(lenv.combine(3, 4).combine(0, 1), 1, renv, 1);
+    else whole.contract(lenv, 1, renv.combine(3, 4).combine(0, 1), 1);
+    whole = whole.simplify().exchange(2, 3).combine(2, 3).combine(0, 1);
+    vector<T> v;
+    std::vector<double> w(std::max<int64_t>(d, NEV) + whole.dimension(0), 0.0);
+    int n = whole.eig(w.data(), v, cfg.sector);
+    for (int i = 0; i < std::min(n, NEV); ++i) vals[i] = w[i];
+    DMRGX_CHECK((int64_t)v.size() == d, "exact diagonalisation: eigenvector length differs from the psi dimension");
+    DevBuf tmp(std::max<int64_t>(d, 1) * sizeof(T));
+    DMRGX_HIP(hipMemcpyAsync(tmp.ptr, v.data(), d * sizeof(T), hipMemcpyHostToDevice, rt().stream));
+    psi = qtensor<T>((const T*)tmp.ptr, dim, sym);
+    DMRGX_HIP(hipStreamSynchronize(rt().stream));
+    return;
+  }
+  vector<vector<int>> rmap, lmap, dim_ret, sym_ret, dim2;
+  DMRGX_CHECK(renv.get_map(rmap, dim_ret, sym_ret, dim, sym, 'N', 'T', r_num, 0, false, alpha_on()),
+              "get_map (rmap) failed");
+  const int64_t d_store = get_dimension(dim_ret, sym_ret);
+  vector<vector<int>> psym = sym;
+  DMRGX_CHECK(lenv.get_map(lmap, dim2, psym, dim_ret, sym_ret, 'N', 'T', l_num, 0, true, alpha_on()),
+              "get_map (lmap) failed");
+  vector<int32_t> rm, lm;
+  flatten_map(rmap, rm);
+  flatten_map(lmap, lm);
+  vector<int64_t> roff(renv.off_), rsz(renv.nblocks()), loff(lenv.off_), lsz(lenv.nblocks());
+  for (int b = 0; b < renv.nblocks(); ++b) rsz[b] = renv.bsize(b);
+  for (int b = 0; b < lenv.nblocks(); ++b) lsz[b] = lenv.bsize(b);
+  const int dt = dtype_of<T>::value;
+  auto plan = make_heff(dt, cfg.prec, renv.data(), roff.data(), rsz.data(), renv.nblocks(), renv.total_, rm.data(),
+                        (int)rmap.size(), lenv.data(), loff.data(), lsz.data(), lenv.nblocks(), lenv.total_, lm.data(),
+                        (int)lmap.size(), d, d_store);
+  std::unique_ptr<HeffPlan> plan64;
+  if (cfg.prec != PREC_FP64)
+    plan64 = make_heff(dt, PREC_FP64, renv.data(), roff.data(), rsz.data(), renv.nblocks(), renv.total_, rm.data(),
+                       (int)rmap.size(), lenv.data(), loff.data(), lsz.data(), lenv.nblocks(), lenv.total_, lm.data(),
+                       (int)lmap.size(), d, d_store);
+  hipStream_t s = rt().stream;
+  HeffPlan* hp = plan.get();
+  MatVec op = [hp, s](const double* x, double* y) { hp->apply(x, y, s); };
+  const bool cx = dt == DT_C64;
+  const int ncv = (cx ? 6 : 4) * cfg.nev;  // src/dsaupd.cpp:76, src/znaupd.cpp:94
+  DevBuf evec(std::max<int64_t>(d, 1) * sizeof(T));
+  const double tol = cfg.prec == PREC_FP64 ? cfg.tol : cfg.tol_lo;
+  uint64_t seed = cfg.seed * 0x9E3779B97F4A7C15ull + (uint64_t)log.size() + 1;
+  EigshResult r = eigsh(op, d, cx, cfg.nev, ncv, tol, cfg.max_restart, seed, nullptr, (double*)evec.ptr);
+  for (int i = 0; i < std::min<int>(NEV, (int)r.evals.size()); ++i) vals[i] = r.evals[i];
+  lg.n_matvec = r.n_matvec;
+  lg.flops = plan->flops * r.n_matvec;
+  if (plan64) {
+    HeffPlan* hp64 = plan64.get();
+    MatVec op64 = [hp64, s](const double* x, double* y) { hp64->apply(x, y, s); };
+    int pm = 0;
+    vals[0] = krylov_polish(op64, d, cx, cfg.refine_k, (double*)evec.ptr, &pm);
+    lg.n_polish = pm;
+    lg.flops += plan64->flops * pm;
+  }
+  psi = qtensor<T>((const T*)evec.ptr, dim, sym);
+  DMRGX_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+void Engine<T>::finish(StepLog& lg, const vector<double>& s, int cutoff) {
+  // print_singular, src/useful.cpp:75-102
+  double ent = 0;
+  for (double x : s) ent += -x * x * std::log(x * x + TOL);
+  double err = 1;
+  int c = std::min<int>((int)s.size(), cutoff);
+  for (int i = 0; i < c; ++i) err -= s[i] * s[i];
+  lg.entropy = ent;
+  lg.trunc = err;
+  lg.s.assign(s.begin(), s.begin() + std::min<size_t>(s.size(), NSI));
+  log.push_back(lg);
+  if (cfg.verbose)
+    std::printf("l=%d r=%d d=%lld E0=%.12f trunc=%.3e matvec=%d t=%.3fs\n", lg.l, lg.r, (long long)lg.d, lg.e[0],
+                lg.trunc, lg.n_matvec, lg.t_total);
+  if (on_step) on_step(lg);
+}
+
+// src/core.cpp:88-172
+template <typename T>
+double Engine<T>::two_sites(int l, int r, int cutoff) {
+  auto t0 = clk::now();
+  StepLog lg;
+  lg.l = l;
+  lg.r = r;
+  qtensor<T> lenv = mpo_[l], renv = mpo_[r];
+  vector<vector<int>> dim, sym;
+  qtensor<T> psi;
+  auto t1 = clk::now();
+  solve(lenv, renv, 1, 1, 1, 2, dim, sym, lg.e, psi, lg, true, 0);
+  auto t2 = clk::now();
+  qtensor<T> U, V;
+  qtensor<double> S;
+  vector<double> s = psi.svd(U, S, V, 1, cutoff);
+  auto t3 = clk::now();
+  mps_[l] = U;
+  mps_.center(l + 1) = S;
+  mps_[r] = V;
+  qtensor<T> vec;
+  vec.contract(lenv, U, 'N', 'N');
+  U.conjugate();
+  mps_(l).contract(U, vec, 'T', 'N');
+  vec.contract(renv, V, 'N', 'T');
+  V.conjugate();
+  mps_(r).contract(V, vec, 'N', 'N');
+  auto t4 = clk::now();
+  lg.t_env = secs(t0, t1) + secs(t3, t4);
+  lg.t_eig = secs(t1, t2);
+  lg.t_svd = secs(t2, t3);
+  lg.t_total = secs(t0, t4);
+  finish(lg, s, cutoff);
+  return lg.e[0];
+}
+
+// src/core.cpp:182-283
+template <typename T>
+double Engine<T>::update_two(int l, int r, int cutoff) {
+  auto t0 = clk::now();
+  StepLog lg;
+  lg.l = l;
+  lg.r = r;
+  qtensor<T> lenv = make_lenv(l), renv = make_renv(r);
+  vector<vector<int>> dim, sym;
+  qtensor<T> psi;
+  auto t1 = clk::now();
+  solve(lenv, renv, 2, 2, 2, 3, dim, sym, lg.e, psi, lg, false, 0);
+  auto t2 = clk::now();
+  qtensor<T> U, V;
+  qtensor<double> S;
+  vector<double> s = psi.svd(U, S, V, 2, cutoff);
+  auto t3 = clk::now();
+  mps_[l] = U;
+  mps_.center(l + 1) = S;
+  mps_[r] = V;
+  qtensor<T> vec;
+  vec.contract(lenv, U, 'N', 'N', 2);
+  U.conjugate();
+  mps_(l).contract(U, vec, 'T', 'N', 2);
+  vec.contract(renv, V, 'N', 'T', 2);
+  V.conjugate();
+  mps_(r).contract(V, vec, 'N', 'N', 2);
+  auto t4 = clk::now();
+  lg.t_env = secs(t0, t1) + secs(t3, t4);
+  lg.t_eig = secs(t1, t2);
+  lg.t_svd = secs(t2, t3);
+  lg.t_total = secs(t0, t4);
+  finish(lg, s, cutoff);
+  return lg.e[0];
+}
+
+// src/core.cpp:293-378
+template <typename T>
+double Engine<T>::move2right(int l, int r, int cutoff) {
+  auto t0 = clk::now();
+  StepLog lg;
+  lg.l = l;
+  lg.r = r;
+  qtensor<T> lenv = make_lenv(l), renv = mps_(r);
+  vector<vector<int>> dim, sym;
+  qtensor<T> psi;
+  auto t1 = clk::now();
+  solve(lenv, renv, 2, 1, 1, 3, dim, sym, lg.e, psi, lg, true, 1);
+  auto t2 = clk::now();
+  qtensor<T> U, V;
+  qtensor<double> S;
+  vector<double> s = psi.svd(U, S, V, 2, cutoff);
+  auto t3 = clk::now();
+  mps_[l] = U;
+  mps_.center(l + 1) = S;
+  qtensor<T> vec;
+  vec.contract(lenv, U, 'N', 'N', 2);
+  U.conjugate();
+  mps_(l).contract(U, vec, 'T', 'N', 2);
+  auto t4 = clk::now();
+  lg.t_env = secs(t0, t1) + secs(t3, t4);
+  lg.t_eig = secs(t1, t2);
+  lg.t_svd = secs(t2, t3);
+  lg.t_total = secs(t0, t4);
+  finish(lg, s, cutoff);
+  return lg.e[0];
+}
+
+// src/core.cpp:388-474
+template <typename T>
+double Engine<T>::move2left(int l, int r, int cutoff) {
+  auto t0 = clk::now();
+  StepLog lg;
+  lg.l = l;
+  lg.r = r;
+  qtensor<T> lenv = mps_(l), renv = make_renv(r);
+  vector<vector<int>> dim, sym;
+  qtensor<T> psi;
+  auto t1 = clk::now();
+  solve(lenv, renv, 1, 2, 2, 2, dim, sym, lg.e, psi, lg, true, 2);
+  auto t2 = clk::now();
+  qtensor<T> U, V;
+  qtensor<double> S;
+  vector<double> s = psi.svd(U, S, V, 1, cutoff);
+  auto t3 = clk::now();
+  mps_.center(l + 1) = S;
+  mps_[r] = V;
+  qtensor<T> vec;
+  vec.contract(renv, V, 'N', 'T', 2);
+  V.conjugate();
+  mps_(r).contract(V, vec, 'N', 'N', 2);
+  auto t4 = clk::now();
+  lg.t_env = secs(t0, t1) + secs(t3, t4);
+  lg.t_eig = secs(t1, t2);
+  lg.t_svd = secs(t2, t3);
+  lg.t_total = secs(t0, t4);
+  finish(lg, s, cutoff);
+  return lg.e[0];
+}
+
+// single-site schedule, src/functions.cpp:157-227
+template <typename T>
+double Engine<T>::dmrg(int cutoff, int sweep) {
+  const int L = mps_.size();
+  int pre_cutoff = sweep == -1 ? cutoff : 10;
+  int pre_sweep = cutoff / 100;
+  double e = two_sites(0, L - 1, cutoff * 10);
+  for (int i = 1; i < L / 2; ++i) e = update_two(i, L - i - 1, pre_cutoff);
+  if (sweep != -1) {
+    for (int i = L / 2; i < L - 1; ++i) e = move2right(i, i + 1, pre_cutoff);
+    for (int s = 0; s < pre_sweep - 1; ++s) {
+      for (int i = L - 2; i > 0; --i) e = move2left(i - 1, i, 100 * (s + 1));
+      for (int i = 1; i < L - 1; ++i) e = move2right(i, i + 1, 100 * (s + 1));
+    }
+  }
+  for (int s = 0; s < sweep; ++s) {
+    for (int i = L - 2; i > 0; --i) e = move2left(i - 1, i, cutoff);
+    for (int i = 1; i < L - 1; ++i) e = move2right(i, i + 1, cutoff);
+  }
+  if (sweep != -1)
+    for (int i = L - 2; i >= L / 2; --i) e = move2left(i - 1, i, cutoff);
+  return e;
+}
+
+// two-site schedule, src/functions.cpp:234-299 (ladder=false skips the
+// pre-sweep cutoff ladder 100, 200, ...)
+template <typename T>
+double Engine<T>::dmrg2(int cutoff, int sweep, bool ladder) {
+  const int L = mps_.size();
+  int pre_cutoff = sweep == -1 ? cutoff : 10;
+  int pre_sweep = ladder ? cutoff / 100 : 0;
+  double e = two_sites(0, L - 1, 0);
+  for (int i = 1; i < L / 2; ++i) e = update_two(i, L - i - 1, pre_cutoff);
+  if (sweep != -1) {
+    for (int i = L / 2; i < L - 2; ++i) e = update_two(i, i + 1, pre_cutoff);
+    for (int s = 0; s < pre_sweep - 1; ++s) {
+      for (int i = L - 3; i > 1; --i) e = update_two(i - 1, i, 100 * (s + 1));
+      for (int i = 2; i < L - 2; ++i) e = update_two(i, i + 1, 100 * (s + 1));
+    }
+  }
+  for (int s = 0; s < sweep; ++s) {
+    for (int i = L - 3; i > 1; --i) e = update_two(i - 1, i, cutoff);
+    for (int i = 2; i < L - 2; ++i) e = update_two(i, i + 1, cutoff);
+  }
+  if (sweep != -1)
+    for (int i = L - 3; i >= L / 2; --i) e = update_two(i - 1, i, cutoff);
+  return e;
+}
+
+template class Engine<double>;
+template class Engine<cplx>;
+
+}  // namespace dmrgx

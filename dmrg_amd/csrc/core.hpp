@@ -1,0 +1,70 @@
+// DMRG step functions and sweep schedules over device-resident qtensors
+// (src/core.cpp, src/functions.cpp).
+#pragma once
+
+#include <functional>
+#include <string>
+#include <vector>
+
+#include "heff.hpp"
+#include "qtensor.hpp"
+
+namespace dmrgx {
+
+struct RunConfig {
+  int prec = PREC_FP64;        // arithmetic of the H_eff matvecs inside the eigensolver
+  bool physical_signs = true;  // bosonic models: get_map alpha = 1 (reference rule when false)
+  bool fermion = false;        // the model is fermionic (the reference's FERMION switch)
+  int sector = 0;
+  uint64_t seed = 1234;
+  double tol = 1e-10;          // Lanczos tolerance (FP64 eigensolve)
+  double tol_lo = 1e-6;        // Lanczos tolerance of a reduced-precision eigensolve
+  int refine_k = 8;            // FP64 Krylov polish steps after a reduced-precision eigensolve
+  bool warm_start = false;     // start Lanczos from the previous two-site state
+  int nev = NEV;
+  int max_restart = 5000;
+  bool verbose = false;
+};
+
+struct StepLog {
+  int l = 0, r = 0;
+  double e[NEV] = {0, 0, 0, 0, 0, 0};
+  double entropy = 0, trunc = 0;
+  int64_t d = 0;
+  int n_matvec = 0, n_polish = 0;
+  double flops = 0;  // algorithmic matvec flops of this step (all matvecs)
+  double t_total = 0, t_eig = 0, t_svd = 0, t_env = 0;
+  std::vector<double> s;  // leading singular values
+};
+
+template <typename T>
+class Engine {
+ public:
+  Engine(const mpo<T>& m, const RunConfig& cfg);
+  double two_sites(int l, int r, int cutoff);
+  double update_two(int l, int r, int cutoff);
+  double move2right(int l, int r, int cutoff);
+  double move2left(int l, int r, int cutoff);
+  double dmrg(int cutoff, int sweep);
+  double dmrg2(int cutoff, int sweep, bool ladder = true);
+
+  mpo<T> mpo_;
+  mps<T> mps_;
+  RunConfig cfg;
+  std::vector<StepLog> log;
+  std::function<void(const StepLog&)> on_step;
+
+ private:
+  bool alpha_on() const { return !cfg.physical_signs || cfg.fermion; }
+  qtensor<T> make_lenv(int l);
+  qtensor<T> make_renv(int r);
+  // Lanczos on (lenv, renv) with r_num / l_num contracted legs; fills vals and psi
+  void solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, int rnum, int r_num, int l_num,
+             vector<vector<int>>& dim, vector<vector<int>>& sym, double* vals, qtensor<T>& psi, StepLog& lg,
+             bool allow_ed, int ed_which);
+  void finish(StepLog& lg, const vector<double>& s, int cutoff);
+  std::shared_ptr<DevBuf> last_psi_;
+  vector<vector<int>> last_dim_, last_sym_;
+};
+
+}  // namespace dmrgx

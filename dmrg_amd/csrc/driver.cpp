@@ -1,0 +1,288 @@
+// C entry points of the host mirror: full DMRG runs, synthetic operators for
+// benchmarking, and structure-only get_map / generate_dim_sym.
+#include <chrono>
+#include <cstring>
+
+#include "../../include/dmrgx_driver.h"
+#include "core.hpp"
+#include "heff.hpp"
+#include "lanczos.hpp"
+#include "qtensor.hpp"
+
+using namespace dmrgx;
+
+#define CAPI_BEGIN try {
+#define CAPI_END                      \
+  return 0;                           \
+  }                                   \
+  catch (const dmrgx::Error& e) {     \
+    set_last_error(e.what());         \
+    return e.code;                    \
+  }                                   \
+  catch (const std::exception& e) {   \
+    set_last_error(e.what());         \
+    return -2;                        \
+  }
+
+namespace {
+
+template <typename T>
+void run_engine(Engine<T>& eng, const dmrgx_run_config* c, dmrgx_run_stats* st, double* log, int cap, int* n) {
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  // sweep boundaries: remember the step index at which each full sweep starts
+  const int L = c->L;
+  double e = c->schedule == DMRGX_SCHEDULE_DMRG ? eng.dmrg(c->m, c->sweeps) : eng.dmrg2(c->m, c->sweeps, c->ladder != 0);
+  auto t1 = clk::now();
+  std::memset(st, 0, sizeof(*st));
+  st->energy = e;
+  st->seconds = std::chrono::duration<double>(t1 - t0).count();
+  st->steps = (int)eng.log.size();
+  for (auto& g : eng.log) {
+    st->matvecs += g.n_matvec;
+    st->polish_matvecs += g.n_polish;
+    st->matvec_flops += g.flops;
+    st->t_eig += g.t_eig;
+    st->t_svd += g.t_svd;
+    st->t_env += g.t_env;
+  }
+  if (!eng.log.empty()) {
+    st->trunc_err = eng.log.back().trunc;
+    st->entropy = eng.log.back().entropy;
+  }
+  // full sweeps are the last `sweeps` blocks of steps before the final half sweep
+  if (c->sweeps > 0) {
+    const int per = c->schedule == DMRGX_SCHEDULE_DMRG ? 2 * (L - 2) : 2 * (L - 4);
+    const int fin = c->schedule == DMRGX_SCHEDULE_DMRG ? (L - 2) - (L / 2) + 1 : (L - 3) - (L / 2) + 1;
+    int end = (int)eng.log.size() - std::max(fin, 0);
+    int beg = end - per * c->sweeps;
+    if (beg >= 0) {
+      double tot = 0, last = 0;
+      for (int i = beg; i < end; ++i) {
+        tot += eng.log[i].t_total;
+        if (i >= end - per) last += eng.log[i].t_total;
+      }
+      st->sweep_seconds = tot / c->sweeps;
+      st->last_sweep_seconds = last;
+    }
+  }
+  int k = 0;
+  for (auto& g : eng.log) {
+    if (!log || k >= cap) break;
+    double* row = log + (size_t)k * DMRGX_STEP_LOG_WIDTH;
+    row[0] = g.l;
+    row[1] = g.r;
+    for (int i = 0; i < NEV; ++i) row[2 + i] = g.e[i];
+    row[8] = g.trunc;
+    row[9] = g.entropy;
+    row[10] = g.n_matvec;
+    row[11] = g.n_polish;
+    row[12] = g.t_total;
+    row[13] = g.t_eig;
+    row[14] = g.t_svd;
+    row[15] = g.t_env;
+    row[16] = (double)g.d;
+    ++k;
+  }
+  if (n) *n = (int)eng.log.size();
+}
+
+RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
+  RunConfig r;
+  r.prec = c->prec;
+  r.physical_signs = c->sign_rule == DMRGX_SIGNS_PHYSICAL;
+  r.fermion = fermion;
+  r.sector = c->sector;
+  r.seed = c->seed;
+  r.tol = c->tol > 0 ? c->tol : 1e-10;
+  r.tol_lo = c->tol_lo > 0 ? c->tol_lo : 1e-6;
+  r.refine_k = c->refine_k > 0 ? c->refine_k : 8;
+  r.verbose = c->verbose != 0;
+  return r;
+}
+
+// desc = [nleg, nsec[nleg], dims..., nblk, sym..., (dir...)]
+struct Desc {
+  vector<vector<int>> dim, sym;
+  vector<int> dir;
+};
+Desc read_desc(const int32_t* d, bool with_dir) {
+  Desc r;
+  int p = 0;
+  int nleg = d[p++];
+  DMRGX_CHECK(nleg >= 0 && nleg <= 8, "desc: 0..8 legs");
+  vector<int> ns(nleg);
+  for (int i = 0; i < nleg; ++i) ns[i] = d[p++];
+  r.dim.resize(nleg);
+  for (int i = 0; i < nleg; ++i)
+    for (int k = 0; k < ns[i]; ++k) r.dim[i].push_back(d[p++]);
+  int nb = d[p++];
+  r.sym.assign(nb, vector<int>(nleg));
+  for (int b = 0; b < nb; ++b)
+    for (int i = 0; i < nleg; ++i) r.sym[b][i] = d[p++];
+  if (with_dir) {
+    r.dir.resize(nleg);
+    for (int i = 0; i < nleg; ++i) r.dir[i] = d[p++];
+  }
+  return r;
+}
+int write_desc(const vector<vector<int>>& dim, const vector<vector<int>>& sym, int32_t* out, int cap) {
+  vector<int32_t> v;
+  v.push_back((int)dim.size());
+  for (auto& x : dim) v.push_back((int)x.size());
+  for (auto& x : dim) v.insert(v.end(), x.begin(), x.end());
+  v.push_back((int)sym.size());
+  for (auto& s : sym) v.insert(v.end(), s.begin(), s.end());
+  DMRGX_CHECK((int)v.size() <= cap, "description buffer too small");
+  std::memcpy(out, v.data(), v.size() * 4);
+  return (int)v.size();
+}
+qtensor<double> structure_only(const Desc& d) {
+  qtensor<double> t;
+  t.dim_ = d.dim;
+  t.sym_ = d.sym;
+  t.dir_ = d.dir.empty() ? vector<int>(d.dim.size(), 0) : d.dir;
+  t.layout();
+  return t;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dmrgx_run_dmrg(const dmrgx_run_config* c, dmrgx_run_stats* st, double* step_log, int log_cap, int* n_steps) {
+  CAPI_BEGIN
+  ensure_init();
+  DMRGX_CHECK(c && st, "run_dmrg: null arguments");
+  DMRGX_CHECK(c->L >= 4, "run_dmrg: L must be at least 4");
+  switch (c->model) {
+    case DMRGX_MODEL_XYZ: {
+      set_symmetry(2);
+      qtensor<cplx> H = H_XYZ(c->p[0], c->p[1], c->p[2]);
+      mpo<cplx> m(c->L, H);
+      m.edge();
+      Engine<cplx> eng(m, make_cfg(c, false));
+      run_engine(eng, c, st, step_log, log_cap, n_steps);
+      break;
+    }
+    case DMRGX_MODEL_KITAEV: {
+      set_symmetry(2);
+      qtensor<double> H = H_Kitaev(c->p[0], c->p[1], c->p[2]);
+      mpo<double> m(c->L, H);
+      m.edge(H.left(), H.right());
+      Engine<double> eng(m, make_cfg(c, true));
+      run_engine(eng, c, st, step_log, log_cap, n_steps);
+      break;
+    }
+    case DMRGX_MODEL_POTTS3: {
+      set_symmetry(3);
+      qtensor<cplx> H = H_Potts3(c->p[0], c->p[1], c->p[2], c->p[3]);
+      mpo<cplx> m(c->L, H);
+      m.edge();
+      Engine<cplx> eng(m, make_cfg(c, false));
+      run_engine(eng, c, st, step_log, log_cap, n_steps);
+      break;
+    }
+    case DMRGX_MODEL_POTTS2: {
+      set_symmetry(2);
+      qtensor<double> H = H_Potts2(c->p[0], c->p[1]);
+      mpo<double> m(c->L, H);
+      m.edge();
+      Engine<double> eng(m, make_cfg(c, false));
+      run_engine(eng, c, st, step_log, log_cap, n_steps);
+      break;
+    }
+    default:
+      DMRGX_CHECK(false, "run_dmrg: unknown model");
+  }
+  CAPI_END
+}
+
+int dmrgx_synth_heff_create(int S, int D, const int* bond, int n_sec, int dtype, int prec, uint64_t seed, int fermion,
+                            dmrgx_heff** plan, int64_t* n_psi) {
+  CAPI_BEGIN
+  ensure_init();
+  DMRGX_CHECK(S >= 1 && n_sec == S && D >= 1, "synth_heff: need one bond dimension per sector");
+  set_symmetry(S);
+  auto build = [&](bool right, auto tag) {
+    using T = decltype(tag);
+    qtensor<T> t;
+    vector<int> phys(S, 1), bd(bond, bond + n_sec);
+    if (right) t.dim_ = {phys, bd, {D}, phys, bd};
+    else t.dim_ = {bd, phys, {D}, bd, phys};
+    t.dir_ = right ? vector<int>{0, -1, 0, 0, 1} : vector<int>{-1, 0, 0, 1, 0};
+    for (int a = 0; a < S; ++a)
+      for (int b = 0; b < S; ++b)
+        for (int c = 0; c < S; ++c)
+          for (int e = 0; e < S; ++e) t.sym_.push_back({a, b, 0, c, e});
+    t.layout();
+    t.allocate();
+    dev_random((double*)t.data(), t.total_, dtype == DT_C64, seed + (right ? 17 : 29), rt().stream);
+    return t;
+  };
+  auto make = [&](auto tag) {
+    using T = decltype(tag);
+    qtensor<T> R = build(true, tag), Lq = build(false, tag);
+    vector<vector<int>> dim, sym, rmap, lmap, dim_ret, sym_ret, dim2;
+    generate_dim_sym(dim, sym, Lq, 2, R, 2, 0);
+    int64_t d = get_dimension(dim, sym);
+    R.get_map(rmap, dim_ret, sym_ret, dim, sym, 'N', 'T', 2, 0, false, fermion != 0);
+    int64_t ds = get_dimension(dim_ret, sym_ret);
+    Lq.get_map(lmap, dim2, sym, dim_ret, sym_ret, 'N', 'T', 3, 0, true, fermion != 0);
+    vector<int32_t> rm, lm;
+    for (auto& x : rmap) rm.insert(rm.end(), x.begin(), x.end());
+    for (auto& x : lmap) lm.insert(lm.end(), x.begin(), x.end());
+    vector<int64_t> rs(R.nblocks()), ls(Lq.nblocks());
+    for (int b = 0; b < R.nblocks(); ++b) rs[b] = R.bsize(b);
+    for (int b = 0; b < Lq.nblocks(); ++b) ls[b] = Lq.bsize(b);
+    auto h = new dmrgx_heff;
+    h->p = make_heff(dtype, prec, R.data(), R.off_.data(), rs.data(), R.nblocks(), R.total_, rm.data(),
+                     (int)rmap.size(), Lq.data(), Lq.off_.data(), ls.data(), Lq.nblocks(), Lq.total_, lm.data(),
+                     (int)lmap.size(), d, ds);
+    h->keep.push_back(R.buf_);
+    h->keep.push_back(Lq.buf_);
+    *plan = h;
+    *n_psi = d;
+  };
+  if (dtype == DT_C64) make(cplx());
+  else make(double());
+  CAPI_END
+}
+
+int dmrgx_get_map(int sym_n, const int32_t* tdesc, const int32_t* adesc, char ta, char tb, int num, int beta, int back,
+                  int fermion, const int32_t* symret_in, int32_t* map_out, int map_cap, int* n_map, int32_t* ret_desc,
+                  int ret_cap) {
+  CAPI_BEGIN
+  set_symmetry(sym_n);
+  Desc t = read_desc(tdesc, true), a = read_desc(adesc, false);
+  qtensor<double> q = structure_only(t);
+  vector<vector<int>> map, dim_ret, sym_ret;
+  if (back) {
+    int nb = symret_in[0];
+    int nleg = (int)(t.dim.size() + a.dim.size()) - 2 * num;
+    sym_ret.assign(nb, vector<int>(nleg));
+    for (int b = 0; b < nb; ++b)
+      for (int i = 0; i < nleg; ++i) sym_ret[b][i] = symret_in[1 + b * nleg + i];
+  }
+  DMRGX_CHECK(q.get_map(map, dim_ret, sym_ret, a.dim, a.sym, ta, tb, num, beta, back != 0, fermion != 0),
+              "get_map: dimensions differ");
+  DMRGX_CHECK((int)map.size() <= map_cap, "get_map: map buffer too small");
+  for (size_t i = 0; i < map.size(); ++i)
+    for (int k = 0; k < 7; ++k) map_out[i * 7 + k] = map[i][k];
+  *n_map = (int)map.size();
+  write_desc(dim_ret, sym_ret, ret_desc, ret_cap);
+  CAPI_END
+}
+
+int dmrgx_generate_dim_sym(int sym_n, const int32_t* ldesc, int lnum, const int32_t* rdesc, int rnum, int sector,
+                           int32_t* out_desc, int cap) {
+  CAPI_BEGIN
+  set_symmetry(sym_n);
+  qtensor<double> l = structure_only(read_desc(ldesc, true)), r = structure_only(read_desc(rdesc, true));
+  vector<vector<int>> dim, sym;
+  generate_dim_sym(dim, sym, l, lnum, r, rnum, sector);
+  write_desc(dim, sym, out_desc, cap);
+  CAPI_END
+}
+
+}  // extern "C"

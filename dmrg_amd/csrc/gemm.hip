@@ -530,9 +530,16 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
   return plan;
 }
 
-void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s) {
+void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, hipEvent_t ev_begin,
+                   hipEvent_t ev_end) {
   const size_t eb = (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);
   for (auto& z : p.zero_ranges) fill_zero((char*)C + z.first * eb, (size_t)z.second * eb, s);
+  if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
+  struct EndRec {
+    hipEvent_t e;
+    hipStream_t s;
+    ~EndRec() { if (e) (void)hipEventRecord(e, s); }
+  } end_rec{ev_end, s};
   if (!p.n_items) return;
   const int per = p.grid / 8;
   if (p.prec == PREC_FP64) {

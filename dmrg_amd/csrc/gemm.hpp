@@ -57,7 +57,8 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks, int
 // Execute: A, B, C are device pointers with storage precision of the plan
 // (double for PREC_FP64, float for PREC_FP32 / PREC_BF16X3; complex
 // interleaved when dtype == DT_C64).
-void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s);
+void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s,
+                   hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 
 // Precision conversion helpers (n counts real scalars).
 void convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);

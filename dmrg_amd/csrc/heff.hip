@@ -105,15 +105,17 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
   return h;
 }
 
-void HeffPlan::apply(const double* in, double* out, hipStream_t s) {
+void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev) {
   const int c = dtype == DT_C64 ? 2 : 1;
+  hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr, e2 = ev ? ev[2] : nullptr,
+             e3 = ev ? ev[3] : nullptr;
   if (prec == PREC_FP64) {
-    run_gemm_plan(*rplan, renv, in, store->ptr, s);
-    run_gemm_plan(*lplan, lenv, store->ptr, out, s);
+    run_gemm_plan(*rplan, renv, in, store->ptr, s, e0, e1);
+    run_gemm_plan(*lplan, lenv, store->ptr, out, s, e2, e3);
   } else {
     convert_f64_to_f32(in, (float*)in_lo->ptr, n_psi * c, s);
-    run_gemm_plan(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s);
-    run_gemm_plan(*lplan, lenv_lo->ptr, store->ptr, out_lo->ptr, s);
+    run_gemm_plan(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s, e0, e1);
+    run_gemm_plan(*lplan, lenv_lo->ptr, store->ptr, out_lo->ptr, s, e2, e3);
     convert_f32_to_f64((const float*)out_lo->ptr, out, n_psi * c, s);
   }
 }
@@ -121,10 +123,6 @@ void HeffPlan::apply(const double* in, double* out, hipStream_t s) {
 }  // namespace dmrgx
 
 using namespace dmrgx;
-
-struct dmrgx_heff {
-  std::unique_ptr<HeffPlan> p;
-};
 
 #define CAPI_BEGIN try {
 #define CAPI_END                      \
@@ -204,6 +202,33 @@ int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out) {
   DMRGX_CHECK(plan && plan->p, "heff_apply: null plan");
   plan->p->apply((const double*)in, (double*)out, rt().stream);
   DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  CAPI_END
+}
+
+int dmrgx_heff_time(dmrgx_heff* plan, const void* in, void* out, int iters, double* ms) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p && iters > 0, "heff_time: bad arguments");
+  hipStream_t s = rt().stream;
+  std::vector<hipEvent_t> ev(4 * iters + 2);
+  for (auto& e : ev) DMRGX_HIP(hipEventCreate(&e));
+  DMRGX_HIP(hipEventRecord(ev[4 * iters], s));
+  for (int i = 0; i < iters; ++i) plan->p->apply((const double*)in, (double*)out, s, &ev[4 * i]);
+  DMRGX_HIP(hipEventRecord(ev[4 * iters + 1], s));
+  DMRGX_HIP(hipStreamSynchronize(s));
+  float tot = 0, r = 0, l = 0, x;
+  DMRGX_HIP(hipEventElapsedTime(&tot, ev[4 * iters], ev[4 * iters + 1]));
+  for (int i = 0; i < iters; ++i) {
+    DMRGX_HIP(hipEventElapsedTime(&x, ev[4 * i], ev[4 * i + 1]));
+    r += x;
+    DMRGX_HIP(hipEventElapsedTime(&x, ev[4 * i + 2], ev[4 * i + 3]));
+    l += x;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  ms[0] = tot / iters;
+  ms[1] = r / iters;
+  ms[2] = l / iters;
+  ms[3] = plan->p->rplan->flops;
+  ms[4] = plan->p->lplan->flops;
   CAPI_END
 }
 

@@ -19,7 +19,8 @@ struct HeffPlan {
   std::unique_ptr<DevBuf> store, in_lo, out_lo;
   std::unique_ptr<GemmPlan> rplan, lplan;
   double flops = 0;
-  void apply(const double* in, double* out, hipStream_t s);
+  // ev: optional {R begin, R end, L begin, L end} events around the two GEMM launches
+  void apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev = nullptr);
 };
 
 // get_map tuples -> GEMM tasks (M = blk_size / col, N = row, K = col).
@@ -36,3 +37,9 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
                                     int64_t n_store);
 
 }  // namespace dmrgx
+
+// opaque C handle: the plan plus any device buffers it owns (synthetic envs)
+struct dmrgx_heff {
+  std::unique_ptr<dmrgx::HeffPlan> p;
+  std::vector<std::shared_ptr<dmrgx::DevBuf>> keep;
+};

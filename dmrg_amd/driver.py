@@ -1,8 +1,183 @@
-"""ctypes declarations of the sweep-driver entry points (include/dmrgx_driver.h)."""
+"""ctypes bindings of include/dmrgx_driver.h: full DMRG runs on the GPU and the
+structure-only bookkeeping entry points (get_map, generate_dim_sym)."""
 from __future__ import annotations
 
-DRIVER_EXPORTS: list[str] = []
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
+STEP_LOG_WIDTH = 17
+DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym"]
+
+
+class RunConfig(C.Structure):
+    _fields_ = [("model", C.c_int), ("p", C.c_double * 4), ("L", C.c_int), ("m", C.c_int), ("sweeps", C.c_int),
+                ("sector", C.c_int), ("schedule", C.c_int), ("prec", C.c_int), ("sign_rule", C.c_int),
+                ("ladder", C.c_int), ("seed", C.c_uint64), ("tol", C.c_double), ("tol_lo", C.c_double),
+                ("refine_k", C.c_int), ("verbose", C.c_int)]
+
+
+class RunStats(C.Structure):
+    _fields_ = [("energy", C.c_double), ("seconds", C.c_double), ("sweep_seconds", C.c_double),
+                ("last_sweep_seconds", C.c_double), ("steps", C.c_int), ("matvecs", C.c_longlong),
+                ("polish_matvecs", C.c_longlong), ("matvec_flops", C.c_double), ("trunc_err", C.c_double),
+                ("entropy", C.c_double), ("t_eig", C.c_double), ("t_svd", C.c_double), ("t_env", C.c_double)]
 
 
 def declare(lib):
+    i32p = C.POINTER(C.c_int32)
+    lib.dmrgx_run_dmrg.argtypes = [C.POINTER(RunConfig), C.POINTER(RunStats), C.POINTER(C.c_double), C.c_int,
+                                   C.POINTER(C.c_int)]
+    lib.dmrgx_run_dmrg.restype = C.c_int
+    lib.dmrgx_synth_heff_create.argtypes = [C.c_int, C.c_int, C.POINTER(C.c_int), C.c_int, C.c_int, C.c_int,
+                                            C.c_uint64, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]
+    lib.dmrgx_synth_heff_create.restype = C.c_int
+    lib.dmrgx_get_map.argtypes = [C.c_int, i32p, i32p, C.c_char, C.c_char, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  i32p, i32p, C.c_int, C.POINTER(C.c_int), i32p, C.c_int]
+    lib.dmrgx_get_map.restype = C.c_int
+    lib.dmrgx_generate_dim_sym.argtypes = [C.c_int, i32p, C.c_int, i32p, C.c_int, C.c_int, i32p, C.c_int]
+    lib.dmrgx_generate_dim_sym.restype = C.c_int
     return lib
+
+
+@dataclass
+class RunResult:
+    energy: float
+    stats: dict
+    steps: np.ndarray  # (n_steps, STEP_LOG_WIDTH)
+
+
+def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
+             sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
+             verbose=False) -> RunResult:
+    from . import PREC_NAMES, check, init, lib
+    init()
+    cfg = RunConfig()
+    cfg.model = MODELS[model]
+    for i, v in enumerate(list(params)[:4]):
+        cfg.p[i] = float(v)
+    cfg.L, cfg.m, cfg.sweeps, cfg.sector = int(L), int(m), int(sweeps), int(sector)
+    cfg.schedule = 1 if schedule == "dmrg2" else 0
+    cfg.prec = PREC_NAMES[prec]
+    cfg.sign_rule = 0 if sign_rule == "physical" else 1
+    cfg.ladder = 1 if ladder else 0
+    cfg.seed = seed
+    cfg.tol, cfg.tol_lo, cfg.refine_k, cfg.verbose = tol, tol_lo, refine_k, 1 if verbose else 0
+    st = RunStats()
+    cap = 20000
+    log = np.zeros((cap, STEP_LOG_WIDTH))
+    n = C.c_int()
+    check(lib().dmrgx_run_dmrg(C.byref(cfg), C.byref(st), log.ctypes.data_as(C.POINTER(C.c_double)), cap,
+                               C.byref(n)), "dmrgx_run_dmrg")
+    stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
+    return RunResult(st.energy, stats, log[:n.value].copy())
+
+
+def encode_desc(dim, sym, dirs=None):
+    v = [len(dim)] + [len(d) for d in dim]
+    for d in dim:
+        v += list(d)
+    v.append(len(sym))
+    for s in sym:
+        v += list(s)
+    if dirs is not None:
+        v += list(dirs)
+    return np.asarray(v, dtype=np.int32)
+
+
+def decode_desc(a):
+    a = list(a)
+    p = 0
+    nleg = a[p]
+    p += 1
+    ns = a[p:p + nleg]
+    p += nleg
+    dim = []
+    for k in ns:
+        dim.append(a[p:p + k])
+        p += k
+    nb = a[p]
+    p += 1
+    sym = [a[p + b * nleg: p + (b + 1) * nleg] for b in range(nb)]
+    return dim, sym
+
+
+def get_map(symmetry, tdim, tsym, tdir, dim, sym, ta="N", tb="T", num=1, beta=0, back=False, sym_ret=None,
+            fermion=True):
+    """Structure-only qtensor::get_map through the library (integer bookkeeping)."""
+    from . import check, lib
+    td = encode_desc(tdim, tsym, tdir)
+    ad = encode_desc(dim, sym)
+    nleg = len(tdim) + len(dim) - 2 * num
+    sr = np.zeros(1, dtype=np.int32)
+    if back:
+        sr = np.asarray([len(sym_ret)] + [x for s in sym_ret for x in s], dtype=np.int32)
+    cap = max(16, 4 * len(tsym) * len(sym) + 16)
+    out = np.zeros(cap * 7, dtype=np.int32)
+    rcap = 1 << 16
+    ret = np.zeros(rcap, dtype=np.int32)
+    n = C.c_int()
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))
+    check(lib().dmrgx_get_map(symmetry, P(td), P(ad), ta.encode(), tb.encode(), num, int(beta), 1 if back else 0,
+                              1 if fermion else 0, P(sr), P(out), cap, C.byref(n), P(ret), rcap), "dmrgx_get_map")
+    maps = out[:n.value * 7].reshape(-1, 7).tolist()
+    d, s = decode_desc(ret)
+    assert len(d) == nleg
+    return maps, d, s
+
+
+def generate_dim_sym(symmetry, ldim, lsym, ldir, lnum, rdim, rsym, rdir, rnum, sector):
+    from . import check, lib
+    P = lambda x: x.ctypes.data_as(C.POINTER(C.c_int32))
+    ld = encode_desc(ldim, lsym, ldir)
+    rd = encode_desc(rdim, rsym, rdir)
+    out = np.zeros(1 << 16, dtype=np.int32)
+    check(lib().dmrgx_generate_dim_sym(symmetry, P(ld), lnum, P(rd), rnum, sector, P(out), len(out)),
+          "dmrgx_generate_dim_sym")
+    return decode_desc(out)
+
+
+class SynthHeff:
+    """A config-shaped synthetic two-site operator built by the library."""
+
+    def __init__(self, S, D, bond, complex_=True, prec="fp64", seed=20261203, fermion=False):
+        from . import C64, PREC_NAMES, R64, check, init, lib
+        init()
+        b = (C.c_int * len(bond))(*bond)
+        h = C.c_void_p()
+        n = C.c_int64()
+        check(lib().dmrgx_synth_heff_create(S, D, b, len(bond), C64 if complex_ else R64, PREC_NAMES[prec], seed,
+                                            1 if fermion else 0, C.byref(h), C.byref(n)), "dmrgx_synth_heff_create")
+        self.h = h
+        self.n_psi = n.value
+        self.dtype = C64 if complex_ else R64
+
+    @property
+    def flops(self):
+        from . import lib
+        return lib().dmrgx_heff_flops(self.h)
+
+    def time(self, x, y, iters):
+        from . import check, lib
+        ms = np.zeros(5)
+        check(lib().dmrgx_heff_time(self.h, x.ptr, y.ptr, int(iters), ms.ctypes.data_as(C.POINTER(C.c_double))),
+              "dmrgx_heff_time")
+        return ms
+
+    def apply(self, x, y):
+        from . import check, lib
+        check(lib().dmrgx_heff_apply(self.h, x.ptr, y.ptr), "dmrgx_heff_apply")
+
+    def close(self):
+        from . import lib
+        if self.h:
+            lib().dmrgx_heff_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -72,6 +72,10 @@ int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out);
 /* algorithmic flops of one apply: sum over tasks of c*M*N*K (c = 2 real, 8 complex) */
 double dmrgx_heff_flops(const dmrgx_heff* plan);
 int dmrgx_heff_destroy(dmrgx_heff* plan);
+/* Run `iters` applies on the library stream with HIP events around each GEMM
+ * launch: ms[0] = mean ms per apply, ms[1] / ms[2] = mean ms of the R / L pass
+ * GEMM kernel, ms[3] / ms[4] = algorithmic flops of the R / L pass. */
+int dmrgx_heff_time(dmrgx_heff* plan, const void* in, void* out, int iters, double* ms);
 
 /* ---- eigensolver -------------------------------------------------------------
  * Replaces znaupd(n, nev, Evals, Evecs, lanczos<T>&) (src/dsaupd.cpp:65-155,

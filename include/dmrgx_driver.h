@@ -1,0 +1,81 @@
+/*
+ * dmrgx driver API: the C++ host mirror of the reference's sweep drivers
+ * (dmrg / dmrg2, src/functions.cpp:157-299) and step functions
+ * (src/core.cpp:88-474) running every numerical kernel through dmrgx.h, plus
+ * the integer bookkeeping entry points used to check it bit for bit.
+ * Same conventions as dmrgx.h (0 = success, dmrgx_last_error()).
+ */
+#ifndef DMRGX_DRIVER_H
+#define DMRGX_DRIVER_H
+
+#include <stdint.h>
+
+#include "dmrgx.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { DMRGX_MODEL_XYZ = 0, DMRGX_MODEL_KITAEV = 1, DMRGX_MODEL_POTTS3 = 2, DMRGX_MODEL_POTTS2 = 3 };
+enum { DMRGX_SCHEDULE_DMRG = 0, DMRGX_SCHEDULE_DMRG2 = 1 };
+enum { DMRGX_SIGNS_PHYSICAL = 0, DMRGX_SIGNS_REFERENCE = 1 };
+
+typedef struct dmrgx_run_config {
+  int model;        /* DMRGX_MODEL_* */
+  double p[4];      /* xyz: jx jy jz; kitaev: t p u; potts3: f j p t; potts2: h j */
+  int L, m, sweeps, sector;   /* the reference's l= m= n= s= (src/functions.cpp:46-66) */
+  int schedule;     /* DMRGX_SCHEDULE_DMRG (the mains' dmrg) or _DMRG2 (two-site) */
+  int prec;         /* matvec arithmetic: DMRGX_FP64 / DMRGX_FP32 / DMRGX_BF16X3 */
+  int sign_rule;    /* DMRGX_SIGNS_PHYSICAL (alpha=1 for bosonic models) or _REFERENCE */
+  int ladder;       /* 1: pre-sweep cutoff ladder 100,200,.. as the reference; 0: skip */
+  uint64_t seed;    /* Lanczos start vectors */
+  double tol;       /* FP64 Lanczos tolerance (reference: 1e-10) */
+  double tol_lo;    /* reduced-precision Lanczos tolerance */
+  int refine_k;     /* FP64 Krylov polish steps after a reduced-precision solve */
+  int verbose;
+} dmrgx_run_config;
+
+typedef struct dmrgx_run_stats {
+  double energy;          /* value returned by the last step */
+  double seconds;         /* whole run */
+  double sweep_seconds;   /* mean wall seconds of the full sweeps (0 if none) */
+  double last_sweep_seconds;
+  int steps;
+  long long matvecs, polish_matvecs;
+  double matvec_flops;    /* algorithmic flops of all matvecs */
+  double trunc_err, entropy;   /* of the last step */
+  double t_eig, t_svd, t_env;  /* summed step phases, seconds */
+} dmrgx_run_stats;
+
+/* One row per step: l, r, E0..E5, trunc, entropy, matvecs, polish matvecs,
+ * t_total, t_eig, t_svd, t_env, d (17 doubles). */
+#define DMRGX_STEP_LOG_WIDTH 17
+int dmrgx_run_dmrg(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, double* step_log, int log_cap,
+                   int* n_steps);
+
+/* Synthetic two-site operator with the block structure of a Z_S model at bond
+ * sectors `bond` (n_sec entries), MPO bond D, physical sectors of dimension 1,
+ * every sector combination present, uniform(-1,1) entries from `seed`.  The
+ * plan owns its environments. */
+int dmrgx_synth_heff_create(int S, int D, const int* bond, int n_sec, int dtype, int prec, uint64_t seed,
+                            int fermion, dmrgx_heff** plan, int64_t* n_psi);
+
+/* qtensor<T>::get_map on structure only (src/qtensor.cpp:997-1119).
+ * Tensor / array descriptions: desc = [nleg, nsec[nleg], dims (per leg, per
+ * sector), nblk, sym (nblk x nleg), dir (nleg, tensor only)].  `symret_in`
+ * ([nblk, sym...]) is read when back != 0.  Writes n_map tuples of 7 int32
+ * and the result description (dims + syms, no dirs) into ret_desc. */
+int dmrgx_get_map(int symmetry, const int32_t* tensor_desc, const int32_t* array_desc, char ta, char tb, int num,
+                  int beta, int back, int fermion, const int32_t* symret_in, int32_t* map_out, int map_cap,
+                  int* n_map, int32_t* ret_desc, int ret_cap);
+
+/* generate_dim_sym (src/useful.cpp:151-194) on structure only; lenv / renv
+ * descriptions as above (dirs included); out_desc as get_map's ret_desc. */
+int dmrgx_generate_dim_sym(int symmetry, const int32_t* lenv_desc, int lnum, const int32_t* renv_desc, int rnum,
+                           int sector, int32_t* out_desc, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DMRGX_DRIVER_H */

@@ -1,0 +1,80 @@
+"""Full DMRG runs on the GPU through the C++ host mirror (dmrgx_run_dmrg)
+against the oracle's restatement of the same schedule and against exact
+energies (BdG closed form, exact diagonalisation)."""
+import numpy as np
+import pytest
+
+from dmrg_amd import driver
+from oracle import exact, models
+from oracle.dmrg import Runner
+
+pytestmark = pytest.mark.gpu
+
+KITAEV = (-1.0, -1.0, 0.25)          # t=Delta=1, mu=0.5 in the reference's (T, P, U)
+XYZ = (1.0, 0.8, 0.5)
+POTTS = (1.0, 1.0, 0.0, 3.14159265359 / 8)
+
+
+def _oracle(model, L, m, sweeps, schedule, sign_rule, params):
+    kw = {"kitaev": dict(t=params[0], p=params[1], u=params[2]),
+          "xyz": dict(jx=params[0], jy=params[1], jz=params[2]),
+          "potts3": dict(f=params[0], j=params[1], p=params[2], t=params[3])}[model]
+    mpo, n, ferm = models.make_model(model, L, **kw)
+    R = Runner(mpo, n, ferm, sign_rule=sign_rule)
+    e = R.dmrg2(m, sweeps) if schedule == "dmrg2" else R.dmrg(m, sweeps)
+    return e, R
+
+
+@pytest.mark.parametrize("schedule", ["dmrg2", "dmrg"])
+def test_kitaev_closed_form(gpu, schedule):
+    L = 12
+    print(f"case kitaev L={L} {schedule}", flush=True)
+    res = driver.run_dmrg("kitaev", L, 32, 2, KITAEV, schedule=schedule)
+    e_bdg, _ = exact.kitaev_bdg_e0(L, *KITAEV)
+    e_ed = exact.kitaev_ed(L, *KITAEV)[0]
+    assert abs(res.energy - e_ed) < 1e-9, (res.energy, e_ed, e_bdg)
+
+
+@pytest.mark.parametrize("schedule", ["dmrg2", "dmrg"])
+def test_xyz_exact_diagonalisation(gpu, schedule):
+    L = 10
+    print(f"case xyz L={L} {schedule}", flush=True)
+    res = driver.run_dmrg("xyz", L, 64, 2, XYZ, schedule=schedule)
+    e_ed = exact.xyz_e0(L, *XYZ)[0]
+    assert abs(res.energy - e_ed) < 1e-9, (res.energy, e_ed)
+
+
+def test_potts3_exact_diagonalisation(gpu):
+    L = 7
+    print(f"case potts3 L={L}", flush=True)
+    res = driver.run_dmrg("potts3", L, 81, 2, POTTS)
+    W = models.H_Potts3(*POTTS).remove_symmetry().val[0]
+    e_ed = exact.mpo_e0(W, L, 3, 3, 0)[0]
+    assert abs(res.energy - e_ed) < 1e-9, (res.energy, e_ed)
+
+
+@pytest.mark.parametrize("sign_rule", ["physical", "reference"])
+def test_xyz_matches_oracle_step_by_step(gpu, sign_rule):
+    """Energies of every step agree with the oracle's schedule (same
+    schedule, same truncation); the reference sign rule is non-Hermitian for
+    bosonic models only in single-site steps, so the two-site schedule is used."""
+    L, m = 10, 16
+    print(f"case xyz oracle-vs-gpu {sign_rule}", flush=True)
+    res = driver.run_dmrg("xyz", L, m, 1, XYZ, schedule="dmrg2", sign_rule=sign_rule)
+    e, R = _oracle("xyz", L, m, 1, "dmrg2", sign_rule, XYZ)
+    assert len(R.log) == len(res.steps)
+    for g, o in zip(res.steps, R.log):
+        assert (int(g[0]), int(g[1])) == (o.l, o.r)
+        assert int(g[16]) == o.dim
+        assert abs(g[2] - o.energies[0]) < 1e-8 * max(1, abs(o.energies[0])), (g[:3], o.energies[0])
+    assert abs(res.energy - e) < 1e-8
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3"])
+def test_mixed_precision_energy(gpu, prec):
+    L, m = 16, 32
+    print(f"case xyz mixed {prec}", flush=True)
+    ref = driver.run_dmrg("xyz", L, m, 2, XYZ, ladder=False)
+    mix = driver.run_dmrg("xyz", L, m, 2, XYZ, ladder=False, prec=prec,
+                          tol_lo=1e-7 if prec == "fp32" else 1e-5, refine_k=10)
+    assert abs(mix.energy - ref.energy) < 1e-8, (mix.energy, ref.energy)
