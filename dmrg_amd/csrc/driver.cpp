@@ -3,7 +3,7 @@
 #include <chrono>
 #include <cstring>
 
-#include "../../include/dmrgx_driver.h"
+#include "dmrgx_driver.h"
 #include "core.hpp"
 #include "heff.hpp"
 #include "lanczos.hpp"

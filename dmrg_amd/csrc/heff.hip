@@ -4,7 +4,7 @@
 
 #include <algorithm>
 
-#include "../../include/dmrgx.h"
+#include "dmrgx.h"
 #include "common.hpp"
 #include "gemm.hpp"
 #include "heff.hpp"

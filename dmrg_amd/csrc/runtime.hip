@@ -7,7 +7,7 @@
 #include <unordered_map>
 
 #include "common.hpp"
-#include "../../include/dmrgx.h"
+#include "dmrgx.h"
 
 namespace dmrgx {
 

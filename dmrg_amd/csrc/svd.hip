@@ -14,7 +14,7 @@
 #include <numeric>
 #include <cstring>
 
-#include "../../include/dmrgx.h"
+#include "dmrgx.h"
 #include "common.hpp"
 #include "svd.hpp"
 

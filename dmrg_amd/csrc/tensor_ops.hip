@@ -5,7 +5,7 @@
 #include <algorithm>
 #include <cmath>
 
-#include "../../include/dmrgx.h"
+#include "dmrgx.h"
 #include "common.hpp"
 #include "tensor_ops.hpp"
 
