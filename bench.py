@@ -31,7 +31,7 @@ L_CHAIN = 100
 JXYZ = (1.0, 0.8, 0.5)
 
 
-def gpu_leg(steps, warmup):
+def gpu_leg(steps, warmup, only=None):
     import dmrg_amd
     from dmrg_amd.driver import SynthHeff
 
@@ -39,6 +39,8 @@ def gpu_leg(steps, warmup):
     bond = [M // S] * S
     out = {}
     for prec in ("fp64", "fp32", "bf16x3"):
+        if only and prec not in only:
+            continue
         h = SynthHeff(S, D, bond, complex_=True, prec=prec, seed=20261203)
         rng = np.random.default_rng(1)
         x = rng.uniform(-1, 1, h.n_psi) + 1j * rng.uniform(-1, 1, h.n_psi)
@@ -107,11 +109,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", type=int, default=0, help="run this many full sweeps of the L=100 m=512 chain")
+    ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
               file=sys.stderr)
         sys.exit(2)
+    if a.only:
+        gpu_leg(a.steps, a.warmup, a.only.split(","))
+        return
     g = gpu_leg(a.steps, a.warmup)
     mixed, fp64 = g["bf16x3"], g["fp64"]
     # dominant kernel: the BF16x3 grouped GEMM (R and L pass launches)

@@ -4,8 +4,8 @@
 // contraction).  A work item is one 64x64 output tile together with a K-chunk
 // of the concatenated segments of all tasks that write that output block, so
 // the sum over contracted sectors happens in registers, uneven sector shapes
-// share a launch, and long K is split across work items (atomic accumulate)
-// to fill all CUs.  Three arithmetic modes:
+// share a launch, and long K is split across work items (partial tiles in a
+// workspace, summed by one reduction pass) to fill all CUs.  Three arithmetic modes:
 //   PREC_FP64   v_mfma_f64_16x16x4f64, FP64 storage
 //   PREC_FP32   v_mfma_f32_16x16x4f32, FP32 storage
 //   PREC_BF16X3 v_mfma_f32_16x16x32_bf16 with each FP32 operand split into
@@ -24,6 +24,9 @@ namespace dmrgx {
 GemmPlan::~GemmPlan() {
   delete d_items;
   delete d_segs;
+  delete d_ws;
+  delete d_red;
+  delete d_red_start;
 }
 
 namespace {
@@ -52,8 +55,30 @@ struct Mfma4<float> {
   }
 };
 
-__device__ __forceinline__ void atomic_add(double* p, double v) { unsafeAtomicAdd(p, v); }
-__device__ __forceinline__ void atomic_add(float* p, float v) { unsafeAtomicAdd(p, v); }
+// Sum the K-chunk partial tiles of the split output blocks into C.
+template <typename S, bool CPLX>
+__global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* __restrict__ C,
+                                                   const RedTask* __restrict__ t, int n,
+                                                   const int64_t* __restrict__ start) {
+  int lo = 0, hi = n - 1;
+  const int64_t b = blockIdx.x;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const RedTask r = t[lo];
+  const int64_t e = (b - start[lo]) * 256 + threadIdx.x;
+  if (e >= r.len) return;
+  constexpr int c = CPLX ? 2 : 1;
+#pragma unroll
+  for (int q = 0; q < c; ++q) {
+    S acc = 0;
+    for (int k = 0; k < r.nch; ++k) acc += W[c * (r.w_off + k * r.len + e) + q];
+    if (r.beta) C[c * (r.c_off + e) + q] += acc;
+    else C[c * (r.c_off + e) + q] = acc;
+  }
+}
 
 // Walks the (segment, k-tile) sequence of one work item.
 struct Cursor {
@@ -88,7 +113,7 @@ template <typename S, bool CPLX, bool AKC, bool BKC>
 __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items, int nitems,
                                               int per_xcd, const GemmSeg* __restrict__ segs,
                                               const S* __restrict__ A, const S* __restrict__ B,
-                                              S* __restrict__ C) {
+                                              S* __restrict__ C, S* __restrict__ W) {
   constexpr int BK = 16;
   constexpr int LDK = BK + 1;
   constexpr int NP = CPLX ? 2 : 1;
@@ -193,8 +218,9 @@ __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items
 
   // epilogue: lane holds column lane%16 of each 16x16 tile; the row of
   // accumulator r is 4*(lane/16)+r for f32 and r*4+lane/16 for f64 MFMA
-  const bool atomic = it.flags & ITEM_ATOMIC;
-  const bool beta = it.flags & ITEM_BETA;
+  const bool partial = it.flags & ITEM_PARTIAL;
+  const bool beta = !partial && (it.flags & ITEM_BETA);
+  auto* D = partial ? W : C;
   constexpr bool F64 = sizeof(S) == 8;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -208,14 +234,12 @@ __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items
           int64_t e = it.c_off + (int64_t)row * it.N + col;
           if (CPLX) {
             S vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
-            if (atomic) { atomic_add(&C[2 * e], vr); atomic_add(&C[2 * e + 1], vi); }
-            else if (beta) { C[2 * e] += vr; C[2 * e + 1] += vi; }
-            else { C[2 * e] = vr; C[2 * e + 1] = vi; }
+            if (beta) { D[2 * e] += vr; D[2 * e + 1] += vi; }
+            else { D[2 * e] = vr; D[2 * e + 1] = vi; }
           } else {
             S v = acc[0][i][j][r];
-            if (atomic) atomic_add(&C[e], v);
-            else if (beta) C[e] += v;
-            else C[e] = v;
+            if (beta) D[e] += v;
+            else D[e] = v;
           }
         }
       }
@@ -235,7 +259,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(const GemmItem* __restrict__
                                                     int per_xcd, const GemmSeg* __restrict__ segs,
                                                     const float* __restrict__ A,
                                                     const float* __restrict__ B,
-                                                    float* __restrict__ C) {
+                                                    float* __restrict__ C, float* __restrict__ W) {
   constexpr int BK = 32;
   constexpr int LDK = BK + 8;  // 80-byte rows keep 16-byte fragment reads aligned
   constexpr int NP = CPLX ? 2 : 1;
@@ -350,8 +374,9 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(const GemmItem* __restrict__
       }
   }
 
-  const bool atomic = it.flags & ITEM_ATOMIC;
-  const bool beta = it.flags & ITEM_BETA;
+  const bool partial = it.flags & ITEM_PARTIAL;
+  const bool beta = !partial && (it.flags & ITEM_BETA);
+  auto* D = partial ? W : C;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -364,17 +389,219 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(const GemmItem* __restrict__
           int64_t e = it.c_off + (int64_t)row * it.N + col;
           if (CPLX) {
             float vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
-            if (atomic) { atomic_add(&C[2 * e], vr); atomic_add(&C[2 * e + 1], vi); }
-            else if (beta) { C[2 * e] += vr; C[2 * e + 1] += vi; }
-            else { C[2 * e] = vr; C[2 * e + 1] = vi; }
+            if (beta) { D[2 * e] += vr; D[2 * e + 1] += vi; }
+            else { D[2 * e] = vr; D[2 * e + 1] = vi; }
           } else {
             float v = acc[0][i][j][r];
-            if (atomic) atomic_add(&C[e], v);
-            else if (beta) C[e] += v;
-            else C[e] = v;
+            if (beta) D[e] += v;
+            else D[e] = v;
           }
         }
       }
+}
+
+
+// ---------------------------------------------------------------------------
+// BF16x3 on pre-split operands: A, B are bfloat16 planes (re_hi, re_lo[, im_hi,
+// im_lo]); 128x128 output tiles, each wavefront 64x64 (4x4 MFMA 16x16x32
+// tiles); 16-byte vector loads straight into LDS when a segment is aligned.
+constexpr int PT = 128, PBK = 32, PLDK = PBK + 8;
+
+__device__ __forceinline__ uint4 xor_sign(uint4 v) {
+  v.x ^= 0x80008000u; v.y ^= 0x80008000u; v.z ^= 0x80008000u; v.w ^= 0x80008000u;
+  return v;
+}
+
+template <bool CPLX>
+__global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ items, int nitems, int per_xcd,
+                                                  const GemmSeg* __restrict__ segs,
+                                                  const unsigned short* __restrict__ A, int64_t as,
+                                                  const unsigned short* __restrict__ B, int64_t bs,
+                                                  float* __restrict__ C, float* __restrict__ W) {
+  constexpr int NPL = CPLX ? 4 : 2;
+  constexpr int NP = CPLX ? 2 : 1;
+  constexpr int CPR = PBK / 8;               // 16-byte chunks per row per plane (4)
+  constexpr int CH = PT * CPR * NPL;         // chunks per operand per k-step
+  constexpr int CPT = CH / NT;               // chunks per thread (8 complex, 4 real)
+  const int bid = blockIdx.x;
+  const int item_id = (bid & 7) * per_xcd + (bid >> 3);
+  if (item_id >= nitems) return;
+  const GemmItem it = items[item_id];
+
+  __shared__ __attribute__((aligned(16))) unsigned short sA[NPL][PT * PLDK];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[NPL][PT * PLDK];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+  f4v acc[NP][4][4];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[p][i][j] = f4v{0, 0, 0, 0};
+
+  uint4 ra[CPT], rb[CPT];
+  Cursor cur;
+  cur.start(segs, it.seg_begin, it.seg_end);
+  bool avec = false, bvec = false;
+  auto seg_flags = [&](const GemmSeg& s) {
+    avec = ((s.a_off | s.K | s.k0) & 7) == 0;
+    bvec = ((s.b_off | s.K | s.k0) & 7) == 0;
+  };
+  if (cur.valid()) seg_flags(cur.s);
+  auto gload = [&](const Cursor& c) {
+    const GemmSeg& s = c.s;
+    const bool neg = s.alpha < 0;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int ch = t + NT * j;
+      const int plane = ch / (PT * CPR);
+      const int rem = ch % (PT * CPR);
+      const int row = rem / CPR, kc = (rem % CPR) * 8;
+      const int gk = c.k + kc;
+      // A
+      {
+        const int gr = it.m0 + row;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (gr < it.M) {
+          const unsigned short* src = A + plane * as + s.a_off + (int64_t)gr * s.K + gk;
+          if (avec && gk + 8 <= s.k1) {
+            v = *reinterpret_cast<const uint4*>(src);
+          } else {
+            unsigned short h[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) h[e] = (gk + e < s.k1) ? src[e] : (unsigned short)0;
+            v.x = h[0] | ((unsigned)h[1] << 16); v.y = h[2] | ((unsigned)h[3] << 16);
+            v.z = h[4] | ((unsigned)h[5] << 16); v.w = h[6] | ((unsigned)h[7] << 16);
+          }
+          if (neg) v = xor_sign(v);
+        }
+        ra[j] = v;
+      }
+      // B
+      {
+        const int gn = it.n0 + row;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (gn < it.N) {
+          const unsigned short* src = B + plane * bs + s.b_off + (int64_t)gn * s.K + gk;
+          if (bvec && gk + 8 <= s.k1) {
+            v = *reinterpret_cast<const uint4*>(src);
+          } else {
+            unsigned short h[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) h[e] = (gk + e < s.k1) ? src[e] : (unsigned short)0;
+            v.x = h[0] | ((unsigned)h[1] << 16); v.y = h[2] | ((unsigned)h[3] << 16);
+            v.z = h[4] | ((unsigned)h[5] << 16); v.w = h[6] | ((unsigned)h[7] << 16);
+          }
+        }
+        rb[j] = v;
+      }
+    }
+  };
+  if (cur.valid()) gload(cur);
+  while (cur.valid()) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+      const int ch = t + NT * j;
+      const int plane = ch / (PT * CPR);
+      const int rem = ch % (PT * CPR);
+      const int row = rem / CPR, kc = (rem % CPR) * 8;
+      *reinterpret_cast<uint4*>(&sA[plane][row * PLDK + kc]) = ra[j];
+      *reinterpret_cast<uint4*>(&sB[plane][row * PLDK + kc]) = rb[j];
+    }
+    __syncthreads();
+    const int prev_seg = cur.seg;
+    cur.next<PBK>(segs);
+    if (cur.valid()) {
+      if (cur.seg != prev_seg) seg_flags(cur.s);
+      gload(cur);
+    }
+    const int kx = (lane >> 4) * 8;
+    bf8v a[NPL][4];
+#pragma unroll
+    for (int p = 0; p < NPL; ++p)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        a[p][i] = *reinterpret_cast<const bf8v*>(&sA[p][(wm + i * 16 + (lane & 15)) * PLDK + kx]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bf8v b[NPL];
+#pragma unroll
+      for (int p = 0; p < NPL; ++p)
+        b[p] = *reinterpret_cast<const bf8v*>(&sB[p][(wn + j * 16 + (lane & 15)) * PLDK + kx]);
+      bf8v nbh, nbl;
+      if (CPLX) {
+        nbh = __builtin_bit_cast(bf8v, __builtin_bit_cast(u16x8, b[2]) ^ (unsigned short)0x8000);
+        nbl = __builtin_bit_cast(bf8v, __builtin_bit_cast(u16x8, b[3]) ^ (unsigned short)0x8000);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        f4v r = acc[0][i][j];
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0], r, 0, 0, 0);
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1], r, 0, 0, 0);
+        if (CPLX) {
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3][i], nbh, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], nbl, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], nbh, r, 0, 0, 0);
+        }
+        r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0], r, 0, 0, 0);
+        acc[0][i][j] = r;
+        if (CPLX) {
+          f4v q = acc[NP - 1][i][j];
+          q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[2], q, 0, 0, 0);
+          q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[3], q, 0, 0, 0);
+          q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[3][i], b[0], q, 0, 0, 0);
+          q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[1], q, 0, 0, 0);
+          q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2], q, 0, 0, 0);
+          q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0], q, 0, 0, 0);
+          acc[NP - 1][i][j] = q;
+        }
+      }
+    }
+  }
+
+  const bool partial = it.flags & ITEM_PARTIAL;
+  const bool beta = !partial && (it.flags & ITEM_BETA);
+  auto* D = partial ? W : C;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int row = it.m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        int col = it.n0 + wn + j * 16 + (lane & 15);
+        if (row < it.M && col < it.N) {
+          int64_t e = it.c_off + (int64_t)row * it.N + col;
+          if (CPLX) {
+            float vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
+            if (beta) { D[2 * e] += vr; D[2 * e + 1] += vi; }
+            else { D[2 * e] = vr; D[2 * e + 1] = vi; }
+          } else {
+            float v = acc[0][i][j][r];
+            if (beta) D[e] += v;
+            else D[e] = v;
+          }
+        }
+      }
+}
+
+template <typename SRC>
+__global__ void k_split_planes(const SRC* __restrict__ src, unsigned short* __restrict__ dst, int64_t n,
+                               int64_t stride, int cplx) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) {
+    for (int part = 0; part <= cplx; ++part) {
+      double x = (double)src[cplx ? 2 * i + part : i];
+      float xf = (float)x;
+      unsigned short h = f2bf_rne(xf);
+      unsigned short l = f2bf_rne((float)(x - (double)bf2f(h)));
+      dst[(2 * part) * stride + i] = h;
+      dst[(2 * part + 1) * stride + i] = l;
+    }
+  }
 }
 
 __global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
@@ -395,7 +622,7 @@ void launch4(const GemmPlan& p, const void* A, const void* B, void* C, hipStream
   const bool akc = (p.ta == 'N' || p.ta == 'n'), bkc = (p.tb == 'T' || p.tb == 't');
   dim3 g(p.grid), b(NT);
 #define L4(AK, BKc) hipLaunchKernelGGL((k_gemm4<S, CPLX, AK, BKc>), g, b, 0, s, it, p.n_items, per, sg, \
-                                       (const S*)A, (const S*)B, (S*)C)
+                                       (const S*)A, (const S*)B, (S*)C, (S*)(p.d_ws ? p.d_ws->ptr : nullptr))
   if (akc && bkc) L4(true, true);
   else if (akc) L4(true, false);
   else if (bkc) L4(false, true);
@@ -410,7 +637,8 @@ void launch_bf(const GemmPlan& p, const void* A, const void* B, void* C, hipStre
   const bool akc = (p.ta == 'N' || p.ta == 'n'), bkc = (p.tb == 'T' || p.tb == 't');
   dim3 g(p.grid), b(NT);
 #define LB(AK, BKc) hipLaunchKernelGGL((k_gemm_bf16x3<CPLX, AK, BKc>), g, b, 0, s, it, p.n_items, per, sg, \
-                                       (const float*)A, (const float*)B, (float*)C)
+                                       (const float*)A, (const float*)B, (float*)C, \
+                                       (float*)(p.d_ws ? p.d_ws->ptr : nullptr))
   if (akc && bkc) LB(true, true);
   else if (akc) LB(true, false);
   else if (bkc) LB(false, true);
@@ -423,9 +651,14 @@ void launch_bf(const GemmPlan& p, const void* A, const void* B, void* C, hipStre
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
-                                         int split_target) {
+                                         int split_target, bool planes) {
   ensure_init();
   auto plan = std::make_unique<GemmPlan>();
+  plan->planes = planes;
+  plan->tile = planes ? 128 : 64;
+  const int BM = plan->tile, BN = plan->tile;
+  DMRGX_CHECK(!planes || (prec == PREC_BF16X3 && (ta == 'N' || ta == 'n') && (tb == 'T' || tb == 't')),
+              "pre-split plans are BF16x3 with ('N','T') operands");
   plan->dtype = dtype;
   plan->prec = prec;
   plan->ta = ta;
@@ -460,19 +693,26 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     tiles_total += tl;
     work += (double)tl * g.ksum;
   }
-  int target = split_target > 0 ? split_target : 4 * std::max(rt().num_cu, 1);
+  int target = split_target > 0 ? split_target : 2 * std::max(rt().num_cu, 1);
   int64_t kchunk = (int64_t)std::max(8.0 * BKq, work / std::max<double>(target, 1.0));
   kchunk = (kchunk + BKq - 1) / BKq * BKq;
 
   std::vector<GemmItem> items;
   std::vector<GemmSeg> segs;
+  std::vector<RedTask> red;
   for (auto& g : groups) {
     int nch = (int)std::max<int64_t>(1, (g.ksum + kchunk - 1) / kchunk);
     int64_t ch = (g.ksum + nch - 1) / nch;
     ch = (ch + BKq - 1) / BKq * BKq;
     nch = (int)((g.ksum + ch - 1) / ch);
     const bool shared = nch > 1;
-    if (shared && g.beta == 0) plan->zero_ranges.push_back({g.c_off, (int64_t)g.M * g.N});
+    const int64_t len = (int64_t)g.M * g.N;
+    int64_t w_off = 0;
+    if (shared) {
+      w_off = plan->ws_elems;
+      plan->ws_elems += len * nch;
+      red.push_back(RedTask{g.c_off, w_off, len, nch, g.beta});
+    }
     // cut the concatenated K of the group's tasks into chunks of length ch
     std::vector<std::pair<int, int>> chunk_rng;  // seg range per chunk
     size_t ti = 0;
@@ -501,14 +741,14 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
       for (int n0 = 0; n0 < g.N; n0 += BN)
         for (int m0 = 0; m0 < g.M; m0 += BM) {
           GemmItem itm{};
-          itm.c_off = g.c_off;
+          itm.c_off = shared ? w_off + (int64_t)c * len : g.c_off;
           itm.M = g.M;
           itm.N = g.N;
           itm.m0 = m0;
           itm.n0 = n0;
           itm.seg_begin = chunk_rng[c].first;
           itm.seg_end = chunk_rng[c].second;
-          itm.flags = shared ? ITEM_ATOMIC : (g.beta ? ITEM_BETA : 0);
+          itm.flags = shared ? ITEM_PARTIAL : (g.beta ? ITEM_BETA : 0);
           items.push_back(itm);
         }
   }
@@ -527,32 +767,108 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
                                hipMemcpyHostToDevice, rt().stream));
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
+  if (!red.empty()) {
+    const size_t eb = (prec == PREC_FP64 ? 8 : 4) * (dtype == DT_C64 ? 2 : 1);
+    plan->d_ws = new DevBuf(plan->ws_elems * eb);
+    std::vector<int64_t> start(red.size());
+    int64_t acc = 0;
+    for (size_t i = 0; i < red.size(); ++i) {
+      start[i] = acc;
+      acc += (red[i].len + 255) / 256;
+    }
+    plan->n_red = (int)red.size();
+    plan->red_blocks = (int)acc;
+    plan->d_red = new DevBuf(red.size() * sizeof(RedTask));
+    plan->d_red_start = new DevBuf(red.size() * 8);
+    DMRGX_HIP(hipMemcpyAsync(plan->d_red->ptr, red.data(), red.size() * sizeof(RedTask), hipMemcpyHostToDevice,
+                             rt().stream));
+    DMRGX_HIP(hipMemcpyAsync(plan->d_red_start->ptr, start.data(), start.size() * 8, hipMemcpyHostToDevice,
+                             rt().stream));
+    DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  }
   return plan;
+}
+
+void reduce_partials(const GemmPlan& p, void* C, hipStream_t s) {
+  if (!p.n_red) return;
+  auto t = (const RedTask*)p.d_red->ptr;
+  auto st = (const int64_t*)p.d_red_start->ptr;
+  dim3 g(p.red_blocks), b(256);
+  const bool cx = p.dtype == DT_C64;
+  if (p.prec == PREC_FP64) {
+    if (cx) hipLaunchKernelGGL((k_reduce_ws<double, true>), g, b, 0, s, (const double*)p.d_ws->ptr, (double*)C, t, p.n_red, st);
+    else hipLaunchKernelGGL((k_reduce_ws<double, false>), g, b, 0, s, (const double*)p.d_ws->ptr, (double*)C, t, p.n_red, st);
+  } else {
+    if (cx) hipLaunchKernelGGL((k_reduce_ws<float, true>), g, b, 0, s, (const float*)p.d_ws->ptr, (float*)C, t, p.n_red, st);
+    else hipLaunchKernelGGL((k_reduce_ws<float, false>), g, b, 0, s, (const float*)p.d_ws->ptr, (float*)C, t, p.n_red, st);
+  }
+  DMRGX_HIP(hipGetLastError());
+}
+
+void reduce_partials(const GemmPlan& p, void* C, hipStream_t s);
+
+void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
+                     float* C, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
+  DMRGX_CHECK(p.planes, "run_gemm_planes needs a pre-split plan");
+  DMRGX_CHECK(a_stride % 8 == 0 && b_stride % 8 == 0, "plane strides must be multiples of 8");
+  const size_t eb = 4 * (p.dtype == DT_C64 ? 2 : 1);
+  for (auto& z : p.zero_ranges) fill_zero((char*)C + z.first * eb, (size_t)z.second * eb, s);
+  if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
+  float* wsp = (float*)(p.d_ws ? p.d_ws->ptr : nullptr);
+  if (p.n_items) {
+    const int per = p.grid / 8;
+    auto it = (const GemmItem*)p.d_items->ptr;
+    auto sg = (const GemmSeg*)p.d_segs->ptr;
+    if (p.dtype == DT_C64)
+      hipLaunchKernelGGL(k_gemm_bf3p<true>, dim3(p.grid), dim3(NT), 0, s, it, p.n_items, per, sg,
+                         (const unsigned short*)A, a_stride, (const unsigned short*)B, b_stride, C, wsp);
+    else
+      hipLaunchKernelGGL(k_gemm_bf3p<false>, dim3(p.grid), dim3(NT), 0, s, it, p.n_items, per, sg,
+                         (const unsigned short*)A, a_stride, (const unsigned short*)B, b_stride, C, wsp);
+    DMRGX_HIP(hipGetLastError());
+  }
+  if (ev_end) DMRGX_HIP(hipEventRecord(ev_end, s));
+  reduce_partials(p, C, s);
+}
+
+void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_split_planes<double>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, n,
+                     stride, cplx ? 1 : 0);
+  DMRGX_HIP(hipGetLastError());
+}
+
+void split_planes_f32(const float* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s) {
+  if (n <= 0) return;
+  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_split_planes<float>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, n,
+                     stride, cplx ? 1 : 0);
+  DMRGX_HIP(hipGetLastError());
 }
 
 void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, hipEvent_t ev_begin,
                    hipEvent_t ev_end) {
   const size_t eb = (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);
   for (auto& z : p.zero_ranges) fill_zero((char*)C + z.first * eb, (size_t)z.second * eb, s);
+  // events bracket the GEMM kernel only; the split-K reduction follows ev_end
   if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
-  struct EndRec {
-    hipEvent_t e;
-    hipStream_t s;
-    ~EndRec() { if (e) (void)hipEventRecord(e, s); }
-  } end_rec{ev_end, s};
-  if (!p.n_items) return;
-  const int per = p.grid / 8;
-  if (p.prec == PREC_FP64) {
-    if (p.dtype == DT_C64) launch4<double, true>(p, A, B, C, s, per);
-    else launch4<double, false>(p, A, B, C, s, per);
-  } else if (p.prec == PREC_FP32) {
-    if (p.dtype == DT_C64) launch4<float, true>(p, A, B, C, s, per);
-    else launch4<float, false>(p, A, B, C, s, per);
-  } else {
-    if (p.dtype == DT_C64) launch_bf<true>(p, A, B, C, s, per);
-    else launch_bf<false>(p, A, B, C, s, per);
+  if (p.n_items) {
+    const int per = p.grid / 8;
+    if (p.prec == PREC_FP64) {
+      if (p.dtype == DT_C64) launch4<double, true>(p, A, B, C, s, per);
+      else launch4<double, false>(p, A, B, C, s, per);
+    } else if (p.prec == PREC_FP32) {
+      if (p.dtype == DT_C64) launch4<float, true>(p, A, B, C, s, per);
+      else launch4<float, false>(p, A, B, C, s, per);
+    } else {
+      if (p.dtype == DT_C64) launch_bf<true>(p, A, B, C, s, per);
+      else launch_bf<false>(p, A, B, C, s, per);
+    }
+    DMRGX_HIP(hipGetLastError());
   }
-  DMRGX_HIP(hipGetLastError());
+  if (ev_end) DMRGX_HIP(hipEventRecord(ev_end, s));
+  reduce_partials(p, C, s);
 }
 
 void convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s) {

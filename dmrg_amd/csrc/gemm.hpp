@@ -34,16 +34,31 @@ struct GemmItem {
   int pad;
 };
 
-enum : int { ITEM_ATOMIC = 1, ITEM_BETA = 2 };
+// ITEM_PARTIAL: the item writes a K-chunk partial tile into the plan's
+// workspace (c_off is then a workspace offset); a reduction pass sums the
+// chunks into C.  ITEM_BETA: single-writer tile accumulating onto C.
+enum : int { ITEM_PARTIAL = 1, ITEM_BETA = 2 };
+
+struct RedTask {
+  int64_t c_off, w_off, len;  // len = M*N elements
+  int nch, beta;
+};
 
 struct GemmPlan {
   int dtype = DT_R64;
   int prec = PREC_FP64;
   char ta = 'N', tb = 'T';
+  int tile = 64;    // output tile edge (64, or 128 for the pre-split BF16x3 kernel)
+  bool planes = false;  // operands are pre-split bfloat16 planes
   int n_items = 0, n_segs = 0, grid = 0;
   DevBuf* d_items = nullptr;
   DevBuf* d_segs = nullptr;
   std::vector<std::pair<int64_t, int64_t>> zero_ranges;  // [off, len) in C elements
+  int64_t ws_elems = 0;                 // split-K workspace, C elements
+  DevBuf* d_ws = nullptr;
+  int n_red = 0, red_blocks = 0;
+  DevBuf* d_red = nullptr;              // RedTask array
+  DevBuf* d_red_start = nullptr;        // first block of each task
   double flops = 0;        // algorithmic real flops (c*M*N*K summed)
   ~GemmPlan();
 };
@@ -52,13 +67,23 @@ struct GemmPlan {
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
-                                         int split_target = 0);
+                                         int split_target = 0, bool planes = false);
 
 // Execute: A, B, C are device pointers with storage precision of the plan
 // (double for PREC_FP64, float for PREC_FP32 / PREC_BF16X3; complex
 // interleaved when dtype == DT_C64).
 void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s,
                    hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+
+// Pre-split BF16x3 execution ('N','T' only): A and B are bfloat16 planes
+// (re_hi, re_lo[, im_hi, im_lo]) of a_stride / b_stride elements each (both
+// multiples of 8); C is FP32 (complex interleaved when dtype == DT_C64).
+void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
+                     float* C, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+// Split n elements (complex if cplx) into hi/lo bfloat16 planes of stride `stride`.
+void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);
+void split_planes_f32(const float* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);
+inline int64_t plane_stride(int64_t n) { return (n + 7) / 8 * 8; }
 
 // Precision conversion helpers (n counts real scalars).
 void convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);

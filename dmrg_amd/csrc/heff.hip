@@ -87,13 +87,28 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     DMRGX_CHECK(g.c_off + (int64_t)g.M * g.N <= n_psi, "heff: lmap writes past the end of psi");
     DMRGX_CHECK(g.a_off + (int64_t)g.M * g.K <= lenv_len, "heff: lmap block past the end of lenv");
   }
-  h->rplan = make_gemm_plan(rt_, dtype, prec, 'N', 'T', uncovered(rt_, n_store));
-  h->lplan = make_gemm_plan(lt_, dtype, prec, 'N', 'T', uncovered(lt_, n_psi));
+  const bool planes = prec == PREC_BF16X3;
+  h->rplan = make_gemm_plan(rt_, dtype, prec, 'N', 'T', uncovered(rt_, n_store), 0, planes);
+  h->lplan = make_gemm_plan(lt_, dtype, prec, 'N', 'T', uncovered(lt_, n_psi), 0, planes);
   h->flops = h->rplan->flops + h->lplan->flops;
   const int c = dtype == DT_C64 ? 2 : 1;
   hipStream_t s = rt().stream;
   if (prec == PREC_FP64) {
     h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 8 * c);
+  } else if (planes) {
+    const int npl = 2 * c;
+    h->renv_ps = plane_stride(renv_len);
+    h->lenv_ps = plane_stride(lenv_len);
+    h->psi_ps = plane_stride(n_psi);
+    h->store_ps = plane_stride(n_store);
+    h->renv_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->renv_ps, 8) * npl * 2);
+    h->lenv_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->lenv_ps, 8) * npl * 2);
+    h->in_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->psi_ps, 8) * npl * 2);
+    h->store_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->store_ps, 8) * npl * 2);
+    split_planes_f64((const double*)renv, (uint16_t*)h->renv_pl->ptr, renv_len, h->renv_ps, c == 2, s);
+    split_planes_f64((const double*)lenv, (uint16_t*)h->lenv_pl->ptr, lenv_len, h->lenv_ps, c == 2, s);
+    h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 4 * c);
+    h->out_lo = std::make_unique<DevBuf>(std::max<int64_t>(n_psi, 1) * 4 * c);
   } else {
     h->renv_lo = to_f32(renv, renv_len * c, s);
     h->lenv_lo = to_f32(lenv, lenv_len * c, s);
@@ -112,6 +127,14 @@ void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* e
   if (prec == PREC_FP64) {
     run_gemm_plan(*rplan, renv, in, store->ptr, s, e0, e1);
     run_gemm_plan(*lplan, lenv, store->ptr, out, s, e2, e3);
+  } else if (prec == PREC_BF16X3) {
+    split_planes_f64(in, (uint16_t*)in_pl->ptr, n_psi, psi_ps, c == 2, s);
+    run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
+                    (float*)store->ptr, s, e0, e1);
+    split_planes_f32((const float*)store->ptr, (uint16_t*)store_pl->ptr, n_store, store_ps, c == 2, s);
+    run_gemm_planes(*lplan, (const uint16_t*)lenv_pl->ptr, lenv_ps, (const uint16_t*)store_pl->ptr, store_ps,
+                    (float*)out_lo->ptr, s, e2, e3);
+    convert_f32_to_f64((const float*)out_lo->ptr, out, n_psi * c, s);
   } else {
     convert_f64_to_f32(in, (float*)in_lo->ptr, n_psi * c, s);
     run_gemm_plan(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s, e0, e1);

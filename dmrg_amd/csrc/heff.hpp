@@ -17,6 +17,9 @@ struct HeffPlan {
   const void* lenv = nullptr;
   std::unique_ptr<DevBuf> renv_lo, lenv_lo;  // FP32 copies for reduced precision
   std::unique_ptr<DevBuf> store, in_lo, out_lo;
+  // BF16x3: bfloat16 planes of the environments, psi and store
+  std::unique_ptr<DevBuf> renv_pl, lenv_pl, in_pl, store_pl;
+  int64_t renv_ps = 0, lenv_ps = 0, psi_ps = 0, store_ps = 0;
   std::unique_ptr<GemmPlan> rplan, lplan;
   double flops = 0;
   // ev: optional {R begin, R end, L begin, L end} events around the two GEMM launches

@@ -18,12 +18,17 @@ namespace {
 
 constexpr int G = 8;  // Krylov columns handled per pass of the dot / combine kernels
 constexpr int DOT_BLOCKS_MAX = 1024;
+constexpr int PARTIAL_DOUBLES = (DOT_BLOCKS_MAX + 64) * 2 * G;
 
-// Partial <V_j, w> (conjugate-linear in V) for columns j0..j0+g-1.
+// Partial <X_c, w> (conjugate-linear in X) for columns c of group blockIdx.y:
+// X_c = V_c for c < k, and X_k = w itself when with_self (the squared norm).
 template <bool CPLX>
-__global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int64_t ld, int j0, int g,
+__global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int64_t ld, int k, int with_self,
                                                const double* __restrict__ w, int64_t n,
                                                double* __restrict__ partial) {
+  const int j0 = blockIdx.y * G;
+  const int ktot = k + with_self;
+  const int g = min(G, ktot - j0);
   double re[G], im[G];
 #pragma unroll
   for (int c = 0; c < G; ++c) re[c] = im[c] = 0;
@@ -35,7 +40,7 @@ __global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int
 #pragma unroll
       for (int c = 0; c < G; ++c)
         if (c < g) {
-          const double* v = V + (int64_t)(j0 + c) * ld;
+          const double* v = (j0 + c < k) ? V + (int64_t)(j0 + c) * ld : w;
           double vr = v[2 * i], vi = v[2 * i + 1];
           re[c] += vr * wr + vi * wi;
           im[c] += vr * wi - vi * wr;
@@ -44,7 +49,7 @@ __global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int
       double wr = w[i];
 #pragma unroll
       for (int c = 0; c < G; ++c)
-        if (c < g) re[c] += V[(int64_t)(j0 + c) * ld + i] * wr;
+        if (c < g) re[c] += ((j0 + c < k) ? V[(int64_t)(j0 + c) * ld + i] : w[i]) * wr;
     }
   }
   __shared__ double red[2 * G][256];
@@ -59,18 +64,20 @@ __global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int
       for (int q = 0; q < 2 * G; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 2 * G) partial[blockIdx.x * 2 * G + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 2 * G)
+    partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 2 * G + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_reduce_partials(const double* __restrict__ partial, int nblk, int j0, int g,
-                                  double* __restrict__ out) {
-  // one wave per value (2*g values)
+// one workgroup per column group: sum the per-block partials
+__global__ void k_reduce_partials(const double* __restrict__ partial, int nblk, int ktot, double* __restrict__ out) {
+  const int j0 = blockIdx.x * G;
   const int q = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const int g = min(G, ktot - j0);
   if (q >= 2 * g) return;
   double s = 0;
-  for (int b = lane; b < nblk; b += 64) s += partial[b * 2 * G + q];
+  for (int b = lane; b < nblk; b += 64) s += partial[((int64_t)blockIdx.x * nblk + b) * 2 * G + q];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-  if (lane == 0) out[2 * j0 + q] = s;  // out[2j] = re, out[2j+1] = im
+  if (lane == 0) out[2 * j0 + q] = s;  // out[2c] = re, out[2c+1] = im
 }
 
 // w -= sum_j V_j h_j
@@ -95,6 +102,32 @@ __global__ __launch_bounds__(256) void k_gemv_sub(const double* __restrict__ V, 
       double a = 0;
       for (int j = 0; j < k; ++j) a += V[(int64_t)j * ld + i] * h[2 * j];
       w[i] -= a;
+    }
+  }
+}
+
+// out = (w - sum_j V_j h_j) * a
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_gemv_sub_scale(const double* __restrict__ V, int64_t ld, int k,
+                                                        const double* __restrict__ h, const double* __restrict__ w,
+                                                        double* __restrict__ out, int64_t n, double a) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    if (CPLX) {
+      double ar = 0, ai = 0;
+      for (int j = 0; j < k; ++j) {
+        const double* v = V + (int64_t)j * ld;
+        double vr = v[2 * i], vi = v[2 * i + 1], hr = h[2 * j], hi = h[2 * j + 1];
+        ar += vr * hr - vi * hi;
+        ai += vr * hi + vi * hr;
+      }
+      out[2 * i] = (w[2 * i] - ar) * a;
+      out[2 * i + 1] = (w[2 * i + 1] - ai) * a;
+    } else {
+      double s = 0;
+      for (int j = 0; j < k; ++j) s += V[(int64_t)j * ld + i] * h[2 * j];
+      out[i] = (w[i] - s) * a;
     }
   }
 }
@@ -160,16 +193,18 @@ struct Work {
   ~Work() { delete partial; delete h; }
 };
 
-// h[0..k) = V[0..k)^H w   (device result in h, complex pairs)
+// h[0..k) = V[0..k)^H w, plus h[k] = <w, w> when with_self (device, complex pairs)
 void vdots(const double* V, int64_t ld, int k, const double* w, int64_t n, bool cplx, double* h,
-           double* partial, hipStream_t s) {
-  int nblk = (int)std::min<int64_t>(DOT_BLOCKS_MAX, std::max<int64_t>(1, (n + 1023) / 1024));
-  for (int j0 = 0; j0 < k; j0 += G) {
-    int g = std::min(G, k - j0);
-    if (cplx) hipLaunchKernelGGL(k_vdots<true>, dim3(nblk), dim3(256), 0, s, V, ld, j0, g, w, n, partial);
-    else hipLaunchKernelGGL(k_vdots<false>, dim3(nblk), dim3(256), 0, s, V, ld, j0, g, w, n, partial);
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(64 * 2 * G), 0, s, partial, nblk, j0, g, h);
-  }
+           double* partial, hipStream_t s, bool with_self = false) {
+  const int ktot = k + (with_self ? 1 : 0);
+  if (ktot <= 0) return;
+  const int ng = (ktot + G - 1) / G;
+  int nblk = (int)std::min<int64_t>(DOT_BLOCKS_MAX / ng + 1, std::max<int64_t>(1, (n + 1023) / 1024));
+  nblk = std::max(1, std::min(nblk, DOT_BLOCKS_MAX));
+  dim3 grid(nblk, ng);
+  if (cplx) hipLaunchKernelGGL(k_vdots<true>, grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
+  else hipLaunchKernelGGL(k_vdots<false>, grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
+  hipLaunchKernelGGL(k_reduce_partials, dim3(ng), dim3(64 * 2 * G), 0, s, partial, nblk, ktot, h);
 }
 
 void gemv_sub(const double* V, int64_t ld, int k, const double* h, double* w, int64_t n, bool cplx,
@@ -192,7 +227,7 @@ void dev_scale(double* v, int64_t n, bool cplx, double a, hipStream_t s) {
 }
 
 void dev_dot(const double* a, const double* b, int64_t n, bool cplx, double out[2], hipStream_t s) {
-  DevBuf partial(DOT_BLOCKS_MAX * 2 * G * sizeof(double));
+  DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
   DevBuf h(2 * G * sizeof(double));
   vdots(a, 0, 1, b, n, cplx, (double*)h.ptr, (double*)partial.ptr, s);
   DMRGX_HIP(hipMemcpyAsync(out, h.ptr, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -326,7 +361,7 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
   }
 
   DevBuf Vb((size_t)(ncv + 1) * nr * 8), Tmp((size_t)ncv * nr * 8);
-  DevBuf partial(DOT_BLOCKS_MAX * 2 * G * sizeof(double));
+  DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
   DevBuf hbuf(2 * (ncv + 2) * sizeof(double)), h2buf(2 * (ncv + 2) * sizeof(double));
   DevBuf Yb((size_t)ncv * ncv * sizeof(double));
   double* V = (double*)Vb.ptr;
@@ -355,29 +390,29 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
       double* vj = V + (int64_t)j * ld;
       op(vj, w);
       res.n_matvec++;
-      // classical Gram-Schmidt against v_0..v_j, twice
+      // classical Gram-Schmidt against v_0..v_j, twice; the second subtraction is
+      // fused with the normalisation and |w'|^2 - |h2|^2 gives beta^2
       vdots(V, ld, j + 1, w, n, cplx, h, (double*)partial.ptr, s);
       gemv_sub(V, ld, j + 1, h, w, n, cplx, s);
-      vdots(V, ld, j + 1, w, n, cplx, h2, (double*)partial.ptr, s);
-      gemv_sub(V, ld, j + 1, h2, w, n, cplx, s);
-      double nw[2];
-      {
-        vdots(w, 0, 1, w, n, cplx, h2 + 2 * (j + 1), (double*)partial.ptr, s);
-      }
+      vdots(V, ld, j + 1, w, n, cplx, h2, (double*)partial.ptr, s, true);
       DMRGX_HIP(hipMemcpyAsync(hh.data(), h, 2 * (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
       DMRGX_HIP(hipMemcpyAsync(hh2.data(), h2, 2 * (j + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
       DMRGX_HIP(hipStreamSynchronize(s));
-      nw[0] = hh2[2 * (j + 1)];
       double alpha = hh[2 * j] + hh2[2 * j];
-      double beta = std::sqrt(std::max(nw[0], 0.0));
+      double nw2 = hh2[2 * (j + 1)], hsq = 0;
+      for (int q = 0; q <= j; ++q) hsq += hh2[2 * q] * hh2[2 * q] + hh2[2 * q + 1] * hh2[2 * q + 1];
+      double beta = std::sqrt(std::max(nw2 - hsq, 0.0));
       T[(size_t)j * m + j] = alpha;
       if (j + 1 < m) {
         T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = beta;
       }
       beta_last = beta;
       double* vn = V + (int64_t)(j + 1) * ld;
-      if (beta > 1e-300) {
-        hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, w, vn, nr, 1.0 / beta);
+      if (beta > 1e-13 * std::max(std::fabs(alpha), std::sqrt(std::max(nw2, 0.0))) && beta > 1e-300) {
+        if (cplx) hipLaunchKernelGGL(k_gemv_sub_scale<true>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,
+                                     (const double*)h2, (const double*)w, vn, n, 1.0 / beta);
+        else hipLaunchKernelGGL(k_gemv_sub_scale<false>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,
+                                (const double*)h2, (const double*)w, vn, n, 1.0 / beta);
       } else {
         // invariant subspace: continue with a fresh random direction
         dev_random(vn, n, cplx, seed + 7919ull * (j + 1) + restart, s);
@@ -445,7 +480,7 @@ double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, i
   const int64_t nr = n * (cplx ? 2 : 1);
   k = (int)std::max<int64_t>(1, std::min<int64_t>(k, n));
   DevBuf Vb((size_t)(k + 1) * nr * 8), wb(nr * 8);
-  DevBuf partial(DOT_BLOCKS_MAX * 2 * G * sizeof(double));
+  DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
   DevBuf hbuf(2 * (k + 2) * 8), h2buf(2 * (k + 2) * 8), Yb((size_t)k * 8);
   double* V = (double*)Vb.ptr;
   double* w = (double*)wb.ptr;

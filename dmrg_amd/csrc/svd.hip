@@ -22,7 +22,9 @@ namespace dmrgx {
 namespace {
 
 constexpr double TOLZERO = 1e-10;  // global.h:19, all-zero block test (src/tensor.cpp:764)
-constexpr double JTOL = 1e-15;      // off-diagonal convergence threshold
+// off-diagonal convergence threshold: |w_p^H w_q| <= tol |w_p| |w_q| with
+// tol = max(rows, cols) * eps (the one-sided Jacobi test of Demmel and Veselic)
+constexpr double EPS = 2.220446049250313e-16;
 constexpr int MAX_SWEEPS = 40;
 
 struct JBlk {
@@ -32,6 +34,7 @@ struct JBlk {
   int rows, cols;   // original A shape
   int special;      // 0 normal, 1 all-zero, 2 one-by-one
   int pad;
+  double tol;
   int64_t a_off, w_off, q_off;
   int64_t pair_base;  // first global pair slot
   int64_t elem_base;  // first flat element for init / finalize kernels (n*m + n*n)
@@ -128,7 +131,7 @@ __global__ __launch_bounds__(256) void k_jround(double* __restrict__ W, double* 
   if (al <= 0 || be <= 0 || ag == 0) return;
   const double ratio = ag / sqrt(al * be);
   if (lane == 0) atomicMax(&off_flag[b], (unsigned long long)__double_as_longlong(ratio));
-  if (ratio <= JTOL) return;
+  if (ratio <= k.tol) return;
   // rotation: w_i' = c w_i - s e^{-i phi} w_j,  w_j' = s e^{i phi} w_i + c w_j
   const double zeta = (be - al) / (2 * ag);
   const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
@@ -277,95 +280,139 @@ __global__ __launch_bounds__(256) void k_jfinal(const double* __restrict__ W, co
   else D[out] = vr;
 }
 
-// Orthonormal completion: vectors k in [k0, k1) of X (length L each; element
-// (k, i) at off + k*ks + i*is) are re-orthogonalised against 0..k-1 (modified
-// Gram-Schmidt, twice); failures are replaced by unit vectors.  One WG per task.
+// Re-orthonormalisation and completion of vectors [good, k1) of X (length L
+// each; element (k, i) at off + k*ks + i*is): Householder QR of the first k0
+// vectors (X = Q R); vector j in [good, k0) becomes Q e_j times the phase of
+// R_jj (the Gram-Schmidt orthonormalisation of the original direction, same
+// phase), and vector j in [k0, k1) becomes the complement vector Q e_j.  The
+// reflectors live in a workspace of L*k0 elements at w_off.  One workgroup of
+// 1024 threads per task; target vectors are built one per wavefront.
 struct CompTask {
-  int64_t off, ks, is;
-  int L, k0, k1, pad;
+  int64_t off, ks, is, w_off;
+  int L, k0, k1, good;  // QR of vectors [0,k0); rewrite vectors [good,k1)
 };
 
+constexpr int CNT = 1024;
+
 template <bool CPLX>
-__global__ __launch_bounds__(256) void k_complete(double* __restrict__ X, const CompTask* __restrict__ tasks) {
+__global__ __launch_bounds__(CNT) void k_complete(double* __restrict__ X, double* __restrict__ Wk,
+                                                   const CompTask* __restrict__ tasks) {
   const CompTask t = tasks[blockIdx.x];
-  __shared__ double red[2][256];
-  __shared__ double coef[2];
-  int next_unit = 0;
-  const int tid = threadIdx.x;
-  auto el = [&](int k, int i) -> double* { return X + (CPLX ? 2 : 1) * (t.off + (int64_t)k * t.ks + (int64_t)i * t.is); };
-  auto dot = [&](int a, int b, double& rr, double& ri) {  // <x_a, x_b>
-    double sr = 0, si = 0;
-    for (int i = tid; i < t.L; i += 256) {
-      double* pa = el(a, i);
-      double* pb = el(b, i);
-      if (CPLX) { sr += pa[0] * pb[0] + pa[1] * pb[1]; si += pa[0] * pb[1] - pa[1] * pb[0]; }
-      else sr += pa[0] * pb[0];
-    }
-    red[0][tid] = sr;
-    red[1][tid] = si;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = CNT / 64;
+  constexpr int C = CPLX ? 2 : 1;
+  __shared__ double red[2][CNT / 64];
+  __shared__ double vn2_s;
+  double* W = Wk + C * t.w_off;  // column-major L x k0 (vector j at W + C*j*L)
+  auto xel = [&](int k, int i) -> double* { return X + C * (t.off + (int64_t)k * t.ks + (int64_t)i * t.is); };
+  // copy the kept vectors
+  for (int64_t e = tid; e < (int64_t)t.L * t.k0; e += CNT) {
+    int j = (int)(e / t.L), i = (int)(e % t.L);
+    const double* x = xel(j, i);
+    W[C * e] = x[0];
+    if (CPLX) W[C * e + 1] = x[1];
+  }
+  __syncthreads();
+  // Householder QR; reflector i stored in W[i:, i] with its squared norm in vn2[i] (LDS-free: recomputed)
+  extern __shared__ double vn2_all[];
+  for (int i = 0; i < t.k0; ++i) {
+    double* col = W + C * (int64_t)i * t.L;
+    // |x|^2 over rows i..L-1
+    double a = 0;
+    for (int r = i + tid; r < t.L; r += CNT) a += CPLX ? col[2 * r] * col[2 * r] + col[2 * r + 1] * col[2 * r + 1] : col[r] * col[r];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (lane == 0) red[0][wave] = a;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) { red[0][tid] += red[0][tid + s]; red[1][tid] += red[1][tid + s]; }
-      __syncthreads();
+    if (tid == 0) {
+      double nrm2 = 0;
+      for (int q = 0; q < nw; ++q) nrm2 += red[0][q];
+      double nrm = sqrt(nrm2);
+      double x0r = col[C * i], x0i = CPLX ? col[C * i + 1] : 0;
+      double ax0 = CPLX ? hypot(x0r, x0i) : fabs(x0r);
+      double pr = ax0 > 0 ? x0r / ax0 : 1, pi = ax0 > 0 ? x0i / ax0 : 0;  // e^{i arg x0}
+      // beta = -e^{i arg x0} |x|;  v = x - beta e_1  =>  v_0 = x0 + e^{i arg x0}|x|
+      col[C * i] = x0r + pr * nrm;
+      if (CPLX) col[C * i + 1] = x0i + pi * nrm;
+      // |v|^2 = 2|x|(|x| + |x0|); R_ii = beta = -e^{i arg x0}|x|
+      vn2_all[i] = 2 * nrm * (nrm + ax0);
+      vn2_all[t.k0 + 2 * i] = -pr;
+      vn2_all[t.k0 + 2 * i + 1] = -pi;
     }
-    rr = red[0][0];
-    ri = red[1][0];
     __syncthreads();
-  };
-  for (int k = t.k0; k < t.k1; ++k) {
-    double n0r, n0i;
-    dot(k, k, n0r, n0i);
-    bool use_unit = !(n0r > 0.25);
-    for (int attempt = 0; attempt < t.L + 1; ++attempt) {
-      if (use_unit) {
-        if (next_unit >= t.L) break;
-        for (int i = tid; i < t.L; i += 256) {
-          double* p = el(k, i);
-          p[0] = i == next_unit ? 1.0 : 0.0;
-          if (CPLX) p[1] = 0.0;
+    const double vn2 = vn2_all[i];
+    if (vn2 <= 0) continue;
+    // apply H_i to the remaining kept vectors, one wavefront per column
+    for (int c = i + 1 + wave; c < t.k0; c += nw) {
+      double* y = W + C * (int64_t)c * t.L;
+      double dr = 0, di = 0;  // v^H y
+      for (int r = i + lane; r < t.L; r += 64) {
+        if (CPLX) {
+          dr += col[2 * r] * y[2 * r] + col[2 * r + 1] * y[2 * r + 1];
+          di += col[2 * r] * y[2 * r + 1] - col[2 * r + 1] * y[2 * r];
+        } else {
+          dr += col[r] * y[r];
         }
-        __syncthreads();
-        ++next_unit;
       }
-      double before;
-      {
-        double a, b2;
-        dot(k, k, a, b2);
-        before = a;
-      }
-      for (int pass = 0; pass < 2; ++pass)
-        for (int j = 0; j < k; ++j) {
-          double cr, ci;
-          dot(j, k, cr, ci);  // <x_j, x_k>
-          for (int i = tid; i < t.L; i += 256) {
-            double* pk = el(k, i);
-            double* pj = el(j, i);
-            if (CPLX) {
-              pk[0] -= cr * pj[0] - ci * pj[1];
-              pk[1] -= cr * pj[1] + ci * pj[0];
-            } else {
-              pk[0] -= cr * pj[0];
-            }
-          }
-          __syncthreads();
+      for (int o = 32; o > 0; o >>= 1) { dr += __shfl_xor(dr, o, 64); if (CPLX) di += __shfl_xor(di, o, 64); }
+      const double fr = 2 * dr / vn2, fi = 2 * di / vn2;
+      for (int r = i + lane; r < t.L; r += 64) {
+        if (CPLX) {
+          y[2 * r] -= col[2 * r] * fr - col[2 * r + 1] * fi;
+          y[2 * r + 1] -= col[2 * r] * fi + col[2 * r + 1] * fr;
+        } else {
+          y[r] -= col[r] * fr;
         }
-      double a, b2;
-      dot(k, k, a, b2);
-      // accept unless the candidate lies (numerically) inside the span so far
-      if (a > 1e-6 * before && a > 1e-24) {
-        double inv = 1.0 / sqrt(a);
-        for (int i = tid; i < t.L; i += 256) {
-          double* p = el(k, i);
-          p[0] *= inv;
-          if (CPLX) p[1] *= inv;
-        }
-        __syncthreads();
-        break;
       }
-      use_unit = true;
+    }
+    __syncthreads();
+  }
+  // Q e_j = H_0 H_1 ... H_{k0-1} e_j for j = good .. k1-1
+  for (int j = t.good + wave; j < t.k1; j += nw) {
+    for (int r = lane; r < t.L; r += 64) {
+      double* x = xel(j, r);
+      x[0] = r == j ? 1.0 : 0.0;
+      if (CPLX) x[1] = 0.0;
+    }
+    for (int i = t.k0 - 1; i >= 0; --i) {
+      const double vn2 = vn2_all[i];
+      if (vn2 <= 0) continue;
+      const double* col = W + C * (int64_t)i * t.L;
+      double dr = 0, di = 0;
+      for (int r = i + lane; r < t.L; r += 64) {
+        const double* x = xel(j, r);
+        if (CPLX) {
+          dr += col[2 * r] * x[0] + col[2 * r + 1] * x[1];
+          di += col[2 * r] * x[1] - col[2 * r + 1] * x[0];
+        } else {
+          dr += col[r] * x[0];
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) { dr += __shfl_xor(dr, o, 64); if (CPLX) di += __shfl_xor(di, o, 64); }
+      const double fr = 2 * dr / vn2, fi = 2 * di / vn2;
+      for (int r = i + lane; r < t.L; r += 64) {
+        double* x = xel(j, r);
+        if (CPLX) {
+          x[0] -= col[2 * r] * fr - col[2 * r + 1] * fi;
+          x[1] -= col[2 * r] * fi + col[2 * r + 1] * fr;
+        } else {
+          x[0] -= col[r] * fr;
+        }
+      }
+    }
+    if (j < t.k0) {  // keep the phase of the original direction: multiply by R_jj/|R_jj|
+      const double qr = vn2_all[t.k0 + 2 * j], qi = vn2_all[t.k0 + 2 * j + 1];
+      for (int r = lane; r < t.L; r += 64) {
+        double* x = xel(j, r);
+        if (CPLX) {
+          double a = x[0], b = x[1];
+          x[0] = a * qr - b * qi;
+          x[1] = a * qi + b * qr;
+        } else {
+          x[0] *= qr;
+        }
+      }
     }
   }
-  (void)coef;
+  (void)vn2_s;
 }
 
 template <bool CPLX>
@@ -425,6 +472,7 @@ int block_svd(int dtype, const void* A, void* U, double* S, void* V, const std::
     j.n = j.transposed ? k.rows : k.cols;
     j.npl = j.n + (j.n & 1);
     j.special = (k.rows * k.cols == 0) ? 1 : (amax[b] <= TOLZERO ? 1 : (k.rows * k.cols == 1 ? 2 : 0));
+    j.tol = std::max(k.rows, k.cols) * EPS;
     j.a_off = k.a_off;
     j.w_off = woff;
     j.q_off = qoff;
@@ -464,13 +512,13 @@ int block_svd(int dtype, const void* A, void* U, double* S, void* V, const std::
       DMRGX_HIP(hipGetLastError());
       DMRGX_HIP(hipMemcpyAsync(flags.data(), dflag.ptr, nb * 8, hipMemcpyDeviceToHost, s));
       DMRGX_HIP(hipStreamSynchronize(s));
-      double worst = 0;
-      for (auto f : flags) {
+      bool done = true;
+      for (int b = 0; b < nb; ++b) {
         double d;
-        memcpy(&d, &f, 8);
-        worst = std::max(worst, d);
+        memcpy(&d, &flags[b], 8);
+        if (!jb[b].special && d > jb[b].tol) done = false;
       }
-      if (worst <= JTOL) break;
+      if (done) break;
     }
   }
   // singular values (column norms) and per-block descending order
@@ -542,42 +590,50 @@ int block_svd(int dtype, const void* A, void* U, double* S, void* V, const std::
                             (const double*)dsig.ptr, (double*)U, (double*)V);
     DMRGX_HIP(hipGetLastError());
   }
-  // completion / cleanup of vectors beyond the numerical rank
-  std::vector<CompTask> ct;
+  // completion of vectors beyond the numerical rank (singular values below
+  // 1e-8 of the largest, or zero): Householder complement of the kept ones
+  std::vector<CompTask> tu, tv;
+  int64_t wu = 0, wv = 0;
+  int maxk = 0;
   for (int b = 0; b < nb; ++b) {
     const JBlk& j = jb[b];
     if (j.special) continue;
     int64_t c0 = col_base[b];
     double smax = j.n ? sig[c0 + perm[c0]] : 0.0;
     int good = 0;
-    while (good < j.n && sig[c0 + perm[c0 + good]] > 1e-6 * smax && sig[c0 + perm[c0 + good]] > 0) ++good;
-    // vectors built from W: U columns (not transposed) or V rows (transposed)
+    while (good < j.n && sig[c0 + perm[c0 + good]] > 1e-8 * smax && sig[c0 + perm[c0 + good]] > 0) ++good;
+    int npos = 0;
+    while (npos < j.n && sig[c0 + perm[c0 + npos]] > 0) ++npos;
     int nu = need_u.empty() ? j.rows : std::min(need_u[b], j.rows);
     int nv = need_v.empty() ? j.cols : std::min(need_v[b], j.cols);
-    if (!j.transposed) {
-      if (nu > good) ct.push_back(CompTask{blocks[b].u_off, 1, j.rows, j.rows, good, nu, 0});
-    } else {
-      if (nv > good) ct.push_back(CompTask{blocks[b].v_off, j.cols, 1, j.cols, good, nv, 0});
+    const int need = j.transposed ? nv : nu;
+    if (need > good) {
+      const int k0 = std::min(npos, need);
+      if (!j.transposed) {
+        tu.push_back(CompTask{blocks[b].u_off, 1, j.rows, wu, j.rows, k0, need, good});
+        wu += (int64_t)j.rows * std::max(k0, 1);
+      } else {
+        tv.push_back(CompTask{blocks[b].v_off, j.cols, 1, wv, j.cols, k0, need, good});
+        wv += (int64_t)j.cols * std::max(k0, 1);
+      }
+      maxk = std::max(maxk, k0);
     }
   }
-  if (!ct.empty()) {
-    // U tasks and V tasks live in different arrays: launch per array
-    std::vector<CompTask> tu, tv;
-    for (auto& c : ct) (c.ks == 1 ? tu : tv).push_back(c);
-    auto launch = [&](std::vector<CompTask>& v, void* X) {
-      if (v.empty()) return;
-      DevBuf d(v.size() * sizeof(CompTask));
-      DMRGX_HIP(hipMemcpyAsync(d.ptr, v.data(), v.size() * sizeof(CompTask), hipMemcpyHostToDevice, s));
-      if (cplx) hipLaunchKernelGGL(k_complete<true>, dim3((unsigned)v.size()), dim3(256), 0, s, (double*)X,
-                                   (const CompTask*)d.ptr);
-      else hipLaunchKernelGGL(k_complete<false>, dim3((unsigned)v.size()), dim3(256), 0, s, (double*)X,
-                              (const CompTask*)d.ptr);
-      DMRGX_HIP(hipGetLastError());
-      DMRGX_HIP(hipStreamSynchronize(s));
-    };
-    launch(tu, U);
-    launch(tv, V);
-  }
+  auto launch = [&](std::vector<CompTask>& v, int64_t wlen, void* X) {
+    if (v.empty()) return;
+    DevBuf d(v.size() * sizeof(CompTask)), wk(std::max<int64_t>(wlen, 1) * 8 * cw);
+    DMRGX_HIP(hipMemcpyAsync(d.ptr, v.data(), v.size() * sizeof(CompTask), hipMemcpyHostToDevice, s));
+    size_t shm = 3 * (size_t)std::max(maxk, 1) * sizeof(double);
+    DMRGX_CHECK(shm <= 64 * 1024, "block_svd: too many kept vectors for the completion kernel");
+    if (cplx) hipLaunchKernelGGL(k_complete<true>, dim3((unsigned)v.size()), dim3(CNT), shm, s, (double*)X,
+                                 (double*)wk.ptr, (const CompTask*)d.ptr);
+    else hipLaunchKernelGGL(k_complete<false>, dim3((unsigned)v.size()), dim3(CNT), shm, s, (double*)X,
+                            (double*)wk.ptr, (const CompTask*)d.ptr);
+    DMRGX_HIP(hipGetLastError());
+    DMRGX_HIP(hipStreamSynchronize(s));
+  };
+  launch(tu, wu, U);
+  launch(tv, wv, V);
   DMRGX_HIP(hipStreamSynchronize(s));
   return sweeps;
 }
