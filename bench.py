@@ -58,6 +58,48 @@ def gpu_leg(steps, warmup, only=None):
     return out
 
 
+def svd_leg(reps=5):
+    """dmrgx_block_svd on the two sector blocks of the L=100 m=512 two-site
+    state (512 x 512 complex each) with a DMRG-like spectrum exp(-30 k / 512)."""
+    import ctypes as C
+    import dmrg_amd
+
+    dmrg_amd.init(0)
+    rng = np.random.default_rng(7)
+    n = M * 2 // S  # (m d / S) rows and columns per sector block (d = 2)
+    blocks = []
+    for _ in range(S):
+        q1 = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+        q2 = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))[0]
+        blocks.append((q1 * np.exp(-30.0 * np.arange(n) / n)) @ q2.conj().T)
+    nb = len(blocks)
+    rows = np.full(nb, n, np.int32)
+    cols = np.full(nb, n, np.int32)
+    off = np.arange(nb, dtype=np.int64) * n * n
+    soff = np.arange(nb, dtype=np.int64) * n
+    A = dmrg_amd.DeviceArray.from_numpy(np.concatenate([b.ravel() for b in blocks]))
+    U = dmrg_amd.DeviceArray(nb * n * n, np.complex128)
+    V = dmrg_amd.DeviceArray(nb * n * n, np.complex128)
+    Sv = dmrg_amd.DeviceArray(nb * n, np.float64)
+    P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+    lib = dmrg_amd.lib()
+
+    def run():
+        dmrg_amd.check(lib.dmrgx_block_svd(dmrg_amd.C64, nb, P(rows, C.c_int32), P(cols, C.c_int32), A.ptr,
+                                           P(off, C.c_int64), U.ptr, P(off, C.c_int64), Sv.ptr,
+                                           P(soff, C.c_int64), V.ptr, P(off, C.c_int64)), "block_svd")
+
+    run()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        run()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    s = Sv.numpy()[:n]
+    ref = np.exp(-30.0 * np.arange(n) / n)
+    return {"svd_ms": round(ms, 3), "sweeps": int(lib.dmrgx_block_svd_sweeps()), "blocks": f"{nb} x {n}x{n} complex",
+            "max_sv_err": float(np.max(np.abs(s - ref)))}
+
+
 def cpu_leg(seconds):
     """The oracle's av() (numpy, one BLAS GEMM per get_map task) on the same
     shapes, timed on this host's cores for about `seconds`."""
@@ -110,11 +152,15 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", type=int, default=0, help="run this many full sweeps of the L=100 m=512 chain")
     ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
+    ap.add_argument("--part", default="", choices=["", "svd"], help="time one part of a sweep step alone")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
               file=sys.stderr)
         sys.exit(2)
+    if a.part == "svd":
+        print(json.dumps(svd_leg(a.steps)), flush=True)
+        return
     if a.only:
         gpu_leg(a.steps, a.warmup, a.only.split(","))
         return

@@ -26,7 +26,7 @@ EXPORTS = [
     "dmrgx_init", "dmrgx_finalize", "dmrgx_last_error", "dmrgx_device_info", "dmrgx_malloc",
     "dmrgx_free", "dmrgx_memcpy_h2d", "dmrgx_memcpy_d2h", "dmrgx_memset_zero", "dmrgx_synchronize",
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
-    "dmrgx_heff_destroy", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_gemm", "dmrgx_leg_contract",
+    "dmrgx_heff_destroy", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time",
 ]
 
@@ -77,6 +77,7 @@ def _declare(lib):
         "dmrgx_eigsh": ([vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, vp, dp, vp, vp, C.c_int,
                          C.POINTER(C.c_int)], C.c_int),
         "dmrgx_block_svd": ([C.c_int, C.c_int, i32p, i32p, vp, i64p, vp, i64p, vp, i64p, vp, i64p], C.c_int),
+        "dmrgx_block_svd_sweeps": ([], C.c_int),
         "dmrgx_block_gemm": ([C.c_int, C.c_char, C.c_char, vp, vp, vp, i64p, C.c_int], C.c_int),
         "dmrgx_leg_contract": ([C.c_int, vp, vp, vp, i64p, C.c_int], C.c_int),
         "dmrgx_permute": ([C.c_int, vp, vp, i64p, C.c_int], C.c_int),

@@ -10,6 +10,7 @@
 #include <cstdio>
 
 #include "lanczos.hpp"
+#include "svd.hpp"
 
 namespace dmrgx {
 
@@ -128,6 +129,7 @@ void Engine<T>::finish(StepLog& lg, const vector<double>& s, int cutoff) {
   if (cfg.verbose)
     std::printf("l=%d r=%d d=%lld E0=%.12f trunc=%.3e matvec=%d t=%.3fs\n", lg.l, lg.r, (long long)lg.d, lg.e[0],
                 lg.trunc, lg.n_matvec, lg.t_total);
+  if (cfg.verbose) std::fflush(stdout);
   if (on_step) on_step(lg);
 }
 
@@ -162,6 +164,8 @@ double Engine<T>::two_sites(int l, int r, int cutoff) {
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);
   lg.t_svd = secs(t2, t3);
+  lg.svd_sweeps = block_svd_last_sweeps();
+  lg.t_block_svd = block_svd_last_seconds();
   lg.t_total = secs(t0, t4);
   finish(lg, s, cutoff);
   return lg.e[0];
@@ -198,6 +202,8 @@ double Engine<T>::update_two(int l, int r, int cutoff) {
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);
   lg.t_svd = secs(t2, t3);
+  lg.svd_sweeps = block_svd_last_sweeps();
+  lg.t_block_svd = block_svd_last_seconds();
   lg.t_total = secs(t0, t4);
   finish(lg, s, cutoff);
   return lg.e[0];
@@ -230,6 +236,8 @@ double Engine<T>::move2right(int l, int r, int cutoff) {
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);
   lg.t_svd = secs(t2, t3);
+  lg.svd_sweeps = block_svd_last_sweeps();
+  lg.t_block_svd = block_svd_last_seconds();
   lg.t_total = secs(t0, t4);
   finish(lg, s, cutoff);
   return lg.e[0];
@@ -262,6 +270,8 @@ double Engine<T>::move2left(int l, int r, int cutoff) {
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);
   lg.t_svd = secs(t2, t3);
+  lg.svd_sweeps = block_svd_last_sweeps();
+  lg.t_block_svd = block_svd_last_seconds();
   lg.t_total = secs(t0, t4);
   finish(lg, s, cutoff);
   return lg.e[0];

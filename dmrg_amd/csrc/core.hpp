@@ -34,6 +34,8 @@ struct StepLog {
   int n_matvec = 0, n_polish = 0;
   double flops = 0;  // algorithmic matvec flops of this step (all matvecs)
   double t_total = 0, t_eig = 0, t_svd = 0, t_env = 0;
+  int svd_sweeps = 0;     // Jacobi sweeps of the block SVD
+  double t_block_svd = 0;  // the block SVD alone (inside t_svd)
   std::vector<double> s;  // leading singular values
 };
 

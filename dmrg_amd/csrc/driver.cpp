@@ -82,6 +82,8 @@ void run_engine(Engine<T>& eng, const dmrgx_run_config* c, dmrgx_run_stats* st, 
     row[14] = g.t_svd;
     row[15] = g.t_env;
     row[16] = (double)g.d;
+    row[17] = g.svd_sweeps;
+    row[18] = g.t_block_svd;
     ++k;
   }
   if (n) *n = (int)eng.log.size();

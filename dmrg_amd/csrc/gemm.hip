@@ -441,60 +441,72 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
       for (int j = 0; j < 4; ++j) acc[p][i][j] = f4v{0, 0, 0, 0};
 
   uint4 ra[CPT], rb[CPT];
+  // per-chunk element offsets, recomputed once per segment
+  int64_t offA[CPT], offB[CPT];
+  unsigned okA = 0, okB = 0;  // bit j: chunk j's row is inside the block
   Cursor cur;
   cur.start(segs, it.seg_begin, it.seg_end);
   bool avec = false, bvec = false;
-  auto seg_flags = [&](const GemmSeg& s) {
+  auto seg_setup = [&](const GemmSeg& s) {
     avec = ((s.a_off | s.K | s.k0) & 7) == 0;
     bvec = ((s.b_off | s.K | s.k0) & 7) == 0;
-  };
-  if (cur.valid()) seg_flags(cur.s);
-  auto gload = [&](const Cursor& c) {
-    const GemmSeg& s = c.s;
-    const bool neg = s.alpha < 0;
+    okA = okB = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
       const int ch = t + NT * j;
       const int plane = ch / (PT * CPR);
       const int rem = ch % (PT * CPR);
       const int row = rem / CPR, kc = (rem % CPR) * 8;
-      const int gk = c.k + kc;
-      // A
-      {
-        const int gr = it.m0 + row;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (gr < it.M) {
-          const unsigned short* src = A + plane * as + s.a_off + (int64_t)gr * s.K + gk;
-          if (avec && gk + 8 <= s.k1) {
-            v = *reinterpret_cast<const uint4*>(src);
-          } else {
-            unsigned short h[8];
+      const int gr = it.m0 + row, gn = it.n0 + row;
+      offA[j] = plane * as + s.a_off + (int64_t)gr * s.K + kc;
+      offB[j] = plane * bs + s.b_off + (int64_t)gn * s.K + kc;
+      if (gr < it.M) okA |= 1u << j;
+      if (gn < it.N) okB |= 1u << j;
+    }
+  };
+  if (cur.valid()) seg_setup(cur.s);
+  auto load8 = [&](const unsigned short* src, int gk, int k1) {
+    unsigned short h[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) h[e] = (gk + e < s.k1) ? src[e] : (unsigned short)0;
-            v.x = h[0] | ((unsigned)h[1] << 16); v.y = h[2] | ((unsigned)h[3] << 16);
-            v.z = h[4] | ((unsigned)h[5] << 16); v.w = h[6] | ((unsigned)h[7] << 16);
-          }
+    for (int e = 0; e < 8; ++e) h[e] = (gk + e < k1) ? src[e] : (unsigned short)0;
+    uint4 v;
+    v.x = h[0] | ((unsigned)h[1] << 16); v.y = h[2] | ((unsigned)h[3] << 16);
+    v.z = h[4] | ((unsigned)h[5] << 16); v.w = h[6] | ((unsigned)h[7] << 16);
+    return v;
+  };
+  auto gload = [&](const Cursor& c) {
+    const GemmSeg& s = c.s;
+    const bool neg = s.alpha < 0;
+    const bool full = c.k + PBK <= s.k1;
+    if (full && avec && bvec) {
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (okA & (1u << j)) v = *reinterpret_cast<const uint4*>(A + offA[j] + c.k);
+        ra[j] = neg ? xor_sign(v) : v;
+        uint4 w = make_uint4(0, 0, 0, 0);
+        if (okB & (1u << j)) w = *reinterpret_cast<const uint4*>(B + offB[j] + c.k);
+        rb[j] = w;
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < CPT; ++j) {
+        const int ch = t + NT * j;
+        const int kc = ((ch % (PT * CPR)) % CPR) * 8;
+        const int gk = c.k + kc;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (okA & (1u << j)) {
+          const unsigned short* src = A + offA[j] + c.k;
+          v = (avec && gk + 8 <= s.k1) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
           if (neg) v = xor_sign(v);
         }
         ra[j] = v;
-      }
-      // B
-      {
-        const int gn = it.n0 + row;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (gn < it.N) {
-          const unsigned short* src = B + plane * bs + s.b_off + (int64_t)gn * s.K + gk;
-          if (bvec && gk + 8 <= s.k1) {
-            v = *reinterpret_cast<const uint4*>(src);
-          } else {
-            unsigned short h[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) h[e] = (gk + e < s.k1) ? src[e] : (unsigned short)0;
-            v.x = h[0] | ((unsigned)h[1] << 16); v.y = h[2] | ((unsigned)h[3] << 16);
-            v.z = h[4] | ((unsigned)h[5] << 16); v.w = h[6] | ((unsigned)h[7] << 16);
-          }
+        uint4 w = make_uint4(0, 0, 0, 0);
+        if (okB & (1u << j)) {
+          const unsigned short* src = B + offB[j] + c.k;
+          w = (bvec && gk + 8 <= s.k1) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
         }
-        rb[j] = v;
+        rb[j] = w;
       }
     }
   };
@@ -514,7 +526,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
     const int prev_seg = cur.seg;
     cur.next<PBK>(segs);
     if (cur.valid()) {
-      if (cur.seg != prev_seg) seg_flags(cur.s);
+      if (cur.seg != prev_seg) seg_setup(cur.s);
       gload(cur);
     }
     const int kx = (lane >> 4) * 8;

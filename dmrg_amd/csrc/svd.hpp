@@ -19,6 +19,9 @@ struct SvdBlock {
 // many leading columns of U / rows of V must be orthonormal to machine
 // precision (columns beyond the numerical rank are completed); pass rows /
 // cols for full matrices.  Reports sweeps used.
+// Jacobi sweeps and wall seconds of the last block_svd call on this thread.
+int block_svd_last_sweeps();
+double block_svd_last_seconds();
 int block_svd(int dtype, const void* A, void* U, double* S, void* V, const std::vector<SvdBlock>& blocks,
               const std::vector<int>& need_u, const std::vector<int>& need_v, hipStream_t s);
 

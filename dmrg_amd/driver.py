@@ -8,7 +8,7 @@ from dataclasses import dataclass
 import numpy as np
 
 MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
-STEP_LOG_WIDTH = 17
+STEP_LOG_WIDTH = 19
 DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym"]
 
 

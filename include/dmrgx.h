@@ -101,10 +101,15 @@ int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
  * S_b descending (min(rows,cols), at s_off[b] in S), V_b full (cols x cols,
  * at v_off[b]).  All-zero blocks (every |x| <= 1e-10) give identity U, V and
  * S = 0; 1x1 blocks give U = 1, S = |a|, V = a/|a| (src/tensor.cpp:764-782).
- * FP64 batched one-sided Jacobi; A is not modified.  All pointers device. */
+ * FP64: Householder QR of each block, then batched blocked one-sided Jacobi on
+ * R^H; A is not modified.  All pointers device. */
 int dmrgx_block_svd(int dtype, int n_blk, const int32_t* rows, const int32_t* cols,
                     const void* A, const int64_t* a_off, void* U, const int64_t* u_off,
                     double* S, const int64_t* s_off, void* V, const int64_t* v_off);
+
+/* Jacobi sweeps taken by the last dmrgx_block_svd call on this host thread
+ * (diagnostic; no counterpart in the reference, whose ?gesvd reports none). */
+int dmrgx_block_svd_sweeps(void);
 
 /* ---- block contraction kernels used by the environment updates --------------
  * dmrgx_block_gemm: the batched form of qtensor<T>::contract(A, B, ta, tb, num)

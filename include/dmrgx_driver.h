@@ -48,8 +48,9 @@ typedef struct dmrgx_run_stats {
 } dmrgx_run_stats;
 
 /* One row per step: l, r, E0..E5, trunc, entropy, matvecs, polish matvecs,
- * t_total, t_eig, t_svd, t_env, d (17 doubles). */
-#define DMRGX_STEP_LOG_WIDTH 17
+ * t_total, t_eig, t_svd, t_env, d, SVD Jacobi sweeps, t_block_svd (19
+ * doubles; t_block_svd is the blockwise SVD alone, inside t_svd). */
+#define DMRGX_STEP_LOG_WIDTH 19
 int dmrgx_run_dmrg(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, double* step_log, int log_cap,
                    int* n_steps);
 

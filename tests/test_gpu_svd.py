@@ -83,3 +83,31 @@ def test_block_svd_rank_deficient_completion(gpu):
         assert np.max(np.abs(v @ v.T - np.eye(v.shape[0]))) < 1e-12
         k = len(s)
         assert np.max(np.abs((u[:, :k] * s) @ v[:k, :] - x)) < 1e-11 * np.max(np.abs(x))
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_block_svd_large_graded_sectors(gpu, cplx):
+    """Sector blocks at DMRG sizes with an exponentially decaying spectrum: a
+    square 512 block, a tall block with more than 1024 rows (the wide panel
+    path) and a wide one whose dimensions are not multiples of the panel or
+    column-block widths."""
+    rng = np.random.default_rng(21)
+
+    def graded(r, c, decay):
+        k = min(r, c)
+        q1 = np.linalg.qr(rng.standard_normal((r, k)) + (1j * rng.standard_normal((r, k)) if cplx else 0))[0]
+        q2 = np.linalg.qr(rng.standard_normal((c, k)) + (1j * rng.standard_normal((c, k)) if cplx else 0))[0]
+        return (q1 * np.exp(-decay * np.arange(k) / k)) @ q2.conj().T
+
+    blocks = [graded(512, 512, 30.0), graded(1100, 1030, 25.0), graded(203, 517, 20.0)]
+    res = _svd(gpu, blocks, cplx)
+    for a, (u, s, v) in zip(blocks, res):
+        ref = np.linalg.svd(a, compute_uv=False)
+        # backward-stable bound: both this SVD (Jacobi stopping tolerance
+        # max(rows, cols) eps) and LAPACK's are accurate to ~dim * eps * s_max
+        assert np.max(np.abs(s - ref)) < 4 * max(a.shape) * 2.2e-16 * ref[0]
+        assert np.all(np.diff(s) <= 1e-15 * s[0])
+        k = len(s)
+        assert np.max(np.abs((u[:, :k] * s) @ v[:k, :] - a)) < 1e-12 * np.max(np.abs(a))
+        assert np.max(np.abs(u.conj().T @ u - np.eye(u.shape[0]))) < 1e-12
+        assert np.max(np.abs(v @ v.conj().T - np.eye(v.shape[0]))) < 1e-12
