@@ -9,8 +9,11 @@ that step (32 R-pass + 32 L-pass tasks of (1280,256,256) / (256,256,1280),
 library on the device.  The mixed-precision path (BF16x3 products, FP32
 accumulate) is the headline value; FP64 and FP32 run on the same shapes.
 
-Prints one JSON line.  ``--sweep`` adds full DMRG sweeps of the same model
-(FP64 and mixed) with the energy difference; it takes several minutes.
+Prints one JSON line.  Its ``sweep`` object is a full two-site DMRG of the
+same model at m=512 on a shorter chain (``--sweep-L``, default 20; 0 sweeps
+skips it), FP64 against BF16x3 with FP64 Lanczos polish: mean sweep seconds,
+E0 of both and their difference.  ``--part svd`` / ``--part copy`` time the
+step SVD alone / the 256 MiB copy used to calibrate the HBM byte counters.
 """
 from __future__ import annotations
 
@@ -25,6 +28,15 @@ import numpy as np
 METRIC = "DMRG sweep s + H_eff matvec TFLOP/s, mixed vs FP64 GPU at |ΔE0|<1e-8"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # MI355X dense matrix peaks, TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
+
+# HBM bytes per launch of k_gemm_bf3p<complex> on this workload (mean of the
+# R- and L-pass launches), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# of `bench.py --only bf16x3`; each counter is scaled by what it reads for a
+# plain 256 MiB copy kernel (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).
+TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 71907.0 * 2.0, 73728.0 * 1.0
+# algorithmic bytes per launch: env blocks (5 x 1024^2 complex) and the
+# input operand as four bf16 planes (8 B per complex), output FP32 complex
+ALG_BYTES = (5 * 1024 * 1024 + 524288) * 8 + 5 * 1024 * 1024 * 8
 
 S, D, M = 2, 5, 512
 L_CHAIN = 100
@@ -127,20 +139,47 @@ def cpu_leg(seconds):
                       f"same m=512 Z2 D=5 shapes, {el:.1f} s"}
 
 
-def sweep_leg(sweeps):
+def sweep_leg(L, sweeps):
+    """Full two-site DMRG (the reference's dmrg2 schedule: growth, m ladder,
+    sweeps, final half sweep) of the XYZ chain at m=512, FP64 and BF16x3 with
+    FP64 polish; sweep_s is the mean wall time of the full sweeps at m=512."""
     from dmrg_amd import driver
-    res = {}
+    res = {"L": L, "m": M, "sweeps": sweeps}
     for prec in ("fp64", "bf16x3"):
         t0 = time.perf_counter()
-        r = driver.run_dmrg("xyz", L_CHAIN, M, sweeps, JXYZ, schedule="dmrg2", prec=prec,
-                            tol_lo=1e-5, refine_k=10)
-        res[prec] = dict(energy=r.energy, sweep_s=r.stats["sweep_seconds"], total_s=time.perf_counter() - t0,
-                         matvecs=r.stats["matvecs"], polish=r.stats["polish_matvecs"])
-        print(f"[bench] sweep {prec}: E0={r.energy:.12f} sweep {r.stats['sweep_seconds']:.2f} s", file=sys.stderr,
+        r = driver.run_dmrg("xyz", L, M, sweeps, JXYZ, schedule="dmrg2", prec=prec, tol_lo=1e-5, refine_k=10)
+        st = r.stats
+        res[prec] = dict(energy=r.energy, sweep_s=round(st["sweep_seconds"], 4), total_s=round(time.perf_counter() - t0, 3),
+                         matvecs=st["matvecs"], polish=st["polish_matvecs"], trunc_err=st["trunc_err"],
+                         t_eig=round(st["t_eig"], 3), t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3))
+        print(f"[bench] sweep {prec} L={L}: E0={r.energy:.12f} sweep {st['sweep_seconds']:.2f} s "
+              f"(eig {st['t_eig']:.2f} svd {st['t_svd']:.2f} env {st['t_env']:.2f} s in all steps)", file=sys.stderr,
               flush=True)
     res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
-    res["speedup"] = res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12)
+    res["speedup"] = round(res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12), 3)
     return res
+
+
+def copy_leg(reps):
+    """A plain copy of 256 MiB through dmrgx_copy_boxes (one box): the
+    calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    import dmrg_amd
+
+    dmrg_amd.init(0)
+    n = 32 * 1024 * 1024  # float64 elements
+    src = dmrg_amd.DeviceArray(n, np.float64)
+    dst = dmrg_amd.DeviceArray(n, np.float64)
+    src.zero()
+    task = np.array([0, 0, 0, 0, 0, 0, 1, 1, n, 0], dtype=np.int64)
+    lib = dmrg_amd.lib()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dmrg_amd.check(lib.dmrgx_copy_boxes(dmrg_amd.R64, src.ptr, dst.ptr,
+                                            task.ctypes.data_as(C.POINTER(C.c_int64)), 1), "copy_boxes")
+    lib.dmrgx_synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    return {"copy_bytes": 8 * n, "ms": round(ms, 4), "GB/s (read+write)": round(16 * n / ms / 1e6, 1)}
 
 
 def main():
@@ -150,9 +189,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sweep", type=int, default=0, help="run this many full sweeps of the L=100 m=512 chain")
+    ap.add_argument("--sweep", type=int, default=1, help="full sweeps in the sweep comparison (0 skips it)")
+    ap.add_argument("--sweep-L", type=int, default=20, help="chain length of the sweep comparison at m=512")
     ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
-    ap.add_argument("--part", default="", choices=["", "svd"], help="time one part of a sweep step alone")
+    ap.add_argument("--part", default="", choices=["", "svd", "copy"],
+                    help="time one part alone: the step SVD, or the copy kernel used to calibrate PMC counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
@@ -160,6 +201,9 @@ def main():
         sys.exit(2)
     if a.part == "svd":
         print(json.dumps(svd_leg(a.steps)), flush=True)
+        return
+    if a.part == "copy":
+        print(json.dumps(copy_leg(a.steps)), flush=True)
         return
     if a.only:
         gpu_leg(a.steps, a.warmup, a.only.split(","))
@@ -192,14 +236,16 @@ def main():
         "mixed_vs_fp64_matvec": round(fp64["ms"] / mixed["ms"], 3),
         "fp64_roofline_frac": round(fp64["flops"] / (fp64["ms"] * 1e-3) / 1e12 / PEAK["fp64"], 4),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["bf16x3"], "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK["bf16x3"], 5), "traffic": None,
-                     "kernel": "k_gemm_bf16x3 (R and L pass launches)",
+                     "frac": round(achieved / PEAK["bf16x3"], 5),
+                     "traffic": round((TRAFFIC_FETCH_KIB + TRAFFIC_WRITE_KIB) * 1024),
+                     "algorithmic_bytes": ALG_BYTES,
+                     "kernel": "k_gemm_bf3p<complex> (R and L pass launches)",
                      "issued_tflops": round(3 * achieved, 3)},
         "cpu_baseline": None,
         "sweep": None,
     }
     if a.sweep:
-        line["sweep"] = sweep_leg(a.sweep)
+        line["sweep"] = sweep_leg(a.sweep_L, a.sweep)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.cpu_seconds)
     print(json.dumps(line), flush=True)
