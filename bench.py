@@ -70,6 +70,42 @@ def gpu_leg(steps, warmup, only=None):
     return out
 
 
+def configs_leg(reps=5, warmup=2):
+    """Matvec throughput at the other large configurations' shapes:
+    Z3 chiral Potts at m=1024 with uneven sectors (0.40, 0.33, 0.27 m; d=3,
+    MPO bond 4; 243 + 243 tasks) and the XYZ chain at m=2048 (Z2, MPO bond 5;
+    FP32 is that configuration's forward precision)."""
+    import dmrg_amd
+    from dmrg_amd.driver import SynthHeff
+
+    dmrg_amd.init(0)
+    cases = {
+        "potts3_m1024_uneven": (3, 4, [410, 338, 276], ("fp64", "fp32", "bf16x3")),
+        "xyz_m2048": (2, 5, [1024, 1024], ("fp64", "fp32", "bf16x3")),
+    }
+    out = {}
+    for name, (S_, D_, bond, precs) in cases.items():
+        res = {"sectors": bond, "S": S_, "D": D_}
+        for prec in precs:
+            h = SynthHeff(S_, D_, bond, complex_=True, prec=prec, seed=20261203)
+            rng = np.random.default_rng(3)
+            x = rng.uniform(-1, 1, h.n_psi) + 1j * rng.uniform(-1, 1, h.n_psi)
+            xd = dmrg_amd.DeviceArray.from_numpy(x)
+            yd = dmrg_amd.DeviceArray(h.n_psi, np.complex128)
+            h.time(xd, yd, warmup)
+            ms = h.time(xd, yd, reps)
+            res[prec] = {"ms": round(ms[0], 3), "tflops": round(h.flops / (ms[0] * 1e-3) / 1e12, 2)}
+            res["flop"] = h.flops
+            res["n_psi"] = h.n_psi
+            print(f"[bench] {name} {prec}: {ms[0]:.3f} ms/matvec, {res[prec]['tflops']:.1f} TFLOP/s", file=sys.stderr,
+                  flush=True)
+            xd.free()
+            yd.free()
+            h.close()
+        out[name] = res
+    return out
+
+
 def svd_leg(reps=5):
     """dmrgx_block_svd on the two sector blocks of the L=100 m=512 two-site
     state (512 x 512 complex each) with a DMRG-like spectrum exp(-30 k / 512)."""
@@ -188,6 +224,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-configs", action="store_true", help="skip the Potts m=1024 / XYZ m=2048 matvec shapes")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", type=int, default=1, help="full sweeps in the sweep comparison (0 skips it)")
     ap.add_argument("--sweep-L", type=int, default=20, help="chain length of the sweep comparison at m=512")
@@ -242,8 +279,11 @@ def main():
                      "kernel": "k_gemm_bf3p<complex> (R and L pass launches)",
                      "issued_tflops": round(3 * achieved, 3)},
         "cpu_baseline": None,
+        "configs": None,
         "sweep": None,
     }
+    if not a.no_configs:
+        line["configs"] = configs_leg()
     if a.sweep:
         line["sweep"] = sweep_leg(a.sweep_L, a.sweep)
     if not a.no_cpu:

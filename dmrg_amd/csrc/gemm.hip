@@ -80,6 +80,12 @@ __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* _
   }
 }
 
+// FP32 partial tiles of the split output blocks summed into FP64 or bf16 planes.
+template <bool CPLX, int OUT>
+__global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__ W, void* __restrict__ Cv,
+                                                       int64_t cps, const RedTask* __restrict__ t, int n,
+                                                       const int64_t* __restrict__ start);
+
 // Walks the (segment, k-tile) sequence of one work item.
 struct Cursor {
   int seg, seg_end, k;
@@ -401,6 +407,43 @@ __global__ __launch_bounds__(NT) void k_gemm_bf16x3(const GemmItem* __restrict__
 }
 
 
+// x -> bf16 hi, lo planes (hi = bf16(x), lo = bf16(x - hi)) of part 0 (re) or 1 (im)
+__device__ __forceinline__ void store_split(unsigned short* P, int64_t ps, int64_t e, int part, float x) {
+  const unsigned short h = f2bf_rne(x);
+  const unsigned short l = f2bf_rne((float)((double)x - (double)bf2f(h)));
+  P[(2 * part) * ps + e] = h;
+  P[(2 * part + 1) * ps + e] = l;
+}
+
+template <bool CPLX, int OUT>
+__global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__ W, void* __restrict__ Cv,
+                                                       int64_t cps, const RedTask* __restrict__ t, int n,
+                                                       const int64_t* __restrict__ start) {
+  int lo = 0, hi = n - 1;
+  const int64_t b = blockIdx.x;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const RedTask r = t[lo];
+  const int64_t e = (b - start[lo]) * 256 + threadIdx.x;
+  if (e >= r.len) return;
+  constexpr int c = CPLX ? 2 : 1;
+#pragma unroll
+  for (int q = 0; q < c; ++q) {
+    float acc = 0;
+    for (int k = 0; k < r.nch; ++k) acc += W[c * (r.w_off + k * r.len + e) + q];
+    if (OUT == OUT_F64) {
+      double* C = (double*)Cv;
+      if (r.beta) C[c * (r.c_off + e) + q] += acc;
+      else C[c * (r.c_off + e) + q] = acc;
+    } else {
+      store_split((unsigned short*)Cv, cps, r.c_off + e, q, acc);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // BF16x3 on pre-split operands: A, B are bfloat16 planes (re_hi, re_lo[, im_hi,
 // im_lo]); 128x128 output tiles, each wavefront 64x64 (4x4 MFMA 16x16x32
@@ -412,12 +455,12 @@ __device__ __forceinline__ uint4 xor_sign(uint4 v) {
   return v;
 }
 
-template <bool CPLX>
+template <bool CPLX, int OUT>
 __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ items, int nitems, int per_xcd,
                                                   const GemmSeg* __restrict__ segs,
                                                   const unsigned short* __restrict__ A, int64_t as,
                                                   const unsigned short* __restrict__ B, int64_t bs,
-                                                  float* __restrict__ C, float* __restrict__ W) {
+                                                  void* __restrict__ Cv, int64_t cps, float* __restrict__ W) {
   constexpr int NPL = CPLX ? 4 : 2;
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int CPR = PBK / 8;               // 16-byte chunks per row per plane (4)
@@ -575,7 +618,6 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
 
   const bool partial = it.flags & ITEM_PARTIAL;
   const bool beta = !partial && (it.flags & ITEM_BETA);
-  auto* D = partial ? W : C;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -586,14 +628,29 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
         int col = it.n0 + wn + j * 16 + (lane & 15);
         if (row < it.M && col < it.N) {
           int64_t e = it.c_off + (int64_t)row * it.N + col;
-          if (CPLX) {
-            float vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
-            if (beta) { D[2 * e] += vr; D[2 * e + 1] += vi; }
-            else { D[2 * e] = vr; D[2 * e + 1] = vi; }
-          } else {
-            float v = acc[0][i][j][r];
-            if (beta) D[e] += v;
-            else D[e] = v;
+          const float vr = acc[0][i][j][r], vi = CPLX ? acc[NP - 1][i][j][r] : 0.0f;
+          if (partial || OUT == OUT_F32) {
+            float* D = partial ? W : (float*)Cv;
+            if (CPLX) {
+              if (beta) { D[2 * e] += vr; D[2 * e + 1] += vi; }
+              else { D[2 * e] = vr; D[2 * e + 1] = vi; }
+            } else {
+              if (beta) D[e] += vr;
+              else D[e] = vr;
+            }
+          } else if (OUT == OUT_F64) {
+            double* D = (double*)Cv;
+            if (CPLX) {
+              if (beta) { D[2 * e] += vr; D[2 * e + 1] += vi; }
+              else { D[2 * e] = vr; D[2 * e + 1] = vi; }
+            } else {
+              if (beta) D[e] += vr;
+              else D[e] = vr;
+            }
+          } else {  // OUT_PLANES: the plan has no accumulating outputs
+            unsigned short* P = (unsigned short*)Cv;
+            store_split(P, cps, e, 0, vr);
+            if (CPLX) store_split(P, cps, e, 1, vi);
           }
         }
       }
@@ -725,6 +782,9 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
       plan->ws_elems += len * nch;
       red.push_back(RedTask{g.c_off, w_off, len, nch, g.beta});
     }
+    if (g.beta) plan->any_beta = true;
+    if (shared) {
+    }
     // cut the concatenated K of the group's tasks into chunks of length ch
     std::vector<std::pair<int, int>> chunk_rng;  // seg range per chunk
     size_t ti = 0;
@@ -820,27 +880,56 @@ void reduce_partials(const GemmPlan& p, void* C, hipStream_t s) {
 void reduce_partials(const GemmPlan& p, void* C, hipStream_t s);
 
 void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
-                     float* C, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
+                     void* C, int out, int64_t c_ps, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
   DMRGX_CHECK(p.planes, "run_gemm_planes needs a pre-split plan");
   DMRGX_CHECK(a_stride % 8 == 0 && b_stride % 8 == 0, "plane strides must be multiples of 8");
-  const size_t eb = 4 * (p.dtype == DT_C64 ? 2 : 1);
-  for (auto& z : p.zero_ranges) fill_zero((char*)C + z.first * eb, (size_t)z.second * eb, s);
+  DMRGX_CHECK(out != OUT_PLANES || !p.any_beta, "run_gemm_planes: plane output cannot accumulate");
+  const int cw = p.dtype == DT_C64 ? 2 : 1;
+  for (auto& z : p.zero_ranges) {
+    if (out == OUT_PLANES)
+      for (int q = 0; q < 2 * cw; ++q) fill_zero_u16((uint16_t*)C + q * c_ps + z.first, (size_t)z.second, s);
+    else fill_zero((char*)C + z.first * (out == OUT_F64 ? 8 : 4) * cw, (size_t)z.second * (out == OUT_F64 ? 8 : 4) * cw, s);
+  }
   if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
   float* wsp = (float*)(p.d_ws ? p.d_ws->ptr : nullptr);
   if (p.n_items) {
     const int per = p.grid / 8;
     auto it = (const GemmItem*)p.d_items->ptr;
     auto sg = (const GemmSeg*)p.d_segs->ptr;
-    if (p.dtype == DT_C64)
-      hipLaunchKernelGGL(k_gemm_bf3p<true>, dim3(p.grid), dim3(NT), 0, s, it, p.n_items, per, sg,
-                         (const unsigned short*)A, a_stride, (const unsigned short*)B, b_stride, C, wsp);
-    else
-      hipLaunchKernelGGL(k_gemm_bf3p<false>, dim3(p.grid), dim3(NT), 0, s, it, p.n_items, per, sg,
-                         (const unsigned short*)A, a_stride, (const unsigned short*)B, b_stride, C, wsp);
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(p.grid), dim3(NT), 0, s, it, p.n_items, per, sg, (const unsigned short*)A,
+                         a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp);
+    };
+    if (p.dtype == DT_C64) {
+      if (out == OUT_F32) go(k_gemm_bf3p<true, OUT_F32>);
+      else if (out == OUT_PLANES) go(k_gemm_bf3p<true, OUT_PLANES>);
+      else go(k_gemm_bf3p<true, OUT_F64>);
+    } else {
+      if (out == OUT_F32) go(k_gemm_bf3p<false, OUT_F32>);
+      else if (out == OUT_PLANES) go(k_gemm_bf3p<false, OUT_PLANES>);
+      else go(k_gemm_bf3p<false, OUT_F64>);
+    }
     DMRGX_HIP(hipGetLastError());
   }
   if (ev_end) DMRGX_HIP(hipEventRecord(ev_end, s));
-  reduce_partials(p, C, s);
+  if (out == OUT_F32) {
+    reduce_partials(p, C, s);
+  } else if (p.n_red) {
+    auto t = (const RedTask*)p.d_red->ptr;
+    auto st = (const int64_t*)p.d_red_start->ptr;
+    auto go = [&](auto kern) {
+      hipLaunchKernelGGL(kern, dim3(p.red_blocks), dim3(256), 0, s, (const float*)p.d_ws->ptr, C, c_ps, t, p.n_red,
+                         st);
+    };
+    if (p.dtype == DT_C64) {
+      if (out == OUT_PLANES) go(k_reduce_ws_out<true, OUT_PLANES>);
+      else go(k_reduce_ws_out<true, OUT_F64>);
+    } else {
+      if (out == OUT_PLANES) go(k_reduce_ws_out<false, OUT_PLANES>);
+      else go(k_reduce_ws_out<false, OUT_F64>);
+    }
+    DMRGX_HIP(hipGetLastError());
+  }
 }
 
 void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s) {

@@ -44,6 +44,10 @@ struct RedTask {
   int nch, beta;
 };
 
+// Output format of the pre-split BF16x3 kernel: FP32, bf16 hi/lo planes
+// (re_hi, re_lo[, im_hi, im_lo], stride c_ps; outputs may not accumulate), or FP64.
+enum { OUT_F32 = 0, OUT_PLANES = 1, OUT_F64 = 2 };
+
 struct GemmPlan {
   int dtype = DT_R64;
   int prec = PREC_FP64;
@@ -60,6 +64,7 @@ struct GemmPlan {
   DevBuf* d_red = nullptr;              // RedTask array
   DevBuf* d_red_start = nullptr;        // first block of each task
   double flops = 0;        // algorithmic real flops (c*M*N*K summed)
+  bool any_beta = false;   // some output block accumulates onto existing C
   ~GemmPlan();
 };
 
@@ -77,9 +82,11 @@ void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hip
 
 // Pre-split BF16x3 execution ('N','T' only): A and B are bfloat16 planes
 // (re_hi, re_lo[, im_hi, im_lo]) of a_stride / b_stride elements each (both
-// multiples of 8); C is FP32 (complex interleaved when dtype == DT_C64).
+// multiples of 8); C is FP32 or FP64 (complex interleaved when dtype ==
+// DT_C64) or, for OUT_PLANES, bf16 planes of stride c_ps.
 void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
-                     float* C, hipStream_t s, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+                     void* C, int out, int64_t c_ps, hipStream_t s, hipEvent_t ev_begin = nullptr,
+                     hipEvent_t ev_end = nullptr);
 // Split n elements (complex if cplx) into hi/lo bfloat16 planes of stride `stride`.
 void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);
 void split_planes_f32(const float* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);

@@ -107,8 +107,8 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     h->store_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->store_ps, 8) * npl * 2);
     split_planes_f64((const double*)renv, (uint16_t*)h->renv_pl->ptr, renv_len, h->renv_ps, c == 2, s);
     split_planes_f64((const double*)lenv, (uint16_t*)h->lenv_pl->ptr, lenv_len, h->lenv_ps, c == 2, s);
-    h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 4 * c);
-    h->out_lo = std::make_unique<DevBuf>(std::max<int64_t>(n_psi, 1) * 4 * c);
+    // an FP32 `store` only when the R pass accumulates (then it cannot write planes)
+    if (h->rplan->any_beta) h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 4 * c);
   } else {
     h->renv_lo = to_f32(renv, renv_len * c, s);
     h->lenv_lo = to_f32(lenv, lenv_len * c, s);
@@ -128,13 +128,19 @@ void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* e
     run_gemm_plan(*rplan, renv, in, store->ptr, s, e0, e1);
     run_gemm_plan(*lplan, lenv, store->ptr, out, s, e2, e3);
   } else if (prec == PREC_BF16X3) {
+    // the R pass writes `store` straight as bf16 planes and the L pass writes
+    // the FP64 result (split-K partial tiles are converted by the reduction)
     split_planes_f64(in, (uint16_t*)in_pl->ptr, n_psi, psi_ps, c == 2, s);
-    run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
-                    (float*)store->ptr, s, e0, e1);
-    split_planes_f32((const float*)store->ptr, (uint16_t*)store_pl->ptr, n_store, store_ps, c == 2, s);
-    run_gemm_planes(*lplan, (const uint16_t*)lenv_pl->ptr, lenv_ps, (const uint16_t*)store_pl->ptr, store_ps,
-                    (float*)out_lo->ptr, s, e2, e3);
-    convert_f32_to_f64((const float*)out_lo->ptr, out, n_psi * c, s);
+    if (!rplan->any_beta) {
+      run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
+                      store_pl->ptr, OUT_PLANES, store_ps, s, e0, e1);
+    } else {
+      run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
+                      store->ptr, OUT_F32, 0, s, e0, e1);
+      split_planes_f32((const float*)store->ptr, (uint16_t*)store_pl->ptr, n_store, store_ps, c == 2, s);
+    }
+    run_gemm_planes(*lplan, (const uint16_t*)lenv_pl->ptr, lenv_ps, (const uint16_t*)store_pl->ptr, store_ps, out,
+                    OUT_F64, 0, s, e2, e3);
   } else {
     convert_f64_to_f32(in, (float*)in_lo->ptr, n_psi * c, s);
     run_gemm_plan(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s, e0, e1);

@@ -15,6 +15,7 @@ struct EigshResult {
   std::vector<double> evals;  // nev lowest, ascending
   int n_matvec = 0;
   int n_restart = 0;
+  int n_reorth = 0;  // steps that needed a second Gram-Schmidt pass
   bool converged = false;
   double max_resid = 0;
 };

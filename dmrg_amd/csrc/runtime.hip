@@ -108,6 +108,21 @@ void fill_zero(void* p, size_t bytes, hipStream_t s) {
   DMRGX_HIP(hipGetLastError());
 }
 
+__global__ void k_fill_zero_u16(uint16_t* __restrict__ p, size_t n) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0;
+}
+
+void fill_zero_u16(uint16_t* p, size_t n, hipStream_t s) {
+  if (!n) return;
+  DMRGX_CHECK(((uintptr_t)p & 1) == 0, "fill_zero_u16: range not 2-byte aligned");
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_zero_u16, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
+  DMRGX_HIP(hipGetLastError());
+}
+
 void copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
   if (bytes) DMRGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
 }
