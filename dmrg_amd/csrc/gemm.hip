@@ -491,8 +491,8 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
   cur.start(segs, it.seg_begin, it.seg_end);
   bool avec = false, bvec = false;
   auto seg_setup = [&](const GemmSeg& s) {
-    avec = ((s.a_off | s.K | s.k0) & 7) == 0;
-    bvec = ((s.b_off | s.K | s.k0) & 7) == 0;
+    avec = ((s.a_off | s.lda | s.k0) & 7) == 0;
+    bvec = ((s.b_off | s.ldb | s.k0) & 7) == 0;
     okA = okB = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
@@ -501,8 +501,8 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
       const int rem = ch % (PT * CPR);
       const int row = rem / CPR, kc = (rem % CPR) * 8;
       const int gr = it.m0 + row, gn = it.n0 + row;
-      offA[j] = plane * as + s.a_off + (int64_t)gr * s.K + kc;
-      offB[j] = plane * bs + s.b_off + (int64_t)gn * s.K + kc;
+      offA[j] = plane * as + s.a_off + (int64_t)gr * s.lda + kc;
+      offB[j] = plane * bs + s.b_off + (int64_t)gn * s.ldb + kc;
       if (gr < it.M) okA |= 1u << j;
       if (gn < it.N) okB |= 1u << j;
     }
@@ -532,22 +532,27 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
         rb[j] = w;
       }
     } else {
+      // a segment that ends at its row end may read the whole last 8-chunk: an
+      // aligned row stride (lda, ldb multiples of 8) is either K itself or the
+      // zero-padded round-up of K
+      const bool tail = c.s.k1 == c.s.K;
 #pragma unroll
       for (int j = 0; j < CPT; ++j) {
         const int ch = t + NT * j;
         const int kc = ((ch % (PT * CPR)) % CPR) * 8;
         const int gk = c.k + kc;
+        const bool whole = gk + 8 <= s.k1 || (tail && gk < s.k1);
         uint4 v = make_uint4(0, 0, 0, 0);
         if (okA & (1u << j)) {
           const unsigned short* src = A + offA[j] + c.k;
-          v = (avec && gk + 8 <= s.k1) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
+          v = (avec && whole) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
           if (neg) v = xor_sign(v);
         }
         ra[j] = v;
         uint4 w = make_uint4(0, 0, 0, 0);
         if (okB & (1u << j)) {
           const unsigned short* src = B + offB[j] + c.k;
-          w = (bvec && gk + 8 <= s.k1) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
+          w = (bvec && whole) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
         }
         rb[j] = w;
       }
@@ -654,6 +659,33 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
           }
         }
       }
+}
+
+template <typename SRC>
+__global__ void k_split_padded(const SRC* __restrict__ src, unsigned short* __restrict__ dst, int64_t ps,
+                               const PadBlk* __restrict__ blk, int nb, int64_t total, int cplx) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; e < total; e += st) {
+    int lo = 0, hi = nb - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (blk[mid].dst_off <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    const PadBlk b = blk[lo];
+    const int64_t loc = e - b.dst_off;
+    const int row = (int)(loc / b.Kp), col = (int)(loc % b.Kp);
+    const bool ok = row < b.rows && col < b.K;
+    const int64_t si = b.src_off + (int64_t)row * b.K + col;
+    for (int part = 0; part <= cplx; ++part) {
+      const double x = ok ? (double)src[cplx ? 2 * si + part : si] : 0.0;
+      const unsigned short h = f2bf_rne((float)x);
+      const unsigned short l = f2bf_rne((float)(x - (double)bf2f(h)));
+      dst[(2 * part) * ps + e] = h;
+      dst[(2 * part + 1) * ps + e] = l;
+    }
+  }
 }
 
 template <typename SRC>
@@ -773,32 +805,29 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     int nch = (int)std::max<int64_t>(1, (g.ksum + kchunk - 1) / kchunk);
     int64_t ch = (g.ksum + nch - 1) / nch;
     ch = (ch + BKq - 1) / BKq * BKq;
-    nch = (int)((g.ksum + ch - 1) / ch);
-    const bool shared = nch > 1;
-    const int64_t len = (int64_t)g.M * g.N;
-    int64_t w_off = 0;
-    if (shared) {
-      w_off = plan->ws_elems;
-      plan->ws_elems += len * nch;
-      red.push_back(RedTask{g.c_off, w_off, len, nch, g.beta});
-    }
     if (g.beta) plan->any_beta = true;
-    if (shared) {
-    }
-    // cut the concatenated K of the group's tasks into chunks of length ch
+    // cut the concatenated K of the group's tasks into chunks of about ch; a
+    // cut inside a task falls on a multiple of 8 so that every segment starts
+    // on a 16-byte boundary of an aligned operand row
     std::vector<std::pair<int, int>> chunk_rng;  // seg range per chunk
     size_t ti = 0;
     int64_t kin = 0;  // position inside current task
-    for (int c = 0; c < nch; ++c) {
+    while (ti < g.t.size()) {
       int sb = (int)segs.size();
       int64_t need = ch;
       while (need > 0 && ti < g.t.size()) {
         const GemmTask& k = tasks_in[g.t[ti]];
         int64_t take = std::min<int64_t>(need, k.K - kin);
+        if (take < k.K - kin) {
+          take = take / 8 * 8;
+          if (take == 0) break;
+        }
         GemmSeg s{};
         s.a_off = k.a_off;
         s.b_off = k.b_off;
         s.K = k.K;
+        s.lda = k.lda > 0 ? k.lda : k.K;
+        s.ldb = k.ldb > 0 ? k.ldb : k.K;
         s.k0 = (int)kin;
         s.k1 = (int)(kin + take);
         s.alpha = k.alpha;
@@ -807,7 +836,17 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
         kin += take;
         if (kin >= k.K) { ++ti; kin = 0; }
       }
+      DMRGX_CHECK((int)segs.size() > sb, "gemm plan: empty K chunk");
       chunk_rng.push_back({sb, (int)segs.size()});
+    }
+    nch = (int)chunk_rng.size();
+    const bool shared = nch > 1;
+    const int64_t len = (int64_t)g.M * g.N;
+    int64_t w_off = 0;
+    if (shared) {
+      w_off = plan->ws_elems;
+      plan->ws_elems += len * nch;
+      red.push_back(RedTask{g.c_off, w_off, len, nch, g.beta});
     }
     for (int c = 0; c < nch; ++c)
       for (int n0 = 0; n0 < g.N; n0 += BN)
@@ -930,6 +969,66 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
     }
     DMRGX_HIP(hipGetLastError());
   }
+}
+
+int64_t PadLayout::map(int64_t src_off) const {
+  auto it = std::lower_bound(blk.begin(), blk.end(), src_off,
+                             [](const PadBlk& b, int64_t v) { return b.src_off < v; });
+  DMRGX_CHECK(it != blk.end() && it->src_off == src_off, "padded layout: unknown block");
+  return it->dst_off;
+}
+
+int PadLayout::kp(int64_t src_off) const {
+  auto it = std::lower_bound(blk.begin(), blk.end(), src_off,
+                             [](const PadBlk& b, int64_t v) { return b.src_off < v; });
+  DMRGX_CHECK(it != blk.end() && it->src_off == src_off, "padded layout: unknown block");
+  return it->Kp;
+}
+
+std::unique_ptr<PadLayout> make_pad_layout(std::vector<PadBlk> blocks) {
+  auto L = std::make_unique<PadLayout>();
+  std::sort(blocks.begin(), blocks.end(), [](const PadBlk& a, const PadBlk& b) { return a.src_off < b.src_off; });
+  int64_t pos = 0;
+  for (auto& b : blocks) {
+    if (!L->blk.empty() && L->blk.back().src_off == b.src_off) {
+      DMRGX_CHECK(L->blk.back().rows == b.rows && L->blk.back().K == b.K,
+                  "padded layout: one block seen with two shapes");
+      continue;
+    }
+    PadBlk q = b;
+    q.Kp = (b.K + 7) / 8 * 8;
+    q.dst_off = pos;
+    q.pad = 0;
+    if (q.dst_off != q.src_off || q.Kp != q.K) L->identity = false;
+    pos += (int64_t)q.rows * q.Kp;
+    L->blk.push_back(q);
+  }
+  L->total = pos;
+  if (!L->blk.empty()) {
+    L->d_blk = std::make_unique<DevBuf>(L->blk.size() * sizeof(PadBlk));
+    DMRGX_HIP(hipMemcpyAsync(L->d_blk->ptr, L->blk.data(), L->blk.size() * sizeof(PadBlk), hipMemcpyHostToDevice,
+                             rt().stream));
+    DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  }
+  return L;
+}
+
+template <typename SRC>
+static void split_padded(const SRC* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s) {
+  if (L.total <= 0) return;
+  DMRGX_CHECK(ps >= L.total, "split_padded: plane stride below the padded length");
+  const int64_t blocks = std::min<int64_t>((L.total + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_split_padded<SRC>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, ps,
+                     (const PadBlk*)L.d_blk->ptr, (int)L.blk.size(), L.total, cplx ? 1 : 0);
+  DMRGX_HIP(hipGetLastError());
+}
+
+void split_padded_f64(const double* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s) {
+  split_padded(src, dst, ps, L, cplx, s);
+}
+
+void split_padded_f32(const float* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s) {
+  split_padded(src, dst, ps, L, cplx, s);
 }
 
 void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s) {

@@ -16,6 +16,7 @@ struct GemmTask {
   int M, N, K;
   int alpha;  // +1 / -1 (the reference's fermion sign)
   int beta;   // 0: first write overwrites, 1: accumulate onto C
+  int lda = 0, ldb = 0;  // row strides of A and B ('N','T' pre-split plans); 0 means K
 };
 
 // Device work decomposition (shared with the kernels).
@@ -24,6 +25,8 @@ struct GemmSeg {
   int K;        // the task's full K (leading dimension when k-contiguous)
   int k0, k1;   // k range handled by this segment
   int alpha;
+  int lda, ldb;  // row strides of A and B in the pre-split kernel (>= K)
+  int pad;
 };
 struct GemmItem {
   int64_t c_off;
@@ -91,6 +94,27 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
 void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);
 void split_planes_f32(const float* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);
 inline int64_t plane_stride(int64_t n) { return (n + 7) / 8 * 8; }
+
+// Padded plane layout of an operand: its blocks (rows x K, row-major at
+// src_off) are stored with rows of Kp = K rounded up to a multiple of 8,
+// zero-filled, so that every row starts on a 16-byte boundary of a bf16 plane.
+struct PadBlk {
+  int64_t src_off, dst_off;
+  int rows, K, Kp, pad;
+};
+struct PadLayout {
+  std::vector<PadBlk> blk;   // ascending src_off
+  int64_t total = 0;         // padded elements
+  bool identity = true;      // dst_off == src_off and Kp == K for every block
+  std::unique_ptr<DevBuf> d_blk;
+  int64_t map(int64_t src_off) const;  // padded offset of the block at src_off
+  int kp(int64_t src_off) const;
+};
+// Layout of the distinct operand blocks {src_off, rows, K} (duplicates allowed).
+std::unique_ptr<PadLayout> make_pad_layout(std::vector<PadBlk> blocks);
+// src (FP64 or FP32, complex if cplx) -> padded hi/lo bf16 planes of stride ps.
+void split_padded_f64(const double* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s);
+void split_padded_f32(const float* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s);
 
 // Precision conversion helpers (n counts real scalars).
 void convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);

@@ -88,8 +88,31 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     DMRGX_CHECK(g.a_off + (int64_t)g.M * g.K <= lenv_len, "heff: lmap block past the end of lenv");
   }
   const bool planes = prec == PREC_BF16X3;
-  h->rplan = make_gemm_plan(rt_, dtype, prec, 'N', 'T', uncovered(rt_, n_store), 0, planes);
-  h->lplan = make_gemm_plan(lt_, dtype, prec, 'N', 'T', uncovered(lt_, n_psi), 0, planes);
+  const auto rzero = uncovered(rt_, n_store), lzero = uncovered(lt_, n_psi);
+  if (planes) {
+    // every operand block gets rows of a multiple of 8 bf16 elements
+    auto lay = [](const std::vector<GemmTask>& t, bool a) {
+      std::vector<PadBlk> b;
+      for (auto& g : t) b.push_back(PadBlk{a ? g.a_off : g.b_off, 0, a ? g.M : g.N, g.K, 0, 0});
+      return make_pad_layout(std::move(b));
+    };
+    h->renv_lay = lay(rt_, true);
+    h->psi_lay = lay(rt_, false);
+    h->lenv_lay = lay(lt_, true);
+    h->store_lay = lay(lt_, false);
+    auto remap = [](std::vector<GemmTask>& t, const PadLayout& A, const PadLayout& B) {
+      for (auto& g : t) {
+        g.lda = A.kp(g.a_off);
+        g.ldb = B.kp(g.b_off);
+        g.a_off = A.map(g.a_off);
+        g.b_off = B.map(g.b_off);
+      }
+    };
+    remap(rt_, *h->renv_lay, *h->psi_lay);
+    remap(lt_, *h->lenv_lay, *h->store_lay);
+  }
+  h->rplan = make_gemm_plan(rt_, dtype, prec, 'N', 'T', rzero, 0, planes);
+  h->lplan = make_gemm_plan(lt_, dtype, prec, 'N', 'T', lzero, 0, planes);
   h->flops = h->rplan->flops + h->lplan->flops;
   const int c = dtype == DT_C64 ? 2 : 1;
   hipStream_t s = rt().stream;
@@ -97,18 +120,21 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 8 * c);
   } else if (planes) {
     const int npl = 2 * c;
-    h->renv_ps = plane_stride(renv_len);
-    h->lenv_ps = plane_stride(lenv_len);
-    h->psi_ps = plane_stride(n_psi);
-    h->store_ps = plane_stride(n_store);
+    // the R pass can write `store` as planes when its padded layout is the
+    // plain one and no output accumulates; otherwise it writes FP32 and the
+    // planes are split from it
+    h->store_direct = h->store_lay->identity && !h->rplan->any_beta && h->store_lay->total == n_store;
+    h->renv_ps = plane_stride(h->renv_lay->total);
+    h->lenv_ps = plane_stride(h->lenv_lay->total);
+    h->psi_ps = plane_stride(h->psi_lay->total);
+    h->store_ps = plane_stride(h->store_direct ? n_store : h->store_lay->total);
     h->renv_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->renv_ps, 8) * npl * 2);
     h->lenv_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->lenv_ps, 8) * npl * 2);
     h->in_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->psi_ps, 8) * npl * 2);
     h->store_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->store_ps, 8) * npl * 2);
-    split_planes_f64((const double*)renv, (uint16_t*)h->renv_pl->ptr, renv_len, h->renv_ps, c == 2, s);
-    split_planes_f64((const double*)lenv, (uint16_t*)h->lenv_pl->ptr, lenv_len, h->lenv_ps, c == 2, s);
-    // an FP32 `store` only when the R pass accumulates (then it cannot write planes)
-    if (h->rplan->any_beta) h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 4 * c);
+    split_padded_f64((const double*)renv, (uint16_t*)h->renv_pl->ptr, h->renv_ps, *h->renv_lay, c == 2, s);
+    split_padded_f64((const double*)lenv, (uint16_t*)h->lenv_pl->ptr, h->lenv_ps, *h->lenv_lay, c == 2, s);
+    if (!h->store_direct) h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 4 * c);
   } else {
     h->renv_lo = to_f32(renv, renv_len * c, s);
     h->lenv_lo = to_f32(lenv, lenv_len * c, s);
@@ -130,14 +156,14 @@ void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* e
   } else if (prec == PREC_BF16X3) {
     // the R pass writes `store` straight as bf16 planes and the L pass writes
     // the FP64 result (split-K partial tiles are converted by the reduction)
-    split_planes_f64(in, (uint16_t*)in_pl->ptr, n_psi, psi_ps, c == 2, s);
-    if (!rplan->any_beta) {
+    split_padded_f64(in, (uint16_t*)in_pl->ptr, psi_ps, *psi_lay, c == 2, s);
+    if (store_direct) {
       run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
                       store_pl->ptr, OUT_PLANES, store_ps, s, e0, e1);
     } else {
       run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
                       store->ptr, OUT_F32, 0, s, e0, e1);
-      split_planes_f32((const float*)store->ptr, (uint16_t*)store_pl->ptr, n_store, store_ps, c == 2, s);
+      split_padded_f32((const float*)store->ptr, (uint16_t*)store_pl->ptr, store_ps, *store_lay, c == 2, s);
     }
     run_gemm_planes(*lplan, (const uint16_t*)lenv_pl->ptr, lenv_ps, (const uint16_t*)store_pl->ptr, store_ps, out,
                     OUT_F64, 0, s, e2, e3);

@@ -20,6 +20,9 @@ struct HeffPlan {
   // BF16x3: bfloat16 planes of the environments, psi and store
   std::unique_ptr<DevBuf> renv_pl, lenv_pl, in_pl, store_pl;
   int64_t renv_ps = 0, lenv_ps = 0, psi_ps = 0, store_ps = 0;
+  // padded plane layouts of the four operands (rows rounded up to 8 elements)
+  std::unique_ptr<PadLayout> renv_lay, lenv_lay, psi_lay, store_lay;
+  bool store_direct = false;  // the R pass writes store's planes itself
   std::unique_ptr<GemmPlan> rplan, lplan;
   double flops = 0;
   // ev: optional {R begin, R end, L begin, L end} events around the two GEMM launches

@@ -37,6 +37,7 @@ def _run(gpu, case, prec):
     ("z2_m64_complex", 2, 5, 64, np.complex128, None),
     ("z3_uneven_complex", 3, 4, 60, np.complex128, [27, 20, 13]),
     ("z2_ragged_real", 2, 5, 70, np.float64, [37, 33]),
+    ("z3_m200_uneven_complex", 3, 4, 200, np.complex128, [81, 67, 52]),
 ])
 def test_heff_synthetic(gpu, prec, shape):
     name, S, D, m, dt, uneven = shape
@@ -46,12 +47,13 @@ def test_heff_synthetic(gpu, prec, shape):
     assert err < TOL[prec], f"{name} {prec}: rel err {err:.3e}"
 
 
+@pytest.mark.parametrize("prec", ["fp64", "bf16x3"])
 @pytest.mark.parametrize("sign_rule", ["physical", "reference"])
-def test_heff_real_envs_xyz(gpu, sign_rule):
-    print(f"case xyz L=8 m=16 sign={sign_rule}", flush=True)
+def test_heff_real_envs_xyz(gpu, sign_rule, prec):
+    print(f"case xyz L=8 m=16 sign={sign_rule} prec={prec}", flush=True)
     case, _ = dmrg_case("xyz", 8, 16, 1, sign_rule=sign_rule, jx=1.0, jy=0.8, jz=0.5)
-    err, _ = _run(gpu, case, "fp64")
-    assert err < TOL["fp64"]
+    err, _ = _run(gpu, case, prec)
+    assert err < TOL[prec]
 
 
 def test_heff_real_envs_kitaev(gpu):
