@@ -178,12 +178,16 @@ def cpu_leg(seconds):
 def sweep_leg(L, sweeps):
     """Full two-site DMRG (the reference's dmrg2 schedule: growth, m ladder,
     sweeps, final half sweep) of the XYZ chain at m=512, FP64 and BF16x3 with
-    FP64 polish; sweep_s is the mean wall time of the full sweeps at m=512."""
+    FP64 polish; sweep_s is the mean wall time of the full sweeps at m=512;
+    fidelity is |<psi_bf16x3|psi_fp64>| of the final states."""
     from dmrg_amd import driver
     res = {"L": L, "m": M, "sweeps": sweeps}
+    states = {}
     for prec in ("fp64", "bf16x3"):
         t0 = time.perf_counter()
-        r = driver.run_dmrg("xyz", L, M, sweeps, JXYZ, schedule="dmrg2", prec=prec, tol_lo=1e-5, refine_k=10)
+        r = driver.run_dmrg("xyz", L, M, sweeps, JXYZ, schedule="dmrg2", prec=prec, tol_lo=1e-5, refine_k=10,
+                            keep_state=True)
+        states[prec] = r.state
         st = r.stats
         res[prec] = dict(energy=r.energy, sweep_s=round(st["sweep_seconds"], 4), total_s=round(time.perf_counter() - t0, 3),
                          matvecs=st["matvecs"], polish=st["polish_matvecs"], trunc_err=st["trunc_err"],
@@ -192,6 +196,10 @@ def sweep_leg(L, sweeps):
               f"(eig {st['t_eig']:.2f} svd {st['t_svd']:.2f} env {st['t_env']:.2f} s in all steps)", file=sys.stderr,
               flush=True)
     res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
+    # |<psi_bf16x3|psi_fp64>| of the two final MPSs (the reference's calc(mps, mps))
+    res["fidelity"] = states["bf16x3"].fidelity(states["fp64"])
+    for st in states.values():
+        st.close()
     res["speedup"] = round(res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12), 3)
     return res
 

@@ -7,9 +7,18 @@
 #include "core.hpp"
 #include "heff.hpp"
 #include "lanczos.hpp"
+#include "observables.hpp"
 #include "qtensor.hpp"
 
 using namespace dmrgx;
+
+// a finished run's MPS (centre absorbed), one of r / c set
+struct dmrgx_state {
+  int model = 0, L = 0;
+  bool signs = false;  // graded signs on: fermionic model, or the reference's sign rule
+  std::unique_ptr<mps<double>> r;
+  std::unique_ptr<mps<cplx>> c;
+};
 
 #define CAPI_BEGIN try {
 #define CAPI_END                      \
@@ -25,6 +34,38 @@ using namespace dmrgx;
   }
 
 namespace {
+
+bool model_complex(int model) { return model == DMRGX_MODEL_XYZ || model == DMRGX_MODEL_POTTS3; }
+bool model_fermion(int model) { return model == DMRGX_MODEL_KITAEV; }
+int model_symmetry(int model) { return model == DMRGX_MODEL_POTTS3 ? 3 : 2; }
+
+// the model MPOs of src/operators.cpp with their edges (include/mpo.h:45-59);
+// sets the charge group of the model
+mpo<cplx> model_mpo_c(int model, const double* p, int L) {
+  set_symmetry(model_symmetry(model));
+  qtensor<cplx> H;
+  if (model == DMRGX_MODEL_XYZ) H = H_XYZ(p[0], p[1], p[2]);
+  else if (model == DMRGX_MODEL_POTTS3) H = H_Potts3(p[0], p[1], p[2], p[3]);
+  else DMRGX_CHECK(false, "unknown complex model");
+  mpo<cplx> m(L, H);
+  m.edge();
+  return m;
+}
+
+mpo<double> model_mpo_r(int model, const double* p, int L) {
+  set_symmetry(model_symmetry(model));
+  if (model == DMRGX_MODEL_KITAEV) {
+    qtensor<double> H = H_Kitaev(p[0], p[1], p[2]);
+    mpo<double> m(L, H);
+    m.edge(H.left(), H.right());
+    return m;
+  }
+  DMRGX_CHECK(model == DMRGX_MODEL_POTTS2, "unknown real model");
+  qtensor<double> H = H_Potts2(p[0], p[1]);
+  mpo<double> m(L, H);
+  m.edge();
+  return m;
+}
 
 template <typename T>
 void run_engine(Engine<T>& eng, const dmrgx_run_config* c, dmrgx_run_stats* st, double* log, int cap, int* n) {
@@ -153,50 +194,90 @@ qtensor<double> structure_only(const Desc& d) {
 extern "C" {
 
 int dmrgx_run_dmrg(const dmrgx_run_config* c, dmrgx_run_stats* st, double* step_log, int log_cap, int* n_steps) {
+  return dmrgx_run_dmrg_state(c, st, step_log, log_cap, n_steps, nullptr);
+}
+
+int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double* step_log, int log_cap,
+                         int* n_steps, dmrgx_state** state) {
   CAPI_BEGIN
   ensure_init();
   DMRGX_CHECK(c && st, "run_dmrg: null arguments");
   DMRGX_CHECK(c->L >= 4, "run_dmrg: L must be at least 4");
-  switch (c->model) {
-    case DMRGX_MODEL_XYZ: {
-      set_symmetry(2);
-      qtensor<cplx> H = H_XYZ(c->p[0], c->p[1], c->p[2]);
-      mpo<cplx> m(c->L, H);
-      m.edge();
-      Engine<cplx> eng(m, make_cfg(c, false));
-      run_engine(eng, c, st, step_log, log_cap, n_steps);
-      break;
-    }
-    case DMRGX_MODEL_KITAEV: {
-      set_symmetry(2);
-      qtensor<double> H = H_Kitaev(c->p[0], c->p[1], c->p[2]);
-      mpo<double> m(c->L, H);
-      m.edge(H.left(), H.right());
-      Engine<double> eng(m, make_cfg(c, true));
-      run_engine(eng, c, st, step_log, log_cap, n_steps);
-      break;
-    }
-    case DMRGX_MODEL_POTTS3: {
-      set_symmetry(3);
-      qtensor<cplx> H = H_Potts3(c->p[0], c->p[1], c->p[2], c->p[3]);
-      mpo<cplx> m(c->L, H);
-      m.edge();
-      Engine<cplx> eng(m, make_cfg(c, false));
-      run_engine(eng, c, st, step_log, log_cap, n_steps);
-      break;
-    }
-    case DMRGX_MODEL_POTTS2: {
-      set_symmetry(2);
-      qtensor<double> H = H_Potts2(c->p[0], c->p[1]);
-      mpo<double> m(c->L, H);
-      m.edge();
-      Engine<double> eng(m, make_cfg(c, false));
-      run_engine(eng, c, st, step_log, log_cap, n_steps);
-      break;
-    }
-    default:
-      DMRGX_CHECK(false, "run_dmrg: unknown model");
+  if (state) *state = nullptr;
+  std::unique_ptr<dmrgx_state> out;
+  if (state) {
+    out = std::make_unique<dmrgx_state>();
+    out->model = c->model;
+    out->L = c->L;
+    out->signs = model_fermion(c->model) || c->sign_rule == DMRGX_SIGNS_REFERENCE;
   }
+  if (model_complex(c->model)) {
+    mpo<cplx> m = model_mpo_c(c->model, c->p, c->L);
+    Engine<cplx> eng(m, make_cfg(c, model_fermion(c->model)));
+    run_engine(eng, c, st, step_log, log_cap, n_steps);
+    if (out) out->c = std::make_unique<mps<cplx>>(prepared(eng.mps_));
+  } else {
+    mpo<double> m = model_mpo_r(c->model, c->p, c->L);
+    Engine<double> eng(m, make_cfg(c, model_fermion(c->model)));
+    run_engine(eng, c, st, step_log, log_cap, n_steps);
+    if (out) out->r = std::make_unique<mps<double>>(prepared(eng.mps_));
+  }
+  if (state) *state = out.release();
+  CAPI_END
+}
+
+int dmrgx_state_norm(const dmrgx_state* a, double* out) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && out, "state_norm: null arguments");
+  set_symmetry(model_symmetry(a->model));
+  if (a->c) {
+    cplx v = calc(*a->c, a->signs);
+    out[0] = v.real();
+    out[1] = v.imag();
+  } else {
+    out[0] = calc(*a->r, a->signs);
+    out[1] = 0;
+  }
+  CAPI_END
+}
+
+int dmrgx_state_overlap(const dmrgx_state* a, const dmrgx_state* b, double* out) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && b && out, "state_overlap: null arguments");
+  DMRGX_CHECK(a->model == b->model && a->L == b->L, "state_overlap: states of different models or lengths");
+  set_symmetry(model_symmetry(a->model));
+  if (a->c) {
+    cplx v = calc(*a->c, *b->c, a->signs);
+    out[0] = v.real();
+    out[1] = v.imag();
+  } else {
+    out[0] = calc(*a->r, *b->r, a->signs);
+    out[1] = 0;
+  }
+  CAPI_END
+}
+
+int dmrgx_state_expect(const dmrgx_state* a, int model, const double* p, double* out) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && p && out, "state_expect: null arguments");
+  DMRGX_CHECK(model_complex(model) == (a->c != nullptr) && model_symmetry(model) == model_symmetry(a->model),
+              "state_expect: operator and state differ in element type or symmetry");
+  if (a->c) {
+    mpo<cplx> m = model_mpo_c(model, p, a->L);
+    cplx v = calc(*a->c, m, a->signs);
+    out[0] = v.real();
+    out[1] = v.imag();
+  } else {
+    mpo<double> m = model_mpo_r(model, p, a->L);
+    out[0] = calc(*a->r, m, a->signs);
+    out[1] = 0;
+  }
+  CAPI_END
+}
+
+int dmrgx_state_destroy(dmrgx_state* a) {
+  CAPI_BEGIN
+  delete a;
   CAPI_END
 }
 

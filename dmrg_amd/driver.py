@@ -9,7 +9,9 @@ import numpy as np
 
 MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
 STEP_LOG_WIDTH = 19
-DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym"]
+DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym",
+                  "dmrgx_run_dmrg_state", "dmrgx_state_norm", "dmrgx_state_overlap", "dmrgx_state_expect",
+                  "dmrgx_state_destroy"]
 
 
 class RunConfig(C.Structure):
@@ -39,7 +41,66 @@ def declare(lib):
     lib.dmrgx_get_map.restype = C.c_int
     lib.dmrgx_generate_dim_sym.argtypes = [C.c_int, i32p, C.c_int, i32p, C.c_int, C.c_int, i32p, C.c_int]
     lib.dmrgx_generate_dim_sym.restype = C.c_int
+    dp = C.POINTER(C.c_double)
+    lib.dmrgx_run_dmrg_state.argtypes = [C.POINTER(RunConfig), C.POINTER(RunStats), dp, C.c_int,
+                                         C.POINTER(C.c_int), C.POINTER(C.c_void_p)]
+    lib.dmrgx_run_dmrg_state.restype = C.c_int
+    lib.dmrgx_state_norm.argtypes = [C.c_void_p, dp]
+    lib.dmrgx_state_norm.restype = C.c_int
+    lib.dmrgx_state_overlap.argtypes = [C.c_void_p, C.c_void_p, dp]
+    lib.dmrgx_state_overlap.restype = C.c_int
+    lib.dmrgx_state_expect.argtypes = [C.c_void_p, C.c_int, dp, dp]
+    lib.dmrgx_state_expect.restype = C.c_int
+    lib.dmrgx_state_destroy.argtypes = [C.c_void_p]
+    lib.dmrgx_state_destroy.restype = C.c_int
     return lib
+
+
+class State:
+    """A finished run's MPS on the device (centre absorbed) and the
+    reference's calc functions on it."""
+
+    def __init__(self, handle, model):
+        self.h = handle
+        self.model = model
+
+    def _call(self, fn, *args):
+        from . import check
+        out = np.zeros(2)
+        check(fn(*args, out.ctypes.data_as(C.POINTER(C.c_double))), fn.__name__)
+        return complex(out[0], out[1])
+
+    def norm2(self) -> complex:
+        """calc(mps) = <a|a>"""
+        from . import lib
+        return self._call(lib().dmrgx_state_norm, self.h)
+
+    def overlap(self, other: "State") -> complex:
+        """calc(mps, mps) = <other|self>"""
+        from . import lib
+        return self._call(lib().dmrgx_state_overlap, self.h, other.h)
+
+    def expect(self, params, model=None) -> complex:
+        """calc(mps, mpo) = <a|H(model, params)|a>"""
+        from . import lib
+        p = (C.c_double * 4)(*([float(x) for x in params] + [0.0] * (4 - len(params))))
+        return self._call(lib().dmrgx_state_expect, self.h, MODELS[model or self.model], p)
+
+    def fidelity(self, other: "State") -> float:
+        """|<b|a>| / sqrt(<a|a> <b|b>)"""
+        return abs(self.overlap(other)) / np.sqrt(abs(self.norm2()) * abs(other.norm2()))
+
+    def close(self):
+        from . import lib
+        if self.h:
+            lib().dmrgx_state_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass
@@ -47,11 +108,12 @@ class RunResult:
     energy: float
     stats: dict
     steps: np.ndarray  # (n_steps, STEP_LOG_WIDTH)
+    state: State | None = None
 
 
 def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
              sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
-             verbose=False) -> RunResult:
+             verbose=False, keep_state=False) -> RunResult:
     from . import PREC_NAMES, check, init, lib
     init()
     cfg = RunConfig()
@@ -69,10 +131,12 @@ def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, sched
     cap = 20000
     log = np.zeros((cap, STEP_LOG_WIDTH))
     n = C.c_int()
-    check(lib().dmrgx_run_dmrg(C.byref(cfg), C.byref(st), log.ctypes.data_as(C.POINTER(C.c_double)), cap,
-                               C.byref(n)), "dmrgx_run_dmrg")
+    h = C.c_void_p()
+    check(lib().dmrgx_run_dmrg_state(C.byref(cfg), C.byref(st), log.ctypes.data_as(C.POINTER(C.c_double)), cap,
+                                     C.byref(n), C.byref(h) if keep_state else None), "dmrgx_run_dmrg")
     stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
-    return RunResult(st.energy, stats, log[:n.value].copy())
+    state = State(h, model) if keep_state else None
+    return RunResult(st.energy, stats, log[:n.value].copy(), state)
 
 
 def encode_desc(dim, sym, dirs=None):

@@ -54,6 +54,25 @@ typedef struct dmrgx_run_stats {
 int dmrgx_run_dmrg(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, double* step_log, int log_cap,
                    int* n_steps);
 
+/* ---- finished states and observables ------------------------------------------
+ * dmrgx_run_dmrg_state runs like dmrgx_run_dmrg and also returns the final MPS
+ * (device-resident, the centre absorbed as mps::prep_calc does,
+ * src/mps.cpp:33-54).  The observables are the reference's calc functions
+ * (src/functions.cpp:394-511), out = {re, im}:
+ *   dmrgx_state_norm:    calc(mps)        = <a|a>
+ *   dmrgx_state_overlap: calc(mps, mps)   = <b|a>  (same model, chain length)
+ *   dmrgx_state_expect:  calc(mps, mpo)   = <a|H(model, p)|a>, H built by the
+ *                        model builder with couplings p (same symmetry as a);
+ *                        linear builders give Hellmann-Feynman derivatives,
+ *                        e.g. XYZ with p = {0, 0, 1} is dH/dJz. */
+typedef struct dmrgx_state dmrgx_state;
+int dmrgx_run_dmrg_state(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, double* step_log, int log_cap,
+                         int* n_steps, dmrgx_state** state);
+int dmrgx_state_norm(const dmrgx_state* a, double* out);
+int dmrgx_state_overlap(const dmrgx_state* a, const dmrgx_state* b, double* out);
+int dmrgx_state_expect(const dmrgx_state* a, int model, const double* p, double* out);
+int dmrgx_state_destroy(dmrgx_state* a);
+
 /* Synthetic two-site operator with the block structure of a Z_S model at bond
  * sectors `bond` (n_sec entries), MPO bond D, physical sectors of dimension 1,
  * every sector combination present, uniform(-1,1) entries from `seed`.  The

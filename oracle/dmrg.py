@@ -330,3 +330,57 @@ class Runner:
             for i in range(L - 3, L // 2 - 1, -1):
                 e = self.update_two(i - 1, i, cutoff)
         return e
+
+
+# ---------------------------------------------------------------------------
+# Observables: mps::prep_calc and the three calc functions.
+
+
+def prepared(mps: MPS):
+    """mps::prep_calc (src/mps.cpp:33-54) on a copy: the centre singular
+    values absorbed into the site at the cut; returns the site tensors."""
+    st = list(mps.state)
+    S = mps.singular.copy()
+    site = st[mps.cut]
+    st[mps.cut] = QTensor().contract_gemm(S if site.dtype == np.float64 else S.comp(), site)
+    return st
+
+
+def calc_overlap(l_st, r_st, fermion=True):
+    """calc(mps, mps) = <r|l> (src/functions.cpp:428-460)."""
+    n = len(l_st)
+    bra = r_st[0].copy()
+    bra.conjugate()
+    left = QTensor().contract_gemm(bra, l_st[0], "T", "N", 1)
+    for i in range(1, n - 1):
+        end = QTensor().contract_gemm(left, l_st[i], "N", "N", 1)
+        bra = r_st[i].copy()
+        bra.conjugate()
+        left = QTensor().contract_gemm(bra, end, "T", "N", 2)
+    bra = r_st[n - 1].copy()
+    bra.conjugate()
+    end = QTensor().contract_gemm(bra, l_st[n - 1], "N", "T", 1)
+    return left.trace(end, fermion)
+
+
+def calc_expect(st, mpo, fermion=True):
+    """calc(mps, mpo) = <m|O|m> (src/functions.cpp:467-511): the transfer
+    matrix with the MPO leg, contracted with change = true at every interior
+    site, simplified, and closed by the graded trace."""
+    n = len(st)
+    end = QTensor().contract_gemm(mpo[0], st[0], "N", "N", 1)
+    tmp = st[0].copy()
+    tmp.conjugate()
+    left = QTensor().contract_gemm(tmp, end, "T", "N", 1)
+    for i in range(1, n - 1):
+        tmp = QTensor().contract_index(left, 1, mpo[i], 0, True, fermion)
+        left = tmp.exchange(3, 4)
+        end = QTensor().contract_gemm(left, st[i], "N", "N", 2)
+        tmp = st[i].copy()
+        tmp.conjugate()
+        left = QTensor().contract_gemm(tmp, end, "T", "N", 2).simplify()
+    store = QTensor().contract_gemm(mpo[n - 1], st[n - 1], "N", "T", 1)
+    tmp = st[n - 1].copy()
+    tmp.conjugate()
+    end = QTensor().contract_gemm(tmp, store, "N", "N", 1).simplify()
+    return left.trace(end, fermion)

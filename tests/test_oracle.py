@@ -128,3 +128,32 @@ def test_svd_selection_rule_pure_function_of_s():
         got = [U.dim[1][0], U.dim[1][1]]
         assert got == want, (sv, cutoff, got, want)
         assert np.allclose(s, sorted(np.concatenate([sv, sv]), reverse=True))
+
+
+def test_oracle_calc_physical_xyz_observables():
+    """calc(mps) = 1, calc(mps, mpo) = E0 and the Hellmann-Feynman derivative
+    <sum sz sz> = dE0/dJz by central differences of ED (XYZ, physical rule:
+    no graded signs)."""
+    from oracle.dmrg import calc_expect, calc_overlap, prepared
+    L = 8
+    mpo, n, ferm = models.make_model("xyz", L, **XYZ)
+    R = Runner(mpo, n, ferm, sign_rule="physical")
+    e = R.dmrg2(32, 2)
+    st = prepared(R.mps)
+    assert abs(calc_overlap(st, st, False) - 1) < 1e-10
+    assert abs(calc_expect(st, mpo, False) - e) < 1e-9
+    dz, _, _ = models.make_model("xyz", L, jx=0.0, jy=0.0, jz=1.0)
+    hf = np.real(calc_expect(st, dz, False))
+    d = 1e-4
+    fd = (exact.xyz_e0(L, 1.0, 0.8, 0.5 + d)[0] - exact.xyz_e0(L, 1.0, 0.8, 0.5 - d)[0]) / (2 * d)
+    assert abs(hf - fd) < 1e-6
+
+
+def test_oracle_calc_fermionic_norm():
+    """The graded calc(mps) of a Kitaev ground state is 1."""
+    from oracle.dmrg import calc_overlap, prepared
+    mpo, n, ferm = models.make_model("kitaev", 8, **KITAEV)
+    R = Runner(mpo, n, ferm)
+    R.dmrg2(16, 2)
+    st = prepared(R.mps)
+    assert abs(calc_overlap(st, st, True) - 1) < 1e-10
