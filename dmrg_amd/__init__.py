@@ -26,7 +26,7 @@ EXPORTS = [
     "dmrgx_init", "dmrgx_finalize", "dmrgx_last_error", "dmrgx_device_info", "dmrgx_malloc",
     "dmrgx_free", "dmrgx_memcpy_h2d", "dmrgx_memcpy_d2h", "dmrgx_memset_zero", "dmrgx_synchronize",
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
-    "dmrgx_heff_destroy", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
+    "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time",
 ]
 
@@ -74,6 +74,7 @@ def _declare(lib):
         "dmrgx_heff_flops": ([vp], C.c_double),
         "dmrgx_heff_time": ([vp, vp, vp, C.c_int, dp], C.c_int),
         "dmrgx_heff_destroy": ([vp], C.c_int),
+        "dmrgx_heff_adjoint": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "dmrgx_eigsh": ([vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, vp, dp, vp, vp, C.c_int,
                          C.POINTER(C.c_int)], C.c_int),
         "dmrgx_block_svd": ([C.c_int, C.c_int, i32p, i32p, vp, i64p, vp, i64p, vp, i64p, vp, i64p], C.c_int),
@@ -189,6 +190,12 @@ class Heff:
 
     def apply(self, x: DeviceArray, y: DeviceArray):
         check(lib().dmrgx_heff_apply(self.h, x.ptr, y.ptr), "dmrgx_heff_apply")
+
+    def adjoint(self, psi: DeviceArray, g_out: DeviceArray, g_psi=None, g_renv=None, g_lenv=None):
+        """Cotangents of psi and of the two environments for a cotangent of the output."""
+        ptr = lambda a: a.ptr if a is not None else None
+        check(lib().dmrgx_heff_adjoint(self.h, psi.ptr, g_out.ptr, ptr(g_psi), ptr(g_renv), ptr(g_lenv)),
+              "dmrgx_heff_adjoint")
 
     def time(self, x: DeviceArray, y: DeviceArray, iters: int):
         """(ms per apply, ms R-pass kernel, ms L-pass kernel, R flops, L flops)"""

@@ -9,6 +9,7 @@
 #include "gemm.hpp"
 #include "heff.hpp"
 #include "lanczos.hpp"
+#include "tensor_ops.hpp"
 
 namespace dmrgx {
 
@@ -88,6 +89,10 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     DMRGX_CHECK(g.a_off + (int64_t)g.M * g.K <= lenv_len, "heff: lmap block past the end of lenv");
   }
   const bool planes = prec == PREC_BF16X3;
+  h->rtasks = rt_;
+  h->ltasks = lt_;
+  h->renv_len = renv_len;
+  h->lenv_len = lenv_len;
   const auto rzero = uncovered(rt_, n_store), lzero = uncovered(lt_, n_psi);
   if (planes) {
     // every operand block gets rows of a multiple of 8 bf16 elements
@@ -144,6 +149,118 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
   }
   DMRGX_HIP(hipStreamSynchronize(s));
   return h;
+}
+
+namespace {
+
+// gA = g conj(B) (M x K): C' = A'(M x N) B'(N x K), 'N','N'
+std::vector<GemmTask> adj_tasks_a(const std::vector<GemmTask>& t) {
+  std::vector<GemmTask> o;
+  for (auto& g : t) {
+    GemmTask a{};
+    a.c_off = g.a_off;
+    a.a_off = g.c_off;
+    a.b_off = g.b_off;
+    a.M = g.M;
+    a.N = g.K;
+    a.K = g.N;
+    a.alpha = g.alpha;
+    a.beta = 0;
+    o.push_back(a);
+  }
+  return o;
+}
+
+// gB = g^T conj(A) (N x K): C' = A'^T B' with A' = g stored M x N ('T') and B' = conj(A) stored M x K
+std::vector<GemmTask> adj_tasks_b(const std::vector<GemmTask>& t) {
+  std::vector<GemmTask> o;
+  for (auto& g : t) {
+    GemmTask b{};
+    b.c_off = g.b_off;
+    b.a_off = g.c_off;
+    b.b_off = g.a_off;
+    b.M = g.N;
+    b.N = g.K;
+    b.K = g.M;
+    b.alpha = g.alpha;
+    b.beta = 0;
+    o.push_back(b);
+  }
+  return o;
+}
+
+}  // namespace
+
+void HeffPlan::adjoint(const double* psi, const double* g_out, double* g_psi, double* g_renv, double* g_lenv,
+                       hipStream_t s) {
+  const bool cx = dtype == DT_C64;
+  const int c = cx ? 2 : 1;
+  const int aprec = prec == PREC_FP64 ? PREC_FP64 : PREC_FP32;
+  const size_t eb = (aprec == PREC_FP64 ? 8 : 4) * c;
+  if (!ad_gr) {
+    auto ta = adj_tasks_a(rtasks), tb = adj_tasks_b(rtasks), la = adj_tasks_a(ltasks), lb = adj_tasks_b(ltasks);
+    ad_gr = make_gemm_plan(ta, dtype, aprec, 'N', 'N', uncovered(ta, renv_len));
+    ad_gp = make_gemm_plan(tb, dtype, aprec, 'T', 'N', uncovered(tb, n_psi));
+    ad_gl = make_gemm_plan(la, dtype, aprec, 'N', 'N', uncovered(la, lenv_len));
+    ad_gs = make_gemm_plan(lb, dtype, aprec, 'T', 'N', uncovered(lb, n_store));
+    if (aprec != PREC_FP64 && prec == PREC_BF16X3)
+      ad_fr = make_gemm_plan(rtasks, dtype, PREC_FP32, 'N', 'T', uncovered(rtasks, n_store));
+  }
+  // operands in the adjoint precision
+  auto as_prec = [&](const double* x, int64_t n, std::unique_ptr<DevBuf>& keep) -> const void* {
+    if (aprec == PREC_FP64) return x;
+    keep = std::make_unique<DevBuf>(std::max<int64_t>(n, 1) * eb);
+    convert_f64_to_f32(x, (float*)keep->ptr, n * c, s);
+    return keep->ptr;
+  };
+  std::unique_ptr<DevBuf> kp, kg, kr, kl;
+  const void* P = as_prec(psi, n_psi, kp);
+  const void* G = as_prec(g_out, n_psi, kg);
+  const void* R = aprec == PREC_FP64 ? renv : (renv_lo ? renv_lo->ptr : as_prec((const double*)renv, renv_len, kr));
+  const void* Lv = aprec == PREC_FP64 ? lenv : (lenv_lo ? lenv_lo->ptr : as_prec((const double*)lenv, lenv_len, kl));
+  // store = R pass of psi
+  DevBuf st(std::max<int64_t>(n_store, 1) * eb);
+  run_gemm_plan(ad_fr ? *ad_fr : *rplan, R, P, st.ptr, s);
+  // conjugated copies of the B operands (complex only)
+  auto conj_copy = [&](const void* x, int64_t n) {
+    auto b = std::make_unique<DevBuf>(std::max<int64_t>(n, 1) * eb);
+    copy_d2d(b->ptr, x, n * eb, s);
+    if (cx) {
+      if (aprec == PREC_FP64) conj_inplace(b->ptr, n, s);
+      else conj_inplace_f32(b->ptr, n, s);
+    }
+    return b;
+  };
+  DevBuf gst(std::max<int64_t>(n_store, 1) * eb);
+  {
+    auto cl = conj_copy(Lv, lenv_len);
+    run_gemm_plan(*ad_gs, G, cl->ptr, gst.ptr, s);  // g_store = g^T conj(lenv)
+  }
+  auto out64 = [&](const DevBuf& src, double* dst, int64_t n) {
+    if (aprec == PREC_FP64) copy_d2d(dst, src.ptr, n * 8 * c, s);
+    else convert_f32_to_f64((const float*)src.ptr, dst, n * c, s);
+  };
+  if (g_lenv) {
+    auto cs = conj_copy(st.ptr, n_store);
+    DevBuf o(std::max<int64_t>(lenv_len, 1) * eb);
+    run_gemm_plan(*ad_gl, G, cs->ptr, o.ptr, s);  // g_lenv = g conj(store)
+    out64(o, g_lenv, lenv_len);
+    DMRGX_HIP(hipStreamSynchronize(s));
+  }
+  if (g_psi) {
+    auto cr = conj_copy(R, renv_len);
+    DevBuf o(std::max<int64_t>(n_psi, 1) * eb);
+    run_gemm_plan(*ad_gp, gst.ptr, cr->ptr, o.ptr, s);  // g_psi = g_store^T conj(renv)
+    out64(o, g_psi, n_psi);
+    DMRGX_HIP(hipStreamSynchronize(s));
+  }
+  if (g_renv) {
+    auto cp = conj_copy(P, n_psi);
+    DevBuf o(std::max<int64_t>(renv_len, 1) * eb);
+    run_gemm_plan(*ad_gr, gst.ptr, cp->ptr, o.ptr, s);  // g_renv = g_store conj(psi)
+    out64(o, g_renv, renv_len);
+  }
+  DMRGX_HIP(hipStreamSynchronize(s));
 }
 
 void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev) {
@@ -257,6 +374,15 @@ int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out) {
   DMRGX_CHECK(plan && plan->p, "heff_apply: null plan");
   plan->p->apply((const double*)in, (double*)out, rt().stream);
   DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  CAPI_END
+}
+
+int dmrgx_heff_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, void* g_psi, void* g_renv,
+                       void* g_lenv) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p && psi && g_out, "heff_adjoint: null arguments");
+  plan->p->adjoint((const double*)psi, (const double*)g_out, (double*)g_psi, (double*)g_renv, (double*)g_lenv,
+                   rt().stream);
   CAPI_END
 }
 

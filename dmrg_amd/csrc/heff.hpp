@@ -25,8 +25,21 @@ struct HeffPlan {
   bool store_direct = false;  // the R pass writes store's planes itself
   std::unique_ptr<GemmPlan> rplan, lplan;
   double flops = 0;
+  int64_t renv_len = 0, lenv_len = 0;
+  std::vector<GemmTask> rtasks, ltasks;  // the get_map tasks (element offsets of the plain layout)
+  // adjoint plans, built on the first adjoint call: gradients of the env
+  // blocks (gR, gL) and of the pass inputs (gP for psi, gS for store), plus an
+  // FP32 R pass when the forward plan works on bf16 planes
+  std::unique_ptr<GemmPlan> ad_gr, ad_gl, ad_gp, ad_gs, ad_fr;
   // ev: optional {R begin, R end, L begin, L end} events around the two GEMM launches
   void apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev = nullptr);
+  // Adjoint of out = H_eff(psi; renv, lenv): for a cotangent g of out, the
+  // cotangents of psi, renv and lenv (each may be null): for every task
+  // C += alpha A B^T, gA += alpha g conj(B) and gB += alpha g^T conj(A), so
+  // that Re<g, dout> = Re<g_psi, dpsi> + Re<g_renv, drenv> + Re<g_lenv, dlenv>.
+  // FP64 for an FP64 plan, FP32 otherwise; outputs are FP64, overwritten.
+  void adjoint(const double* psi, const double* g_out, double* g_psi, double* g_renv, double* g_lenv,
+               hipStream_t s);
 };
 
 // get_map tuples -> GEMM tasks (M = blk_size / col, N = row, K = col).

@@ -254,6 +254,18 @@ std::vector<double> box_maxabs(int dtype, const void* src, const std::vector<Box
   return out;
 }
 
+__global__ void k_conj_f32(float* __restrict__ p, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += st) p[2 * i + 1] = -p[2 * i + 1];
+}
+
+void conj_inplace_f32(void* p, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_conj_f32, dim3((unsigned)std::min<int64_t>(4096, blocks_for(n))), dim3(256), 0, s, (float*)p, n);
+  DMRGX_HIP(hipGetLastError());
+}
+
 void conj_inplace(void* p, int64_t n, hipStream_t s) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_conj, dim3((unsigned)std::min<int64_t>(4096, blocks_for(n))), dim3(256), 0, s, (double*)p, n);

@@ -37,6 +37,7 @@ void copy_boxes(int dtype, const void* src, void* dst, const std::vector<BoxTask
 std::vector<double> box_maxabs(int dtype, const void* src, const std::vector<BoxTask>& t, hipStream_t s);
 
 void conj_inplace(void* p, int64_t n, hipStream_t s);  // complex elements
+void conj_inplace_f32(void* p, int64_t n, hipStream_t s);  // complex FP32 elements
 void scale_real(double* p, int64_t nreal, double a, hipStream_t s);
 
 }  // namespace dmrgx

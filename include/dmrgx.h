@@ -69,6 +69,16 @@ int dmrgx_heff_create(dmrgx_heff** plan, int dtype, int prec,
                       int n_lblk, const int32_t* lmap, int n_ltask,
                       int64_t n_psi, int64_t n_store);
 int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out);
+/* Adjoint of out = H_eff(psi) with respect to psi and the two environments
+ * (no counterpart in the reference, which has no derivatives): for a
+ * cotangent g_out of out, writes g_psi (n_psi), g_renv and g_lenv (the
+ * environments' lengths, same block layout) such that
+ * Re<g_out, d out> = Re<g_psi, d psi> + Re<g_renv, d renv> + Re<g_lenv, d lenv>
+ * (<a, b> = sum conj(a) b).  Any of g_psi / g_renv / g_lenv may be NULL.
+ * FP64 for FP64 plans, FP32 MFMA for FP32 / BF16X3 plans; all buffers FP64
+ * (complex interleaved), device pointers. */
+int dmrgx_heff_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, void* g_psi, void* g_renv,
+                       void* g_lenv);
 /* algorithmic flops of one apply: sum over tasks of c*M*N*K (c = 2 real, 8 complex) */
 double dmrgx_heff_flops(const dmrgx_heff* plan);
 int dmrgx_heff_destroy(dmrgx_heff* plan);
