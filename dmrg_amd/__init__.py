@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -31,6 +32,7 @@ EXPORTS = [
 ]
 
 _lib = None
+TORCH_LOADED_FIRST = False
 
 
 class DmrgxError(RuntimeError):
@@ -49,6 +51,10 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     _declare(lib)
     if path is None:
         _lib = lib
+        # libdmrgx binds to whichever HIP runtime the process loaded first
+        # (torch ships its own under the same soname); see dmrg_amd.autograd
+        global TORCH_LOADED_FIRST
+        TORCH_LOADED_FIRST = "torch" in sys.modules
     return lib
 
 

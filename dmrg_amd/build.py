@@ -53,6 +53,35 @@ def build(verbose: bool = True) -> Path:
     return OUT
 
 
+TORCH_SRC = PKG / "torch_ext" / "torch_heff.cpp"
+TORCH_OUT = PKG / "dmrgx_torch.so"
+
+
+def build_torch(verbose: bool = True) -> Path:
+    """The torch::autograd::Function module (``dmrg_amd/dmrgx_torch.so``),
+    linked against libdmrgx.so in the same directory."""
+    from torch.utils import cpp_extension
+
+    bdir = OBJ / "torch"
+    bdir.mkdir(parents=True, exist_ok=True)
+    cpp_extension.load(
+        name="dmrgx_torch", sources=[str(TORCH_SRC)], build_directory=str(bdir),
+        extra_include_paths=[str(PKG.parent / "include")],
+        extra_cflags=["-O2"],
+        # $ORIGIN escaped for ninja ($$) and for the shell it runs (\\$)
+        extra_ldflags=[f"-L{PKG}", "-ldmrgx", "-Wl,-rpath,\\$$ORIGIN", "-Wl,-rpath,\\$$ORIGIN/../.."],
+        verbose=False, is_python_module=True)
+    built = bdir / "dmrgx_torch.so"
+    tmp = TORCH_OUT.with_suffix(".so.tmp")
+    tmp.write_bytes(built.read_bytes())
+    os.replace(tmp, TORCH_OUT)
+    if verbose:
+        print(f"built {TORCH_OUT}")
+    return TORCH_OUT
+
+
 if __name__ == "__main__":
     build()
+    if "--torch" in sys.argv:
+        build_torch()
     sys.exit(0)

@@ -14,10 +14,10 @@ using namespace dmrgx;
 
 // a finished run's MPS (centre absorbed), one of r / c set
 struct dmrgx_state {
-  int model = 0, L = 0;
+  int model = 0, L = 0, sector = 0;
   bool signs = false;  // graded signs on: fermionic model, or the reference's sign rule
-  std::unique_ptr<mps<double>> r;
-  std::unique_ptr<mps<cplx>> c;
+  std::unique_ptr<mps<double>> r, raw_r;  // prepared (centre absorbed) and as left by the run
+  std::unique_ptr<mps<cplx>> c, raw_c;
 };
 
 #define CAPI_BEGIN try {
@@ -63,6 +63,26 @@ mpo<double> model_mpo_r(int model, const double* p, int L) {
   DMRGX_CHECK(model == DMRGX_MODEL_POTTS2, "unknown real model");
   qtensor<double> H = H_Potts2(p[0], p[1]);
   mpo<double> m(L, H);
+  m.edge();
+  return m;
+}
+
+template <typename T>
+mps<T> without_stores(const mps<T>& m) {
+  mps<T> r;
+  r.cut_ = m.cut_;
+  r.state_ = m.state_;
+  r.singular_ = m.singular_;
+  r.store_.assign(m.state_.size(), qtensor<T>());
+  return r;
+}
+
+// the MPO of H(p + dp) - H(p) (the builders are linear in the couplings)
+template <typename T>
+mpo<T> mpo_difference(mpo<T>& a, mpo<T>& b, int L) {
+  qtensor<T> d;
+  d.plus(a.operators_[0], b.operators_[0], T(1), T(-1));
+  mpo<T> m(L, d);
   m.edge();
   return m;
 }
@@ -210,17 +230,24 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double*
     out->model = c->model;
     out->L = c->L;
     out->signs = model_fermion(c->model) || c->sign_rule == DMRGX_SIGNS_REFERENCE;
+    out->sector = c->sector;
   }
   if (model_complex(c->model)) {
     mpo<cplx> m = model_mpo_c(c->model, c->p, c->L);
     Engine<cplx> eng(m, make_cfg(c, model_fermion(c->model)));
     run_engine(eng, c, st, step_log, log_cap, n_steps);
-    if (out) out->c = std::make_unique<mps<cplx>>(prepared(eng.mps_));
+    if (out) {
+      out->c = std::make_unique<mps<cplx>>(prepared(eng.mps_));
+      out->raw_c = std::make_unique<mps<cplx>>(without_stores(eng.mps_));
+    }
   } else {
     mpo<double> m = model_mpo_r(c->model, c->p, c->L);
     Engine<double> eng(m, make_cfg(c, model_fermion(c->model)));
     run_engine(eng, c, st, step_log, log_cap, n_steps);
-    if (out) out->r = std::make_unique<mps<double>>(prepared(eng.mps_));
+    if (out) {
+      out->r = std::make_unique<mps<double>>(prepared(eng.mps_));
+      out->raw_r = std::make_unique<mps<double>>(without_stores(eng.mps_));
+    }
   }
   if (state) *state = out.release();
   CAPI_END
@@ -272,6 +299,26 @@ int dmrgx_state_expect(const dmrgx_state* a, int model, const double* p, double*
     out[0] = calc(*a->r, m, a->signs);
     out[1] = 0;
   }
+  CAPI_END
+}
+
+int dmrgx_state_derivative(const dmrgx_state* a, const double* p, const double* dp, int prec, double* out) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && p && dp && out, "state_derivative: null arguments");
+  DMRGX_CHECK(prec == DMRGX_FP64 || prec == DMRGX_FP32, "state_derivative: prec must be FP64 or FP32");
+  double q[4];
+  for (int i = 0; i < 4; ++i) q[i] = p[i] + dp[i];
+  double e = 0;
+  if (a->raw_c) {
+    mpo<cplx> W = model_mpo_c(a->model, p, a->L), Wq = model_mpo_c(a->model, q, a->L);
+    mpo<cplx> dW = mpo_difference(Wq, W, a->L);
+    out[0] = centre_derivative(*a->raw_c, W, dW, a->sector, a->signs, prec, &e);
+  } else {
+    mpo<double> W = model_mpo_r(a->model, p, a->L), Wq = model_mpo_r(a->model, q, a->L);
+    mpo<double> dW = mpo_difference(Wq, W, a->L);
+    out[0] = centre_derivative(*a->raw_r, W, dW, a->sector, a->signs, prec, &e);
+  }
+  out[1] = e;
   CAPI_END
 }
 

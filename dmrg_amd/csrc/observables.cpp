@@ -2,6 +2,11 @@
 // contracted site by site on the device through the qtensor mirror.
 #include "observables.hpp"
 
+#include <map>
+
+#include "heff.hpp"
+#include "lanczos.hpp"
+
 namespace dmrgx {
 
 template <>
@@ -88,6 +93,156 @@ T calc(const mps<T>& m, mpo<T>& o, bool fermion) {
   end = end.simplify();
   return left.trace(end, fermion);
 }
+
+namespace {
+
+// left store of site k from the one of k-1 and the site tensor (the
+// env update of update_two / move2right, src/core.cpp:266-268)
+template <typename T>
+qtensor<T> lenv_of(const qtensor<T>& store, qtensor<T>& W) {
+  qtensor<T> t;
+  t.contract(store, 1, W, 0);
+  return t.exchange(3, 4);
+}
+template <typename T>
+qtensor<T> renv_of(const qtensor<T>& store, qtensor<T>& W) {
+  qtensor<T> ope = W.exchange(0, 2);
+  qtensor<T> t;
+  t.contract(store, 1, ope, 0, true);
+  return t.exchange(0, 1);
+}
+template <typename T>
+qtensor<T> left_sandwich(const qtensor<T>& env, const qtensor<T>& U, int num) {
+  qtensor<T> vec, c = U, out;
+  vec.contract(env, U, 'N', 'N', num);
+  c.conjugate();
+  out.contract(c, vec, 'T', 'N', num);
+  return out;
+}
+template <typename T>
+qtensor<T> right_sandwich(const qtensor<T>& env, const qtensor<T>& V, int num) {
+  qtensor<T> vec, c = V, out;
+  vec.contract(env, V, 'N', 'T', num);
+  c.conjugate();
+  out.contract(c, vec, 'N', 'N', num);
+  return out;
+}
+template <typename T>
+qtensor<T> add(const qtensor<T>& a, const qtensor<T>& b) {
+  qtensor<T> r;
+  r.plus(a, b, T(1), T(1));
+  return r;
+}
+
+// Re sum conj(g) x over the blocks of x, g laid out as the blocks of ref
+template <typename T>
+double block_dot(const qtensor<T>& ref, const void* g, const qtensor<T>& x) {
+  auto code = [&](const vector<int>& s) {
+    int64_t c = 0;
+    for (size_t j = 0; j < ref.dim_.size(); ++j) c = c * ref.dim_[j].size() + s[j];
+    return c;
+  };
+  std::map<int64_t, int> at;
+  for (int i = 0; i < ref.nblocks(); ++i) at[code(ref.sym_[i])] = i;
+  const bool cx = dtype_of<T>::value == DT_C64;
+  double acc = 0;
+  for (int j = 0; j < x.nblocks(); ++j) {
+    auto it = at.find(code(x.sym_[j]));
+    if (it == at.end()) continue;  // the energy does not depend on a block the environment lacks
+    double o[2];
+    dev_dot((const double*)g + (cx ? 2 : 1) * ref.off_[it->second], (const double*)x.block(j), x.bsize(j), cx, o,
+            rt().stream);
+    acc += o[0];
+  }
+  return acc;
+}
+
+}  // namespace
+
+template <typename T>
+double centre_derivative(const mps<T>& m, mpo<T>& W, mpo<T>& dW, int sector, bool alpha, int prec, double* energy) {
+  const int n = m.size();
+  const int l = m.cut_ - 1, r = m.cut_;
+  DMRGX_CHECK(l >= 1 && r <= n - 2, "centre_derivative: the centre must have a site on each side");
+  // stores from the two ends and their tangents along dW
+  qtensor<T> S = left_sandwich(W[0], m.state_[0], 1), dS = left_sandwich(dW[0], m.state_[0], 1);
+  for (int k = 1; k < l; ++k) {
+    qtensor<T> e = lenv_of(S, W[k]);
+    qtensor<T> de = add(lenv_of(dS, W[k]), lenv_of(S, dW[k]));
+    S = left_sandwich(e, m.state_[k], 2);
+    dS = left_sandwich(de, m.state_[k], 2);
+  }
+  qtensor<T> Lv = lenv_of(S, W[l]), dL = add(lenv_of(dS, W[l]), lenv_of(S, dW[l]));
+  qtensor<T> Q = right_sandwich(W[n - 1], m.state_[n - 1], 1), dQ = right_sandwich(dW[n - 1], m.state_[n - 1], 1);
+  for (int k = n - 2; k > r; --k) {
+    qtensor<T> e = renv_of(Q, W[k]);
+    qtensor<T> de = add(renv_of(dQ, W[k]), renv_of(Q, dW[k]));
+    Q = right_sandwich(e, m.state_[k], 2);
+    dQ = right_sandwich(de, m.state_[k], 2);
+  }
+  qtensor<T> Rv = renv_of(Q, W[r]), dR = add(renv_of(dQ, W[r]), renv_of(Q, dW[r]));
+  // the two-site state U_l S V_r in the block order of the eigensolver
+  vector<vector<int>> dim, sym;
+  generate_dim_sym(dim, sym, Lv, 2, Rv, 2, sector);
+  const int64_t d = get_dimension(dim, sym);
+  qtensor<T> us, two;
+  us.contract(m.state_[l], centre_as<T>(m.singular_));
+  two.contract(us, m.state_[r]);
+  qtensor<T> psi;
+  psi.dim_ = dim;
+  psi.sym_ = sym;
+  psi.dir_.assign(dim.size(), 0);
+  psi.layout();
+  psi.allocate();
+  {
+    auto code = [&](const vector<int>& s) {
+      int64_t c = 0;
+      for (size_t j = 0; j < dim.size(); ++j) c = c * dim[j].size() + s[j];
+      return c;
+    };
+    std::map<int64_t, int> at;
+    for (int i = 0; i < two.nblocks(); ++i) at[code(two.sym_[i])] = i;
+    for (int b = 0; b < psi.nblocks(); ++b) {
+      auto it = at.find(code(psi.sym_[b]));
+      if (it == at.end()) fill_zero(psi.block(b), psi.bsize(b) * sizeof(T), rt().stream);
+      else {
+        DMRGX_CHECK(two.bsize(it->second) == psi.bsize(b), "centre_derivative: two-site block shapes differ");
+        copy_d2d(psi.block(b), two.block(it->second), psi.bsize(b) * sizeof(T), rt().stream);
+      }
+    }
+  }
+  // H_eff plan of (Lv, Rv) and the adjoint for the output cotangent psi
+  vector<vector<int>> rmap, lmap, dim_ret, sym_ret, dim2;
+  DMRGX_CHECK(Rv.get_map(rmap, dim_ret, sym_ret, dim, sym, 'N', 'T', 2, 0, false, alpha), "get_map (rmap)");
+  const int64_t d_store = get_dimension(dim_ret, sym_ret);
+  vector<vector<int>> psym = sym;
+  DMRGX_CHECK(Lv.get_map(lmap, dim2, psym, dim_ret, sym_ret, 'N', 'T', 3, 0, true, alpha), "get_map (lmap)");
+  vector<int32_t> rm, lm;
+  for (auto& t : rmap) rm.insert(rm.end(), t.begin(), t.end());
+  for (auto& t : lmap) lm.insert(lm.end(), t.begin(), t.end());
+  vector<int64_t> rsz(Rv.nblocks()), lsz(Lv.nblocks());
+  for (int b = 0; b < Rv.nblocks(); ++b) rsz[b] = Rv.bsize(b);
+  for (int b = 0; b < Lv.nblocks(); ++b) lsz[b] = Lv.bsize(b);
+  const int dt = dtype_of<T>::value;
+  auto plan = make_heff(dt, prec, Rv.data(), Rv.off_.data(), rsz.data(), Rv.nblocks(), Rv.total_, rm.data(),
+                        (int)rmap.size(), Lv.data(), Lv.off_.data(), lsz.data(), Lv.nblocks(), Lv.total_, lm.data(),
+                        (int)lmap.size(), d, d_store);
+  hipStream_t s = rt().stream;
+  const bool cx = dt == DT_C64;
+  DevBuf out(std::max<int64_t>(d, 1) * sizeof(T)), gR(std::max<int64_t>(Rv.total_, 1) * sizeof(T)),
+      gL(std::max<int64_t>(Lv.total_, 1) * sizeof(T));
+  plan->apply((const double*)psi.data(), (double*)out.ptr, s);
+  double e[2], nn[2];
+  dev_dot((const double*)psi.data(), (const double*)out.ptr, d, cx, e, s);
+  dev_dot((const double*)psi.data(), (const double*)psi.data(), d, cx, nn, s);
+  plan->adjoint((const double*)psi.data(), (const double*)psi.data(), nullptr, (double*)gR.ptr, (double*)gL.ptr, s);
+  const double der = block_dot(Lv, gL.ptr, dL) + block_dot(Rv, gR.ptr, dR);
+  if (energy) *energy = e[0] / nn[0];
+  return der / nn[0];
+}
+
+template double centre_derivative(const mps<double>&, mpo<double>&, mpo<double>&, int, bool, int, double*);
+template double centre_derivative(const mps<cplx>&, mpo<cplx>&, mpo<cplx>&, int, bool, int, double*);
 
 template mps<double> prepared(const mps<double>&);
 template mps<cplx> prepared(const mps<cplx>&);

@@ -25,4 +25,17 @@ T calc(const mps<T>& l, const mps<T>& r, bool fermion = true);
 template <typename T>
 T calc(const mps<T>& m, mpo<T>& o, bool fermion = true);
 
+// Derivative of the two-site energy E = <psi|H_eff(W)|psi> / <psi|psi> at the
+// centre of a finished (unprepared) MPS along an MPO direction dW (the MPO
+// of the coupling change; the model builders are linear in the couplings).
+// The environments and their tangents along dW are recomputed from the MPS
+// site by site (no stored environments), and the derivative is
+// Re(<g_L, dL> + <g_R, dR>) with g_L, g_R the adjoint cotangents of the two
+// environments for the output cotangent psi.  `alpha` is get_map's sign
+// switch; `prec` the arithmetic of the adjoint (FP64, or FP32 MFMA).
+// Also returns E in *energy.
+template <typename T>
+double centre_derivative(const mps<T>& m, mpo<T>& W, mpo<T>& dW, int sector, bool alpha, int prec,
+                         double* energy);
+
 }  // namespace dmrgx

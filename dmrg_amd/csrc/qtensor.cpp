@@ -1003,29 +1003,25 @@ qtensor<T>& qtensor<T>::plus(const qtensor<T>& A, const qtensor<T>& B, T alpha, 
     for (size_t j = 0; j < A.dim_.size(); ++j) c = c * A.dim_[j].size() + s[j];
     return c;
   };
-  vector<T> xa = A.values(), xb = B.values();
   std::map<int64_t, std::pair<int, int>> m;
   for (int i = 0; i < A.nblocks(); ++i) m[code(A.sym_[i])].first = i + 1;
   for (int i = 0; i < B.nblocks(); ++i) m[code(B.sym_[i])].second = i + 1;
-  vector<T> out;
+  vector<AxpbySeg> segs;
   for (auto& kv : m) {
     int ia = kv.second.first - 1, ib = kv.second.second - 1;
-    const vector<int>& s = ia >= 0 ? A.sym_[ia] : B.sym_[ib];
-    r.sym_.push_back(s);
-    int64_t n = ia >= 0 ? A.bsize(ia) : B.bsize(ib);
-    for (int64_t e = 0; e < n; ++e) {
-      T v = T(0);
-      if (ia >= 0) v += alpha * xa[A.off_[ia] + e];
-      if (ib >= 0) v += beta * xb[B.off_[ib] + e];
-      out.push_back(v);
-    }
+    r.sym_.push_back(ia >= 0 ? A.sym_[ia] : B.sym_[ib]);
   }
   r.layout();
   r.allocate();
-  if (!out.empty()) {
-    DMRGX_HIP(hipMemcpyAsync(r.data(), out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice, rt().stream));
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  int k = 0;
+  for (auto& kv : m) {
+    int ia = kv.second.first - 1, ib = kv.second.second - 1;
+    segs.push_back(AxpbySeg{r.off_[k], ia >= 0 ? A.off_[ia] : -1, ib >= 0 ? B.off_[ib] : -1, r.bsize(k)});
+    ++k;
   }
+  const cplx a(alpha), b(beta);
+  blocks_axpby(dtype_of<T>::value == DT_C64, A.data(), B.data(), r.data(), segs, r.total_, a.real(), a.imag(),
+               b.real(), b.imag(), rt().stream);
   *this = r;
   return *this;
 }

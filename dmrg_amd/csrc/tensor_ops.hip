@@ -254,6 +254,50 @@ std::vector<double> box_maxabs(int dtype, const void* src, const std::vector<Box
   return out;
 }
 
+// out[seg] = alpha A[seg] + beta B[seg] over block segments; a_off / b_off < 0
+// means that operand has no such block (it counts as zero)
+template <bool CPLX>
+__global__ void k_axpby_seg(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ O,
+                            const AxpbySeg* __restrict__ seg, int nseg, int64_t total, double ar, double ai,
+                            double br, double bi) {
+  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t st = (int64_t)gridDim.x * blockDim.x;
+  for (; e < total; e += st) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (seg[mid].out_off <= e) lo = mid;
+      else hi = mid - 1;
+    }
+    const AxpbySeg g = seg[lo];
+    const int64_t k = e - g.out_off;
+    if (CPLX) {
+      double xr = 0, xi = 0, yr = 0, yi = 0;
+      if (g.a_off >= 0) { xr = A[2 * (g.a_off + k)]; xi = A[2 * (g.a_off + k) + 1]; }
+      if (g.b_off >= 0) { yr = B[2 * (g.b_off + k)]; yi = B[2 * (g.b_off + k) + 1]; }
+      O[2 * e] = ar * xr - ai * xi + br * yr - bi * yi;
+      O[2 * e + 1] = ar * xi + ai * xr + br * yi + bi * yr;
+    } else {
+      const double x = g.a_off >= 0 ? A[g.a_off + k] : 0.0, y = g.b_off >= 0 ? B[g.b_off + k] : 0.0;
+      O[e] = ar * x + br * y;
+    }
+  }
+}
+
+void blocks_axpby(bool cplx, const void* A, const void* B, void* out, const std::vector<AxpbySeg>& segs,
+                  int64_t total, double ar, double ai, double br, double bi, hipStream_t s) {
+  if (total <= 0 || segs.empty()) return;
+  DevBuf d(segs.size() * sizeof(AxpbySeg));
+  DMRGX_HIP(hipMemcpyAsync(d.ptr, segs.data(), segs.size() * sizeof(AxpbySeg), hipMemcpyHostToDevice, s));
+  const unsigned g = (unsigned)std::min<int64_t>(4096, blocks_for(total));
+  if (cplx) hipLaunchKernelGGL(k_axpby_seg<true>, dim3(g), dim3(256), 0, s, (const double*)A, (const double*)B,
+                               (double*)out, (const AxpbySeg*)d.ptr, (int)segs.size(), total, ar, ai, br, bi);
+  else hipLaunchKernelGGL(k_axpby_seg<false>, dim3(g), dim3(256), 0, s, (const double*)A, (const double*)B,
+                          (double*)out, (const AxpbySeg*)d.ptr, (int)segs.size(), total, ar, ai, br, bi);
+  DMRGX_HIP(hipGetLastError());
+  DMRGX_HIP(hipStreamSynchronize(s));  // d is released at scope end
+}
+
 __global__ void k_conj_f32(float* __restrict__ p, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;

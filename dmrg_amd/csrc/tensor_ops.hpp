@@ -36,6 +36,12 @@ void copy_boxes(int dtype, const void* src, void* dst, const std::vector<BoxTask
 // max |x| over each source box (host result)
 std::vector<double> box_maxabs(int dtype, const void* src, const std::vector<BoxTask>& t, hipStream_t s);
 
+struct AxpbySeg {
+  int64_t out_off, a_off, b_off, len;  // element offsets; a_off / b_off < 0: block absent
+};
+// out = alpha A + beta B block by block (complex alpha, beta when cplx)
+void blocks_axpby(bool cplx, const void* A, const void* B, void* out, const std::vector<AxpbySeg>& segs,
+                  int64_t total, double ar, double ai, double br, double bi, hipStream_t s);
 void conj_inplace(void* p, int64_t n, hipStream_t s);  // complex elements
 void conj_inplace_f32(void* p, int64_t n, hipStream_t s);  // complex FP32 elements
 void scale_real(double* p, int64_t nreal, double a, hipStream_t s);

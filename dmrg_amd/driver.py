@@ -11,7 +11,7 @@ MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
 STEP_LOG_WIDTH = 19
 DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym",
                   "dmrgx_run_dmrg_state", "dmrgx_state_norm", "dmrgx_state_overlap", "dmrgx_state_expect",
-                  "dmrgx_state_destroy"]
+                  "dmrgx_state_destroy", "dmrgx_state_derivative"]
 
 
 class RunConfig(C.Structure):
@@ -51,6 +51,8 @@ def declare(lib):
     lib.dmrgx_state_overlap.restype = C.c_int
     lib.dmrgx_state_expect.argtypes = [C.c_void_p, C.c_int, dp, dp]
     lib.dmrgx_state_expect.restype = C.c_int
+    lib.dmrgx_state_derivative.argtypes = [C.c_void_p, dp, dp, C.c_int, dp]
+    lib.dmrgx_state_derivative.restype = C.c_int
     lib.dmrgx_state_destroy.argtypes = [C.c_void_p]
     lib.dmrgx_state_destroy.restype = C.c_int
     return lib
@@ -85,6 +87,16 @@ class State:
         from . import lib
         p = (C.c_double * 4)(*([float(x) for x in params] + [0.0] * (4 - len(params))))
         return self._call(lib().dmrgx_state_expect, self.h, MODELS[model or self.model], p)
+
+    def derivative(self, params, dparams, prec="fp64"):
+        """(dE0/dt along dparams at params, E0 of the centre) from the adjoint
+        H_eff kernels with recomputed environment tangents."""
+        from . import PREC_NAMES, check, lib
+        pad = lambda v: (C.c_double * 4)(*([float(x) for x in v] + [0.0] * (4 - len(v))))
+        out = np.zeros(2)
+        check(lib().dmrgx_state_derivative(self.h, pad(params), pad(dparams), PREC_NAMES[prec],
+                                           out.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_state_derivative")
+        return float(out[0]), float(out[1])
 
     def fidelity(self, other: "State") -> float:
         """|<b|a>| / sqrt(<a|a> <b|b>)"""

@@ -71,6 +71,14 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, do
 int dmrgx_state_norm(const dmrgx_state* a, double* out);
 int dmrgx_state_overlap(const dmrgx_state* a, const dmrgx_state* b, double* out);
 int dmrgx_state_expect(const dmrgx_state* a, int model, const double* p, double* out);
+/* dmrgx_state_derivative: d E0 / dt along the coupling direction dp at the
+ * couplings p (the state's own), from the adjoint H_eff kernels at the
+ * state's centre: the environments and their tangents along the MPO of
+ * H(p + dp) - H(p) are recomputed from the MPS site by site, and
+ * dE0/dt = Re(<g_L, dL> + <g_R, dR>) with g_L, g_R the environment
+ * cotangents for the output cotangent psi (Hellmann-Feynman at fixed MPS
+ * basis).  out = {dE0/dt, E0 of the centre}; prec DMRGX_FP64 or DMRGX_FP32. */
+int dmrgx_state_derivative(const dmrgx_state* a, const double* p, const double* dp, int prec, double* out);
 int dmrgx_state_destroy(dmrgx_state* a);
 
 /* Synthetic two-site operator with the block structure of a Z_S model at bond
