@@ -95,6 +95,21 @@ def configs_leg(reps=5, warmup=2):
             h.time(xd, yd, warmup)
             ms = h.time(xd, yd, reps)
             res[prec] = {"ms": round(ms[0], 3), "tflops": round(h.flops / (ms[0] * 1e-3) / 1e12, 2)}
+            if name == "xyz_m2048" and prec == "fp32":
+                # the adjoint (cotangents of psi and both environments): one R pass
+                # plus four gradient GEMM passes = 2.5 matvecs of algorithmic flop
+                gp = dmrg_amd.DeviceArray(h.n_psi, np.complex128)
+                h.adjoint_synth(xd, yd, gp)
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    h.adjoint_synth(xd, yd, gp)
+                ams = (time.perf_counter() - t0) / reps * 1e3
+                res["fp32_adjoint"] = {"ms": round(ams, 3), "tflops": round(2.5 * h.flops / (ams * 1e-3) / 1e12, 2),
+                                       "note": "cotangents of psi, renv, lenv; host-timed, includes the operand "
+                                               "conjugation and FP32 conversions"}
+                print(f"[bench] {name} fp32 adjoint: {ams:.3f} ms, {res['fp32_adjoint']['tflops']:.1f} TFLOP/s",
+                      file=sys.stderr, flush=True)
+                gp.free()
             res["flop"] = h.flops
             res["n_psi"] = h.n_psi
             print(f"[bench] {name} {prec}: {ms[0]:.3f} ms/matvec, {res[prec]['tflops']:.1f} TFLOP/s", file=sys.stderr,

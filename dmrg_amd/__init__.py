@@ -27,7 +27,7 @@ EXPORTS = [
     "dmrgx_init", "dmrgx_finalize", "dmrgx_last_error", "dmrgx_device_info", "dmrgx_malloc",
     "dmrgx_free", "dmrgx_memcpy_h2d", "dmrgx_memcpy_d2h", "dmrgx_memset_zero", "dmrgx_synchronize",
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
-    "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
+    "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time",
 ]
 
@@ -81,6 +81,7 @@ def _declare(lib):
         "dmrgx_heff_time": ([vp, vp, vp, C.c_int, dp], C.c_int),
         "dmrgx_heff_destroy": ([vp], C.c_int),
         "dmrgx_heff_adjoint": ([vp, vp, vp, vp, vp, vp], C.c_int),
+        "dmrgx_heff_sizes": ([vp, i64p], C.c_int),
         "dmrgx_eigsh": ([vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, vp, dp, vp, vp, C.c_int,
                          C.POINTER(C.c_int)], C.c_int),
         "dmrgx_block_svd": ([C.c_int, C.c_int, i32p, i32p, vp, i64p, vp, i64p, vp, i64p, vp, i64p], C.c_int),

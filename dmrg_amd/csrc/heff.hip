@@ -377,6 +377,16 @@ int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out) {
   CAPI_END
 }
 
+int dmrgx_heff_sizes(const dmrgx_heff* plan, int64_t* out) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p && out, "heff_sizes: null arguments");
+  out[0] = plan->p->n_psi;
+  out[1] = plan->p->n_store;
+  out[2] = plan->p->renv_len;
+  out[3] = plan->p->lenv_len;
+  CAPI_END
+}
+
 int dmrgx_heff_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, void* g_psi, void* g_renv,
                        void* g_lenv) {
   CAPI_BEGIN

@@ -246,8 +246,30 @@ class SynthHeff:
         from . import check, lib
         check(lib().dmrgx_heff_apply(self.h, x.ptr, y.ptr), "dmrgx_heff_apply")
 
+    def sizes(self):
+        """(n_psi, n_store, renv length, lenv length)"""
+        from . import check, lib
+        out = np.zeros(4, np.int64)
+        check(lib().dmrgx_heff_sizes(self.h, out.ctypes.data_as(C.POINTER(C.c_int64))), "dmrgx_heff_sizes")
+        return tuple(int(v) for v in out)
+
+    def adjoint_synth(self, psi, g_out, g_psi):
+        """Cotangents of psi and of both (plan-owned) environments; the
+        environment cotangent buffers are kept between calls."""
+        from . import DeviceArray, check, lib
+        npd = np.complex128 if self.dtype != 0 else np.float64
+        if getattr(self, "_genv", None) is None:
+            _, _, rl, ll = self.sizes()
+            self._genv = (DeviceArray(rl, npd), DeviceArray(ll, npd))
+        gr, gl = self._genv
+        check(lib().dmrgx_heff_adjoint(self.h, psi.ptr, g_out.ptr, g_psi.ptr, gr.ptr, gl.ptr), "dmrgx_heff_adjoint")
+
     def close(self):
         from . import lib
+        if getattr(self, "_genv", None) is not None:
+            for b in self._genv:
+                b.free()
+            self._genv = None
         if self.h:
             lib().dmrgx_heff_destroy(self.h)
             self.h = None

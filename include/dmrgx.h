@@ -79,6 +79,8 @@ int dmrgx_heff_apply(dmrgx_heff* plan, const void* in, void* out);
  * (complex interleaved), device pointers. */
 int dmrgx_heff_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, void* g_psi, void* g_renv,
                        void* g_lenv);
+/* {n_psi, n_store, renv length, lenv length} in elements */
+int dmrgx_heff_sizes(const dmrgx_heff* plan, int64_t* out);
 /* algorithmic flops of one apply: sum over tasks of c*M*N*K (c = 2 real, 8 complex) */
 double dmrgx_heff_flops(const dmrgx_heff* plan);
 int dmrgx_heff_destroy(dmrgx_heff* plan);
