@@ -27,7 +27,7 @@ EXPORTS = [
     "dmrgx_init", "dmrgx_finalize", "dmrgx_last_error", "dmrgx_device_info", "dmrgx_malloc",
     "dmrgx_free", "dmrgx_memcpy_h2d", "dmrgx_memcpy_d2h", "dmrgx_memset_zero", "dmrgx_synchronize",
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
-    "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
+    "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_eigsh_batched", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time",
 ]
 
@@ -84,6 +84,8 @@ def _declare(lib):
         "dmrgx_heff_sizes": ([vp, i64p], C.c_int),
         "dmrgx_eigsh": ([vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, vp, dp, vp, vp, C.c_int,
                          C.POINTER(C.c_int)], C.c_int),
+        "dmrgx_eigsh_batched": ([vp, C.c_int, i64p, C.c_int, C.c_int, C.c_double, C.c_int,
+                                 C.POINTER(C.c_uint64), dp, vp, vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
         "dmrgx_block_svd": ([C.c_int, C.c_int, i32p, i32p, vp, i64p, vp, i64p, vp, i64p, vp, i64p], C.c_int),
         "dmrgx_block_svd_sweeps": ([], C.c_int),
         "dmrgx_block_gemm": ([C.c_int, C.c_char, C.c_char, vp, vp, vp, i64p, C.c_int], C.c_int),
@@ -223,6 +225,26 @@ class Heff:
                                 ev.ctypes.data_as(C.POINTER(C.c_double)), vec.ptr,
                                 refine.h if refine is not None else None, refine_k, info), "dmrgx_eigsh")
         return ev, vec, list(info)
+
+    def eigsh_batched(self, lens, nev=6, ncv=None, tol=1e-10, max_restart=1000, seeds=None, refine=None,
+                      refine_k=6):
+        """dmrgx_eigsh_batched over chains of lengths `lens` concatenated in
+        this plan: (evals n_chains x nev, the concatenated vectors, info n_chains x 4)."""
+        if ncv is None:
+            ncv = (6 if self.dtype == C64 else 4) * nev
+        n = len(lens)
+        ln = np.asarray(lens, dtype=np.int64)
+        sd = np.asarray(seeds if seeds is not None else [1234] * n, dtype=np.uint64)
+        npd = np.complex128 if self.dtype == C64 else np.float64
+        vec = DeviceArray(self.n_psi, npd)
+        ev = np.zeros((n, max(nev, 1)))
+        info = np.zeros((n, 4), dtype=np.int32)
+        check(lib().dmrgx_eigsh_batched(self.h, n, ln.ctypes.data_as(C.POINTER(C.c_int64)), nev, ncv, tol,
+                                        max_restart, sd.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                        ev.ctypes.data_as(C.POINTER(C.c_double)), vec.ptr,
+                                        refine.h if refine is not None else None, refine_k,
+                                        info.ctypes.data_as(C.POINTER(C.c_int))), "dmrgx_eigsh_batched")
+        return ev, vec, info
 
     def close(self):
         if self.h:

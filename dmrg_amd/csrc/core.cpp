@@ -301,29 +301,35 @@ double Engine<T>::dmrg(int cutoff, int sweep) {
   return e;
 }
 
-// two-site schedule, src/functions.cpp:234-299 (ladder=false skips the
-// pre-sweep cutoff ladder 100, 200, ...)
-template <typename T>
-double Engine<T>::dmrg2(int cutoff, int sweep, bool ladder) {
-  const int L = mps_.size();
+// two-site schedule, src/functions.cpp:234-299
+double schedule_dmrg2(int L, int cutoff, int sweep, bool ladder, const std::function<double()>& first,
+                      const std::function<double(int, int, int)>& step) {
   int pre_cutoff = sweep == -1 ? cutoff : 10;
   int pre_sweep = ladder ? cutoff / 100 : 0;
-  double e = two_sites(0, L - 1, 0);
-  for (int i = 1; i < L / 2; ++i) e = update_two(i, L - i - 1, pre_cutoff);
+  double e = first();
+  for (int i = 1; i < L / 2; ++i) e = step(i, L - i - 1, pre_cutoff);
   if (sweep != -1) {
-    for (int i = L / 2; i < L - 2; ++i) e = update_two(i, i + 1, pre_cutoff);
+    for (int i = L / 2; i < L - 2; ++i) e = step(i, i + 1, pre_cutoff);
     for (int s = 0; s < pre_sweep - 1; ++s) {
-      for (int i = L - 3; i > 1; --i) e = update_two(i - 1, i, 100 * (s + 1));
-      for (int i = 2; i < L - 2; ++i) e = update_two(i, i + 1, 100 * (s + 1));
+      for (int i = L - 3; i > 1; --i) e = step(i - 1, i, 100 * (s + 1));
+      for (int i = 2; i < L - 2; ++i) e = step(i, i + 1, 100 * (s + 1));
     }
   }
   for (int s = 0; s < sweep; ++s) {
-    for (int i = L - 3; i > 1; --i) e = update_two(i - 1, i, cutoff);
-    for (int i = 2; i < L - 2; ++i) e = update_two(i, i + 1, cutoff);
+    for (int i = L - 3; i > 1; --i) e = step(i - 1, i, cutoff);
+    for (int i = 2; i < L - 2; ++i) e = step(i, i + 1, cutoff);
   }
   if (sweep != -1)
-    for (int i = L - 3; i >= L / 2; --i) e = update_two(i - 1, i, cutoff);
+    for (int i = L - 3; i >= L / 2; --i) e = step(i - 1, i, cutoff);
   return e;
+}
+
+template <typename T>
+double Engine<T>::dmrg2(int cutoff, int sweep, bool ladder) {
+  const int L = mps_.size();
+  return schedule_dmrg2(
+      L, cutoff, sweep, ladder, [&] { return two_sites(0, L - 1, 0); },
+      [&](int l, int r, int c) { return update_two(l, r, c); });
 }
 
 template class Engine<double>;

@@ -40,6 +40,16 @@ struct StepLog {
 };
 
 template <typename T>
+class BatchEngine;
+
+// The two-site sweep schedule of dmrg2 (src/functions.cpp:234-299): `first`
+// is the infinite-size start on the two edge sites, `step(l, r, cutoff)` an
+// update_two; ladder=false skips the pre-sweep cutoff ladder 100, 200, ...
+// Returns the last step's value.
+double schedule_dmrg2(int L, int cutoff, int sweep, bool ladder, const std::function<double()>& first,
+                      const std::function<double(int, int, int)>& step);
+
+template <typename T>
 class Engine {
  public:
   Engine(const mpo<T>& m, const RunConfig& cfg);
@@ -57,6 +67,7 @@ class Engine {
   std::function<void(const StepLog&)> on_step;
 
  private:
+  friend class BatchEngine<T>;
   bool alpha_on() const { return !cfg.physical_signs || cfg.fermion; }
   qtensor<T> make_lenv(int l);
   qtensor<T> make_renv(int r);

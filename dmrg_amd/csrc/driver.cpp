@@ -4,6 +4,7 @@
 #include <cstring>
 
 #include "dmrgx_driver.h"
+#include "batch.hpp"
 #include "core.hpp"
 #include "heff.hpp"
 #include "lanczos.hpp"
@@ -150,6 +151,64 @@ void run_engine(Engine<T>& eng, const dmrgx_run_config* c, dmrgx_run_stats* st, 
   if (n) *n = (int)eng.log.size();
 }
 
+template <typename T>
+void run_batch(BatchEngine<T>& b, const dmrgx_run_config* c, double* energy, double* trunc, dmrgx_run_stats* st,
+               double* log, int cap, int* n) {
+  using clk = std::chrono::steady_clock;
+  auto t0 = clk::now();
+  double e = b.dmrg2(c->m, c->sweeps, c->ladder != 0);
+  auto t1 = clk::now();
+  std::memset(st, 0, sizeof(*st));
+  st->energy = e;
+  st->seconds = std::chrono::duration<double>(t1 - t0).count();
+  st->steps = (int)b.log.size();
+  for (auto& g : b.log) {
+    st->matvecs += g.n_matvec;
+    st->polish_matvecs += g.n_polish;
+    st->matvec_flops += g.flops;
+    st->t_eig += g.t_eig;
+    st->t_svd += g.t_svd;
+    st->t_env += g.t_env;
+  }
+  const int nc = (int)b.chains.size();
+  for (int i = 0; i < nc; ++i) {
+    const StepLog& g = b.chains[i]->log.back();
+    if (energy) energy[i] = g.e[0];
+    if (trunc) trunc[i] = g.trunc;
+  }
+  if (c->sweeps > 0) {
+    const int L = c->L, per = 2 * (L - 4), fin = (L - 3) - (L / 2) + 1;
+    int end = (int)b.log.size() - std::max(fin, 0), beg = end - per * c->sweeps;
+    if (beg >= 0) {
+      double tot = 0, last = 0;
+      for (int i = beg; i < end; ++i) {
+        tot += b.log[i].t_total;
+        if (i >= end - per) last += b.log[i].t_total;
+      }
+      st->sweep_seconds = tot / c->sweeps;
+      st->last_sweep_seconds = last;
+    }
+  }
+  int k = 0;
+  for (auto& g : b.log) {
+    if (!log || k >= cap) break;
+    double* row = log + (size_t)k * DMRGX_BATCH_LOG_WIDTH;
+    row[0] = g.l;
+    row[1] = g.r;
+    row[2] = (double)g.d;
+    row[3] = g.n_matvec;
+    row[4] = g.n_polish;
+    row[5] = g.flops;
+    row[6] = g.t_total;
+    row[7] = g.t_eig;
+    row[8] = g.t_svd;
+    row[9] = g.t_env;
+    row[10] = g.t_plan;
+    ++k;
+  }
+  if (n) *n = (int)b.log.size();
+}
+
 RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
   RunConfig r;
   r.prec = c->prec;
@@ -250,6 +309,28 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double*
     }
   }
   if (state) *state = out.release();
+  CAPI_END
+}
+
+int dmrgx_run_batch(const dmrgx_run_config* c, int n_chains, const double* p, double* energy, double* trunc,
+                    dmrgx_run_stats* st, double* step_log, int log_cap, int* n_steps) {
+  CAPI_BEGIN
+  ensure_init();
+  DMRGX_CHECK(c && st && p && n_chains > 0, "run_batch: null arguments or no chains");
+  DMRGX_CHECK(c->L >= 6, "run_batch: L must be at least 6");
+  DMRGX_CHECK(c->schedule == DMRGX_SCHEDULE_DMRG2, "run_batch: only the two-site schedule is batched");
+  const RunConfig cfg = make_cfg(c, model_fermion(c->model));
+  if (model_complex(c->model)) {
+    vector<mpo<cplx>> m;
+    for (int i = 0; i < n_chains; ++i) m.push_back(model_mpo_c(c->model, p + 4 * i, c->L));
+    BatchEngine<cplx> b(m, cfg);
+    run_batch(b, c, energy, trunc, st, step_log, log_cap, n_steps);
+  } else {
+    vector<mpo<double>> m;
+    for (int i = 0; i < n_chains; ++i) m.push_back(model_mpo_r(c->model, p + 4 * i, c->L));
+    BatchEngine<double> b(m, cfg);
+    run_batch(b, c, energy, trunc, st, step_log, log_cap, n_steps);
+  }
   CAPI_END
 }
 

@@ -460,6 +460,40 @@ int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
   CAPI_END
 }
 
+int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int nev, int ncv, double tol,
+                        int max_restart, const uint64_t* seeds, double* evals, void* evec, dmrgx_heff* refine,
+                        int refine_k, int* info) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p && len && seeds && evals && evec && n_chains > 0, "eigsh_batched: null arguments");
+  HeffPlan* h = plan->p.get();
+  std::vector<int64_t> ln(len, len + n_chains);
+  int64_t tot = 0;
+  for (auto n : ln) tot += n;
+  DMRGX_CHECK(tot == h->n_psi, "eigsh_batched: chain lengths do not add up to the plan's psi length");
+  hipStream_t s = rt().stream;
+  MatVec op = [h, s](const double* x, double* y) { h->apply(x, y, s); };
+  std::vector<uint64_t> sd(seeds, seeds + n_chains);
+  auto r = eigsh_batched(op, ln, h->dtype == DT_C64, nev, ncv, tol, max_restart, sd, nullptr, (double*)evec);
+  for (int c = 0; c < n_chains; ++c)
+    for (int i = 0; i < nev; ++i) evals[(size_t)c * nev + i] = i < (int)r[c].evals.size() ? r[c].evals[i] : 0.0;
+  int polish_mv = 0;
+  if (refine && refine->p && h->prec != PREC_FP64) {
+    HeffPlan* f = refine->p.get();
+    DMRGX_CHECK(f->n_psi == h->n_psi, "eigsh_batched: refine plan of another size");
+    MatVec op64 = [f, s](const double* x, double* y) { f->apply(x, y, s); };
+    auto e0 = krylov_polish_batched(op64, ln, f->dtype == DT_C64, std::max(refine_k, 1), (double*)evec, &polish_mv);
+    for (int c = 0; c < n_chains; ++c) evals[(size_t)c * nev] = e0[c];
+  }
+  if (info)
+    for (int c = 0; c < n_chains; ++c) {
+      info[4 * c] = r[c].n_matvec;
+      info[4 * c + 1] = r[c].n_restart;
+      info[4 * c + 2] = r[c].converged ? 1 : 0;
+      info[4 * c + 3] = polish_mv;
+    }
+  CAPI_END
+}
+
 int dmrgx_block_gemm(int dtype, char ta, char tb, const void* A, const void* B, void* C, const int64_t* tasks,
                      int n_task) {
   CAPI_BEGIN

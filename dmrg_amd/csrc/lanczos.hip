@@ -240,6 +240,16 @@ double dev_norm2(const double* v, int64_t n, bool cplx, hipStream_t s) {
   return std::sqrt(std::max(o[0], 0.0));
 }
 
+void dev_orthogonalise(const double* V, int64_t ld, int k, double* v, int64_t n, bool cplx, hipStream_t s) {
+  if (k <= 0) return;
+  DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
+  DevBuf h(2 * (k + G) * sizeof(double));
+  for (int pass = 0; pass < 2; ++pass) {
+    vdots(V, ld, k, v, n, cplx, (double*)h.ptr, (double*)partial.ptr, s);
+    gemv_sub(V, ld, k, (const double*)h.ptr, v, n, cplx, s);
+  }
+}
+
 void host_symeig(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& v) {
   v.assign((size_t)n * n, 0.0);
   for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;

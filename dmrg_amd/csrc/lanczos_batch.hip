@@ -1,0 +1,581 @@
+// Thick-restart Lanczos on many independent operators at once.  The chains'
+// vectors are concatenated (chain c owns elements [off_c, off_c + n_c) of
+// every Krylov vector), one matvec call applies every chain's operator, and
+// the Gram-Schmidt reductions and vector updates are segmented by chain in
+// single launches.  The projected matrices, Ritz values and restarts are per
+// chain on the host.  Each chain follows eigsh's algorithm and stopping test
+// (the ARPACK test of src/dsaupd.cpp:65-155 / src/znaupd.cpp:78-197); all
+// chains take the same number of steps, until every chain has converged.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <memory>
+
+#include "common.hpp"
+#include "lanczos.hpp"
+
+namespace dmrgx {
+namespace {
+
+constexpr int G = 8;      // Krylov columns per pass of the dot / combine kernels
+constexpr int CH = 4096;  // elements of one chain handled by one work-group
+constexpr int64_t DENSE_MAX = 64;
+
+// element range [b, e) of the concatenated vector, owned by chain ordinal `chain`
+struct SegBlk {
+  int64_t b, e;
+  int chain, pad;
+};
+
+// partial[(y, blk)][2c + {re, im}] = <X_c, w> over the block, X_c = V_{j0+c} (j0+c < k) or w
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_vdots_seg(const double* __restrict__ V, int64_t ld, int k,
+                                                   const double* __restrict__ w, const SegBlk* __restrict__ blks,
+                                                   double* __restrict__ partial) {
+  const int j0 = blockIdx.y * G;
+  const int g = min(G, k + 1 - j0);
+  const SegBlk B = blks[blockIdx.x];
+  double re[G], im[G];
+#pragma unroll
+  for (int c = 0; c < G; ++c) re[c] = im[c] = 0;
+  for (int64_t i = B.b + threadIdx.x; i < B.e; i += 256) {
+    if (CPLX) {
+      const double wr = w[2 * i], wi = w[2 * i + 1];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) {
+          const double* v = (j0 + c < k) ? V + (int64_t)(j0 + c) * ld : w;
+          const double vr = v[2 * i], vi = v[2 * i + 1];
+          re[c] += vr * wr + vi * wi;
+          im[c] += vr * wi - vi * wr;
+        }
+    } else {
+      const double wr = w[i];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) re[c] += ((j0 + c < k) ? V[(int64_t)(j0 + c) * ld + i] : wr) * wr;
+    }
+  }
+  __shared__ double red[2 * G][256];
+#pragma unroll
+  for (int c = 0; c < G; ++c) {
+    red[2 * c][threadIdx.x] = re[c];
+    red[2 * c + 1][threadIdx.x] = im[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < 2 * G; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 2 * G)
+    partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 2 * G + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// h[chain * hs + 2 j + {re, im}] = sum of the chain's block partials;
+// work-group (chain, column group), one wavefront per real component
+__global__ void k_reduce_seg(const double* __restrict__ partial, int nblk, int ktot, const int* __restrict__ cb,
+                             const int* __restrict__ ce, double* __restrict__ h, int hs) {
+  const int chain = blockIdx.x, j0 = blockIdx.y * G;
+  const int q = threadIdx.x / 64, lane = threadIdx.x & 63;
+  const int g = min(G, ktot - j0);
+  if (q >= 2 * g) return;
+  double s = 0;
+  for (int b = cb[chain] + lane; b < ce[chain]; b += 64) s += partial[((int64_t)blockIdx.y * nblk + b) * 2 * G + q];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) h[(int64_t)chain * hs + 2 * j0 + q] = s;
+}
+
+// out = (w - sum_{j<k} V_j h_chain[j]) * scale[chain]; out may alias w; scale null = 1
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_gemv_sub_seg(const double* __restrict__ V, int64_t ld, int k,
+                                                      const double* __restrict__ h, int hs, const double* w,
+                                                      double* out, const double* __restrict__ scale,
+                                                      const SegBlk* __restrict__ blks) {
+  const SegBlk B = blks[blockIdx.x];
+  const double* hc = h + (int64_t)B.chain * hs;
+  const double a = scale ? scale[B.chain] : 1.0;
+  for (int64_t i = B.b + threadIdx.x; i < B.e; i += 256) {
+    if (CPLX) {
+      double ar = 0, ai = 0;
+      for (int j = 0; j < k; ++j) {
+        const double* v = V + (int64_t)j * ld;
+        const double vr = v[2 * i], vi = v[2 * i + 1], hr = hc[2 * j], hi = hc[2 * j + 1];
+        ar += vr * hr - vi * hi;
+        ai += vr * hi + vi * hr;
+      }
+      const double xr = (w[2 * i] - ar) * a, xi = (w[2 * i + 1] - ai) * a;
+      out[2 * i] = xr;
+      out[2 * i + 1] = xi;
+    } else {
+      double acc = 0;
+      for (int j = 0; j < k; ++j) acc += V[(int64_t)j * ld + i] * hc[2 * j];
+      out[i] = (w[i] - acc) * a;
+    }
+  }
+}
+
+// out column (c0 + c) = sum_{j<m} V_j Y_chain[j][c0 + c]; Y_chain is m x ky, real
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_combine_seg(const double* __restrict__ V, int64_t ld, int m,
+                                                     const double* __restrict__ Y, int ky, int c0, int g,
+                                                     double* __restrict__ out, const SegBlk* __restrict__ blks) {
+  const SegBlk B = blks[blockIdx.x];
+  const double* Yc = Y + (int64_t)B.chain * m * ky;
+  constexpr int C = CPLX ? 2 : 1;
+  for (int64_t i = C * B.b + threadIdx.x; i < C * B.e; i += 256) {
+    double acc[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) acc[c] = 0;
+    for (int j = 0; j < m; ++j) {
+      const double v = V[(int64_t)j * ld + i];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) acc[c] += v * Yc[j * ky + c0 + c];
+    }
+#pragma unroll
+    for (int c = 0; c < G; ++c)
+      if (c < g) out[(int64_t)(c0 + c) * ld + i] = acc[c];
+  }
+}
+
+// x holds e_j (and no longer e_{j-1}) in every listed chain with j < len;
+// one thread per listed chain
+__global__ void k_unit_seg(double* __restrict__ x, const int64_t* __restrict__ off, const int64_t* __restrict__ len,
+                           int nl, int j, int cw) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nl) return;
+  if (j > 0 && j - 1 < len[c]) x[cw * (off[c] + j - 1)] = 0;
+  if (j < len[c]) x[cw * (off[c] + j)] = 1;
+}
+
+// H[c][i][j] (complex pairs) = y_c[i] for the listed chains; work-group per chain
+__global__ void k_gather_col(const double* __restrict__ y, const int64_t* __restrict__ off,
+                             const int64_t* __restrict__ len, int nd, int j, int cw, double* __restrict__ H) {
+  const int c = blockIdx.x;
+  for (int i = threadIdx.x; i < len[c]; i += blockDim.x) {
+    double* h = H + (((int64_t)c * nd + i) * nd + j) * 2;
+    h[0] = y[cw * (off[c] + i)];
+    h[1] = cw == 2 ? y[cw * (off[c] + i) + 1] : 0.0;
+  }
+}
+
+// The segments of a set of chains: work-group blocks of at most CH elements
+// and each chain's range of blocks; `chain` in a block is the ordinal within
+// the set.
+struct Segs {
+  int n = 0, nblk = 0;
+  std::unique_ptr<DevBuf> blk, cb, ce;
+  Segs(const std::vector<int64_t>& off, const std::vector<int64_t>& len, const std::vector<int>& set,
+       hipStream_t s) {
+    n = (int)set.size();
+    std::vector<SegBlk> b;
+    std::vector<int> hb(n), he(n);
+    for (int a = 0; a < n; ++a) {
+      const int c = set[a];
+      hb[a] = (int)b.size();
+      for (int64_t x = off[c]; x < off[c] + len[c]; x += CH)
+        b.push_back(SegBlk{x, std::min<int64_t>(x + CH, off[c] + len[c]), a, 0});
+      he[a] = (int)b.size();
+    }
+    nblk = (int)b.size();
+    blk = std::make_unique<DevBuf>(std::max(nblk, 1) * sizeof(SegBlk));
+    cb = std::make_unique<DevBuf>(std::max(n, 1) * 4);
+    ce = std::make_unique<DevBuf>(std::max(n, 1) * 4);
+    DMRGX_HIP(hipMemcpyAsync(blk->ptr, b.data(), nblk * sizeof(SegBlk), hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipMemcpyAsync(cb->ptr, hb.data(), n * 4, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipMemcpyAsync(ce->ptr, he.data(), n * 4, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipStreamSynchronize(s));  // the host arrays end here
+  }
+  const SegBlk* B() const { return (const SegBlk*)blk->ptr; }
+};
+
+// Segmented Gram-Schmidt helpers over one set of chains; coefficient arrays
+// hold hs doubles (complex pairs) per chain.
+struct SegOps {
+  const Segs& sg;
+  bool cplx;
+  int64_t ld;
+  int hs;
+  hipStream_t s;
+  DevBuf partial;
+  SegOps(const Segs& g, bool c, int64_t l, int maxk, hipStream_t st)
+      : sg(g), cplx(c), ld(l), hs(2 * (maxk + G)), s(st),
+        partial((size_t)std::max(g.nblk, 1) * ((maxk + G) / G + 1) * 2 * G * 8) {}
+  // h[c] = (V[0..k)^H x_c, |x_c|^2), k <= maxk - 1
+  void vdots(const double* V, int k, const double* x, double* h) {
+    const int ng = (k + 1 + G - 1) / G;
+    if (cplx) hipLaunchKernelGGL(k_vdots_seg<true>, dim3(sg.nblk, ng), dim3(256), 0, s, V, ld, k, x, sg.B(),
+                                 (double*)partial.ptr);
+    else hipLaunchKernelGGL(k_vdots_seg<false>, dim3(sg.nblk, ng), dim3(256), 0, s, V, ld, k, x, sg.B(),
+                            (double*)partial.ptr);
+    hipLaunchKernelGGL(k_reduce_seg, dim3(sg.n, ng), dim3(64 * 2 * G), 0, s, (const double*)partial.ptr, sg.nblk,
+                       k + 1, (const int*)sg.cb->ptr, (const int*)sg.ce->ptr, h, hs);
+  }
+  void sub(const double* V, int k, const double* h, const double* x, double* out, const double* scale) {
+    if (cplx) hipLaunchKernelGGL(k_gemv_sub_seg<true>, dim3(sg.nblk), dim3(256), 0, s, V, ld, k, h, hs, x, out,
+                                 scale, sg.B());
+    else hipLaunchKernelGGL(k_gemv_sub_seg<false>, dim3(sg.nblk), dim3(256), 0, s, V, ld, k, h, hs, x, out, scale,
+                            sg.B());
+  }
+  // out columns [0, ky) = V[0..m) Y_c with Y on the device (chains x m x ky)
+  void combine(const double* V, int m, const double* Y, int ky, double* out) {
+    for (int c0 = 0; c0 < ky; c0 += G) {
+      if (cplx) hipLaunchKernelGGL(k_combine_seg<true>, dim3(sg.nblk), dim3(256), 0, s, V, ld, m, Y, ky, c0,
+                                   std::min(G, ky - c0), out, sg.B());
+      else hipLaunchKernelGGL(k_combine_seg<false>, dim3(sg.nblk), dim3(256), 0, s, V, ld, m, Y, ky, c0,
+                              std::min(G, ky - c0), out, sg.B());
+    }
+  }
+  std::vector<double> fetch(const double* h) {
+    std::vector<double> o((size_t)sg.n * hs);
+    DMRGX_HIP(hipMemcpyAsync(o.data(), h, o.size() * 8, hipMemcpyDeviceToHost, s));
+    DMRGX_HIP(hipStreamSynchronize(s));
+    return o;
+  }
+  // x_c /= |x_c| for every chain of the set
+  void normalise(double* x, double* h, double* scl) {
+    vdots(nullptr, 0, x, h);
+    auto hh = fetch(h);
+    std::vector<double> sc(sg.n);
+    for (int a = 0; a < sg.n; ++a) {
+      const double n2 = hh[(size_t)a * hs];
+      DMRGX_CHECK(n2 > 0, "batched Lanczos: zero vector");
+      sc[a] = 1.0 / std::sqrt(n2);
+    }
+    DMRGX_HIP(hipMemcpyAsync(scl, sc.data(), sg.n * 8, hipMemcpyHostToDevice, s));
+    sub(nullptr, 0, h, x, x, scl);
+    DMRGX_HIP(hipStreamSynchronize(s));
+  }
+};
+
+std::vector<int64_t> offsets(const std::vector<int64_t>& len) {
+  std::vector<int64_t> off(len.size());
+  int64_t o = 0;
+  for (size_t c = 0; c < len.size(); ++c) {
+    DMRGX_CHECK(len[c] > 0, "batched Lanczos: empty chain");
+    off[c] = o;
+    o += len[c];
+  }
+  return off;
+}
+
+}  // namespace
+
+std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int nev,
+                                       int ncv, double tol, int max_restart, const std::vector<uint64_t>& seed,
+                                       const double* v0, double* evec) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  const int nc = (int)len.size();
+  DMRGX_CHECK(nc > 0 && (int)seed.size() == nc, "eigsh_batched: one length and one seed per chain");
+  const std::vector<int64_t> off = offsets(len);
+  const int64_t N = off.back() + len.back();
+  const int cw = cplx ? 2 : 1;
+  const int64_t nr = N * cw, ld = nr;
+  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+  std::vector<EigshResult> res(nc);
+
+  // chains diagonalised densely (eigsh's small-operator branch) and Lanczos chains
+  std::vector<int> small, big;
+  for (int c = 0; c < nc; ++c) {
+    const int64_t nev_c = std::min<int64_t>(nev, len[c]);
+    const int64_t ncv_c = std::min<int64_t>(std::max<int64_t>(ncv, nev_c + 2), len[c]);
+    (len[c] <= DENSE_MAX || ncv_c <= nev_c + 1 ? small : big).push_back(c);
+  }
+  if (!small.empty()) {
+    // materialise each small chain's operator column by column, all chains
+    // through the same matvec (the other chains see zero vectors)
+    const int ns = (int)small.size();
+    std::vector<int64_t> so(ns), sl(ns);
+    int nd = 0;
+    for (int a = 0; a < ns; ++a) {
+      so[a] = off[small[a]];
+      sl[a] = len[small[a]];
+      nd = std::max<int>(nd, (int)sl[a]);
+    }
+    DevBuf xb(nr * 8), yb(nr * 8), dso(ns * 8), dsl(ns * 8), Hd((size_t)ns * nd * nd * 16);
+    DMRGX_HIP(hipMemcpyAsync(dso.ptr, so.data(), ns * 8, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipMemcpyAsync(dsl.ptr, sl.data(), ns * 8, hipMemcpyHostToDevice, s));
+    fill_zero(xb.ptr, nr * 8, s);
+    for (int j = 0; j < nd; ++j) {
+      hipLaunchKernelGGL(k_unit_seg, dim3((ns + 63) / 64), dim3(64), 0, s, (double*)xb.ptr,
+                         (const int64_t*)dso.ptr, (const int64_t*)dsl.ptr, ns, j, cw);
+      op((const double*)xb.ptr, (double*)yb.ptr);
+      hipLaunchKernelGGL(k_gather_col, dim3(ns), dim3(64), 0, s, (const double*)yb.ptr, (const int64_t*)dso.ptr,
+                         (const int64_t*)dsl.ptr, nd, j, cw, (double*)Hd.ptr);
+    }
+    std::vector<double> Hh((size_t)ns * nd * nd * 2);
+    DMRGX_HIP(hipMemcpyAsync(Hh.data(), Hd.ptr, Hh.size() * 8, hipMemcpyDeviceToHost, s));
+    DMRGX_HIP(hipStreamSynchronize(s));
+    std::vector<double> x(nr, 0.0);
+    for (int a = 0; a < ns; ++a) {
+      const int c = small[a], n = (int)sl[a];
+      std::vector<dmrgx::cplx> H((size_t)n * n);
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+          const double* q = Hh.data() + (((size_t)a * nd + i) * nd + j) * 2;
+          H[(size_t)i * n + j] = dmrgx::cplx(q[0], q[1]);
+        }
+      std::vector<double> w;
+      std::vector<dmrgx::cplx> Vh;
+      host_heeig(n, H, w, Vh);
+      res[c].evals.assign(w.begin(), w.begin() + std::min(nev, n));
+      res[c].n_matvec = nd;
+      res[c].converged = true;
+      for (int i = 0; i < n; ++i) {
+        if (cplx) {
+          x[2 * (off[c] + i)] = Vh[(size_t)i * n].real();
+          x[2 * (off[c] + i) + 1] = Vh[(size_t)i * n].imag();
+        } else {
+          x[off[c] + i] = Vh[(size_t)i * n].real();
+        }
+      }
+    }
+    // the small chains' eigenvectors; the Lanczos chains' segments follow below
+    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), nr * 8, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipStreamSynchronize(s));
+    if (big.empty()) return res;
+  }
+
+  const int nb = (int)big.size();
+  int64_t nmin = len[big[0]];
+  for (int c : big) nmin = std::min(nmin, len[c]);
+  nev = (int)std::min<int64_t>(nev, nmin);
+  const int m = (int)std::min<int64_t>(std::max(ncv, nev + 2), nmin);
+  const int keep = std::min(m - 2, nev + std::max(2, (m - nev) / 2));
+  Segs sg(off, len, big, s);
+  SegOps ops(sg, cplx, ld, m + 1, s);
+  const int hs = ops.hs;
+  DevBuf Vb((size_t)(m + 1) * nr * 8), Tmp((size_t)m * nr * 8);
+  DevBuf hbuf((size_t)nb * hs * 8), h2buf((size_t)nb * hs * 8), scl((size_t)nb * 8);
+  DevBuf Yb((size_t)nb * m * std::max(keep, 1) * 8);
+  double* V = (double*)Vb.ptr;
+  double* w = (double*)Tmp.ptr;
+  double* h = (double*)hbuf.ptr;
+  double* h2 = (double*)h2buf.ptr;
+  // the small chains' segments stay zero in every Krylov vector
+  if (!small.empty()) {
+    fill_zero(V, (size_t)(m + 1) * nr * 8, s);
+    fill_zero(Tmp.ptr, (size_t)m * nr * 8, s);
+  }
+  if (v0) {
+    copy_d2d(V, v0, nr * 8, s);
+    for (int c : small) fill_zero(V + cw * off[c], len[c] * cw * 8, s);
+  } else {
+    for (int c : big) dev_random(V + cw * off[c], len[c], cplx, seed[c], s);
+  }
+  ops.normalise(V, h, (double*)scl.ptr);
+
+  std::vector<std::vector<double>> T(nb, std::vector<double>((size_t)m * m, 0.0)), theta(nb), Y(nb);
+  std::vector<double> beta_last(nb, 0.0), alpha(nb), b2(nb), nw2(nb), sc(nb);
+  std::vector<int> n_reorth(nb, 0);
+  std::vector<char> conv(nb, 0);
+  int j0 = 0, n_matvec = 0, n_restart = 0;
+  for (int restart = 0; restart <= max_restart; ++restart) {
+    n_restart = restart;
+    for (int j = j0; j < m; ++j) {
+      op(V + (int64_t)j * ld, w);
+      ++n_matvec;
+      // one classical Gram-Schmidt pass fused with |w|^2; a second pass (for
+      // every chain) when any chain shows cancellation |w'|^2 < 1e-8 |w|^2
+      ops.vdots(V, j + 1, w, h);
+      auto hh = ops.fetch(h);
+      bool second = false;
+      for (int a = 0; a < nb; ++a) {
+        const double* q = hh.data() + (size_t)a * hs;
+        double hsq = 0;
+        for (int i = 0; i <= j; ++i) hsq += q[2 * i] * q[2 * i] + q[2 * i + 1] * q[2 * i + 1];
+        alpha[a] = q[2 * j];
+        nw2[a] = q[2 * (j + 1)];
+        b2[a] = nw2[a] - hsq;
+        if (!(b2[a] >= 1e-8 * nw2[a])) second = true;
+      }
+      const double* hfin = h;
+      if (second) {
+        ops.sub(V, j + 1, h, w, w, nullptr);
+        ops.vdots(V, j + 1, w, h2);
+        auto hh2 = ops.fetch(h2);
+        for (int a = 0; a < nb; ++a) {
+          const double* q = hh2.data() + (size_t)a * hs;
+          double hsq = 0;
+          for (int i = 0; i <= j; ++i) hsq += q[2 * i] * q[2 * i] + q[2 * i + 1] * q[2 * i + 1];
+          alpha[a] += q[2 * j];
+          nw2[a] = q[2 * (j + 1)];
+          b2[a] = nw2[a] - hsq;
+          n_reorth[a]++;
+        }
+        hfin = h2;
+      }
+      std::vector<int> lost;
+      for (int a = 0; a < nb; ++a) {
+        double beta = std::sqrt(std::max(b2[a], 0.0));
+        if (!(beta > 1e-13 * std::max(std::fabs(alpha[a]), std::sqrt(std::max(nw2[a], 0.0))) && beta > 1e-300)) {
+          beta = 0;
+          lost.push_back(a);
+        }
+        T[a][(size_t)j * m + j] = alpha[a];
+        if (j + 1 < m) T[a][(size_t)j * m + j + 1] = T[a][(size_t)(j + 1) * m + j] = beta;
+        beta_last[a] = beta;
+        sc[a] = beta > 0 ? 1.0 / beta : 0.0;
+      }
+      DMRGX_HIP(hipMemcpyAsync(scl.ptr, sc.data(), nb * 8, hipMemcpyHostToDevice, s));
+      double* vn = V + (int64_t)(j + 1) * ld;
+      ops.sub(V, j + 1, hfin, w, vn, (const double*)scl.ptr);
+      for (int a : lost) {
+        // invariant subspace of this chain: continue with a fresh random
+        // direction orthogonal to its basis (as in eigsh)
+        const int c = big[a];
+        double* seg = vn + cw * off[c];
+        dev_random(seg, len[c], cplx, seed[c] + 7919ull * (j + 1) + restart, s);
+        dev_orthogonalise(V + cw * off[c], ld, j + 1, seg, len[c], cplx, s);
+        const double z = dev_norm2(seg, len[c], cplx, s);
+        dev_scale(seg, len[c], cplx, 1.0 / z, s);
+      }
+      DMRGX_HIP(hipStreamSynchronize(s));  // sc is rewritten by the next step
+    }
+    // Rayleigh-Ritz per chain
+    bool all = true;
+    for (int a = 0; a < nb; ++a) {
+      std::vector<double> A = T[a];
+      host_symeig(m, A, theta[a], Y[a]);
+      bool ok = true;
+      double mr = 0;
+      for (int i = 0; i < nev; ++i) {
+        const double r = std::fabs(beta_last[a] * Y[a][(size_t)(m - 1) * m + i]);
+        mr = std::max(mr, r);
+        if (r > tol * std::max(std::fabs(theta[a][i]), eps23)) ok = false;
+      }
+      res[big[a]].max_resid = mr;
+      conv[a] = ok;
+      all = all && ok;
+    }
+    if (all || restart == max_restart) break;
+    // thick restart of every chain: its `keep` lowest Ritz vectors and the residual
+    std::vector<double> Yk((size_t)nb * m * keep);
+    for (int a = 0; a < nb; ++a)
+      for (int r = 0; r < m; ++r)
+        for (int k = 0; k < keep; ++k) Yk[((size_t)a * m + r) * keep + k] = Y[a][(size_t)r * m + k];
+    DMRGX_HIP(hipMemcpyAsync(Yb.ptr, Yk.data(), Yk.size() * 8, hipMemcpyHostToDevice, s));
+    ops.combine(V, m, (const double*)Yb.ptr, keep, (double*)Tmp.ptr);
+    DMRGX_HIP(hipMemcpyAsync(V, Tmp.ptr, (size_t)keep * nr * 8, hipMemcpyDeviceToDevice, s));
+    DMRGX_HIP(hipMemcpyAsync(V + (int64_t)keep * ld, V + (int64_t)m * ld, nr * 8, hipMemcpyDeviceToDevice, s));
+    for (int a = 0; a < nb; ++a) {
+      std::fill(T[a].begin(), T[a].end(), 0.0);
+      for (int i = 0; i < keep; ++i) {
+        T[a][(size_t)i * m + i] = theta[a][i];
+        const double sgl = beta_last[a] * Y[a][(size_t)(m - 1) * m + i];
+        T[a][(size_t)i * m + keep] = T[a][(size_t)keep * m + i] = sgl;
+      }
+    }
+    j0 = keep;
+    DMRGX_HIP(hipStreamSynchronize(s));  // Yk ends here
+  }
+  // lowest Ritz vector of every Lanczos chain, normalised, into its segment of evec
+  std::vector<double> y0((size_t)nb * m);
+  for (int a = 0; a < nb; ++a)
+    for (int r = 0; r < m; ++r) y0[(size_t)a * m + r] = Y[a][(size_t)r * m];
+  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
+  double* out = small.empty() ? evec : (double*)Tmp.ptr;
+  ops.combine(V, m, (const double*)Yb.ptr, 1, out);
+  ops.normalise(out, h, (double*)scl.ptr);
+  if (!small.empty())
+    for (int c : big) copy_d2d(evec + cw * off[c], out + cw * off[c], len[c] * cw * 8, s);
+  DMRGX_HIP(hipStreamSynchronize(s));
+  for (int a = 0; a < nb; ++a) {
+    EigshResult& r = res[big[a]];
+    r.evals.assign(theta[a].begin(), theta[a].begin() + nev);
+    r.n_matvec = n_matvec;
+    r.n_restart = n_restart;
+    r.n_reorth = n_reorth[a];
+    r.converged = conv[a];
+  }
+  DMRGX_HIP(hipGetLastError());
+  return res;
+}
+
+std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int k,
+                                          double* x, int* n_matvec) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  const int nc = (int)len.size();
+  DMRGX_CHECK(nc > 0, "krylov_polish_batched: no chains");
+  const std::vector<int64_t> off = offsets(len);
+  const int64_t N = off.back() + len.back();
+  const int cw = cplx ? 2 : 1;
+  const int64_t nr = N * cw;
+  int64_t nmax = 0;
+  for (auto n : len) nmax = std::max(nmax, n);
+  k = (int)std::max<int64_t>(1, std::min<int64_t>(k, nmax));
+  std::vector<int> all(nc);
+  for (int c = 0; c < nc; ++c) all[c] = c;
+  Segs sg(off, len, all, s);
+  SegOps ops(sg, cplx, nr, k + 1, s);
+  const int hs = ops.hs;
+  DevBuf Vb((size_t)(k + 1) * nr * 8), wb(nr * 8), hbuf((size_t)nc * hs * 8), h2buf((size_t)nc * hs * 8),
+      scl((size_t)nc * 8), Yb((size_t)nc * k * 8);
+  double* V = (double*)Vb.ptr;
+  double* w = (double*)wb.ptr;
+  double* h = (double*)hbuf.ptr;
+  double* h2 = (double*)h2buf.ptr;
+  copy_d2d(V, x, nr * 8, s);
+  ops.normalise(V, h, (double*)scl.ptr);
+  // chain c's Krylov space stops growing at kk[c]; its later vectors are zero
+  std::vector<std::vector<double>> T(nc, std::vector<double>((size_t)k * k, 0.0));
+  std::vector<int> kk(nc, k);
+  std::vector<double> sc(nc);
+  for (int j = 0; j < k; ++j) {
+    op(V + (int64_t)j * nr, w);
+    if (n_matvec) (*n_matvec)++;
+    // two full Gram-Schmidt passes (as in krylov_polish); |w''|^2 = |w'|^2 - |h2|^2
+    ops.vdots(V, j + 1, w, h);
+    ops.sub(V, j + 1, h, w, w, nullptr);
+    ops.vdots(V, j + 1, w, h2);
+    auto hh = ops.fetch(h);
+    auto hh2 = ops.fetch(h2);
+    for (int c = 0; c < nc; ++c) {
+      const double* q = hh.data() + (size_t)c * hs;
+      const double* q2 = hh2.data() + (size_t)c * hs;
+      sc[c] = 0;
+      if (j >= kk[c]) continue;
+      const double a = q[2 * j] + q2[2 * j];
+      T[c][(size_t)j * k + j] = a;
+      double hsq = 0;
+      for (int i = 0; i <= j; ++i) hsq += q2[2 * i] * q2[2 * i] + q2[2 * i + 1] * q2[2 * i + 1];
+      const double beta = std::sqrt(std::max(q2[2 * (j + 1)] - hsq, 0.0));
+      if (j + 1 < k) {
+        if (beta < 1e-14 * std::fabs(a) || beta < 1e-300 || j + 1 >= len[c]) {
+          kk[c] = j + 1;
+        } else {
+          T[c][(size_t)j * k + j + 1] = T[c][(size_t)(j + 1) * k + j] = beta;
+          sc[c] = 1.0 / beta;
+        }
+      }
+    }
+    if (j + 1 < k) {
+      DMRGX_HIP(hipMemcpyAsync(scl.ptr, sc.data(), nc * 8, hipMemcpyHostToDevice, s));
+      ops.sub(V, j + 1, h2, w, V + (int64_t)(j + 1) * nr, (const double*)scl.ptr);
+      DMRGX_HIP(hipStreamSynchronize(s));
+    }
+  }
+  std::vector<double> y0((size_t)nc * k, 0.0), th0(nc);
+  for (int c = 0; c < nc; ++c) {
+    const int n = kk[c];
+    std::vector<double> A((size_t)n * n), th, Y;
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = T[c][(size_t)i * k + j];
+    host_symeig(n, A, th, Y);
+    th0[c] = th[0];
+    for (int r = 0; r < n; ++r) y0[(size_t)c * k + r] = Y[(size_t)r * n];
+  }
+  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
+  ops.combine(V, k, (const double*)Yb.ptr, 1, w);
+  ops.normalise(w, h, (double*)scl.ptr);
+  copy_d2d(x, w, nr * 8, s);
+  DMRGX_HIP(hipStreamSynchronize(s));
+  return th0;
+}
+
+}  // namespace dmrgx

@@ -779,46 +779,65 @@ bool qtensor<T>::get_map(vector<vector<int>>& map_ret, vector<vector<int>>& dim_
   return true;
 }
 
-// qtensor::svd, src/qtensor.cpp:1122-1262
+// qtensor::svd, src/qtensor.cpp:1122-1262, in two halves around the block
+// SVD so that several tensors can share one block_svd call (svd_batched).
+namespace {
+
 template <typename T>
-vector<double> qtensor<T>::svd(qtensor<T>& U, qtensor<double>& S, qtensor<T>& V, int num, int cutoff) {
-  DMRGX_CHECK(num > 0 && num < (int)dim_.size(), "svd: number of left legs out of range");
-  qtensor<T> comb = combine(num, (int)dim_.size() - 1).combine(0, num - 1);
+struct SvdJob {
+  qtensor<T> comb;
+  vector<SvdBlock> blocks;  // offsets relative to comb and to this job's U, S, V
+  vector<int> rows, cols, need_u, need_v;
+  int64_t uo = 0, vo = 0, so = 0;
+};
+
+template <typename T>
+SvdJob<T> svd_prepare(const qtensor<T>& t, int num, int cutoff) {
+  DMRGX_CHECK(num > 0 && num < (int)t.dim_.size(), "svd: number of left legs out of range");
+  SvdJob<T> j;
+  j.comb = t.combine(num, (int)t.dim_.size() - 1).combine(0, num - 1);
+  const qtensor<T>& comb = j.comb;
   DMRGX_CHECK(comb.nblocks() > 0, "svd: empty tensor");
   int sym0 = ss(comb.sym_[0][0], comb.sym_[0][1]);
   for (auto& s : comb.sym_) DMRGX_CHECK(ss(s[0], s[1]) == sym0, "svd: symmetry sectors do not have constraint");
   const int nsym = comb.nblocks();
-  // per-block SVD on the GPU
-  vector<SvdBlock> blocks(nsym);
-  vector<int> rows(nsym), cols(nsym);
-  int64_t uo = 0, vo = 0, so = 0;
+  j.blocks.resize(nsym);
+  j.rows.resize(nsym);
+  j.cols.resize(nsym);
   for (int i = 0; i < nsym; ++i) {
     vector<int> sh = comb.shape(i);
-    rows[i] = sh[0];
-    cols[i] = sh[1];
-    blocks[i] = SvdBlock{sh[0], sh[1], comb.off_[i], uo, so, vo};
-    uo += (int64_t)sh[0] * sh[0];
-    vo += (int64_t)sh[1] * sh[1];
-    so += std::min(sh[0], sh[1]);
+    j.rows[i] = sh[0];
+    j.cols[i] = sh[1];
+    j.blocks[i] = SvdBlock{sh[0], sh[1], comb.off_[i], j.uo, j.so, j.vo};
+    j.uo += (int64_t)sh[0] * sh[0];
+    j.vo += (int64_t)sh[1] * sh[1];
+    j.so += std::min(sh[0], sh[1]);
   }
-  vector<int> need_u(nsym), need_v(nsym);
+  j.need_u.resize(nsym);
+  j.need_v.resize(nsym);
   for (int i = 0; i < nsym; ++i) {
-    need_u[i] = cutoff ? std::min(rows[i], cutoff) : rows[i];
-    need_v[i] = cutoff ? std::min(cols[i], cutoff) : cols[i];
+    j.need_u[i] = cutoff ? std::min(j.rows[i], cutoff) : j.rows[i];
+    j.need_v[i] = cutoff ? std::min(j.cols[i], cutoff) : j.cols[i];
   }
-  DevBuf Ub(std::max<int64_t>(uo, 1) * sizeof(T)), Vb(std::max<int64_t>(vo, 1) * sizeof(T)),
-      Sb(std::max<int64_t>(so, 1) * 8);
-  block_svd(dtype_of<T>::value, comb.data(), Ub.ptr, (double*)Sb.ptr, Vb.ptr, blocks, need_u, need_v, rt().stream);
-  vector<double> sall(so);
-  if (so) {
-    DMRGX_HIP(hipMemcpyAsync(sall.data(), Sb.ptr, so * 8, hipMemcpyDeviceToHost, rt().stream));
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
-  }
+  return j;
+}
+
+// truncation and U, S, V from the blocks' factors: Ub / Vb (device) and sall
+// (host) hold this job's full U, V and singular values
+template <typename T>
+vector<double> svd_finish(const vector<vector<int>>& dim, const SvdJob<T>& j, const void* Ub, const void* Vb,
+                          const double* sall, qtensor<T>& U, qtensor<double>& S, qtensor<T>& V, int num,
+                          int cutoff) {
+  const qtensor<T>& comb = j.comb;
+  const vector<SvdBlock>& blocks = j.blocks;
+  const vector<int>& rows = j.rows;
+  const vector<int>& cols = j.cols;
+  const int nsym = comb.nblocks();
   vector<vector<double>> Sv(nsym);
   int total = 0;
   for (int i = 0; i < nsym; ++i) {
     int mn = std::min(rows[i], cols[i]);
-    Sv[i].assign(sall.begin() + blocks[i].s_off, sall.begin() + blocks[i].s_off + mn);
+    Sv[i].assign(sall + blocks[i].s_off, sall + blocks[i].s_off + mn);
     total += mn;
   }
   // global selection by repeated max scan (ties -> lowest sector, strict <)
@@ -899,8 +918,8 @@ vector<double> qtensor<T>::svd(qtensor<T>& U, qtensor<double>& S, qtensor<T>& V,
     int vr = std::min(vrows[i], cols[i]);
     if (vr && cols[i]) tv.push_back(BoxTask{blocks[i].v_off, cols[i], 0, V.off_[i], cols[i], 0, vr, 1, cols[i], 0});
   }
-  copy_boxes(dtype_of<T>::value, Ub.ptr, U.data(), tu, rt().stream);
-  copy_boxes(dtype_of<T>::value, Vb.ptr, V.data(), tv, rt().stream);
+  copy_boxes(dtype_of<T>::value, Ub, U.data(), tu, rt().stream);
+  copy_boxes(dtype_of<T>::value, Vb, V.data(), tv, rt().stream);
   // S blocks: diagonal singular values, resized like tensor::diag(2) + resize
   vector<double> hs(S.total_, 0.0);
   for (int i = 0; i < nsym; ++i) {
@@ -911,11 +930,87 @@ vector<double> qtensor<T>::svd(qtensor<T>& U, qtensor<double>& S, qtensor<T>& V,
     DMRGX_HIP(hipMemcpyAsync(S.data(), hs.data(), hs.size() * 8, hipMemcpyHostToDevice, rt().stream));
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
-  vector<vector<int>> dl(dim_.begin(), dim_.begin() + num), dr(dim_.begin() + num, dim_.end());
+  vector<vector<int>> dl(dim.begin(), dim.begin() + num), dr(dim.begin() + num, dim.end());
   U = U.split(0, dl);
   V = V.split(1, dr);
   return s;
 }
+
+}  // namespace
+
+template <typename T>
+vector<double> qtensor<T>::svd(qtensor<T>& U, qtensor<double>& S, qtensor<T>& V, int num, int cutoff) {
+  SvdJob<T> j = svd_prepare(*this, num, cutoff);
+  DevBuf Ub(std::max<int64_t>(j.uo, 1) * sizeof(T)), Vb(std::max<int64_t>(j.vo, 1) * sizeof(T)),
+      Sb(std::max<int64_t>(j.so, 1) * 8);
+  block_svd(dtype_of<T>::value, j.comb.data(), Ub.ptr, (double*)Sb.ptr, Vb.ptr, j.blocks, j.need_u, j.need_v,
+            rt().stream);
+  vector<double> sall(j.so);
+  if (j.so) {
+    DMRGX_HIP(hipMemcpyAsync(sall.data(), Sb.ptr, j.so * 8, hipMemcpyDeviceToHost, rt().stream));
+    DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  }
+  return svd_finish(dim_, j, Ub.ptr, Vb.ptr, sall.data(), U, S, V, num, cutoff);
+}
+
+template <typename T>
+vector<vector<double>> svd_batched(const vector<const qtensor<T>*>& t, vector<qtensor<T>>& U,
+                                   vector<qtensor<double>>& S, vector<qtensor<T>>& V, int num, int cutoff) {
+  const int n = (int)t.size();
+  hipStream_t st = rt().stream;
+  vector<SvdJob<T>> jobs;
+  jobs.reserve(n);
+  vector<int64_t> ab(n), ub(n), vb(n), sb(n);
+  int64_t at = 0, ut = 0, vt = 0, stt = 0;
+  for (int c = 0; c < n; ++c) {
+    jobs.push_back(svd_prepare(*t[c], num, cutoff));
+    ab[c] = at;
+    ub[c] = ut;
+    vb[c] = vt;
+    sb[c] = stt;
+    at += jobs[c].comb.total_;
+    ut += jobs[c].uo;
+    vt += jobs[c].vo;
+    stt += jobs[c].so;
+  }
+  // every tensor's blocks in one arena and one block_svd call
+  DevBuf Ab(std::max<int64_t>(at, 1) * sizeof(T)), Ub(std::max<int64_t>(ut, 1) * sizeof(T)),
+      Vb(std::max<int64_t>(vt, 1) * sizeof(T)), Sb(std::max<int64_t>(stt, 1) * 8);
+  vector<SvdBlock> blocks;
+  vector<int> need_u, need_v;
+  for (int c = 0; c < n; ++c) {
+    const SvdJob<T>& j = jobs[c];
+    if (j.comb.total_) copy_d2d((T*)Ab.ptr + ab[c], j.comb.data(), j.comb.total_ * sizeof(T), st);
+    for (auto b : j.blocks) {
+      b.a_off += ab[c];
+      b.u_off += ub[c];
+      b.v_off += vb[c];
+      b.s_off += sb[c];
+      blocks.push_back(b);
+    }
+    need_u.insert(need_u.end(), j.need_u.begin(), j.need_u.end());
+    need_v.insert(need_v.end(), j.need_v.begin(), j.need_v.end());
+  }
+  block_svd(dtype_of<T>::value, Ab.ptr, Ub.ptr, (double*)Sb.ptr, Vb.ptr, blocks, need_u, need_v, st);
+  vector<double> sall(stt);
+  if (stt) {
+    DMRGX_HIP(hipMemcpyAsync(sall.data(), Sb.ptr, stt * 8, hipMemcpyDeviceToHost, st));
+    DMRGX_HIP(hipStreamSynchronize(st));
+  }
+  U.resize(n);
+  S.resize(n);
+  V.resize(n);
+  vector<vector<double>> out(n);
+  for (int c = 0; c < n; ++c)
+    out[c] = svd_finish(t[c]->dim_, jobs[c], (const T*)Ub.ptr + ub[c], (const T*)Vb.ptr + vb[c],
+                        sall.data() + sb[c], U[c], S[c], V[c], num, cutoff);
+  return out;
+}
+
+template vector<vector<double>> svd_batched(const vector<const qtensor<double>*>&, vector<qtensor<double>>&,
+                                            vector<qtensor<double>>&, vector<qtensor<double>>&, int, int);
+template vector<vector<double>> svd_batched(const vector<const qtensor<cplx>*>&, vector<qtensor<cplx>>&,
+                                            vector<qtensor<double>>&, vector<qtensor<cplx>>&, int, int);
 
 // qtensor::eig (src/qtensor.cpp:1265-1313) for the tiny exact-diagonalisation
 // branch: eigenvalues ascending, lowest eigenvector; returns the block size.

@@ -110,6 +110,12 @@ class qtensor {
   qtensor& plus(const qtensor& A, const qtensor& B, T alpha = T(1), T beta = T(1));
 };
 
+// qtensor::svd of several tensors with one block SVD call over all their
+// blocks; results as t[c]->svd(U[c], S[c], V[c], num, cutoff)
+template <typename T>
+vector<vector<double>> svd_batched(const vector<const qtensor<T>*>& t, vector<qtensor<T>>& U,
+                                   vector<qtensor<double>>& S, vector<qtensor<T>>& V, int num, int cutoff);
+
 template <typename T>
 void generate_dim_sym(vector<vector<int>>& dim, vector<vector<int>>& sym, const qtensor<T>& lenv, int lnum,
                       const qtensor<T>& renv, int rnum, int sector);

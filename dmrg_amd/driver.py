@@ -11,7 +11,8 @@ MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
 STEP_LOG_WIDTH = 19
 DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym",
                   "dmrgx_run_dmrg_state", "dmrgx_state_norm", "dmrgx_state_overlap", "dmrgx_state_expect",
-                  "dmrgx_state_destroy", "dmrgx_state_derivative"]
+                  "dmrgx_state_destroy", "dmrgx_state_derivative", "dmrgx_run_batch"]
+BATCH_LOG_WIDTH = 11
 
 
 class RunConfig(C.Structure):
@@ -53,6 +54,9 @@ def declare(lib):
     lib.dmrgx_state_expect.restype = C.c_int
     lib.dmrgx_state_derivative.argtypes = [C.c_void_p, dp, dp, C.c_int, dp]
     lib.dmrgx_state_derivative.restype = C.c_int
+    lib.dmrgx_run_batch.argtypes = [C.POINTER(RunConfig), C.c_int, dp, dp, dp, C.POINTER(RunStats), dp, C.c_int,
+                                    C.POINTER(C.c_int)]
+    lib.dmrgx_run_batch.restype = C.c_int
     lib.dmrgx_state_destroy.argtypes = [C.c_void_p]
     lib.dmrgx_state_destroy.restype = C.c_int
     return lib
@@ -123,11 +127,9 @@ class RunResult:
     state: State | None = None
 
 
-def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
-             sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
-             verbose=False, keep_state=False) -> RunResult:
-    from . import PREC_NAMES, check, init, lib
-    init()
+def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo, refine_k,
+            verbose):
+    from . import PREC_NAMES
     cfg = RunConfig()
     cfg.model = MODELS[model]
     for i, v in enumerate(list(params)[:4]):
@@ -139,6 +141,16 @@ def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, sched
     cfg.ladder = 1 if ladder else 0
     cfg.seed = seed
     cfg.tol, cfg.tol_lo, cfg.refine_k, cfg.verbose = tol, tol_lo, refine_k, 1 if verbose else 0
+    return cfg
+
+
+def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
+             sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
+             verbose=False, keep_state=False) -> RunResult:
+    from . import check, init, lib
+    init()
+    cfg = _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo,
+                  refine_k, verbose)
     st = RunStats()
     cap = 20000
     log = np.zeros((cap, STEP_LOG_WIDTH))
@@ -149,6 +161,59 @@ def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, sched
     stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
     state = State(h, model) if keep_state else None
     return RunResult(st.energy, stats, log[:n.value].copy(), state)
+
+
+def splitmix64_stream(seed: int, n: int) -> np.ndarray:
+    """n successive splitmix64 outputs of the generator state `seed`
+    (state += 0x9E3779B97F4A7C15, then the xor-shift-multiply finaliser)."""
+    M = (1 << 64) - 1
+    out = np.zeros(n, dtype=np.uint64)
+    x = seed & M
+    for i in range(n):
+        x = (x + 0x9E3779B97F4A7C15) & M
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        out[i] = z ^ (z >> 31)
+    return out
+
+
+def batch_couplings(n_chains: int, seed: int) -> np.ndarray:
+    """(Jx, Jy, Jz) of n_chains XYZ chains, i.i.d. U[0.5, 1.5): 0.5 + (x >> 11) 2^-53
+    over one splitmix64 stream, chain-major."""
+    x = splitmix64_stream(seed, 3 * n_chains)
+    return (0.5 + (x >> np.uint64(11)).astype(np.float64) * 2.0 ** -53).reshape(n_chains, 3)
+
+
+@dataclass
+class BatchResult:
+    energies: np.ndarray  # last E0 of each chain
+    trunc: np.ndarray     # last truncation error of each chain
+    stats: dict
+    steps: np.ndarray     # (n_steps, BATCH_LOG_WIDTH)
+
+
+def run_batch(model: str, L: int, m: int, sweeps: int, params, sector=0, prec="fp64", sign_rule="physical",
+              ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8, verbose=False) -> BatchResult:
+    """dmrgx_run_batch: chains with couplings params[i] swept in lockstep."""
+    from . import check, init, lib
+    init()
+    p = np.zeros((len(params), 4))
+    for i, q in enumerate(params):
+        p[i, :len(q)] = q
+    cfg = _config(model, L, m, sweeps, (), sector, "dmrg2", prec, sign_rule, ladder, seed, tol, tol_lo, refine_k,
+                  verbose)
+    n = len(params)
+    e, tr = np.zeros(n), np.zeros(n)
+    st = RunStats()
+    cap = 20000
+    log = np.zeros((cap, BATCH_LOG_WIDTH))
+    ns = C.c_int()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    check(lib().dmrgx_run_batch(C.byref(cfg), n, dp(p), dp(e), dp(tr), C.byref(st), dp(log), cap, C.byref(ns)),
+          "dmrgx_run_batch")
+    stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
+    return BatchResult(e, tr, stats, log[:ns.value].copy())
 
 
 def encode_desc(dim, sym, dirs=None):

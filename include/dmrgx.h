@@ -105,6 +105,23 @@ int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
                 uint64_t seed, const void* v0, double* evals, void* evec,
                 dmrgx_heff* refine, int refine_k, int* info);
 
+/* dmrgx_eigsh on many independent operators at once: `plan` maps the
+ * concatenation of n_chains vectors (chain c: len[c] elements, in order) to
+ * itself without coupling them (a plan built from all chains' get_map tasks
+ * with shifted offsets).  Each chain starts from its own seeded vector
+ * (seeds[c]) and follows dmrgx_eigsh's algorithm and convergence test; the
+ * chains' Gram-Schmidt reductions and vector updates are segmented launches
+ * and every matvec is one call of `plan` for all chains; all chains take the
+ * same number of steps until the last one has converged.  Chains of at most
+ * 64 elements are diagonalised densely.  evals: n_chains x nev; evec: the
+ * concatenated lowest vectors; refine / refine_k as in dmrgx_eigsh (one
+ * batched polish); info: n_chains x {matvecs, restarts, converged, polish
+ * matvecs}.  No counterpart in the reference (one ARPACK call per chain,
+ * src/dsaupd.cpp:65-155, src/znaupd.cpp:78-197). */
+int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int nev, int ncv, double tol,
+                        int max_restart, const uint64_t* seeds, double* evals, void* evec, dmrgx_heff* refine,
+                        int refine_k, int* info);
+
 /* ---- blockwise SVD -----------------------------------------------------------
  * Replaces the per-sector tensor<T>::svd -> ?gesvd loop of qtensor<T>::svd
  * (src/qtensor.cpp:1167-1176, src/tensor.cpp:736-785, src/dgesvd.cpp:32-97,

@@ -54,6 +54,24 @@ typedef struct dmrgx_run_stats {
 int dmrgx_run_dmrg(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, double* step_log, int log_cap,
                    int* n_steps);
 
+/* ---- many chains in lockstep ----------------------------------------------------
+ * dmrgx_run_batch sweeps n_chains independent chains of cfg->model (couplings
+ * p[4 i .. 4 i + 3] for chain i; cfg->p is ignored) with the two-site schedule
+ * (cfg->schedule must be DMRGX_SCHEDULE_DMRG2), every step of every chain in
+ * lockstep: one H_eff plan over all chains' get_map tasks (one grouped launch
+ * per pass), a segmented batched Lanczos over the concatenated two-site
+ * states, one block SVD over all chains' blocks.  The reference has no
+ * counterpart (one chain per process, src/functions.cpp:234-299 per chain);
+ * each chain's result is that of dmrgx_run_dmrg with its couplings, up to
+ * rounding and to the extra Lanczos steps a chain takes while others converge.
+ * energy[i], trunc[i]: chain i's last E0 and truncation error.  stats sum the
+ * batch (matvecs = batched matvec calls, matvec_flops = all chains).  One row
+ * per batch step: l, r, summed dimension, matvec calls, polish calls, flops,
+ * t_total, t_eig, t_svd, t_env, t_plan (plan construction, inside t_eig). */
+#define DMRGX_BATCH_LOG_WIDTH 11
+int dmrgx_run_batch(const dmrgx_run_config* cfg, int n_chains, const double* p, double* energy, double* trunc,
+                    dmrgx_run_stats* stats, double* step_log, int log_cap, int* n_steps);
+
 /* ---- finished states and observables ------------------------------------------
  * dmrgx_run_dmrg_state runs like dmrgx_run_dmrg and also returns the final MPS
  * (device-resident, the centre absorbed as mps::prep_calc does,
