@@ -88,6 +88,30 @@ mpo<T> mpo_difference(mpo<T>& a, mpo<T>& b, int L) {
   return m;
 }
 
+// one row of DMRGX_STEP_LOG_WIDTH doubles per step, at most cap rows
+void write_rows(const std::vector<StepLog>& lg, double* log, int cap) {
+  int k = 0;
+  for (auto& g : lg) {
+    if (!log || k >= cap) break;
+    double* row = log + (size_t)k * DMRGX_STEP_LOG_WIDTH;
+    row[0] = g.l;
+    row[1] = g.r;
+    for (int i = 0; i < NEV; ++i) row[2 + i] = g.e[i];
+    row[8] = g.trunc;
+    row[9] = g.entropy;
+    row[10] = g.n_matvec;
+    row[11] = g.n_polish;
+    row[12] = g.t_total;
+    row[13] = g.t_eig;
+    row[14] = g.t_svd;
+    row[15] = g.t_env;
+    row[16] = (double)g.d;
+    row[17] = g.svd_sweeps;
+    row[18] = g.t_block_svd;
+    ++k;
+  }
+}
+
 template <typename T>
 void run_engine(Engine<T>& eng, const dmrgx_run_config* c, dmrgx_run_stats* st, double* log, int cap, int* n) {
   using clk = std::chrono::steady_clock;
@@ -128,32 +152,13 @@ void run_engine(Engine<T>& eng, const dmrgx_run_config* c, dmrgx_run_stats* st, 
       st->last_sweep_seconds = last;
     }
   }
-  int k = 0;
-  for (auto& g : eng.log) {
-    if (!log || k >= cap) break;
-    double* row = log + (size_t)k * DMRGX_STEP_LOG_WIDTH;
-    row[0] = g.l;
-    row[1] = g.r;
-    for (int i = 0; i < NEV; ++i) row[2 + i] = g.e[i];
-    row[8] = g.trunc;
-    row[9] = g.entropy;
-    row[10] = g.n_matvec;
-    row[11] = g.n_polish;
-    row[12] = g.t_total;
-    row[13] = g.t_eig;
-    row[14] = g.t_svd;
-    row[15] = g.t_env;
-    row[16] = (double)g.d;
-    row[17] = g.svd_sweeps;
-    row[18] = g.t_block_svd;
-    ++k;
-  }
+  write_rows(eng.log, log, cap);
   if (n) *n = (int)eng.log.size();
 }
 
 template <typename T>
 void run_batch(BatchEngine<T>& b, const dmrgx_run_config* c, double* energy, double* trunc, dmrgx_run_stats* st,
-               double* log, int cap, int* n) {
+               double* log, int cap, int* n, double* chain_log) {
   using clk = std::chrono::steady_clock;
   auto t0 = clk::now();
   double e = b.dmrg2(c->m, c->sweeps, c->ladder != 0);
@@ -207,6 +212,9 @@ void run_batch(BatchEngine<T>& b, const dmrgx_run_config* c, double* energy, dou
     ++k;
   }
   if (n) *n = (int)b.log.size();
+  if (chain_log)
+    for (int i = 0; i < nc; ++i)
+      write_rows(b.chains[i]->log, chain_log + (size_t)i * cap * DMRGX_STEP_LOG_WIDTH, cap);
 }
 
 RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
@@ -313,7 +321,7 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double*
 }
 
 int dmrgx_run_batch(const dmrgx_run_config* c, int n_chains, const double* p, double* energy, double* trunc,
-                    dmrgx_run_stats* st, double* step_log, int log_cap, int* n_steps) {
+                    dmrgx_run_stats* st, double* step_log, int log_cap, int* n_steps, double* chain_log) {
   CAPI_BEGIN
   ensure_init();
   DMRGX_CHECK(c && st && p && n_chains > 0, "run_batch: null arguments or no chains");
@@ -324,12 +332,12 @@ int dmrgx_run_batch(const dmrgx_run_config* c, int n_chains, const double* p, do
     vector<mpo<cplx>> m;
     for (int i = 0; i < n_chains; ++i) m.push_back(model_mpo_c(c->model, p + 4 * i, c->L));
     BatchEngine<cplx> b(m, cfg);
-    run_batch(b, c, energy, trunc, st, step_log, log_cap, n_steps);
+    run_batch(b, c, energy, trunc, st, step_log, log_cap, n_steps, chain_log);
   } else {
     vector<mpo<double>> m;
     for (int i = 0; i < n_chains; ++i) m.push_back(model_mpo_r(c->model, p + 4 * i, c->L));
     BatchEngine<double> b(m, cfg);
-    run_batch(b, c, energy, trunc, st, step_log, log_cap, n_steps);
+    run_batch(b, c, energy, trunc, st, step_log, log_cap, n_steps, chain_log);
   }
   CAPI_END
 }

@@ -400,33 +400,29 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
       double* vj = V + (int64_t)j * ld;
       op(vj, w);
       res.n_matvec++;
-      // classical Gram-Schmidt against v_0..v_j: h = V^H w together with |w|^2,
-      // then |w'|^2 = |w|^2 - |h|^2.  One pass leaves components of relative
-      // size eps |w| / |w'| along V; a second pass is made only when
-      // |w'|^2 < 1e-8 |w|^2 (in ground-state Lanczos |w| / |w'| is about
-      // |alpha| / beta, normally below 1e2).  The last subtraction is fused with
-      // the normalisation of v_{j+1}.
-      vdots(V, ld, j + 1, w, n, cplx, h, (double*)partial.ptr, s, true);
-      DMRGX_HIP(hipMemcpyAsync(hh.data(), h, 2 * (j + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
+      // classical Gram-Schmidt against v_0..v_j, always two passes ("twice
+      // is enough", Giraud, Langou and Rozloznik 2005).  One pass amplifies
+      // the basis' loss of orthogonality by about |w| / |w'| per step, and
+      // DMRG operators have |alpha| >> beta (a spectrum far from zero), so a
+      // single pass loses orthogonality within a dozen steps.  The second
+      // pass's |w'|^2 comes with its coefficients; |w''|^2 = |w'|^2 - |h2|^2;
+      // both coefficient sets are fetched with one synchronisation and the
+      // last subtraction is fused with the normalisation of v_{j+1}.
+      vdots(V, ld, j + 1, w, n, cplx, h, (double*)partial.ptr, s, false);
+      gemv_sub(V, ld, j + 1, h, w, n, cplx, s);
+      vdots(V, ld, j + 1, w, n, cplx, h2, (double*)partial.ptr, s, true);
+      DMRGX_HIP(hipMemcpyAsync(hh.data(), h, 2 * (j + 1) * sizeof(double), hipMemcpyDeviceToHost, s));
+      DMRGX_HIP(hipMemcpyAsync(hh2.data(), h2, 2 * (j + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
       DMRGX_HIP(hipStreamSynchronize(s));
-      double alpha = hh[2 * j];
-      double nw2 = hh[2 * (j + 1)], hsq = 0;
-      for (int q = 0; q <= j; ++q) hsq += hh[2 * q] * hh[2 * q] + hh[2 * q + 1] * hh[2 * q + 1];
-      double b2 = nw2 - hsq;
-      const double* hfin = h;
-      if (!(b2 >= 1e-8 * nw2)) {
-        gemv_sub(V, ld, j + 1, h, w, n, cplx, s);
-        vdots(V, ld, j + 1, w, n, cplx, h2, (double*)partial.ptr, s, true);
-        DMRGX_HIP(hipMemcpyAsync(hh2.data(), h2, 2 * (j + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
-        DMRGX_HIP(hipStreamSynchronize(s));
-        alpha += hh2[2 * j];
-        nw2 = hh2[2 * (j + 1)];
-        hsq = 0;
-        for (int q = 0; q <= j; ++q) hsq += hh2[2 * q] * hh2[2 * q] + hh2[2 * q + 1] * hh2[2 * q + 1];
-        b2 = nw2 - hsq;
-        hfin = h2;
-        res.n_reorth++;
-      }
+      const double alpha = hh[2 * j] + hh2[2 * j];
+      const double nw2 = hh2[2 * (j + 1)];
+      double hsq = 0;
+      for (int q = 0; q <= j; ++q) hsq += hh2[2 * q] * hh2[2 * q] + hh2[2 * q + 1] * hh2[2 * q + 1];
+      const double b2 = nw2 - hsq;
+      double w2 = nw2;  // |w|^2 = |h|^2 + |w'|^2
+      for (int q = 0; q <= j; ++q) w2 += hh[2 * q] * hh[2 * q] + hh[2 * q + 1] * hh[2 * q + 1];
+      const double* hfin = h2;
+      res.n_reorth++;
       double beta = std::sqrt(std::max(b2, 0.0));
       T[(size_t)j * m + j] = alpha;
       if (j + 1 < m) {
@@ -434,7 +430,7 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
       }
       beta_last = beta;
       double* vn = V + (int64_t)(j + 1) * ld;
-      if (beta > 1e-13 * std::max(std::fabs(alpha), std::sqrt(std::max(nw2, 0.0))) && beta > 1e-300) {
+      if (beta > 1e-13 * std::max(std::fabs(alpha), std::sqrt(w2)) && beta > 1e-300) {
         if (cplx) hipLaunchKernelGGL(k_gemv_sub_scale<true>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,
                                      hfin, (const double*)w, vn, n, 1.0 / beta);
         else hipLaunchKernelGGL(k_gemv_sub_scale<false>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,

@@ -15,13 +15,13 @@ struct EigshResult {
   std::vector<double> evals;  // nev lowest, ascending
   int n_matvec = 0;
   int n_restart = 0;
-  int n_reorth = 0;  // steps that needed a second Gram-Schmidt pass
+  int n_reorth = 0;  // second Gram-Schmidt passes (one per step)
   bool converged = false;
   double max_resid = 0;
 };
 
 // Lowest `nev` eigenpairs of a Hermitian operator: thick-restart Lanczos with
-// full reorthogonalisation.  Converged when every wanted Ritz pair satisfies
+// full reorthogonalisation (two classical Gram-Schmidt passes per step).  Converged when every wanted Ritz pair satisfies
 // |beta * y_last| <= tol * max(|theta|, eps^(2/3)) (the ARPACK test).  `v0`
 // (device, may be null) is the start vector; otherwise a seeded random one.
 // `evec` (device, n elements) receives the lowest Ritz vector.

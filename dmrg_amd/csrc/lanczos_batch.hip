@@ -20,7 +20,12 @@ namespace dmrgx {
 namespace {
 
 constexpr int G = 8;      // Krylov columns per pass of the dot / combine kernels
-constexpr int CH = 4096;  // elements of one chain handled by one work-group
+// elements of one chain handled by one work-group: enough work-groups to
+// fill the chip (about 1024), between 256 and 4096 elements each
+inline int64_t seg_elems(int64_t total) {
+  int64_t c = (total / 1024 + 255) / 256 * 256;
+  return std::max<int64_t>(256, std::min<int64_t>(4096, c));
+}
 constexpr int64_t DENSE_MAX = 64;
 
 // element range [b, e) of the concatenated vector, owned by chain ordinal `chain`
@@ -171,6 +176,9 @@ struct Segs {
   Segs(const std::vector<int64_t>& off, const std::vector<int64_t>& len, const std::vector<int>& set,
        hipStream_t s) {
     n = (int)set.size();
+    int64_t total = 0;
+    for (int c : set) total += len[c];
+    const int64_t CH = seg_elems(total);
     std::vector<SegBlk> b;
     std::vector<int> hb(n), he(n);
     for (int a = 0; a < n; ++a) {
@@ -229,8 +237,9 @@ struct SegOps {
                               std::min(G, ky - c0), out, sg.B());
     }
   }
-  std::vector<double> fetch(const double* h) {
-    std::vector<double> o((size_t)sg.n * hs);
+  // the coefficient arrays of `count` consecutive sets starting at h
+  std::vector<double> fetch(const double* h, int count = 1) {
+    std::vector<double> o((size_t)count * sg.n * hs);
     DMRGX_HIP(hipMemcpyAsync(o.data(), h, o.size() * 8, hipMemcpyDeviceToHost, s));
     DMRGX_HIP(hipStreamSynchronize(s));
     return o;
@@ -350,12 +359,12 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
   SegOps ops(sg, cplx, ld, m + 1, s);
   const int hs = ops.hs;
   DevBuf Vb((size_t)(m + 1) * nr * 8), Tmp((size_t)m * nr * 8);
-  DevBuf hbuf((size_t)nb * hs * 8), h2buf((size_t)nb * hs * 8), scl((size_t)nb * 8);
+  DevBuf hbuf((size_t)2 * nb * hs * 8), scl((size_t)nb * 8);
   DevBuf Yb((size_t)nb * m * std::max(keep, 1) * 8);
   double* V = (double*)Vb.ptr;
   double* w = (double*)Tmp.ptr;
   double* h = (double*)hbuf.ptr;
-  double* h2 = (double*)h2buf.ptr;
+  double* h2 = h + (size_t)nb * hs;
   // the small chains' segments stay zero in every Krylov vector
   if (!small.empty()) {
     fill_zero(V, (size_t)(m + 1) * nr * 8, s);
@@ -379,36 +388,26 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
     for (int j = j0; j < m; ++j) {
       op(V + (int64_t)j * ld, w);
       ++n_matvec;
-      // one classical Gram-Schmidt pass fused with |w|^2; a second pass (for
-      // every chain) when any chain shows cancellation |w'|^2 < 1e-8 |w|^2
+      // two classical Gram-Schmidt passes (as in eigsh); the second pass's
+      // |w'|^2 comes with its coefficients, both sets in one synchronisation
       ops.vdots(V, j + 1, w, h);
-      auto hh = ops.fetch(h);
-      bool second = false;
+      ops.sub(V, j + 1, h, w, w, nullptr);
+      ops.vdots(V, j + 1, w, h2);
+      auto hh = ops.fetch(h, 2);  // h and h2 are adjacent
       for (int a = 0; a < nb; ++a) {
         const double* q = hh.data() + (size_t)a * hs;
-        double hsq = 0;
-        for (int i = 0; i <= j; ++i) hsq += q[2 * i] * q[2 * i] + q[2 * i + 1] * q[2 * i + 1];
-        alpha[a] = q[2 * j];
-        nw2[a] = q[2 * (j + 1)];
-        b2[a] = nw2[a] - hsq;
-        if (!(b2[a] >= 1e-8 * nw2[a])) second = true;
-      }
-      const double* hfin = h;
-      if (second) {
-        ops.sub(V, j + 1, h, w, w, nullptr);
-        ops.vdots(V, j + 1, w, h2);
-        auto hh2 = ops.fetch(h2);
-        for (int a = 0; a < nb; ++a) {
-          const double* q = hh2.data() + (size_t)a * hs;
-          double hsq = 0;
-          for (int i = 0; i <= j; ++i) hsq += q[2 * i] * q[2 * i] + q[2 * i + 1] * q[2 * i + 1];
-          alpha[a] += q[2 * j];
-          nw2[a] = q[2 * (j + 1)];
-          b2[a] = nw2[a] - hsq;
-          n_reorth[a]++;
+        const double* q2 = hh.data() + (size_t)(nb + a) * hs;
+        double hsq = 0, w2 = q2[2 * (j + 1)];
+        for (int i = 0; i <= j; ++i) {
+          hsq += q2[2 * i] * q2[2 * i] + q2[2 * i + 1] * q2[2 * i + 1];
+          w2 += q[2 * i] * q[2 * i] + q[2 * i + 1] * q[2 * i + 1];
         }
-        hfin = h2;
+        alpha[a] = q[2 * j] + q2[2 * j];
+        nw2[a] = w2;
+        b2[a] = q2[2 * (j + 1)] - hsq;
+        n_reorth[a]++;
       }
+      const double* hfin = h2;
       std::vector<int> lost;
       for (int a = 0; a < nb; ++a) {
         double beta = std::sqrt(std::max(b2[a], 0.0));
@@ -514,12 +513,12 @@ std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<in
   Segs sg(off, len, all, s);
   SegOps ops(sg, cplx, nr, k + 1, s);
   const int hs = ops.hs;
-  DevBuf Vb((size_t)(k + 1) * nr * 8), wb(nr * 8), hbuf((size_t)nc * hs * 8), h2buf((size_t)nc * hs * 8),
-      scl((size_t)nc * 8), Yb((size_t)nc * k * 8);
+  DevBuf Vb((size_t)(k + 1) * nr * 8), wb(nr * 8), hbuf((size_t)2 * nc * hs * 8), scl((size_t)nc * 8),
+      Yb((size_t)nc * k * 8);
   double* V = (double*)Vb.ptr;
   double* w = (double*)wb.ptr;
   double* h = (double*)hbuf.ptr;
-  double* h2 = (double*)h2buf.ptr;
+  double* h2 = h + (size_t)nc * hs;
   copy_d2d(V, x, nr * 8, s);
   ops.normalise(V, h, (double*)scl.ptr);
   // chain c's Krylov space stops growing at kk[c]; its later vectors are zero
@@ -533,11 +532,10 @@ std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<in
     ops.vdots(V, j + 1, w, h);
     ops.sub(V, j + 1, h, w, w, nullptr);
     ops.vdots(V, j + 1, w, h2);
-    auto hh = ops.fetch(h);
-    auto hh2 = ops.fetch(h2);
+    auto hh = ops.fetch(h, 2);  // h and h2 are adjacent
     for (int c = 0; c < nc; ++c) {
       const double* q = hh.data() + (size_t)c * hs;
-      const double* q2 = hh2.data() + (size_t)c * hs;
+      const double* q2 = hh.data() + (size_t)(nc + c) * hs;
       sc[c] = 0;
       if (j >= kk[c]) continue;
       const double a = q[2 * j] + q2[2 * j];

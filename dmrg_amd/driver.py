@@ -55,7 +55,7 @@ def declare(lib):
     lib.dmrgx_state_derivative.argtypes = [C.c_void_p, dp, dp, C.c_int, dp]
     lib.dmrgx_state_derivative.restype = C.c_int
     lib.dmrgx_run_batch.argtypes = [C.POINTER(RunConfig), C.c_int, dp, dp, dp, C.POINTER(RunStats), dp, C.c_int,
-                                    C.POINTER(C.c_int)]
+                                    C.POINTER(C.c_int), dp]
     lib.dmrgx_run_batch.restype = C.c_int
     lib.dmrgx_state_destroy.argtypes = [C.c_void_p]
     lib.dmrgx_state_destroy.restype = C.c_int
@@ -191,6 +191,7 @@ class BatchResult:
     trunc: np.ndarray     # last truncation error of each chain
     stats: dict
     steps: np.ndarray     # (n_steps, BATCH_LOG_WIDTH)
+    chain_steps: np.ndarray  # (n_chains, n_chain_steps, STEP_LOG_WIDTH)
 
 
 def run_batch(model: str, L: int, m: int, sweeps: int, params, sector=0, prec="fp64", sign_rule="physical",
@@ -206,14 +207,16 @@ def run_batch(model: str, L: int, m: int, sweeps: int, params, sector=0, prec="f
     n = len(params)
     e, tr = np.zeros(n), np.zeros(n)
     st = RunStats()
-    cap = 20000
+    cap = 2 * L * (sweeps + m // 100 + 3)  # above the schedule's step count
     log = np.zeros((cap, BATCH_LOG_WIDTH))
+    clog = np.zeros((n, cap, STEP_LOG_WIDTH))
     ns = C.c_int()
     dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
-    check(lib().dmrgx_run_batch(C.byref(cfg), n, dp(p), dp(e), dp(tr), C.byref(st), dp(log), cap, C.byref(ns)),
-          "dmrgx_run_batch")
+    check(lib().dmrgx_run_batch(C.byref(cfg), n, dp(p), dp(e), dp(tr), C.byref(st), dp(log), cap, C.byref(ns),
+                                dp(clog)), "dmrgx_run_batch")
     stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
-    return BatchResult(e, tr, stats, log[:ns.value].copy())
+    k = min(ns.value, cap)
+    return BatchResult(e, tr, stats, log[:k].copy(), clog[:, :k].copy())
 
 
 def encode_desc(dim, sym, dirs=None):

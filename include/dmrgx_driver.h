@@ -67,10 +67,13 @@ int dmrgx_run_dmrg(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, double* 
  * energy[i], trunc[i]: chain i's last E0 and truncation error.  stats sum the
  * batch (matvecs = batched matvec calls, matvec_flops = all chains).  One row
  * per batch step: l, r, summed dimension, matvec calls, polish calls, flops,
- * t_total, t_eig, t_svd, t_env, t_plan (plan construction, inside t_eig). */
+ * t_total, t_eig, t_svd, t_env, t_plan (plan construction, inside t_eig).
+ * chain_log (may be NULL): n_chains x log_cap rows of DMRGX_STEP_LOG_WIDTH,
+ * each chain's own step log as dmrgx_run_dmrg writes it (its times are its
+ * 1/n_chains share of the batch step). */
 #define DMRGX_BATCH_LOG_WIDTH 11
 int dmrgx_run_batch(const dmrgx_run_config* cfg, int n_chains, const double* p, double* energy, double* trunc,
-                    dmrgx_run_stats* stats, double* step_log, int log_cap, int* n_steps);
+                    dmrgx_run_stats* stats, double* step_log, int log_cap, int* n_steps, double* chain_log);
 
 /* ---- finished states and observables ------------------------------------------
  * dmrgx_run_dmrg_state runs like dmrgx_run_dmrg and also returns the final MPS

@@ -131,3 +131,20 @@ def test_run_batch_mixed_precision(gpu, prec):
     ref = driver.run_batch("xyz", L, m, 2, J)
     mix = driver.run_batch("xyz", L, m, 2, J, prec=prec, tol_lo=1e-5, refine_k=10)
     assert np.max(np.abs(mix.energies - ref.energies)) < 1e-8, (mix.energies, ref.energies)
+
+
+def test_merged_plan_applies_each_chain(gpu):
+    """The merged plan maps every chain's segment as that chain's own plan does."""
+    print("case merged apply", flush=True)
+    cases = _cases()
+    h = _merged_plan(gpu, cases, "fp64")
+    rng = np.random.default_rng(3)
+    x = [rng.standard_normal(c.n_psi) + 1j * rng.standard_normal(c.n_psi) for c in cases]
+    X = gpu.DeviceArray.from_numpy(np.concatenate(x))
+    Y = gpu.DeviceArray(h.n_psi, np.complex128)
+    h.apply(X, Y)
+    y = Y.numpy()
+    off = np.cumsum([0] + [c.n_psi for c in cases])
+    for k, c in enumerate(cases):
+        ref = c.oracle_apply(x[k])
+        assert np.max(np.abs(y[off[k]:off[k + 1]] - ref)) < 1e-12 * np.max(np.abs(ref)), k

@@ -5,7 +5,7 @@ import pytest
 import scipy.linalg
 
 from oracle.dmrg import dense_heff
-from tests.cases import dmrg_case
+from tests.cases import dmrg_case, dmrg_step_case
 
 pytestmark = pytest.mark.gpu
 
@@ -51,3 +51,31 @@ def test_eigsh_mixed_with_fp64_polish(gpu, prec):
     ev, vec, info = lo.eigsh(nev=6, tol=1e-6 if prec == "fp32" else 1e-4, refine=hi, refine_k=8)
     assert abs(ev[0] - w[0]) < 1e-8 * max(1.0, abs(w[0])), (ev[0], w[0])
     assert info[3] > 0
+
+
+def test_eigsh_spectrum_far_from_zero(gpu):
+    """A DMRG operator whose spectrum lies far from zero (all eigenvalues
+    within [-38, -10], so |alpha| is about 5 beta at every Lanczos step): the
+    warm-up step l=8 of an open XYZ chain L=32 at bond dimension 10.  One
+    classical Gram-Schmidt pass per step loses orthogonality here within a
+    dozen steps and the convergence test then passes on wrong Ritz pairs."""
+    print("case eigsh far-from-zero spectrum", flush=True)
+    case = dmrg_step_case("xyz", 32, 128, 8, jx=1.1268656294706476, jy=1.4919361875572013, jz=1.0733433358684978)
+    H = dense_heff(case.ctx, case.n_psi, case.dtype)
+    w = scipy.linalg.eigvalsh(H)
+    assert w[-1] < 0 and abs(w[0]) > 30
+    h = _plan(gpu, case, "fp64")
+    ev, vec, info = h.eigsh(nev=6, tol=1e-10, max_restart=5000)
+    assert info[2] == 1
+    assert np.max(np.abs(ev - w[:6])) < 1e-9 * abs(w[0]), (ev, w[:6])
+    x = vec.numpy()
+    assert np.linalg.norm(H @ x - ev[0] * x) < 1e-7 * abs(ev[0])
+    # the same operator three times in one batched solve
+    from tests.test_gpu_batch import _merged_plan
+    hb = _merged_plan(gpu, [case] * 3, "fp64")
+    evb, vb, infob = hb.eigsh_batched([case.n_psi] * 3, nev=6, tol=1e-10, seeds=[1, 2, 3])
+    xb = vb.numpy().reshape(3, -1)
+    for k in range(3):
+        assert infob[k, 2] == 1
+        assert np.max(np.abs(evb[k] - w[:6])) < 1e-9 * abs(w[0]), (k, evb[k], w[:6])
+        assert np.linalg.norm(H @ xb[k] - evb[k, 0] * xb[k]) < 1e-7 * abs(w[0])

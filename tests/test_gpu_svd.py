@@ -111,3 +111,28 @@ def test_block_svd_large_graded_sectors(gpu, cplx):
         assert np.max(np.abs((u[:, :k] * s) @ v[:k, :] - a)) < 1e-12 * np.max(np.abs(a))
         assert np.max(np.abs(u.conj().T @ u - np.eye(u.shape[0]))) < 1e-12
         assert np.max(np.abs(v @ v.conj().T - np.eye(v.shape[0]))) < 1e-12
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_block_svd_many_blocks(gpu, cplx):
+    """128 sector blocks in one call (the blocks of 64 chains' two-site states):
+    every block is factorised as if it were alone."""
+    rng = np.random.default_rng(5)
+
+    def graded(r, c, decay):
+        k = min(r, c)
+        q1 = np.linalg.qr(rng.standard_normal((r, k)) + (1j * rng.standard_normal((r, k)) if cplx else 0))[0]
+        q2 = np.linalg.qr(rng.standard_normal((c, k)) + (1j * rng.standard_normal((c, k)) if cplx else 0))[0]
+        return (q1 * np.exp(-decay * np.arange(k) / k)) @ q2.conj().T
+
+    shapes = [(128, 128), (128, 128), (96, 160), (160, 96)]
+    blocks = [graded(*shapes[i % 4], 10.0 + (i % 7)) for i in range(128)]
+    print(f"case block_svd 128 blocks cplx={cplx}", flush=True)
+    res = _svd(gpu, blocks, cplx)
+    for i, (a, (u, s, v)) in enumerate(zip(blocks, res)):
+        ref = np.linalg.svd(a, compute_uv=False)
+        assert np.max(np.abs(s - ref)) < 4 * max(a.shape) * 2.2e-16 * ref[0], i
+        k = len(s)
+        assert np.max(np.abs((u[:, :k] * s) @ v[:k, :] - a)) < 1e-12 * np.max(np.abs(a)), i
+        assert np.max(np.abs(u.conj().T @ u - np.eye(u.shape[0]))) < 1e-12, i
+        assert np.max(np.abs(v @ v.conj().T - np.eye(v.shape[0]))) < 1e-12, i
