@@ -116,9 +116,13 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   vector<uint64_t> seeds(n);
   for (int c = 0; c < n; ++c)
     seeds[c] = chains[c]->cfg.seed * 0x9E3779B97F4A7C15ull + (uint64_t)chains[c]->log.size() + 1;
-  DevBuf evec(P * sizeof(T));
+  // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64 polish
+  int64_t dmin = d[0];
+  for (auto x : d) dmin = std::min(dmin, x);
+  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, cfg.nev, dmin})) : 1;
+  DevBuf evec(P * sizeof(T) * pv);
   vector<EigshResult> r = eigsh_batched(op, d, cx, cfg.nev, ncv, tol, cfg.max_restart, seeds, nullptr,
-                                        (double*)evec.ptr);
+                                        (double*)evec.ptr, pv);
   int nm = 0;
   for (int c = 0; c < n; ++c) {
     for (int i = 0; i < NEV; ++i) lg[c].e[i] = i < (int)r[c].evals.size() ? r[c].evals[i] : 0.0;
@@ -132,7 +136,9 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
     HeffPlan* hp64 = plan64.get();
     MatVec op64 = [hp64, s](const double* x, double* y) { hp64->apply(x, y, s); };
     int pm = 0;
-    vector<double> e0 = krylov_polish_batched(op64, d, cx, cfg.refine_k, (double*)evec.ptr, &pm);
+    const int kx = std::max(1, cfg.refine_k);
+    vector<double> e0 = subspace_polish_batched(op64, d, cx, pv, kx, (double*)evec.ptr, cfg.tol,
+                                                cfg.polish_cycles, &pm);
     for (int c = 0; c < n; ++c) {
       lg[c].e[0] = e0[c];
       lg[c].n_polish = pm;

@@ -20,6 +20,7 @@ struct RunConfig {
   double tol = 1e-10;          // Lanczos tolerance (FP64 eigensolve)
   double tol_lo = 1e-6;        // Lanczos tolerance of a reduced-precision eigensolve
   int refine_k = 8;            // FP64 Krylov polish steps after a reduced-precision eigensolve
+  int polish_cycles = 30;      // polish cycles at most, until the FP64 residual passes `tol`
   bool warm_start = false;     // start Lanczos from the previous two-site state
   int nev = NEV;
   int max_restart = 5000;

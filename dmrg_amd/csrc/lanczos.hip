@@ -250,85 +250,8 @@ void dev_orthogonalise(const double* V, int64_t ld, int k, double* v, int64_t n,
   }
 }
 
-void host_symeig(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& v) {
-  v.assign((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
-  for (int sweep = 0; sweep < 100; ++sweep) {
-    double off = 0, diag = 0;
-    for (int p = 0; p < n; ++p) {
-      diag += a[(size_t)p * n + p] * a[(size_t)p * n + p];
-      for (int q = p + 1; q < n; ++q) off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
-    }
-    if (off <= 1e-32 * std::max(diag, 1e-300)) break;
-    for (int p = 0; p < n; ++p)
-      for (int q = p + 1; q < n; ++q) {
-        double apq = a[(size_t)p * n + q];
-        if (std::fabs(apq) < 1e-300) continue;
-        double app = a[(size_t)p * n + p], aqq = a[(size_t)q * n + q];
-        double tau = (aqq - app) / (2 * apq);
-        double t = (tau >= 0 ? 1.0 : -1.0) / (std::fabs(tau) + std::sqrt(1 + tau * tau));
-        double c = 1 / std::sqrt(1 + t * t), s = t * c;
-        for (int k = 0; k < n; ++k) {  // columns p,q
-          double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
-          a[(size_t)k * n + p] = c * akp - s * akq;
-          a[(size_t)k * n + q] = s * akp + c * akq;
-        }
-        for (int k = 0; k < n; ++k) {  // rows p,q
-          double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
-          a[(size_t)p * n + k] = c * apk - s * aqk;
-          a[(size_t)q * n + k] = s * apk + c * aqk;
-        }
-        for (int k = 0; k < n; ++k) {
-          double vkp = v[(size_t)k * n + p], vkq = v[(size_t)k * n + q];
-          v[(size_t)k * n + p] = c * vkp - s * vkq;
-          v[(size_t)k * n + q] = s * vkp + c * vkq;
-        }
-      }
-  }
-  std::vector<int> idx(n);
-  for (int i = 0; i < n; ++i) idx[i] = i;
-  std::sort(idx.begin(), idx.end(), [&](int x, int y) { return a[(size_t)x * n + x] < a[(size_t)y * n + y]; });
-  w.resize(n);
-  std::vector<double> vs((size_t)n * n);
-  for (int c = 0; c < n; ++c) {
-    w[c] = a[(size_t)idx[c] * n + idx[c]];
-    for (int k = 0; k < n; ++k) vs[(size_t)k * n + c] = v[(size_t)k * n + idx[c]];
-  }
-  v.swap(vs);
-}
-
-void host_heeig(int n, std::vector<cplx>& a, std::vector<double>& w, std::vector<cplx>& v) {
-  // embed the Hermitian matrix H = X + iY as the real symmetric [[X, -Y], [Y, X]]
-  int m = 2 * n;
-  std::vector<double> r((size_t)m * m);
-  for (int i = 0; i < n; ++i)
-    for (int j = 0; j < n; ++j) {
-      cplx h = 0.5 * (a[(size_t)i * n + j] + std::conj(a[(size_t)j * n + i]));
-      r[(size_t)i * m + j] = h.real();
-      r[(size_t)(i + n) * m + j + n] = h.real();
-      r[(size_t)i * m + j + n] = -h.imag();
-      r[(size_t)(i + n) * m + j] = h.imag();
-    }
-  std::vector<double> wr, vr;
-  host_symeig(m, r, wr, vr);
-  // eigenvalues come in pairs; take every other one
-  w.resize(n);
-  v.assign((size_t)n * n, 0.0);
-  for (int c = 0; c < n; ++c) {
-    w[c] = wr[2 * c];
-    double nrm = 0;
-    for (int k = 0; k < n; ++k) {
-      cplx z(vr[(size_t)k * m + 2 * c], vr[(size_t)(k + n) * m + 2 * c]);
-      v[(size_t)k * n + c] = z;
-      nrm += std::norm(z);
-    }
-    nrm = std::sqrt(nrm);
-    for (int k = 0; k < n; ++k) v[(size_t)k * n + c] /= nrm;
-  }
-}
-
 EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
-                  int max_restart, uint64_t seed, const double* v0, double* evec) {
+                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec) {
   ensure_init();
   hipStream_t s = rt().stream;
   EigshResult res;
@@ -359,12 +282,15 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
     std::vector<dmrgx::cplx> V;
     host_heeig((int)n, H, w, V);
     res.evals.assign(w.begin(), w.begin() + nev);
-    std::vector<double> x(nr);
-    for (int64_t i = 0; i < n; ++i) {
-      if (cplx) { x[2 * i] = V[(size_t)i * n].real(); x[2 * i + 1] = V[(size_t)i * n].imag(); }
-      else x[i] = V[(size_t)i * n].real();
-    }
-    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), nr * 8, hipMemcpyHostToDevice, s));
+    const int nv = (int)std::min<int64_t>(std::max(n_vec, 1), n);
+    std::vector<double> x((size_t)std::max(n_vec, 1) * nr, 0.0);
+    for (int c = 0; c < nv; ++c)
+      for (int64_t i = 0; i < n; ++i) {
+        double* xc = x.data() + (size_t)c * nr;
+        if (cplx) { xc[2 * i] = V[(size_t)i * n + c].real(); xc[2 * i + 1] = V[(size_t)i * n + c].imag(); }
+        else xc[i] = V[(size_t)i * n + c].real();
+      }
+    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), x.size() * 8, hipMemcpyHostToDevice, s));
     DMRGX_HIP(hipStreamSynchronize(s));
     res.converged = true;
     return res;
@@ -483,21 +409,28 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
     // alpha comes from the Gram-Schmidt coefficients like any other step
   }
   res.evals.assign(theta.begin(), theta.begin() + nev);
-  // lowest Ritz vector
-  std::vector<double> y0(m);
-  for (int r = 0; r < m; ++r) y0[r] = Y[(size_t)r * m + 0];
-  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), m * 8, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, m, (const double*)Yb.ptr, 1, 0, 1,
-                     (double*)Tmp.ptr, nr);
+  // the n_vec lowest Ritz vectors, the lowest normalised
+  const int nvec = std::max(1, std::min(n_vec, m));
+  std::vector<double> y0((size_t)m * nvec);
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < nvec; ++c) y0[(size_t)r * nvec + c] = Y[(size_t)r * m + c];
+  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
+  for (int c0 = 0; c0 < nvec; c0 += G)
+    hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, m, (const double*)Yb.ptr, nvec, c0,
+                       std::min(G, nvec - c0), (double*)Tmp.ptr, nr);
   double z = dev_norm2((double*)Tmp.ptr, n, cplx, s);
   hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, (const double*)Tmp.ptr, evec, nr, 1.0 / z);
+  if (nvec > 1) copy_d2d(evec + nr, (const double*)Tmp.ptr + nr, (size_t)(nvec - 1) * nr * 8, s);
   DMRGX_HIP(hipStreamSynchronize(s));
   DMRGX_HIP(hipGetLastError());
   return res;
 }
 
-double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec) {
-  ensure_init();
+namespace {
+
+// one Rayleigh-Ritz polish cycle in K_k(H, x); *resid = |beta_k y_k[0]|, the
+// residual norm of the returned Ritz pair (0 when the space became invariant)
+double polish_cycle(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec, double* resid) {
   hipStream_t s = rt().stream;
   const int64_t nr = n * (cplx ? 2 : 1);
   k = (int)std::max<int64_t>(1, std::min<int64_t>(k, n));
@@ -510,6 +443,7 @@ double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, i
   hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, x, V, nr, 1.0 / nx);
   std::vector<double> T((size_t)k * k, 0.0), hh(2 * (k + 2)), hh2(2 * (k + 2));
   int kk = k;
+  double beta_k = 0;
   for (int j = 0; j < k; ++j) {
     op(V + (int64_t)j * nr, w);
     if (n_matvec) (*n_matvec)++;
@@ -523,6 +457,7 @@ double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, i
     DMRGX_HIP(hipStreamSynchronize(s));
     T[(size_t)j * k + j] = hh[2 * j] + hh2[2 * j];
     double beta = std::sqrt(std::max(hh2[2 * (j + 1)], 0.0));
+    if (j + 1 == k) beta_k = beta;
     if (j + 1 < k) {
       if (beta < 1e-14 * std::fabs(T[(size_t)j * k + j]) || beta < 1e-300) { kk = j + 1; break; }
       T[(size_t)j * k + j + 1] = T[(size_t)(j + 1) * k + j] = beta;
@@ -542,7 +477,22 @@ double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, i
   double z = dev_norm2(w, n, cplx, s);
   hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, w, x, nr, 1.0 / z);
   DMRGX_HIP(hipStreamSynchronize(s));
+  *resid = kk < k ? 0.0 : std::fabs(beta_k * Y[(size_t)(kk - 1) * kk]);
   return th[0];
+}
+
+}  // namespace
+
+double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec, double tol,
+                     int max_cycles) {
+  ensure_init();
+  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+  double theta = 0, resid = 0;
+  for (int c = 0; c < std::max(1, max_cycles); ++c) {
+    theta = polish_cycle(op, n, cplx, k, x, n_matvec, &resid);
+    if (resid <= tol * std::max(std::fabs(theta), eps23)) break;
+  }
+  return theta;
 }
 
 }  // namespace dmrgx

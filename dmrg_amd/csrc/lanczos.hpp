@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "host_eig.hpp"
 
 namespace dmrgx {
 
@@ -24,14 +25,18 @@ struct EigshResult {
 // full reorthogonalisation (two classical Gram-Schmidt passes per step).  Converged when every wanted Ritz pair satisfies
 // |beta * y_last| <= tol * max(|theta|, eps^(2/3)) (the ARPACK test).  `v0`
 // (device, may be null) is the start vector; otherwise a seeded random one.
-// `evec` (device, n elements) receives the lowest Ritz vector.
+// `evec` (device, n_vec x n elements) receives the n_vec lowest Ritz vectors
+// (n_vec <= ncv), the lowest normalised.
 EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
-                  int max_restart, uint64_t seed, const double* v0, double* evec);
+                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec = 1);
 
 // Rayleigh-Ritz polish of an approximate lowest eigenvector in a small Krylov
 // space K_k(H, x) built with `op`; x is replaced by the polished vector and
-// the Ritz value returned.
-double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec);
+// the Ritz value returned.  Cycles repeat (each from the previous polished
+// vector) until the Ritz pair's residual |beta_k y_k| <= tol max(|theta|,
+// eps^(2/3)) (eigsh's test) or max_cycles cycles ran.
+double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec, double tol = 0,
+                     int max_cycles = 1);
 
 // device vector helpers (n = number of elements, complex if cplx)
 void dev_random(double* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s);
@@ -51,15 +56,20 @@ void dev_orthogonalise(const double* V, int64_t ld, int k, double* v, int64_t n,
 // at most 64 elements are diagonalised densely, as in eigsh.
 std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int nev,
                                        int ncv, double tol, int max_restart, const std::vector<uint64_t>& seed,
-                                       const double* v0, double* evec);
-// krylov_polish of every chain's segment of x at once; returns the Ritz values
+                                       const double* v0, double* evec, int n_vec = 1);
+// Rayleigh-Ritz polish of the lowest eigenpair of every chain in FP64 over
+// span{X} + K_k(H, x_0) (X: the p <= 8 lowest approximate eigenvectors of each
+// chain, p columns of the concatenated length, as eigsh writes them; the
+// Krylov vectors H x_0 .. H^k x_0 extend it): X resolves nearly degenerate
+// lowest pairs that a single-vector Krylov polish cannot.  Cycles
+// repeat from the new Ritz vectors until every chain's residual |H x - theta
+// x| <= tol max(|theta|, eps^(2/3)) or max_cycles ran.  X is overwritten
+// (column 0 = the polished lowest vector, normalised); returns the Ritz values.
+std::vector<double> subspace_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int p,
+                                            int k, double* X, double tol, int max_cycles, int* n_matvec);
+// krylov_polish of every chain's segment of x at once (cycles until every
+// chain passes the test); returns the Ritz values
 std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int k,
-                                          double* x, int* n_matvec);
-
-// Symmetric eigen-decomposition of a small dense matrix on the host
-// (cyclic Jacobi).  a: n x n row-major, overwritten; w ascending; v columns.
-void host_symeig(int n, std::vector<double>& a, std::vector<double>& w, std::vector<double>& v);
-// Hermitian version (complex, row-major), used by the tiny exact-diagonalisation branch.
-void host_heeig(int n, std::vector<cplx>& a, std::vector<double>& w, std::vector<cplx>& v);
+                                          double* x, int* n_matvec, double tol = 0, int max_cycles = 1);
 
 }  // namespace dmrgx

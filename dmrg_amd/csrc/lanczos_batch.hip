@@ -14,6 +14,7 @@
 #include <memory>
 
 #include "common.hpp"
+#include "host_pool.hpp"
 #include "lanczos.hpp"
 
 namespace dmrgx {
@@ -146,6 +147,49 @@ __global__ __launch_bounds__(256) void k_combine_seg(const double* __restrict__ 
   }
 }
 
+// out column c = sum_{j<m} V_j Z_chain[j][c] for c < g, Z_chain m x g complex
+// pairs (the real part only for real vectors)
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_combine_z_seg(const double* __restrict__ V, int64_t ld, int m,
+                                                       const double* __restrict__ Z, int g,
+                                                       double* __restrict__ out, const SegBlk* __restrict__ blks) {
+  const SegBlk B = blks[blockIdx.x];
+  const double* Zc = Z + (int64_t)B.chain * m * g * 2;
+  for (int64_t i = B.b + threadIdx.x; i < B.e; i += 256) {
+    double ar[G], ai[G];
+#pragma unroll
+    for (int c = 0; c < G; ++c) ar[c] = ai[c] = 0;
+    for (int j = 0; j < m; ++j) {
+      const double* v = V + (int64_t)j * ld;
+      if (CPLX) {
+        const double vr = v[2 * i], vi = v[2 * i + 1];
+#pragma unroll
+        for (int c = 0; c < G; ++c)
+          if (c < g) {
+            const double zr = Zc[2 * (j * g + c)], zi = Zc[2 * (j * g + c) + 1];
+            ar[c] += vr * zr - vi * zi;
+            ai[c] += vr * zi + vi * zr;
+          }
+      } else {
+        const double vr = v[i];
+#pragma unroll
+        for (int c = 0; c < G; ++c)
+          if (c < g) ar[c] += vr * Zc[2 * (j * g + c)];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < G; ++c)
+      if (c < g) {
+        if (CPLX) {
+          out[(int64_t)c * ld + 2 * i] = ar[c];
+          out[(int64_t)c * ld + 2 * i + 1] = ai[c];
+        } else {
+          out[(int64_t)c * ld + i] = ar[c];
+        }
+      }
+  }
+}
+
 // x holds e_j (and no longer e_{j-1}) in every listed chain with j < len;
 // one thread per listed chain
 __global__ void k_unit_seg(double* __restrict__ x, const int64_t* __restrict__ off, const int64_t* __restrict__ len,
@@ -209,9 +253,18 @@ struct SegOps {
   int hs;
   hipStream_t s;
   DevBuf partial;
-  SegOps(const Segs& g, bool c, int64_t l, int maxk, hipStream_t st)
+  int sets;
+  double* pin = nullptr;  // pinned staging: `sets` coefficient sets, then one scale per chain
+  SegOps(const Segs& g, bool c, int64_t l, int maxk, hipStream_t st, int nsets = 2)
       : sg(g), cplx(c), ld(l), hs(2 * (maxk + G)), s(st),
-        partial((size_t)std::max(g.nblk, 1) * ((maxk + G) / G + 1) * 2 * G * 8) {}
+        partial((size_t)std::max(g.nblk, 1) * ((maxk + G) / G + 1) * 2 * G * 8), sets(std::max(nsets, 2)) {
+    DMRGX_HIP(hipHostMalloc((void**)&pin, ((size_t)sets * sg.n * hs + sg.n) * 8, hipHostMallocDefault));
+  }
+  ~SegOps() {
+    if (pin) (void)hipHostFree(pin);
+  }
+  SegOps(const SegOps&) = delete;
+  SegOps& operator=(const SegOps&) = delete;
   // h[c] = (V[0..k)^H x_c, |x_c|^2), k <= maxk - 1
   void vdots(const double* V, int k, const double* x, double* h) {
     const int ng = (k + 1 + G - 1) / G;
@@ -237,24 +290,36 @@ struct SegOps {
                               std::min(G, ky - c0), out, sg.B());
     }
   }
-  // the coefficient arrays of `count` consecutive sets starting at h
-  std::vector<double> fetch(const double* h, int count = 1) {
-    std::vector<double> o((size_t)count * sg.n * hs);
-    DMRGX_HIP(hipMemcpyAsync(o.data(), h, o.size() * 8, hipMemcpyDeviceToHost, s));
+  // out columns [0, g) = V[0..m) Z_c, Z on the device (chains x m x g complex pairs), g <= 8
+  void combine_z(const double* V, int m, const double* Z, int g, double* out) {
+    if (cplx) hipLaunchKernelGGL(k_combine_z_seg<true>, dim3(sg.nblk), dim3(256), 0, s, V, ld, m, Z, g, out, sg.B());
+    else hipLaunchKernelGGL(k_combine_z_seg<false>, dim3(sg.nblk), dim3(256), 0, s, V, ld, m, Z, g, out, sg.B());
+  }
+  // the coefficient arrays of `count` (<= sets) consecutive sets starting at
+  // h, in pinned memory valid until the next fetch
+  const double* fetch(const double* h, int count = 1) {
+    DMRGX_CHECK(count <= sets, "batched Lanczos: staging too small");
+    DMRGX_HIP(hipMemcpyAsync(pin, h, (size_t)count * sg.n * hs * 8, hipMemcpyDeviceToHost, s));
     DMRGX_HIP(hipStreamSynchronize(s));
-    return o;
+    return pin;
+  }
+  // one scale per chain to the device array d
+  void put(const std::vector<double>& sc, double* d) {
+    double* p = pin + (size_t)sets * sg.n * hs;
+    std::copy(sc.begin(), sc.begin() + sg.n, p);
+    DMRGX_HIP(hipMemcpyAsync(d, p, sg.n * 8, hipMemcpyHostToDevice, s));
   }
   // x_c /= |x_c| for every chain of the set
   void normalise(double* x, double* h, double* scl) {
     vdots(nullptr, 0, x, h);
-    auto hh = fetch(h);
+    const double* hh = fetch(h);
     std::vector<double> sc(sg.n);
     for (int a = 0; a < sg.n; ++a) {
       const double n2 = hh[(size_t)a * hs];
       DMRGX_CHECK(n2 > 0, "batched Lanczos: zero vector");
       sc[a] = 1.0 / std::sqrt(n2);
     }
-    DMRGX_HIP(hipMemcpyAsync(scl, sc.data(), sg.n * 8, hipMemcpyHostToDevice, s));
+    put(sc, scl);
     sub(nullptr, 0, h, x, x, scl);
     DMRGX_HIP(hipStreamSynchronize(s));
   }
@@ -275,7 +340,7 @@ std::vector<int64_t> offsets(const std::vector<int64_t>& len) {
 
 std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int nev,
                                        int ncv, double tol, int max_restart, const std::vector<uint64_t>& seed,
-                                       const double* v0, double* evec) {
+                                       const double* v0, double* evec, int n_vec) {
   ensure_init();
   hipStream_t s = rt().stream;
   const int nc = (int)len.size();
@@ -319,7 +384,8 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
     std::vector<double> Hh((size_t)ns * nd * nd * 2);
     DMRGX_HIP(hipMemcpyAsync(Hh.data(), Hd.ptr, Hh.size() * 8, hipMemcpyDeviceToHost, s));
     DMRGX_HIP(hipStreamSynchronize(s));
-    std::vector<double> x(nr, 0.0);
+    const int nvs = std::max(n_vec, 1);
+    std::vector<double> x((size_t)nvs * nr, 0.0);
     for (int a = 0; a < ns; ++a) {
       const int c = small[a], n = (int)sl[a];
       std::vector<dmrgx::cplx> H((size_t)n * n);
@@ -334,17 +400,20 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
       res[c].evals.assign(w.begin(), w.begin() + std::min(nev, n));
       res[c].n_matvec = nd;
       res[c].converged = true;
-      for (int i = 0; i < n; ++i) {
-        if (cplx) {
-          x[2 * (off[c] + i)] = Vh[(size_t)i * n].real();
-          x[2 * (off[c] + i) + 1] = Vh[(size_t)i * n].imag();
-        } else {
-          x[off[c] + i] = Vh[(size_t)i * n].real();
+      for (int q = 0; q < std::min(nvs, n); ++q) {
+        double* xq = x.data() + (size_t)q * nr;
+        for (int i = 0; i < n; ++i) {
+          if (cplx) {
+            xq[2 * (off[c] + i)] = Vh[(size_t)i * n + q].real();
+            xq[2 * (off[c] + i) + 1] = Vh[(size_t)i * n + q].imag();
+          } else {
+            xq[off[c] + i] = Vh[(size_t)i * n + q].real();
+          }
         }
       }
     }
     // the small chains' eigenvectors; the Lanczos chains' segments follow below
-    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), nr * 8, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), x.size() * 8, hipMemcpyHostToDevice, s));
     DMRGX_HIP(hipStreamSynchronize(s));
     if (big.empty()) return res;
   }
@@ -360,7 +429,8 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
   const int hs = ops.hs;
   DevBuf Vb((size_t)(m + 1) * nr * 8), Tmp((size_t)m * nr * 8);
   DevBuf hbuf((size_t)2 * nb * hs * 8), scl((size_t)nb * 8);
-  DevBuf Yb((size_t)nb * m * std::max(keep, 1) * 8);
+  const int nv = std::max(1, std::min(n_vec, m));
+  DevBuf Yb((size_t)nb * m * std::max(keep, nv) * 8);
   double* V = (double*)Vb.ptr;
   double* w = (double*)Tmp.ptr;
   double* h = (double*)hbuf.ptr;
@@ -393,10 +463,10 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
       ops.vdots(V, j + 1, w, h);
       ops.sub(V, j + 1, h, w, w, nullptr);
       ops.vdots(V, j + 1, w, h2);
-      auto hh = ops.fetch(h, 2);  // h and h2 are adjacent
+      const double* hh = ops.fetch(h, 2);  // h and h2 are adjacent
       for (int a = 0; a < nb; ++a) {
-        const double* q = hh.data() + (size_t)a * hs;
-        const double* q2 = hh.data() + (size_t)(nb + a) * hs;
+        const double* q = hh + (size_t)a * hs;
+        const double* q2 = hh + (size_t)(nb + a) * hs;
         double hsq = 0, w2 = q2[2 * (j + 1)];
         for (int i = 0; i <= j; ++i) {
           hsq += q2[2 * i] * q2[2 * i] + q2[2 * i + 1] * q2[2 * i + 1];
@@ -420,7 +490,7 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
         beta_last[a] = beta;
         sc[a] = beta > 0 ? 1.0 / beta : 0.0;
       }
-      DMRGX_HIP(hipMemcpyAsync(scl.ptr, sc.data(), nb * 8, hipMemcpyHostToDevice, s));
+      ops.put(sc, (double*)scl.ptr);
       double* vn = V + (int64_t)(j + 1) * ld;
       ops.sub(V, j + 1, hfin, w, vn, (const double*)scl.ptr);
       for (int a : lost) {
@@ -435,11 +505,9 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
       }
       DMRGX_HIP(hipStreamSynchronize(s));  // sc is rewritten by the next step
     }
-    // Rayleigh-Ritz per chain
-    bool all = true;
-    for (int a = 0; a < nb; ++a) {
-      std::vector<double> A = T[a];
-      host_symeig(m, A, theta[a], Y[a]);
+    // Rayleigh-Ritz per chain, the chains spread over host threads
+    parallel_for(nb, [&](int a) {
+      host_symeig(m, T[a], theta[a], Y[a]);
       bool ok = true;
       double mr = 0;
       for (int i = 0; i < nev; ++i) {
@@ -449,8 +517,9 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
       }
       res[big[a]].max_resid = mr;
       conv[a] = ok;
-      all = all && ok;
-    }
+    });
+    bool all = true;
+    for (int a = 0; a < nb; ++a) all = all && conv[a];
     if (all || restart == max_restart) break;
     // thick restart of every chain: its `keep` lowest Ritz vectors and the residual
     std::vector<double> Yk((size_t)nb * m * keep);
@@ -472,16 +541,22 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
     j0 = keep;
     DMRGX_HIP(hipStreamSynchronize(s));  // Yk ends here
   }
-  // lowest Ritz vector of every Lanczos chain, normalised, into its segment of evec
-  std::vector<double> y0((size_t)nb * m);
+  // the nv lowest Ritz vectors of every Lanczos chain (the lowest normalised)
+  // into its segments of evec's columns
+  std::vector<double> y0((size_t)nb * m * nv);
   for (int a = 0; a < nb; ++a)
-    for (int r = 0; r < m; ++r) y0[(size_t)a * m + r] = Y[a][(size_t)r * m];
+    for (int r = 0; r < m; ++r)
+      for (int q = 0; q < nv; ++q) y0[((size_t)a * m + r) * nv + q] = Y[a][(size_t)r * m + q];
   DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
-  double* out = small.empty() ? evec : (double*)Tmp.ptr;
-  ops.combine(V, m, (const double*)Yb.ptr, 1, out);
+  double* out = (double*)Tmp.ptr;
+  ops.combine(V, m, (const double*)Yb.ptr, nv, out);
   ops.normalise(out, h, (double*)scl.ptr);
-  if (!small.empty())
-    for (int c : big) copy_d2d(evec + cw * off[c], out + cw * off[c], len[c] * cw * 8, s);
+  if (small.empty()) {
+    copy_d2d(evec, out, (size_t)nv * nr * 8, s);
+  } else {
+    for (int q = 0; q < nv; ++q)
+      for (int c : big) copy_d2d(evec + q * nr + cw * off[c], out + q * nr + cw * off[c], len[c] * cw * 8, s);
+  }
   DMRGX_HIP(hipStreamSynchronize(s));
   for (int a = 0; a < nb; ++a) {
     EigshResult& r = res[big[a]];
@@ -496,7 +571,7 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
 }
 
 std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int k,
-                                          double* x, int* n_matvec) {
+                                          double* x, int* n_matvec, double tol, int max_cycles) {
   ensure_init();
   hipStream_t s = rt().stream;
   const int nc = (int)len.size();
@@ -519,59 +594,176 @@ std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<in
   double* w = (double*)wb.ptr;
   double* h = (double*)hbuf.ptr;
   double* h2 = h + (size_t)nc * hs;
-  copy_d2d(V, x, nr * 8, s);
-  ops.normalise(V, h, (double*)scl.ptr);
-  // chain c's Krylov space stops growing at kk[c]; its later vectors are zero
-  std::vector<std::vector<double>> T(nc, std::vector<double>((size_t)k * k, 0.0));
-  std::vector<int> kk(nc, k);
-  std::vector<double> sc(nc);
-  for (int j = 0; j < k; ++j) {
-    op(V + (int64_t)j * nr, w);
-    if (n_matvec) (*n_matvec)++;
-    // two full Gram-Schmidt passes (as in krylov_polish); |w''|^2 = |w'|^2 - |h2|^2
-    ops.vdots(V, j + 1, w, h);
-    ops.sub(V, j + 1, h, w, w, nullptr);
-    ops.vdots(V, j + 1, w, h2);
-    auto hh = ops.fetch(h, 2);  // h and h2 are adjacent
-    for (int c = 0; c < nc; ++c) {
-      const double* q = hh.data() + (size_t)c * hs;
-      const double* q2 = hh.data() + (size_t)(nc + c) * hs;
-      sc[c] = 0;
-      if (j >= kk[c]) continue;
-      const double a = q[2 * j] + q2[2 * j];
-      T[c][(size_t)j * k + j] = a;
-      double hsq = 0;
-      for (int i = 0; i <= j; ++i) hsq += q2[2 * i] * q2[2 * i] + q2[2 * i + 1] * q2[2 * i + 1];
-      const double beta = std::sqrt(std::max(q2[2 * (j + 1)] - hsq, 0.0));
-      if (j + 1 < k) {
-        if (beta < 1e-14 * std::fabs(a) || beta < 1e-300 || j + 1 >= len[c]) {
-          kk[c] = j + 1;
-        } else {
-          T[c][(size_t)j * k + j + 1] = T[c][(size_t)(j + 1) * k + j] = beta;
-          sc[c] = 1.0 / beta;
+  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+  std::vector<double> th0(nc), resid(nc);
+  for (int cycle = 0; cycle < std::max(1, max_cycles); ++cycle) {
+    copy_d2d(V, x, nr * 8, s);
+    ops.normalise(V, h, (double*)scl.ptr);
+    // chain c's Krylov space stops growing at kk[c]; its later vectors are zero
+    std::vector<std::vector<double>> T(nc, std::vector<double>((size_t)k * k, 0.0));
+    std::vector<int> kk(nc, k);
+    std::vector<double> sc(nc), beta_k(nc, 0.0);
+    for (int j = 0; j < k; ++j) {
+      op(V + (int64_t)j * nr, w);
+      if (n_matvec) (*n_matvec)++;
+      // two full Gram-Schmidt passes (as in krylov_polish); |w''|^2 = |w'|^2 - |h2|^2
+      ops.vdots(V, j + 1, w, h);
+      ops.sub(V, j + 1, h, w, w, nullptr);
+      ops.vdots(V, j + 1, w, h2);
+      const double* hh = ops.fetch(h, 2);  // h and h2 are adjacent
+      for (int c = 0; c < nc; ++c) {
+        const double* q = hh + (size_t)c * hs;
+        const double* q2 = hh + (size_t)(nc + c) * hs;
+        sc[c] = 0;
+        if (j >= kk[c]) continue;
+        const double a = q[2 * j] + q2[2 * j];
+        T[c][(size_t)j * k + j] = a;
+        double hsq = 0;
+        for (int i = 0; i <= j; ++i) hsq += q2[2 * i] * q2[2 * i] + q2[2 * i + 1] * q2[2 * i + 1];
+        const double beta = std::sqrt(std::max(q2[2 * (j + 1)] - hsq, 0.0));
+        if (j + 1 == k) beta_k[c] = beta;
+        if (j + 1 < k) {
+          if (beta < 1e-14 * std::fabs(a) || beta < 1e-300 || j + 1 >= len[c]) {
+            kk[c] = j + 1;
+          } else {
+            T[c][(size_t)j * k + j + 1] = T[c][(size_t)(j + 1) * k + j] = beta;
+            sc[c] = 1.0 / beta;
+          }
         }
       }
+      if (j + 1 < k) {
+        ops.put(sc, (double*)scl.ptr);
+        ops.sub(V, j + 1, h2, w, V + (int64_t)(j + 1) * nr, (const double*)scl.ptr);
+        DMRGX_HIP(hipStreamSynchronize(s));
+      }
     }
-    if (j + 1 < k) {
-      DMRGX_HIP(hipMemcpyAsync(scl.ptr, sc.data(), nc * 8, hipMemcpyHostToDevice, s));
-      ops.sub(V, j + 1, h2, w, V + (int64_t)(j + 1) * nr, (const double*)scl.ptr);
+    std::vector<double> y0((size_t)nc * k, 0.0);
+    parallel_for(nc, [&](int c) {
+      const int n = kk[c];
+      std::vector<double> A((size_t)n * n), th, Y;
+      for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = T[c][(size_t)i * k + j];
+      host_symeig(n, A, th, Y);
+      th0[c] = th[0];
+      resid[c] = kk[c] < k ? 0.0 : std::fabs(beta_k[c] * Y[(size_t)(n - 1) * n]);
+      for (int r = 0; r < n; ++r) y0[(size_t)c * k + r] = Y[(size_t)r * n];
+    });
+    DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
+    ops.combine(V, k, (const double*)Yb.ptr, 1, w);
+    ops.normalise(w, h, (double*)scl.ptr);
+    copy_d2d(x, w, nr * 8, s);
+    DMRGX_HIP(hipStreamSynchronize(s));
+    bool done = true;
+    for (int c = 0; c < nc; ++c) done = done && resid[c] <= tol * std::max(std::fabs(th0[c]), eps23);
+    if (done) break;
+  }
+  return th0;
+}
+
+}  // namespace dmrgx
+
+namespace dmrgx {
+
+std::vector<double> subspace_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int p,
+                                            int k, double* X, double tol, int max_cycles, int* n_matvec) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  const int nc = (int)len.size();
+  DMRGX_CHECK(nc > 0, "subspace_polish: no chains");
+  const std::vector<int64_t> off = offsets(len);
+  const int64_t N = off.back() + len.back();
+  const int64_t nr = N * (cplx ? 2 : 1);
+  DMRGX_CHECK(p >= 1 && p <= G && k >= 0, "subspace_polish: 1 <= p <= 8 and k >= 0");
+  const int q = p + k;  // basis: X (p vectors) and the Krylov vectors H x_0, H^2 x_0, ... (k)
+  std::vector<int> all(nc);
+  for (int c = 0; c < nc; ++c) all[c] = c;
+  Segs sg(off, len, all, s);
+  SegOps ops(sg, cplx, nr, q + 1, s, q);
+  const int hs = ops.hs;
+  DevBuf Bb((size_t)q * nr * 8), HBb((size_t)q * nr * 8), tb(nr * 8), gbuf((size_t)q * nc * hs * 8),
+      scl((size_t)nc * 8), Zb((size_t)nc * q * p * 16);
+  double* Bv = (double*)Bb.ptr;
+  double* HB = (double*)HBb.ptr;
+  double* t = (double*)tb.ptr;
+  double* h = (double*)gbuf.ptr;
+  double* h2 = h + (size_t)nc * hs;
+  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+  std::vector<double> th0(nc, 0.0), sc(nc);
+  for (int cycle = 0; cycle < std::max(1, max_cycles); ++cycle) {
+    // orthonormal basis B (CGS2 per chain; a vector dependent on the earlier
+    // ones becomes zero and is left out of the projection) and HB = H B
+    std::vector<std::vector<char>> valid(nc, std::vector<char>(q, 0));
+    for (int j = 0; j < q; ++j) {
+      const double* src = j < p ? X + (size_t)j * nr : HB + (size_t)(j == p ? 0 : j - 1) * nr;
+      ops.vdots(Bv, j, src, h);
+      ops.sub(Bv, j, h, src, t, nullptr);
+      ops.vdots(Bv, j, t, h2);
+      const double* hh = ops.fetch(h, 2);
+      for (int c = 0; c < nc; ++c) {
+        const double* a = hh + (size_t)c * hs;
+        const double* b = hh + (size_t)(nc + c) * hs;
+        double hsq = 0;
+        for (int i = 0; i < j; ++i) hsq += b[2 * i] * b[2 * i] + b[2 * i + 1] * b[2 * i + 1];
+        const double w2 = a[2 * j], b2 = b[2 * j] - hsq;
+        const bool ok = w2 > 0 && b2 > 1e-20 * w2 && j < len[c];
+        valid[c][j] = ok;
+        sc[c] = ok ? 1.0 / std::sqrt(b2) : 0.0;
+      }
+      ops.put(sc, (double*)scl.ptr);
+      ops.sub(Bv, j, h2, t, Bv + (size_t)j * nr, (const double*)scl.ptr);
+      op(Bv + (size_t)j * nr, HB + (size_t)j * nr);
+      if (n_matvec) (*n_matvec)++;
       DMRGX_HIP(hipStreamSynchronize(s));
     }
+    // G = B^H H B per chain, one column per launch pair
+    for (int j = 0; j < q; ++j) ops.vdots(Bv, q, HB + (size_t)j * nr, h + (size_t)j * nc * hs);
+    const double* gg = ops.fetch(h, q);
+    std::vector<double> Z((size_t)nc * q * p * 2, 0.0);
+    parallel_for(nc, [&](int c) {
+      std::vector<int> idx;
+      for (int j = 0; j < q; ++j)
+        if (valid[c][j]) idx.push_back(j);
+      const int r = (int)idx.size();
+      std::vector<dmrgx::cplx> A((size_t)r * r), W;
+      for (int a = 0; a < r; ++a)
+        for (int b = 0; b < r; ++b) {
+          // column idx[b] of G, row idx[a]: <B_a, H B_b>
+          const double* col = gg + ((size_t)idx[b] * nc + c) * hs;
+          A[(size_t)a * r + b] = dmrgx::cplx(col[2 * idx[a]], col[2 * idx[a] + 1]);
+        }
+      std::vector<double> w;
+      host_heeig(r, A, w, W);
+      th0[c] = w[0];
+      for (int col = 0; col < std::min(p, r); ++col)
+        for (int a = 0; a < r; ++a) {
+          double* z = Z.data() + (((size_t)c * q + idx[a]) * p + col) * 2;
+          z[0] = W[(size_t)a * r + col].real();
+          z[1] = W[(size_t)a * r + col].imag();
+        }
+    });
+    // new X = B Z (p Ritz vectors) and the residual of the lowest, H x - theta x
+    DMRGX_HIP(hipMemcpyAsync(Zb.ptr, Z.data(), Z.size() * 8, hipMemcpyHostToDevice, s));
+    ops.combine_z(Bv, q, (const double*)Zb.ptr, p, X);
+    std::vector<double> z0((size_t)nc * q * 2);
+    for (int c = 0; c < nc; ++c)
+      for (int j = 0; j < q; ++j) {
+        z0[((size_t)c * q + j) * 2] = Z[(((size_t)c * q + j) * p) * 2];
+        z0[((size_t)c * q + j) * 2 + 1] = Z[(((size_t)c * q + j) * p) * 2 + 1];
+      }
+    DMRGX_HIP(hipMemcpyAsync(Zb.ptr, z0.data(), z0.size() * 8, hipMemcpyHostToDevice, s));
+    ops.combine_z(HB, q, (const double*)Zb.ptr, 1, t);
+    std::vector<double> th(nc * hs, 0.0);
+    for (int c = 0; c < nc; ++c) th[(size_t)c * hs] = th0[c];
+    DMRGX_HIP(hipMemcpyAsync(h, th.data(), th.size() * 8, hipMemcpyHostToDevice, s));
+    ops.sub(X, 1, h, t, t, nullptr);
+    ops.vdots(nullptr, 0, t, h2);
+    const double* rr = ops.fetch(h2, 1);
+    bool done = true;
+    for (int c = 0; c < nc; ++c)
+      done = done && std::sqrt(std::max(rr[(size_t)c * hs], 0.0)) <= tol * std::max(std::fabs(th0[c]), eps23);
+    ops.normalise(X, h, (double*)scl.ptr);
+    if (done) break;
   }
-  std::vector<double> y0((size_t)nc * k, 0.0), th0(nc);
-  for (int c = 0; c < nc; ++c) {
-    const int n = kk[c];
-    std::vector<double> A((size_t)n * n), th, Y;
-    for (int i = 0; i < n; ++i)
-      for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = T[c][(size_t)i * k + j];
-    host_symeig(n, A, th, Y);
-    th0[c] = th[0];
-    for (int r = 0; r < n; ++r) y0[(size_t)c * k + r] = Y[(size_t)r * n];
-  }
-  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
-  ops.combine(V, k, (const double*)Yb.ptr, 1, w);
-  ops.normalise(w, h, (double*)scl.ptr);
-  copy_d2d(x, w, nr * 8, s);
   DMRGX_HIP(hipStreamSynchronize(s));
   return th0;
 }
