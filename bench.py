@@ -12,7 +12,10 @@ accumulate) is the headline value; FP64 and FP32 run on the same shapes.
 Prints one JSON line.  Its ``sweep`` object is a full two-site DMRG of the
 same model at m=512 on a shorter chain (``--sweep-L``, default 20; 0 sweeps
 skips it), FP64 against BF16x3 with FP64 Lanczos polish: mean sweep seconds,
-E0 of both and their difference.  ``--part svd`` / ``--part copy`` time the
+E0 of both and their difference.  Its ``batch`` object is the 64-chain
+batched workload (open XYZ chains L=32, m=128, couplings U[0.5, 1.5) from a
+seeded splitmix64 stream) swept in lockstep, FP64 and BF16x3, beside the
+same chains run one at a time.  ``--part svd`` / ``--part copy`` time the
 step SVD alone / the 256 MiB copy used to calibrate the HBM byte counters.
 """
 from __future__ import annotations
@@ -219,6 +222,43 @@ def sweep_leg(L, sweeps):
     return res
 
 
+def batch_leg(n_chains=64, L=32, m=128, sweeps=1):
+    """The batched workload: n_chains independent open XYZ chains with
+    couplings i.i.d. U[0.5, 1.5) from splitmix64 (seed 20261203), swept in
+    lockstep (one H_eff plan and one block SVD over all chains per step,
+    segmented batched Lanczos), FP64 and BF16x3 with the FP64 polish; two
+    chains also run alone (FP64) for the sequential comparison."""
+    from dmrg_amd import driver
+    J = driver.batch_couplings(n_chains, 20261203)
+    res = {"chains": n_chains, "L": L, "m": m, "sweeps": sweeps, "coupling_seed": 20261203}
+    runs = {}
+    for prec in ("fp64", "bf16x3"):
+        t0 = time.perf_counter()
+        b = driver.run_batch("xyz", L, m, sweeps, J, prec=prec, tol_lo=1e-5, refine_k=8)
+        el = time.perf_counter() - t0
+        st = b.stats
+        runs[prec] = b
+        res[prec] = dict(total_s=round(el, 3), sweep_s=round(st["sweep_seconds"], 4), matvec_calls=st["matvecs"],
+                         polish_calls=st["polish_matvecs"], matvec_tflop=round(st["matvec_flops"] / 1e12, 3),
+                         t_eig=round(st["t_eig"], 3), t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3),
+                         chain_sweeps_per_s=round(n_chains * sweeps / max(st["sweep_seconds"] * sweeps, 1e-12), 3))
+        print(f"[bench] batch {prec}: {n_chains} chains L={L} m={m} in {el:.1f} s, sweep {st['sweep_seconds']:.2f} s",
+              file=sys.stderr, flush=True)
+    res["max_dE0_bf16x3_vs_fp64"] = float(np.max(np.abs(runs["bf16x3"].energies - runs["fp64"].energies)))
+    res["max_trunc_err"] = float(np.max(runs["fp64"].trunc))
+    single = []
+    for k in range(2):
+        t0 = time.perf_counter()
+        one = driver.run_dmrg("xyz", L, m, sweeps, J[k])
+        single.append(time.perf_counter() - t0)
+        res.setdefault("single_dE0", []).append(abs(one.energy - runs["fp64"].energies[k]))
+    res["single_fp64_s"] = round(float(np.mean(single)), 3)
+    res["sequential_fp64_s_extrapolated"] = round(float(np.mean(single)) * n_chains, 2)
+    res["speedup_vs_sequential"] = {p: round(res["sequential_fp64_s_extrapolated"] / res[p]["total_s"], 2)
+                                    for p in ("fp64", "bf16x3")}
+    return res
+
+
 def copy_leg(reps):
     """A plain copy of 256 MiB through dmrgx_copy_boxes (one box): the
     calibration kernel for the FETCH_SIZE / WRITE_SIZE counters."""
@@ -251,6 +291,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", type=int, default=1, help="full sweeps in the sweep comparison (0 skips it)")
     ap.add_argument("--sweep-L", type=int, default=20, help="chain length of the sweep comparison at m=512")
+    ap.add_argument("--batch", type=int, default=1, help="sweeps of the 64-chain batched workload (0 skips it)")
     ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
     ap.add_argument("--part", default="", choices=["", "svd", "copy"],
                     help="time one part alone: the step SVD, or the copy kernel used to calibrate PMC counters")
@@ -304,11 +345,14 @@ def main():
         "cpu_baseline": None,
         "configs": None,
         "sweep": None,
+        "batch": None,
     }
     if not a.no_configs:
         line["configs"] = configs_leg()
     if a.sweep:
         line["sweep"] = sweep_leg(a.sweep_L, a.sweep)
+    if a.batch:
+        line["batch"] = batch_leg(sweeps=a.batch)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.cpu_seconds)
     print(json.dumps(line), flush=True)

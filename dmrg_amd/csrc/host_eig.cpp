@@ -187,4 +187,15 @@ void host_heeig(int n, std::vector<cplx>& a, std::vector<double>& w, std::vector
   }
 }
 
+void host_hermitian_eig(int n, std::vector<cplx>& a, bool real, std::vector<double>& w, std::vector<cplx>& v) {
+  if (!real) {
+    host_heeig(n, a, w, v);
+    return;
+  }
+  std::vector<double> ar((size_t)n * n), vr;
+  for (size_t i = 0; i < ar.size(); ++i) ar[i] = a[i].real();
+  host_symeig(n, ar, w, vr);
+  v.assign(vr.begin(), vr.end());
+}
+
 }  // namespace dmrgx

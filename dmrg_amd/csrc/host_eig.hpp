@@ -16,5 +16,8 @@ void host_symeig(int n, std::vector<double>& a, std::vector<double>& w, std::vec
 // Hermitian version (complex, row-major) through the real symmetric embedding
 // [[X, -Y], [Y, X]] of X + iY; used by the tiny exact-diagonalisation branches.
 void host_heeig(int n, std::vector<cplx>& a, std::vector<double>& w, std::vector<cplx>& v);
+// host_heeig, or host_symeig on the real part when `real` (then the
+// eigenvectors are real, never a real vector times a phase)
+void host_hermitian_eig(int n, std::vector<cplx>& a, bool real, std::vector<double>& w, std::vector<cplx>& v);
 
 }  // namespace dmrgx

@@ -280,7 +280,7 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
     }
     std::vector<double> w;
     std::vector<dmrgx::cplx> V;
-    host_heeig((int)n, H, w, V);
+    host_hermitian_eig((int)n, H, !cplx, w, V);
     res.evals.assign(w.begin(), w.begin() + nev);
     const int nv = (int)std::min<int64_t>(std::max(n_vec, 1), n);
     std::vector<double> x((size_t)std::max(n_vec, 1) * nr, 0.0);

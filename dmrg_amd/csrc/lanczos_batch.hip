@@ -396,7 +396,7 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
         }
       std::vector<double> w;
       std::vector<dmrgx::cplx> Vh;
-      host_heeig(n, H, w, Vh);
+      host_hermitian_eig(n, H, !cplx, w, Vh);
       res[c].evals.assign(w.begin(), w.begin() + std::min(nev, n));
       res[c].n_matvec = nd;
       res[c].converged = true;
@@ -724,15 +724,16 @@ std::vector<double> subspace_polish_batched(const MatVec& op, const std::vector<
       for (int j = 0; j < q; ++j)
         if (valid[c][j]) idx.push_back(j);
       const int r = (int)idx.size();
+      DMRGX_CHECK(r > 0, "subspace_polish: empty basis");
       std::vector<dmrgx::cplx> A((size_t)r * r), W;
       for (int a = 0; a < r; ++a)
         for (int b = 0; b < r; ++b) {
           // column idx[b] of G, row idx[a]: <B_a, H B_b>
           const double* col = gg + ((size_t)idx[b] * nc + c) * hs;
-          A[(size_t)a * r + b] = dmrgx::cplx(col[2 * idx[a]], col[2 * idx[a] + 1]);
+          A[(size_t)a * r + b] = dmrgx::cplx(col[2 * idx[a]], cplx ? col[2 * idx[a] + 1] : 0.0);
         }
       std::vector<double> w;
-      host_heeig(r, A, w, W);
+      host_hermitian_eig(r, A, !cplx, w, W);
       th0[c] = w[0];
       for (int col = 0; col < std::min(p, r); ++col)
         for (int a = 0; a < r; ++a) {
