@@ -291,7 +291,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--sweep", type=int, default=1, help="full sweeps in the sweep comparison (0 skips it)")
     ap.add_argument("--sweep-L", type=int, default=20, help="chain length of the sweep comparison at m=512")
-    ap.add_argument("--batch", type=int, default=1, help="sweeps of the 64-chain batched workload (0 skips it)")
+    ap.add_argument("--batch", type=int, default=2, help="sweeps of the 64-chain batched workload (0 skips it)")
     ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
     ap.add_argument("--part", default="", choices=["", "svd", "copy"],
                     help="time one part alone: the step SVD, or the copy kernel used to calibrate PMC counters")
