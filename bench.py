@@ -10,9 +10,9 @@ library on the device.  The mixed-precision path (BF16x3 products, FP32
 accumulate) is the headline value; FP64 and FP32 run on the same shapes.
 
 Prints one JSON line.  Its ``sweep`` object is a full two-site DMRG of the
-same model at m=512 on a shorter chain (``--sweep-L``, default 20; 0 sweeps
-skips it), FP64 against BF16x3 with FP64 Lanczos polish: mean sweep seconds,
-E0 of both and their difference.  Its ``batch`` object is the 64-chain
+same model at m=1024 on a shorter chain (``--sweep-L``, default 24; 2 sweeps,
+the last one timed; 0 sweeps skips it), FP64 against BF16x3 with the FP64
+refinement: sweep seconds, E0 of both and their difference, fidelity.  Its ``batch`` object is the 64-chain
 batched workload (open XYZ chains L=32, m=128, couplings U[0.5, 1.5) from a
 seeded splitmix64 stream) swept in lockstep, FP64 and BF16x3, beside the
 same chains run one at a time.  ``--part svd`` / ``--part copy`` time the
@@ -193,24 +193,26 @@ def cpu_leg(seconds):
                       f"same m=512 Z2 D=5 shapes, {el:.1f} s"}
 
 
-def sweep_leg(L, sweeps):
-    """Full two-site DMRG (the reference's dmrg2 schedule: growth, m ladder,
-    sweeps, final half sweep) of the XYZ chain at m=512, FP64 and BF16x3 with
-    FP64 polish; sweep_s is the mean wall time of the full sweeps at m=512;
-    fidelity is |<psi_bf16x3|psi_fp64>| of the final states."""
+def sweep_leg(L, sweeps, m=1024):
+    """Full two-site DMRG (the reference's dmrg2 schedule: growth at cutoff 10,
+    then full sweeps at m; the pre-sweep cutoff ladder is skipped) of the XYZ
+    chain at m=1024, FP64 against BF16x3 with the FP64 Jacobi-Davidson
+    refinement; sweep_s is the wall time of the last full sweep (steady state
+    at m); fidelity is |<psi_bf16x3|psi_fp64>| of the final states."""
     from dmrg_amd import driver
-    res = {"L": L, "m": M, "sweeps": sweeps}
+    res = {"L": L, "m": m, "sweeps": sweeps, "timed": "last full sweep"}
     states = {}
     for prec in ("fp64", "bf16x3"):
         t0 = time.perf_counter()
-        r = driver.run_dmrg("xyz", L, M, sweeps, JXYZ, schedule="dmrg2", prec=prec, tol_lo=1e-5, refine_k=10,
-                            keep_state=True)
+        r = driver.run_dmrg("xyz", L, m, sweeps, JXYZ, schedule="dmrg2", prec=prec, tol_lo=1e-4, refine_k=8,
+                            ladder=False, keep_state=True)
         states[prec] = r.state
         st = r.stats
-        res[prec] = dict(energy=r.energy, sweep_s=round(st["sweep_seconds"], 4), total_s=round(time.perf_counter() - t0, 3),
-                         matvecs=st["matvecs"], polish=st["polish_matvecs"], trunc_err=st["trunc_err"],
-                         t_eig=round(st["t_eig"], 3), t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3))
-        print(f"[bench] sweep {prec} L={L}: E0={r.energy:.12f} sweep {st['sweep_seconds']:.2f} s "
+        res[prec] = dict(energy=r.energy, sweep_s=round(st["last_sweep_seconds"], 4),
+                         total_s=round(time.perf_counter() - t0, 3), matvecs=st["matvecs"],
+                         polish=st["polish_matvecs"], trunc_err=st["trunc_err"], t_eig=round(st["t_eig"], 3),
+                         t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3))
+        print(f"[bench] sweep {prec} L={L} m={m}: E0={r.energy:.12f} last sweep {st['last_sweep_seconds']:.2f} s "
               f"(eig {st['t_eig']:.2f} svd {st['t_svd']:.2f} env {st['t_env']:.2f} s in all steps)", file=sys.stderr,
               flush=True)
     res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
@@ -234,7 +236,7 @@ def batch_leg(n_chains=64, L=32, m=128, sweeps=1):
     runs = {}
     for prec in ("fp64", "bf16x3"):
         t0 = time.perf_counter()
-        b = driver.run_batch("xyz", L, m, sweeps, J, prec=prec, tol_lo=1e-5, refine_k=8)
+        b = driver.run_batch("xyz", L, m, sweeps, J, prec=prec, tol_lo=1e-4, refine_k=8)
         el = time.perf_counter() - t0
         st = b.stats
         runs[prec] = b
@@ -289,8 +291,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-configs", action="store_true", help="skip the Potts m=1024 / XYZ m=2048 matvec shapes")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--sweep", type=int, default=1, help="full sweeps in the sweep comparison (0 skips it)")
-    ap.add_argument("--sweep-L", type=int, default=20, help="chain length of the sweep comparison at m=512")
+    ap.add_argument("--sweep", type=int, default=2, help="full sweeps in the sweep comparison (0 skips it)")
+    ap.add_argument("--sweep-L", type=int, default=24, help="chain length of the sweep comparison at m=1024")
     ap.add_argument("--batch", type=int, default=2, help="sweeps of the 64-chain batched workload (0 skips it)")
     ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
     ap.add_argument("--part", default="", choices=["", "svd", "copy"],

@@ -119,9 +119,10 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64 polish
   int64_t dmin = d[0];
   for (auto x : d) dmin = std::min(dmin, x);
-  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, cfg.nev, dmin})) : 1;
+  const int nev = plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
+  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, dmin})) : 1;
   DevBuf evec(P * sizeof(T) * pv);
-  vector<EigshResult> r = eigsh_batched(op, d, cx, cfg.nev, ncv, tol, cfg.max_restart, seeds, nullptr,
+  vector<EigshResult> r = eigsh_batched(op, d, cx, nev, ncv, tol, cfg.max_restart, seeds, nullptr,
                                         (double*)evec.ptr, pv);
   int nm = 0;
   for (int c = 0; c < n; ++c) {
@@ -135,17 +136,18 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   if (plan64) {
     HeffPlan* hp64 = plan64.get();
     MatVec op64 = [hp64, s](const double* x, double* y) { hp64->apply(x, y, s); };
-    int pm = 0;
-    const int kx = std::max(1, cfg.refine_k);
-    vector<double> e0 = subspace_polish_batched(op64, d, cx, pv, kx, (double*)evec.ptr, cfg.tol,
-                                                cfg.polish_cycles, &pm);
+    int pm = 0, plo = 0;
+    vector<double> e0 = jd_polish_batched(op, op64, d, cx, pv, (double*)evec.ptr, cfg.tol, cfg.polish_cycles,
+                                          2 * std::max(1, cfg.refine_k), &pm, &plo);
     for (int c = 0; c < n; ++c) {
       lg[c].e[0] = e0[c];
       lg[c].n_polish = pm;
-      lg[c].flops += cflops[c] * pm;
+      lg[c].n_matvec += plo;
+      lg[c].flops += cflops[c] * (pm + plo);
     }
     bl.n_polish = pm;
-    bl.flops += plan64->flops * pm;
+    bl.n_matvec += plo;
+    bl.flops += plan64->flops * pm + plan->flops * plo;
   }
   for (int c = 0; c < n; ++c) psi[c] = qtensor<T>((const T*)evec.ptr + poff[c], dim[c], sym[c]);
   DMRGX_HIP(hipStreamSynchronize(s));

@@ -96,22 +96,24 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
   const bool cx = dt == DT_C64;
   const int ncv = (cx ? 6 : 4) * cfg.nev;  // src/dsaupd.cpp:76, src/znaupd.cpp:94
   // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64 polish
-  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, cfg.nev, d})) : 1;
+  const int nev = plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
+  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, d})) : 1;
   DevBuf evec(std::max<int64_t>(d, 1) * sizeof(T) * pv);
   const double tol = cfg.prec == PREC_FP64 ? cfg.tol : cfg.tol_lo;
   uint64_t seed = cfg.seed * 0x9E3779B97F4A7C15ull + (uint64_t)log.size() + 1;
-  EigshResult r = eigsh(op, d, cx, cfg.nev, ncv, tol, cfg.max_restart, seed, nullptr, (double*)evec.ptr, pv);
+  EigshResult r = eigsh(op, d, cx, nev, ncv, tol, cfg.max_restart, seed, nullptr, (double*)evec.ptr, pv);
   for (int i = 0; i < std::min<int>(NEV, (int)r.evals.size()); ++i) vals[i] = r.evals[i];
   lg.n_matvec = r.n_matvec;
   lg.flops = plan->flops * r.n_matvec;
   if (plan64) {
     HeffPlan* hp64 = plan64.get();
     MatVec op64 = [hp64, s](const double* x, double* y) { hp64->apply(x, y, s); };
-    int pm = 0;
-    const int kx = std::max(1, cfg.refine_k);
-    vals[0] = subspace_polish_batched(op64, {d}, cx, pv, kx, (double*)evec.ptr, cfg.tol, cfg.polish_cycles, &pm)[0];
+    int pm = 0, plo = 0;
+    vals[0] = jd_polish_batched(op, op64, {d}, cx, pv, (double*)evec.ptr, cfg.tol, cfg.polish_cycles,
+                                2 * std::max(1, cfg.refine_k), &pm, &plo)[0];
     lg.n_polish = pm;
-    lg.flops += plan64->flops * pm;
+    lg.n_matvec += plo;
+    lg.flops += plan64->flops * pm + plan->flops * plo;
   }
   psi = qtensor<T>((const T*)evec.ptr, dim, sym);
   DMRGX_HIP(hipStreamSynchronize(s));

@@ -19,10 +19,11 @@ struct RunConfig {
   uint64_t seed = 1234;
   double tol = 1e-10;          // Lanczos tolerance (FP64 eigensolve)
   double tol_lo = 1e-6;        // Lanczos tolerance of a reduced-precision eigensolve
-  int refine_k = 8;            // FP64 Krylov polish steps after a reduced-precision eigensolve
-  int polish_cycles = 30;      // polish cycles at most, until the FP64 residual passes `tol`
+  int refine_k = 8;            // refinement of a reduced-precision eigensolve: 2 refine_k inner steps
+  int polish_cycles = 12;      // FP64 refinement steps at most, until the FP64 residual passes `tol`
   bool warm_start = false;     // start Lanczos from the previous two-site state
   int nev = NEV;
+  int nev_lo = 4;              // eigenpairs converged by a reduced-precision solve
   int max_restart = 5000;
   bool verbose = false;
 };

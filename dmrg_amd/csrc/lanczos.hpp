@@ -67,6 +67,19 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
 // (column 0 = the polished lowest vector, normalised); returns the Ritz values.
 std::vector<double> subspace_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int p,
                                             int k, double* X, double tol, int max_cycles, int* n_matvec);
+// Jacobi-Davidson refinement of the lowest eigenpair of every chain: the FP64
+// Rayleigh-Ritz basis starts as the p approximate eigenvectors X (p columns
+// of the concatenated length, as eigsh writes them) and grows by one vector
+// per outer step, the correction s from a few conjugate-gradient steps on
+// P (H_lo - theta) P s = r with the cheap operator op_lo (P: the projector
+// orthogonal to the basis, r = H64 u - theta u), so each outer step costs one
+// FP64 matvec and at most `inner` low-precision ones.  Stops when every
+// chain's FP64 residual passes tol max(|theta|, eps^(2/3)) or after max_outer
+// steps.  X column 0 receives the refined lowest vector (normalised); returns
+// the Ritz values; *n64 / *nlo count the matvecs of each operator.
+std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, const std::vector<int64_t>& len,
+                                      bool cplx, int p, double* X, double tol, int max_outer, int inner, int* n64,
+                                      int* nlo);
 // krylov_polish of every chain's segment of x at once (cycles until every
 // chain passes the test); returns the Ritz values
 std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int k,

@@ -770,3 +770,183 @@ std::vector<double> subspace_polish_batched(const MatVec& op, const std::vector<
 }
 
 }  // namespace dmrgx
+
+namespace dmrgx {
+
+std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, const std::vector<int64_t>& len,
+                                      bool cplx, int p, double* X, double tol, int max_outer, int inner,
+                                      int* n64, int* nlo) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  const int nc = (int)len.size();
+  DMRGX_CHECK(nc > 0, "jd_polish: no chains");
+  DMRGX_CHECK(p >= 1 && max_outer >= 0 && inner >= 1, "jd_polish: p >= 1, max_outer >= 0, inner >= 1");
+  const std::vector<int64_t> off = offsets(len);
+  const int64_t N = off.back() + len.back();
+  const int64_t nr = N * (cplx ? 2 : 1);
+  const int qmax = p + max_outer;
+  std::vector<int> all(nc);
+  for (int c = 0; c < nc; ++c) all[c] = c;
+  Segs sg(off, len, all, s);
+  SegOps ops(sg, cplx, nr, qmax + 1, s);
+  const int hs = ops.hs;
+  const size_t vb = (size_t)nr * 8;
+  DevBuf Vb(qmax * vb), HVb(qmax * vb), wk(7 * vb), hb((size_t)2 * nc * hs * 8), scl((size_t)nc * 8),
+      Zb((size_t)nc * qmax * 16);
+  double* V = (double*)Vb.ptr;
+  double* HV = (double*)HVb.ptr;
+  double* u = (double*)wk.ptr;
+  double* t1 = u + nr;
+  double* r = t1 + nr;
+  double* sv = r + nr;
+  double* dv = sv + nr;
+  double* wv = dv + nr;
+  double* dn = wv + nr;
+  double* h = (double*)hb.ptr;
+  double* h2 = h + (size_t)nc * hs;
+  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+  // per-chain host state: the projected matrix G = V^H H V and the valid columns
+  std::vector<std::vector<dmrgx::cplx>> G(nc, std::vector<dmrgx::cplx>((size_t)qmax * qmax));
+  std::vector<std::vector<char>> valid(nc, std::vector<char>(qmax, 0));
+  std::vector<double> th0(nc, 0.0), sc(nc), coef((size_t)nc * hs, 0.0);
+  auto put_coef = [&](const std::vector<double>& a, double* d) {  // one complex coefficient (a_c, 0) per chain
+    std::fill(coef.begin(), coef.end(), 0.0);
+    for (int c = 0; c < nc; ++c) coef[(size_t)c * hs] = a[c];
+    DMRGX_HIP(hipMemcpyAsync(d, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipStreamSynchronize(s));
+  };
+  // V_j = the part of src orthogonal to V[0..j) (two passes), normalised per
+  // chain, or zero where it is dependent; then HV_j = H64 V_j and column j of G
+  auto add_vector = [&](int j, const double* src) {
+    ops.vdots(V, j, src, h);
+    ops.sub(V, j, h, src, t1, nullptr);
+    ops.vdots(V, j, t1, h2);
+    const double* hh = ops.fetch(h, 2);
+    for (int c = 0; c < nc; ++c) {
+      const double* a = hh + (size_t)c * hs;
+      const double* b = hh + (size_t)(nc + c) * hs;
+      double hsq = 0;
+      for (int i = 0; i < j; ++i) hsq += b[2 * i] * b[2 * i] + b[2 * i + 1] * b[2 * i + 1];
+      const double w2 = a[2 * j], b2 = b[2 * j] - hsq;
+      const bool ok = w2 > 0 && b2 > 1e-20 * w2 && j < len[c];
+      valid[c][j] = ok;
+      sc[c] = ok ? 1.0 / std::sqrt(b2) : 0.0;
+    }
+    ops.put(sc, (double*)scl.ptr);
+    ops.sub(V, j, h2, t1, V + (size_t)j * nr, (const double*)scl.ptr);
+    op64(V + (size_t)j * nr, HV + (size_t)j * nr);
+    if (n64) (*n64)++;
+    ops.vdots(V, j + 1, HV + (size_t)j * nr, h);
+    const double* g = ops.fetch(h, 1);
+    for (int c = 0; c < nc; ++c) {
+      const double* col = g + (size_t)c * hs;
+      for (int i = 0; i <= j; ++i) {
+        const dmrgx::cplx x(col[2 * i], cplx ? col[2 * i + 1] : 0.0);
+        G[c][(size_t)i * qmax + j] = x;
+        G[c][(size_t)j * qmax + i] = std::conj(x);
+      }
+    }
+  };
+  int q = 0;
+  for (int j = 0; j < p; ++j) add_vector(q++, X + (size_t)j * nr);
+  std::vector<char> done(nc, 0);
+  for (int outer = 0;; ++outer) {
+    // Rayleigh-Ritz in span V: the lowest pair, u = V z, r = H u - theta u
+    std::vector<double> Z((size_t)nc * q * 2, 0.0);
+    parallel_for(nc, [&](int c) {
+      std::vector<int> idx;
+      for (int j = 0; j < q; ++j)
+        if (valid[c][j]) idx.push_back(j);
+      const int n = (int)idx.size();
+      DMRGX_CHECK(n > 0, "jd_polish: empty basis");
+      std::vector<dmrgx::cplx> A((size_t)n * n), W;
+      for (int a = 0; a < n; ++a)
+        for (int b = 0; b < n; ++b) A[(size_t)a * n + b] = G[c][(size_t)idx[a] * qmax + idx[b]];
+      std::vector<double> w;
+      host_hermitian_eig(n, A, !cplx, w, W);
+      th0[c] = w[0];
+      for (int a = 0; a < n; ++a) {
+        Z[((size_t)c * q + idx[a]) * 2] = W[(size_t)a * n].real();
+        Z[((size_t)c * q + idx[a]) * 2 + 1] = W[(size_t)a * n].imag();
+      }
+    });
+    DMRGX_HIP(hipMemcpyAsync(Zb.ptr, Z.data(), Z.size() * 8, hipMemcpyHostToDevice, s));
+    ops.combine_z(V, q, (const double*)Zb.ptr, 1, u);
+    ops.combine_z(HV, q, (const double*)Zb.ptr, 1, r);
+    put_coef(th0, h);
+    ops.sub(u, 1, h, r, r, nullptr);  // r = H u - theta u
+    ops.vdots(nullptr, 0, r, h2);
+    const double* rr0 = ops.fetch(h2, 1);
+    bool all_done = true;
+    std::vector<double> rr(nc);
+    for (int c = 0; c < nc; ++c) {
+      rr[c] = std::max(rr0[(size_t)c * hs], 0.0);
+      done[c] = std::sqrt(rr[c]) <= tol * std::max(std::fabs(th0[c]), eps23);
+      all_done = all_done && done[c];
+    }
+    if (all_done || q >= qmax) break;
+    // correction: CG on P (H_lo - theta) P s = r, P the projector orthogonal
+    // to V (its spectrum on range(P) is >= E_p - theta > 0 near convergence);
+    // converged chains keep s = 0
+    fill_zero(sv, vb, s);
+    copy_d2d(dv, r, vb, s);
+    std::vector<double> rho(nc);
+    for (int c = 0; c < nc; ++c) rho[c] = done[c] ? 0.0 : rr[c];
+    // zero the residual of converged chains so their s stays zero
+    {
+      std::vector<double> keep(nc);
+      for (int c = 0; c < nc; ++c) keep[c] = done[c] ? 0.0 : 1.0;
+      ops.put(keep, (double*)scl.ptr);
+      ops.sub(nullptr, 0, h, dv, dv, (const double*)scl.ptr);
+      copy_d2d(t1, dv, vb, s);  // t1: the CG residual
+    }
+    std::vector<char> live(nc);
+    for (int c = 0; c < nc; ++c) live[c] = !done[c] && rho[c] > 0;
+    for (int it = 0; it < inner; ++it) {
+      op_lo(dv, wv);
+      if (nlo) (*nlo)++;
+      put_coef(th0, h);
+      ops.sub(dv, 1, h, wv, wv, nullptr);   // w = (H_lo - theta) d
+      ops.vdots(V, q, wv, h);
+      ops.sub(V, q, h, wv, wv, nullptr);    // w = P w
+      ops.vdots(dv, 1, wv, h);              // <d, w>
+      const double* dw = ops.fetch(h, 1);
+      std::vector<double> alpha(nc, 0.0);
+      for (int c = 0; c < nc; ++c) {
+        const double x = dw[(size_t)c * hs];
+        if (live[c] && x > 0) alpha[c] = rho[c] / x;
+        else live[c] = 0;
+      }
+      std::vector<double> na(nc);
+      for (int c = 0; c < nc; ++c) na[c] = -alpha[c];
+      put_coef(na, h);
+      ops.sub(dv, 1, h, sv, sv, nullptr);   // s += alpha d
+      put_coef(alpha, h);
+      ops.sub(wv, 1, h, t1, t1, nullptr);   // res -= alpha w
+      ops.vdots(nullptr, 0, t1, h2);
+      const double* rn = ops.fetch(h2, 1);
+      std::vector<double> nb(nc, 0.0);
+      bool any = false;
+      for (int c = 0; c < nc; ++c) {
+        const double x = std::max(rn[(size_t)c * hs], 0.0);
+        if (live[c] && rho[c] > 0) nb[c] = -x / rho[c];
+        rho[c] = x;
+        // an inner solve a few digits deep is enough for the outer step
+        if (live[c] && x <= 1e-6 * rr[c]) live[c] = 0;
+        any = any || live[c];
+      }
+      if (!any || it + 1 == inner) break;
+      put_coef(nb, h);
+      ops.sub(dv, 1, h, t1, dn, nullptr);   // d = res + beta d
+      std::swap(dv, dn);
+    }
+    add_vector(q++, sv);
+  }
+  // the polished lowest vector of every chain into X column 0
+  copy_d2d(X, u, vb, s);
+  ops.normalise(X, h, (double*)scl.ptr);
+  DMRGX_HIP(hipStreamSynchronize(s));
+  return th0;
+}
+
+}  // namespace dmrgx

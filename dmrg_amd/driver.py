@@ -19,7 +19,7 @@ class RunConfig(C.Structure):
     _fields_ = [("model", C.c_int), ("p", C.c_double * 4), ("L", C.c_int), ("m", C.c_int), ("sweeps", C.c_int),
                 ("sector", C.c_int), ("schedule", C.c_int), ("prec", C.c_int), ("sign_rule", C.c_int),
                 ("ladder", C.c_int), ("seed", C.c_uint64), ("tol", C.c_double), ("tol_lo", C.c_double),
-                ("refine_k", C.c_int), ("verbose", C.c_int)]
+                ("refine_k", C.c_int), ("verbose", C.c_int), ("nev_lo", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -128,7 +128,7 @@ class RunResult:
 
 
 def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo, refine_k,
-            verbose):
+            verbose, nev_lo=0):
     from . import PREC_NAMES
     cfg = RunConfig()
     cfg.model = MODELS[model]
@@ -141,16 +141,17 @@ def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladd
     cfg.ladder = 1 if ladder else 0
     cfg.seed = seed
     cfg.tol, cfg.tol_lo, cfg.refine_k, cfg.verbose = tol, tol_lo, refine_k, 1 if verbose else 0
+    cfg.nev_lo = int(nev_lo)
     return cfg
 
 
 def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
              sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
-             verbose=False, keep_state=False) -> RunResult:
+             verbose=False, keep_state=False, nev_lo=0) -> RunResult:
     from . import check, init, lib
     init()
     cfg = _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo,
-                  refine_k, verbose)
+                  refine_k, verbose, nev_lo)
     st = RunStats()
     cap = 20000
     log = np.zeros((cap, STEP_LOG_WIDTH))
@@ -195,7 +196,7 @@ class BatchResult:
 
 
 def run_batch(model: str, L: int, m: int, sweeps: int, params, sector=0, prec="fp64", sign_rule="physical",
-              ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8, verbose=False) -> BatchResult:
+              ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8, verbose=False, nev_lo=0) -> BatchResult:
     """dmrgx_run_batch: chains with couplings params[i] swept in lockstep."""
     from . import check, init, lib
     init()
@@ -203,7 +204,7 @@ def run_batch(model: str, L: int, m: int, sweeps: int, params, sector=0, prec="f
     for i, q in enumerate(params):
         p[i, :len(q)] = q
     cfg = _config(model, L, m, sweeps, (), sector, "dmrg2", prec, sign_rule, ladder, seed, tol, tol_lo, refine_k,
-                  verbose)
+                  verbose, nev_lo)
     n = len(params)
     e, tr = np.zeros(n), np.zeros(n)
     st = RunStats()

@@ -33,6 +33,7 @@ typedef struct dmrgx_run_config {
   double tol_lo;    /* reduced-precision Lanczos tolerance */
   int refine_k;     /* FP64 Krylov polish steps after a reduced-precision solve */
   int verbose;
+  int nev_lo;       /* eigenpairs converged by a reduced-precision solve (0: 4); FP64 solves use 6 */
 } dmrgx_run_config;
 
 typedef struct dmrgx_run_stats {
