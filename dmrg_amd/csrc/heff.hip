@@ -494,6 +494,11 @@ int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int 
   CAPI_END
 }
 
+int dmrgx_set_lanczos_graphs(int on) {
+  set_lanczos_graphs(on != 0);
+  return 0;
+}
+
 int dmrgx_block_gemm(int dtype, char ta, char tb, const void* A, const void* B, void* C, const int64_t* tasks,
                      int n_task) {
   CAPI_BEGIN

@@ -57,6 +57,12 @@ void dev_orthogonalise(const double* V, int64_t ld, int k, double* v, int64_t n,
 std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int nev,
                                        int ncv, double tol, int max_restart, const std::vector<uint64_t>& seed,
                                        const double* v0, double* evec, int n_vec = 1);
+// Whether eigsh_batched replays its Lanczos steps as hipGraphs (default off:
+// a DMRG step's plan lives for a few restart cycles only, and capturing plus
+// instantiating the graphs costs more than the replays save; the results are
+// identical either way).
+bool lanczos_graphs();
+void set_lanczos_graphs(bool on);
 // Rayleigh-Ritz polish of the lowest eigenpair of every chain in FP64 over
 // span{X} + K_k(H, x_0) (X: the p <= 8 lowest approximate eigenvectors of each
 // chain, p columns of the concatenated length, as eigsh writes them; the

@@ -11,6 +11,8 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <atomic>
+#include <functional>
 #include <memory>
 
 #include "common.hpp"
@@ -211,6 +213,43 @@ __global__ void k_gather_col(const double* __restrict__ y, const int64_t* __rest
   }
 }
 
+// Executable graphs indexed by Krylov position: slot j holds the launches of
+// step j, captured from the stream at first use and replayed afterwards.  The
+// captured launches read their scalars from device or pinned memory written
+// before each replay, so a replay equals the eager sequence.
+struct GraphSet {
+  struct Slot {
+    hipStream_t s = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void run(const std::function<void()>& body) {
+      if (!exec) {
+        hipGraph_t g = nullptr;
+        DMRGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+          body();
+        } catch (...) {
+          (void)hipStreamEndCapture(s, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        DMRGX_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        DMRGX_HIP(e);
+      }
+      DMRGX_HIP(hipGraphLaunch(exec, s));
+    }
+    ~Slot() {
+      if (exec) (void)hipGraphExecDestroy(exec);
+    }
+  };
+  std::vector<Slot> slot;
+  GraphSet(int n, hipStream_t s) : slot(n) {
+    for (auto& x : slot) x.s = s;
+  }
+  Slot& operator[](int j) { return slot[j]; }
+};
+
 // The segments of a set of chains: work-group blocks of at most CH elements
 // and each chain's range of blocks; `chain` in a block is the ordinal within
 // the set.
@@ -303,6 +342,16 @@ struct SegOps {
     DMRGX_HIP(hipStreamSynchronize(s));
     return pin;
   }
+  // the copy of fetch without the synchronisation (for graph capture)
+  void fetch_async(const double* h, int count) {
+    DMRGX_CHECK(count <= sets, "batched Lanczos: staging too small");
+    DMRGX_HIP(hipMemcpyAsync(pin, h, (size_t)count * sg.n * hs * 8, hipMemcpyDeviceToHost, s));
+  }
+  // put() in two halves: the host write into pinned staging, then the copy
+  void stage(const std::vector<double>& sc) { std::copy(sc.begin(), sc.begin() + sg.n, pin + (size_t)sets * sg.n * hs); }
+  void put_staged(double* d) {
+    DMRGX_HIP(hipMemcpyAsync(d, pin + (size_t)sets * sg.n * hs, sg.n * 8, hipMemcpyHostToDevice, s));
+  }
   // one scale per chain to the device array d
   void put(const std::vector<double>& sc, double* d) {
     double* p = pin + (size_t)sets * sg.n * hs;
@@ -336,7 +385,12 @@ std::vector<int64_t> offsets(const std::vector<int64_t>& len) {
   return off;
 }
 
+std::atomic<bool> g_graphs{false};
+
 }  // namespace
+
+bool lanczos_graphs() { return g_graphs.load(); }
+void set_lanczos_graphs(bool on) { g_graphs.store(on); }
 
 std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int nev,
                                        int ncv, double tol, int max_restart, const std::vector<uint64_t>& seed,
@@ -453,17 +507,29 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
   std::vector<int> n_reorth(nb, 0);
   std::vector<char> conv(nb, 0);
   int j0 = 0, n_matvec = 0, n_restart = 0;
+  GraphSet gstep(m, s), gfin(m, s);
+  auto run_graph = [&](GraphSet::Slot& g, const std::function<void()>& body) {
+    if (lanczos_graphs()) g.run(body);
+    else body();
+  };
   for (int restart = 0; restart <= max_restart; ++restart) {
     n_restart = restart;
     for (int j = j0; j < m; ++j) {
-      op(V + (int64_t)j * ld, w);
+      // the matvec and two classical Gram-Schmidt passes (as in eigsh); the
+      // second pass's |w'|^2 comes with its coefficients, both sets in one
+      // synchronisation.  The same launches recur at index j in every restart
+      // cycle, so they run as one hipGraph per j, captured at its first use.
+      auto body = [&] {
+        op(V + (int64_t)j * ld, w);
+        ops.vdots(V, j + 1, w, h);
+        ops.sub(V, j + 1, h, w, w, nullptr);
+        ops.vdots(V, j + 1, w, h2);
+        ops.fetch_async(h, 2);  // h and h2 are adjacent
+      };
+      run_graph(gstep[j], body);
       ++n_matvec;
-      // two classical Gram-Schmidt passes (as in eigsh); the second pass's
-      // |w'|^2 comes with its coefficients, both sets in one synchronisation
-      ops.vdots(V, j + 1, w, h);
-      ops.sub(V, j + 1, h, w, w, nullptr);
-      ops.vdots(V, j + 1, w, h2);
-      const double* hh = ops.fetch(h, 2);  // h and h2 are adjacent
+      DMRGX_HIP(hipStreamSynchronize(s));
+      const double* hh = ops.pin;
       for (int a = 0; a < nb; ++a) {
         const double* q = hh + (size_t)a * hs;
         const double* q2 = hh + (size_t)(nb + a) * hs;
@@ -490,9 +556,12 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
         beta_last[a] = beta;
         sc[a] = beta > 0 ? 1.0 / beta : 0.0;
       }
-      ops.put(sc, (double*)scl.ptr);
       double* vn = V + (int64_t)(j + 1) * ld;
-      ops.sub(V, j + 1, hfin, w, vn, (const double*)scl.ptr);
+      ops.stage(sc);
+      run_graph(gfin[j], [&] {
+        ops.put_staged((double*)scl.ptr);
+        ops.sub(V, j + 1, hfin, w, vn, (const double*)scl.ptr);
+      });
       for (int a : lost) {
         // invariant subspace of this chain: continue with a fresh random
         // direction orthogonal to its basis (as in eigsh)
