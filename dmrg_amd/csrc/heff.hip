@@ -3,6 +3,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <atomic>
+#include <map>
+#include <mutex>
+#include <string>
 
 #include "dmrgx.h"
 #include "common.hpp"
@@ -49,6 +54,8 @@ std::vector<std::pair<int64_t, int64_t>> uncovered(const std::vector<GemmTask>& 
   if (pos < n_out) gaps.push_back({pos, n_out - pos});
   return gaps;
 }
+
+static std::atomic<bool> g_autotune{true};
 
 static std::unique_ptr<DevBuf> to_f32(const void* src, int64_t nreal, hipStream_t s) {
   auto b = std::make_unique<DevBuf>(std::max<int64_t>(nreal, 1) * 4);
@@ -116,6 +123,10 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     remap(rt_, *h->renv_lay, *h->psi_lay);
     remap(lt_, *h->lenv_lay, *h->store_lay);
   }
+  h->rexec = rt_;
+  h->lexec = lt_;
+  h->rzero = rzero;
+  h->lzero = lzero;
   h->rplan = make_gemm_plan(rt_, dtype, prec, 'N', 'T', rzero, 0, planes);
   h->lplan = make_gemm_plan(lt_, dtype, prec, 'N', 'T', lzero, 0, planes);
   h->flops = h->rplan->flops + h->lplan->flops;
@@ -148,7 +159,98 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     h->out_lo = std::make_unique<DevBuf>(std::max<int64_t>(n_psi, 1) * 4 * c);
   }
   DMRGX_HIP(hipStreamSynchronize(s));
+  if (g_autotune.load()) h->autotune(s);
   return h;
+}
+
+void HeffPlan::replan(int r_target, int l_target) {
+  const bool planes = prec == PREC_BF16X3;
+  rplan = make_gemm_plan(rexec, dtype, prec, 'N', 'T', rzero, r_target, planes);
+  lplan = make_gemm_plan(lexec, dtype, prec, 'N', 'T', lzero, l_target, planes);
+  r_split = r_target;
+  l_split = l_target;
+}
+
+namespace {
+
+// The tuner's key: precision, element type, and per pass the histogram of
+// task shapes with each of M, N, K rounded to a power of two (so that the
+// slightly different sector sizes of consecutive DMRG steps share a key).
+std::string shape_key(const HeffPlan& h) {
+  auto hist = [](const std::vector<GemmTask>& t) {
+    std::map<std::array<int, 3>, int> m;
+    auto lg = [](int x) { int b = 0; while ((1 << b) < x) ++b; return b; };
+    for (auto& g : t) m[{lg(g.M), lg(g.N), lg(g.K)}]++;
+    std::string k;
+    for (auto& e : m) {
+      int c = 0;
+      while ((1 << c) < e.second) ++c;  // counts rounded to powers of two too
+      k += std::to_string(e.first[0]) + "," + std::to_string(e.first[1]) + "," + std::to_string(e.first[2]) + "x" +
+           std::to_string(c) + ";";
+    }
+    return k;
+  };
+  return std::to_string(h.dtype) + "/" + std::to_string(h.prec) + "/R" + hist(h.rexec) + "/L" + hist(h.lexec);
+}
+
+std::mutex g_tune_mu;
+std::map<std::string, std::pair<int, int>> g_tune;  // key -> (R target, L target)
+
+}  // namespace
+
+void set_heff_autotune(bool on) { g_autotune.store(on); }
+int heff_autotune_keys() {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  return (int)g_tune.size();
+}
+
+void HeffPlan::autotune(hipStream_t s) {
+  // small operators keep the default: timing them costs more than any choice saves
+  if (rexec.empty() || lexec.empty() || n_psi <= 0 || flops < 1e9) return;
+  const std::string key = shape_key(*this);
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto f = g_tune.find(key);
+    if (f != g_tune.end()) {
+      if (f->second.first != r_split || f->second.second != l_split) replan(f->second.first, f->second.second);
+      return;
+    }
+  }
+  // candidates: 2 (the default, first), 1, 4 and 8 work items per CU; each
+  // pass timed with events around its own launch (two applies after one
+  // warm-up) on a zero input; another candidate replaces the default only
+  // when it is at least 5 % faster
+  const int cu = std::max(rt().num_cu, 1);
+  const int cand[4] = {2 * cu, cu, 4 * cu, 8 * cu};
+  const int c = dtype == DT_C64 ? 2 : 1;
+  DevBuf x(std::max<int64_t>(n_psi, 1) * 8 * c), y(std::max<int64_t>(n_psi, 1) * 8 * c);
+  fill_zero(x.ptr, n_psi * 8 * c, s);
+  hipEvent_t ev[4];
+  for (auto& e : ev) DMRGX_HIP(hipEventCreate(&e));
+  double best_r = 1e300, best_l = 1e300;
+  int br = 0, bl = 0;
+  for (int t : cand) {
+    replan(t, t);
+    apply((const double*)x.ptr, (double*)y.ptr, s);
+    float tr = 0, tl = 0;
+    for (int rep = 0; rep < 2; ++rep) {
+      apply((const double*)x.ptr, (double*)y.ptr, s, ev);
+      DMRGX_HIP(hipEventSynchronize(ev[3]));
+      float a = 0, b = 0;
+      DMRGX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+      DMRGX_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
+      tr += a;
+      tl += b;
+    }
+    const double m = t == cand[0] ? 1.0 : 0.95;
+    if (tr < m * best_r) { best_r = tr; br = t; }
+    if (tl < m * best_l) { best_l = tl; bl = t; }
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  replan(br, bl);
+  DMRGX_HIP(hipStreamSynchronize(s));
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[key] = {br, bl};
 }
 
 namespace {
@@ -493,6 +595,13 @@ int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int 
     }
   CAPI_END
 }
+
+int dmrgx_set_autotune(int on) {
+  set_heff_autotune(on != 0);
+  return 0;
+}
+
+int dmrgx_autotune_keys(void) { return heff_autotune_keys(); }
 
 int dmrgx_set_lanczos_graphs(int on) {
   set_lanczos_graphs(on != 0);

@@ -27,6 +27,17 @@ struct HeffPlan {
   double flops = 0;
   int64_t renv_len = 0, lenv_len = 0;
   std::vector<GemmTask> rtasks, ltasks;  // the get_map tasks (element offsets of the plain layout)
+  // the tasks as the pass plans execute them (padded-plane offsets for
+  // BF16x3), the output ranges no task writes, and the chosen split targets
+  std::vector<GemmTask> rexec, lexec;
+  std::vector<std::pair<int64_t, int64_t>> rzero, lzero;
+  int r_split = 0, l_split = 0;
+  // rebuild the two pass plans with the given split-K targets (work items
+  // per launch; 0: two per CU)
+  void replan(int r_target, int l_target);
+  // pick each pass plan's split-K target by timing candidates once per
+  // sector-shape histogram (cached for the process)
+  void autotune(hipStream_t s);
   // adjoint plans, built on the first adjoint call: gradients of the env
   // blocks (gR, gL) and of the pass inputs (gP for psi, gS for store), plus an
   // FP32 R pass when the forward plan works on bf16 planes
@@ -47,6 +58,11 @@ std::vector<GemmTask> tasks_from_map(const int32_t* map, int n_task, const int64
                                      const int64_t* blk_size);
 // C ranges in [0, n_out) that no task writes (they must read as zero).
 std::vector<std::pair<int64_t, int64_t>> uncovered(const std::vector<GemmTask>& t, int64_t n_out);
+
+// Whether make_heff tunes the pass plans (default on) and how many
+// histogram keys the tuner has timed in this process.
+void set_heff_autotune(bool on);
+int heff_autotune_keys();
 
 std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const int64_t* rblk_off,
                                     const int64_t* rblk_size, int n_rblk, int64_t renv_len,

@@ -122,6 +122,20 @@ int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int 
                         int max_restart, const uint64_t* seeds, double* evals, void* evec, dmrgx_heff* refine,
                         int refine_k, int* info);
 
+/* Work-decomposition autotuner of the H_eff plans (on by default).  Each
+ * pass plan (R and L) cuts its output tiles and the concatenated K of every
+ * output block into work items; the number of items per launch decides
+ * split-K depth and load balance.  The first plan with a given sector-shape
+ * histogram (task shapes with M, N, K rounded to powers of two, counts too,
+ * per pass, with precision and element type) of at least 1e9 flop times 1, 2,
+ * 4 and 8 items per CU with HIP events and keeps the fastest for every later
+ * plan with that histogram in the process (the default of 2 per CU is kept
+ * unless another is 5 % faster).  Choices change the summation order only.
+ * dmrgx_autotune_keys: histograms timed so far.  No counterpart in the
+ * reference. */
+int dmrgx_set_autotune(int on);
+int dmrgx_autotune_keys(void);
+
 /* dmrgx_eigsh_batched (and the batched sweeps) run each Lanczos step -- the
  * matvec and both Gram-Schmidt passes, then the normalisation of the next
  * vector -- as hipGraphs captured at their first use and replayed in every
