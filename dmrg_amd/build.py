@@ -53,6 +53,25 @@ def build(verbose: bool = True) -> Path:
     return OUT
 
 
+CLI_SRC = PKG / "cli" / "dmrgx_main.cpp"
+CLI_OUT = PKG / "dmrgx"
+
+
+def build_cli(verbose: bool = True) -> Path:
+    """The command-line driver ``dmrg_amd/dmrgx`` (the reference mains'
+    interface), linked against libdmrgx.so in the same directory."""
+    tmp = CLI_OUT.with_suffix(".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{PKG.parent / 'include'}", str(CLI_SRC), f"-L{PKG}", "-ldmrgx",
+           "-Wl,-rpath,$ORIGIN", "-o", str(tmp)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed:\n{r.stderr[-8000:]}")
+    os.replace(tmp, CLI_OUT)
+    if verbose:
+        print(f"built {CLI_OUT}")
+    return CLI_OUT
+
+
 TORCH_SRC = PKG / "torch_ext" / "torch_heff.cpp"
 TORCH_OUT = PKG / "dmrgx_torch.so"
 
@@ -82,6 +101,7 @@ def build_torch(verbose: bool = True) -> Path:
 
 if __name__ == "__main__":
     build()
+    build_cli()
     if "--torch" in sys.argv:
         build_torch()
     sys.exit(0)

@@ -10,6 +10,7 @@
 #include <cstdio>
 
 #include "lanczos.hpp"
+#include "output.hpp"
 #include "svd.hpp"
 
 namespace dmrgx {
@@ -18,7 +19,14 @@ using clk = std::chrono::steady_clock;
 static double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
 
 template <typename T>
-Engine<T>::Engine(const mpo<T>& m, const RunConfig& c) : mpo_(m), mps_(m), cfg(c) {}
+Engine<T>::Engine(const mpo<T>& m, const RunConfig& c) : mpo_(m), mps_(m), cfg(c) {
+  if (!cfg.output_suffix.empty()) {
+    // src/functions.cpp:112-116, 163-167
+    f_energy_ = std::make_unique<std::ofstream>("energy" + cfg.output_suffix);
+    f_singular_ = std::make_unique<std::ofstream>("singular" + cfg.output_suffix);
+    DMRGX_CHECK(f_energy_->good() && f_singular_->good(), "cannot open the output files for " + cfg.output_suffix);
+  }
+}
 
 template <typename T>
 qtensor<T> Engine<T>::make_lenv(int l) {
@@ -61,6 +69,7 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
     std::vector<double> w(std::max<int64_t>(d, NEV) + whole.dimension(0), 0.0);
     int n = whole.eig(w.data(), v, cfg.sector);
     for (int i = 0; i < std::min(n, NEV); ++i) vals[i] = w[i];
+    lg.n_eval = (int)d;  // src/core.cpp:123
     DMRGX_CHECK((int64_t)v.size() == d, "exact diagonalisation: eigenvector length differs from the psi dimension");
     DevBuf tmp(std::max<int64_t>(d, 1) * sizeof(T));
     DMRGX_HIP(hipMemcpyAsync(tmp.ptr, v.data(), d * sizeof(T), hipMemcpyHostToDevice, rt().stream));
@@ -130,6 +139,12 @@ void Engine<T>::finish(StepLog& lg, const vector<double>& s, int cutoff) {
   lg.entropy = ent;
   lg.trunc = err;
   lg.s.assign(s.begin(), s.begin() + std::min<size_t>(s.size(), NSI));
+  if (f_energy_) {
+    write_energy_line(*f_energy_, lg.l, lg.r, lg.e, lg.n_eval);
+    write_singular_line(*f_singular_, lg.l, lg.r, s, cutoff);
+    f_energy_->flush();
+    f_singular_->flush();
+  }
   log.push_back(lg);
   if (cfg.verbose)
     std::printf("l=%d r=%d d=%lld E0=%.12f trunc=%.3e matvec=%d t=%.3fs\n", lg.l, lg.r, (long long)lg.d, lg.e[0],

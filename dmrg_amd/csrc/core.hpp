@@ -2,7 +2,9 @@
 // (src/core.cpp, src/functions.cpp).
 #pragma once
 
+#include <fstream>
 #include <functional>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -26,6 +28,9 @@ struct RunConfig {
   int nev_lo = 4;              // eigenpairs converged by a reduced-precision solve
   int max_restart = 5000;
   bool verbose = false;
+  // when set, every step appends to "energy" + output_suffix and "singular" +
+  // output_suffix in the working directory, as the reference's drivers do
+  std::string output_suffix;
 };
 
 struct StepLog {
@@ -39,6 +44,7 @@ struct StepLog {
   int svd_sweeps = 0;     // Jacobi sweeps of the block SVD
   double t_block_svd = 0;  // the block SVD alone (inside t_svd)
   std::vector<double> s;  // leading singular values
+  int n_eval = NEV;       // eigenvalues the step reports (d when it diagonalised exactly)
 };
 
 template <typename T>
@@ -79,6 +85,7 @@ class Engine {
              bool allow_ed, int ed_which);
   void finish(StepLog& lg, const vector<double>& s, int cutoff);
   std::shared_ptr<DevBuf> last_psi_;
+  std::unique_ptr<std::ofstream> f_energy_, f_singular_;
   vector<vector<int>> last_dim_, last_sym_;
 };
 

@@ -9,6 +9,7 @@
 #include "heff.hpp"
 #include "lanczos.hpp"
 #include "observables.hpp"
+#include "output.hpp"
 #include "qtensor.hpp"
 
 using namespace dmrgx;
@@ -228,6 +229,7 @@ RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
   r.tol_lo = c->tol_lo > 0 ? c->tol_lo : 1e-6;
   r.refine_k = c->refine_k > 0 ? c->refine_k : 8;
   r.nev_lo = c->nev_lo > 0 ? c->nev_lo : 4;
+  if (c->output) r.output_suffix = c->output;
   r.verbose = c->verbose != 0;
   return r;
 }
@@ -318,6 +320,25 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double*
     }
   }
   if (state) *state = out.release();
+  CAPI_END
+}
+
+int dmrgx_parse_args(int argc, const char* const* argv, int n_para, const char* const* names, int* L, int* m,
+                     int* sweeps, int* sector, double* params, char* suffix, int cap) {
+  CAPI_BEGIN
+  DMRGX_CHECK(argc >= 0 && n_para >= 0 && (argc == 0 || argv) && (n_para == 0 || names), "parse_args: null arguments");
+  std::vector<std::string> nm(names, names + n_para);
+  Params p = set_para_val(argc, argv, nm);
+  if (L) *L = p.sites;
+  if (m) *m = p.cutoff;
+  if (sweeps) *sweeps = p.sweep;
+  if (sector) *sector = p.sector;
+  if (params)
+    for (int i = 0; i < n_para; ++i) params[i] = p.values[i];
+  if (suffix) {
+    DMRGX_CHECK((int)p.suffix.size() < cap, "parse_args: suffix buffer too small");
+    std::memcpy(suffix, p.suffix.c_str(), p.suffix.size() + 1);
+  }
   CAPI_END
 }
 

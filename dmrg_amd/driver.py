@@ -11,7 +11,7 @@ MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
 STEP_LOG_WIDTH = 19
 DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym",
                   "dmrgx_run_dmrg_state", "dmrgx_state_norm", "dmrgx_state_overlap", "dmrgx_state_expect",
-                  "dmrgx_state_destroy", "dmrgx_state_derivative", "dmrgx_run_batch"]
+                  "dmrgx_state_destroy", "dmrgx_state_derivative", "dmrgx_run_batch", "dmrgx_parse_args"]
 BATCH_LOG_WIDTH = 11
 
 
@@ -19,7 +19,7 @@ class RunConfig(C.Structure):
     _fields_ = [("model", C.c_int), ("p", C.c_double * 4), ("L", C.c_int), ("m", C.c_int), ("sweeps", C.c_int),
                 ("sector", C.c_int), ("schedule", C.c_int), ("prec", C.c_int), ("sign_rule", C.c_int),
                 ("ladder", C.c_int), ("seed", C.c_uint64), ("tol", C.c_double), ("tol_lo", C.c_double),
-                ("refine_k", C.c_int), ("verbose", C.c_int), ("nev_lo", C.c_int)]
+                ("refine_k", C.c_int), ("verbose", C.c_int), ("nev_lo", C.c_int), ("output", C.c_char_p)]
 
 
 class RunStats(C.Structure):
@@ -57,6 +57,10 @@ def declare(lib):
     lib.dmrgx_run_batch.argtypes = [C.POINTER(RunConfig), C.c_int, dp, dp, dp, C.POINTER(RunStats), dp, C.c_int,
                                     C.POINTER(C.c_int), dp]
     lib.dmrgx_run_batch.restype = C.c_int
+    lib.dmrgx_parse_args.argtypes = [C.c_int, C.POINTER(C.c_char_p), C.c_int, C.POINTER(C.c_char_p),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     dp, C.c_char_p, C.c_int]
+    lib.dmrgx_parse_args.restype = C.c_int
     lib.dmrgx_state_destroy.argtypes = [C.c_void_p]
     lib.dmrgx_state_destroy.restype = C.c_int
     return lib
@@ -128,7 +132,7 @@ class RunResult:
 
 
 def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo, refine_k,
-            verbose, nev_lo=0):
+            verbose, nev_lo=0, output=None):
     from . import PREC_NAMES
     cfg = RunConfig()
     cfg.model = MODELS[model]
@@ -142,16 +146,17 @@ def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladd
     cfg.seed = seed
     cfg.tol, cfg.tol_lo, cfg.refine_k, cfg.verbose = tol, tol_lo, refine_k, 1 if verbose else 0
     cfg.nev_lo = int(nev_lo)
+    cfg.output = output.encode() if output else None
     return cfg
 
 
 def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
              sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
-             verbose=False, keep_state=False, nev_lo=0) -> RunResult:
+             verbose=False, keep_state=False, nev_lo=0, output=None) -> RunResult:
     from . import check, init, lib
     init()
     cfg = _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo,
-                  refine_k, verbose, nev_lo)
+                  refine_k, verbose, nev_lo, output)
     st = RunStats()
     cap = 20000
     log = np.zeros((cap, STEP_LOG_WIDTH))
@@ -162,6 +167,20 @@ def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, sched
     stats = {k: getattr(st, k) for k, _ in RunStats._fields_}
     state = State(h, model) if keep_state else None
     return RunResult(st.energy, stats, log[:n.value].copy(), state)
+
+
+def parse_args(argv, names):
+    """dmrgx_parse_args (the reference's set_para_val): (L, m, sweeps, sector,
+    parameter values, output file suffix)."""
+    from . import check, lib
+    a = (C.c_char_p * max(len(argv), 1))(*[x.encode() for x in argv])
+    nm = (C.c_char_p * max(len(names), 1))(*[x.encode() for x in names])
+    L, m, n, s = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+    vals = np.zeros(max(len(names), 1))
+    buf = C.create_string_buffer(4096)
+    check(lib().dmrgx_parse_args(len(argv), a, len(names), nm, C.byref(L), C.byref(m), C.byref(n), C.byref(s),
+                                 vals.ctypes.data_as(C.POINTER(C.c_double)), buf, 4096), "dmrgx_parse_args")
+    return L.value, m.value, n.value, s.value, list(vals[:len(names)]), buf.value.decode()
 
 
 def splitmix64_stream(seed: int, n: int) -> np.ndarray:

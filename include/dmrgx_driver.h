@@ -34,6 +34,9 @@ typedef struct dmrgx_run_config {
   int refine_k;     /* FP64 Krylov polish steps after a reduced-precision solve */
   int verbose;
   int nev_lo;       /* eigenpairs converged by a reduced-precision solve (0: 4); FP64 solves use 6 */
+  const char* output;  /* NULL: no files; else every step appends to "energy" + output and
+                          "singular" + output in the working directory in the reference's
+                          formats (print_energy / print_singular, src/useful.cpp:54-102) */
 } dmrgx_run_config;
 
 typedef struct dmrgx_run_stats {
@@ -54,6 +57,14 @@ typedef struct dmrgx_run_stats {
 #define DMRGX_STEP_LOG_WIDTH 19
 int dmrgx_run_dmrg(const dmrgx_run_config* cfg, dmrgx_run_stats* stats, double* step_log, int log_cap,
                    int* n_steps);
+
+/* set_para_val (src/functions.cpp:46-80): reads "key=value" words (l, m, n, s
+ * and the model's n_para parameter names; defaults l=4, m=10, n=-1, s=0, the
+ * first parameter 1 and the others 0) and writes the output file suffix
+ * "_L_<l>_m_<m>_n_<n>_s_<s>_<name>_<value>..._.txt" (values in the default
+ * stream format) into suffix (cap bytes).  Host only: no device is touched. */
+int dmrgx_parse_args(int argc, const char* const* argv, int n_para, const char* const* names, int* L, int* m,
+                     int* sweeps, int* sector, double* params, char* suffix, int cap);
 
 /* ---- many chains in lockstep ----------------------------------------------------
  * dmrgx_run_batch sweeps n_chains independent chains of cfg->model (couplings
