@@ -450,6 +450,13 @@ __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__
 // tiles); 16-byte vector loads straight into LDS when a segment is aligned.
 constexpr int PT = 128, PBK = 32, PLDK = PBK + 8;
 
+// Row of the q-th 64-byte row segment a work-group stages per plane: within
+// each group of 16 rows the 4 x 4 (q % 4, q / 4) order is transposed, so the
+// four rows that 16 consecutive lanes store (4 lanes per row) are 4 rows apart
+// (320 bytes with the 80-byte padded LDS rows: disjoint banks), while each
+// 4-lane group still loads 64 contiguous bytes of one row from memory.
+__device__ __forceinline__ int stage_row(int q) { return (q & ~15) | ((q & 3) << 2) | ((q >> 2) & 3); }
+
 __device__ __forceinline__ uint4 xor_sign(uint4 v) {
   v.x ^= 0x80008000u; v.y ^= 0x80008000u; v.z ^= 0x80008000u; v.w ^= 0x80008000u;
   return v;
@@ -499,7 +506,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
       const int ch = t + NT * j;
       const int plane = ch / (PT * CPR);
       const int rem = ch % (PT * CPR);
-      const int row = rem / CPR, kc = (rem % CPR) * 8;
+      const int row = stage_row(rem / CPR), kc = (rem % CPR) * 8;
       const int gr = it.m0 + row, gn = it.n0 + row;
       offA[j] = plane * as + s.a_off + (int64_t)gr * s.lda + kc;
       offB[j] = plane * bs + s.b_off + (int64_t)gn * s.ldb + kc;
@@ -566,7 +573,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
       const int ch = t + NT * j;
       const int plane = ch / (PT * CPR);
       const int rem = ch % (PT * CPR);
-      const int row = rem / CPR, kc = (rem % CPR) * 8;
+      const int row = stage_row(rem / CPR), kc = (rem % CPR) * 8;
       *reinterpret_cast<uint4*>(&sA[plane][row * PLDK + kc]) = ra[j];
       *reinterpret_cast<uint4*>(&sB[plane][row * PLDK + kc]) = rb[j];
     }
