@@ -216,35 +216,40 @@ void HeffPlan::autotune(hipStream_t s) {
       return;
     }
   }
-  // candidates: 2 (the default, first), 1, 4 and 8 work items per CU; each
-  // pass timed with events around its own launch (two applies after one
-  // warm-up) on a zero input; another candidate replaces the default only
-  // when it is at least 5 % faster
+  // candidates: 2 (the default, first), 1, 4 and 8 work items per CU.  The
+  // whole matvec (both passes with their split-K reductions and zero fills)
+  // is timed with events, two applies after one warm-up on a zero input: the
+  // R pass's candidates first with the L pass at the default, then the L
+  // pass's with the R pass at its best.  Another candidate replaces the
+  // default only when the matvec is at least 5 % faster.
   const int cu = std::max(rt().num_cu, 1);
   const int cand[4] = {2 * cu, cu, 4 * cu, 8 * cu};
   const int c = dtype == DT_C64 ? 2 : 1;
   DevBuf x(std::max<int64_t>(n_psi, 1) * 8 * c), y(std::max<int64_t>(n_psi, 1) * 8 * c);
   fill_zero(x.ptr, n_psi * 8 * c, s);
-  hipEvent_t ev[4];
+  hipEvent_t ev[2];
   for (auto& e : ev) DMRGX_HIP(hipEventCreate(&e));
-  double best_r = 1e300, best_l = 1e300;
-  int br = 0, bl = 0;
-  for (int t : cand) {
-    replan(t, t);
+  auto time_apply = [&](int rt_, int lt_) {
+    replan(rt_, lt_);
     apply((const double*)x.ptr, (double*)y.ptr, s);
-    float tr = 0, tl = 0;
-    for (int rep = 0; rep < 2; ++rep) {
-      apply((const double*)x.ptr, (double*)y.ptr, s, ev);
-      DMRGX_HIP(hipEventSynchronize(ev[3]));
-      float a = 0, b = 0;
-      DMRGX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
-      DMRGX_HIP(hipEventElapsedTime(&b, ev[2], ev[3]));
-      tr += a;
-      tl += b;
-    }
-    const double m = t == cand[0] ? 1.0 : 0.95;
-    if (tr < m * best_r) { best_r = tr; br = t; }
-    if (tl < m * best_l) { best_l = tl; bl = t; }
+    DMRGX_HIP(hipEventRecord(ev[0], s));
+    for (int rep = 0; rep < 2; ++rep) apply((const double*)x.ptr, (double*)y.ptr, s);
+    DMRGX_HIP(hipEventRecord(ev[1], s));
+    DMRGX_HIP(hipEventSynchronize(ev[1]));
+    float ms = 0;
+    DMRGX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
+    return (double)ms;
+  };
+  int br = cand[0], bl = cand[0];
+  double best = time_apply(br, bl);
+  for (int i = 1; i < 4; ++i) {
+    const double t = time_apply(cand[i], bl);
+    if (t < 0.95 * best) { best = t; br = cand[i]; }
+  }
+  const double base_l = best;
+  for (int i = 1; i < 4; ++i) {
+    const double t = time_apply(br, cand[i]);
+    if (t < 0.95 * best && t < base_l) { best = t; bl = cand[i]; }
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   replan(br, bl);
