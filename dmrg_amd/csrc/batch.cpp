@@ -48,6 +48,7 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   vector<int32_t> rm, lm;
   vector<int64_t> roff, rsz, loff, lsz;
   vector<double> cflops(n, 0.0);
+  vector<int> rt_end(n), lt_end(n);  // each chain's tasks end here in the merged lists
   int64_t P = 0, SX = 0, RT = 0, LT = 0;
   for (int c = 0; c < n; ++c) {
     const qtensor<T>& L = lenv[c];
@@ -82,6 +83,8 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
                            t[3], t[4], t[5], t[6]});
       cflops[c] += cf * (double)(L.bsize(t[0]) / t[4]) * t[3] * t[4];
     }
+    rt_end[c] = (int)(rm.size() / 7);
+    lt_end[c] = (int)(lm.size() / 7);
     poff[c] = P;
     P += d[c];
     SX += ds;
@@ -122,8 +125,18 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   const int nev = plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
   const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, dmin})) : 1;
   DevBuf evec(P * sizeof(T) * pv);
+  // converged chains leave the operator at the end of their restart cycle
+  RetireFn retire = [&](const std::vector<char>& active) {
+    std::vector<char> kr(rt_end[n - 1]), kl(lt_end[n - 1]);
+    for (int c = 0; c < n; ++c) {
+      for (int i = c ? rt_end[c - 1] : 0; i < rt_end[c]; ++i) kr[i] = active[c];
+      for (int i = c ? lt_end[c - 1] : 0; i < lt_end[c]; ++i) kl[i] = active[c];
+    }
+    hp->restrict_tasks(kr, kl);
+  };
   vector<EigshResult> r = eigsh_batched(op, d, cx, nev, ncv, tol, cfg.max_restart, seeds, nullptr,
-                                        (double*)evec.ptr, pv);
+                                        (double*)evec.ptr, pv, retire);
+  hp->restrict_tasks({}, {});
   int nm = 0;
   for (int c = 0; c < n; ++c) {
     for (int i = 0; i < NEV; ++i) lg[c].e[i] = i < (int)r[c].evals.size() ? r[c].evals[i] : 0.0;
@@ -132,7 +145,7 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
     nm = std::max(nm, r[c].n_matvec);
   }
   bl.n_matvec = nm;
-  bl.flops = plan->flops * nm;
+  for (int c = 0; c < n; ++c) bl.flops += lg[c].flops;  // retired chains stop adding work
   if (plan64) {
     HeffPlan* hp64 = plan64.get();
     MatVec op64 = [hp64, s](const double* x, double* y) { hp64->apply(x, y, s); };

@@ -171,6 +171,22 @@ void HeffPlan::replan(int r_target, int l_target) {
   l_split = l_target;
 }
 
+void HeffPlan::restrict_tasks(const std::vector<char>& keep_r, const std::vector<char>& keep_l) {
+  if (keep_r.empty() && keep_l.empty()) {
+    replan(r_split, l_split);
+    return;
+  }
+  DMRGX_CHECK(keep_r.size() == rexec.size() && keep_l.size() == lexec.size(), "restrict_tasks: mask sizes");
+  std::vector<GemmTask> r, l;
+  for (size_t i = 0; i < rexec.size(); ++i)
+    if (keep_r[i]) r.push_back(rexec[i]);
+  for (size_t i = 0; i < lexec.size(); ++i)
+    if (keep_l[i]) l.push_back(lexec[i]);
+  const bool planes = prec == PREC_BF16X3;
+  rplan = make_gemm_plan(r, dtype, prec, 'N', 'T', uncovered(r, n_store), r_split, planes);
+  lplan = make_gemm_plan(l, dtype, prec, 'N', 'T', uncovered(l, n_psi), l_split, planes);
+}
+
 namespace {
 
 // The tuner's key: precision, element type, and per pass the histogram of

@@ -35,6 +35,9 @@ struct HeffPlan {
   // rebuild the two pass plans with the given split-K targets (work items
   // per launch; 0: two per CU)
   void replan(int r_target, int l_target);
+  // rebuild the pass plans over the tasks with keep_r[i] / keep_l[i] set
+  // (outputs of the dropped tasks read as zero); empty masks: all tasks
+  void restrict_tasks(const std::vector<char>& keep_r, const std::vector<char>& keep_l);
   // pick each pass plan's split-K target by timing candidates once per
   // sector-shape histogram (cached for the process)
   void autotune(hipStream_t s);

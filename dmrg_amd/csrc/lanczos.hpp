@@ -54,9 +54,16 @@ void dev_orthogonalise(const double* V, int64_t ld, int k, double* v, int64_t n,
 // updates segmented in single launches, and gets its own result; all chains
 // take the same number of steps until the last one has converged.  Chains of
 // at most 64 elements are diagonalised densely, as in eigsh.
+// When `retire` is given, a chain whose Ritz pairs pass the test at the end of
+// a restart cycle is finished there (its result and vectors are final) and
+// retire(active) is called with the chains still iterating, so that the
+// caller can shrink `op` to them; the retired chains' segments of later
+// vectors are ignored (they may read as zero).
+using RetireFn = std::function<void(const std::vector<char>& active)>;
 std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int nev,
                                        int ncv, double tol, int max_restart, const std::vector<uint64_t>& seed,
-                                       const double* v0, double* evec, int n_vec = 1);
+                                       const double* v0, double* evec, int n_vec = 1,
+                                       const RetireFn& retire = nullptr);
 // Whether eigsh_batched replays its Lanczos steps as hipGraphs (default off:
 // a DMRG step's plan lives for a few restart cycles only, and capturing plus
 // instantiating the graphs costs more than the replays save; the results are

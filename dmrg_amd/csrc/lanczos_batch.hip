@@ -358,15 +358,16 @@ struct SegOps {
     std::copy(sc.begin(), sc.begin() + sg.n, p);
     DMRGX_HIP(hipMemcpyAsync(d, p, sg.n * 8, hipMemcpyHostToDevice, s));
   }
-  // x_c /= |x_c| for every chain of the set
-  void normalise(double* x, double* h, double* scl) {
+  // x_c /= |x_c| for every chain of the set (a zero segment stays zero when
+  // allow_zero, else it is an error)
+  void normalise(double* x, double* h, double* scl, bool allow_zero = false) {
     vdots(nullptr, 0, x, h);
     const double* hh = fetch(h);
     std::vector<double> sc(sg.n);
     for (int a = 0; a < sg.n; ++a) {
       const double n2 = hh[(size_t)a * hs];
-      DMRGX_CHECK(n2 > 0, "batched Lanczos: zero vector");
-      sc[a] = 1.0 / std::sqrt(n2);
+      DMRGX_CHECK(n2 > 0 || allow_zero, "batched Lanczos: zero vector");
+      sc[a] = n2 > 0 ? 1.0 / std::sqrt(n2) : 0.0;
     }
     put(sc, scl);
     sub(nullptr, 0, h, x, x, scl);
@@ -394,7 +395,7 @@ void set_lanczos_graphs(bool on) { g_graphs.store(on); }
 
 std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int nev,
                                        int ncv, double tol, int max_restart, const std::vector<uint64_t>& seed,
-                                       const double* v0, double* evec, int n_vec) {
+                                       const double* v0, double* evec, int n_vec, const RetireFn& retire) {
   ensure_init();
   hipStream_t s = rt().stream;
   const int nc = (int)len.size();
@@ -505,9 +506,9 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
   std::vector<std::vector<double>> T(nb, std::vector<double>((size_t)m * m, 0.0)), theta(nb), Y(nb);
   std::vector<double> beta_last(nb, 0.0), alpha(nb), b2(nb), nw2(nb), sc(nb);
   std::vector<int> n_reorth(nb, 0);
-  std::vector<char> conv(nb, 0);
+  std::vector<char> conv(nb, 0), retired(nb, 0);
   int j0 = 0, n_matvec = 0, n_restart = 0;
-  GraphSet gstep(m, s), gfin(m, s);
+  std::unique_ptr<GraphSet> gstep = std::make_unique<GraphSet>(m, s), gfin = std::make_unique<GraphSet>(m, s);
   auto run_graph = [&](GraphSet::Slot& g, const std::function<void()>& body) {
     if (lanczos_graphs()) g.run(body);
     else body();
@@ -526,11 +527,12 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
         ops.vdots(V, j + 1, w, h2);
         ops.fetch_async(h, 2);  // h and h2 are adjacent
       };
-      run_graph(gstep[j], body);
+      run_graph((*gstep)[j], body);
       ++n_matvec;
       DMRGX_HIP(hipStreamSynchronize(s));
       const double* hh = ops.pin;
       for (int a = 0; a < nb; ++a) {
+        if (retired[a]) continue;
         const double* q = hh + (size_t)a * hs;
         const double* q2 = hh + (size_t)(nb + a) * hs;
         double hsq = 0, w2 = q2[2 * (j + 1)];
@@ -546,6 +548,10 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
       const double* hfin = h2;
       std::vector<int> lost;
       for (int a = 0; a < nb; ++a) {
+        if (retired[a]) {  // a finished chain's later vectors are zero
+          sc[a] = 0;
+          continue;
+        }
         double beta = std::sqrt(std::max(b2[a], 0.0));
         if (!(beta > 1e-13 * std::max(std::fabs(alpha[a]), std::sqrt(std::max(nw2[a], 0.0))) && beta > 1e-300)) {
           beta = 0;
@@ -558,7 +564,7 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
       }
       double* vn = V + (int64_t)(j + 1) * ld;
       ops.stage(sc);
-      run_graph(gfin[j], [&] {
+      run_graph((*gfin)[j], [&] {
         ops.put_staged((double*)scl.ptr);
         ops.sub(V, j + 1, hfin, w, vn, (const double*)scl.ptr);
       });
@@ -576,6 +582,7 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
     }
     // Rayleigh-Ritz per chain, the chains spread over host threads
     parallel_for(nb, [&](int a) {
+      if (retired[a]) return;
       host_symeig(m, T[a], theta[a], Y[a]);
       bool ok = true;
       double mr = 0;
@@ -590,6 +597,42 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
     bool all = true;
     for (int a = 0; a < nb; ++a) all = all && conv[a];
     if (all || restart == max_restart) break;
+    // chains that converged in this cycle finish here: their Ritz vectors
+    // now, and the caller may drop them from the operator
+    if (retire) {
+      std::vector<int> done;
+      for (int a = 0; a < nb; ++a)
+        if (conv[a] && !retired[a]) done.push_back(a);
+      if (!done.empty()) {
+        std::vector<double> y0((size_t)nb * m * nv, 0.0);
+        for (int a : done)
+          for (int r = 0; r < m; ++r)
+            for (int q = 0; q < nv; ++q) y0[((size_t)a * m + r) * nv + q] = Y[a][(size_t)r * m + q];
+        DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
+        ops.combine(V, m, (const double*)Yb.ptr, nv, (double*)Tmp.ptr);
+        ops.normalise((double*)Tmp.ptr, h, (double*)scl.ptr, true);
+        for (int a : done) {
+          const int c = big[a];
+          for (int q = 0; q < nv; ++q)
+            copy_d2d(evec + q * nr + cw * off[c], (double*)Tmp.ptr + q * nr + cw * off[c], len[c] * cw * 8, s);
+          EigshResult& r = res[c];
+          r.evals.assign(theta[a].begin(), theta[a].begin() + nev);
+          r.n_matvec = n_matvec;
+          r.n_restart = n_restart;
+          r.n_reorth = n_reorth[a];
+          r.converged = true;
+          retired[a] = 1;
+        }
+        DMRGX_HIP(hipStreamSynchronize(s));  // y0 ends here
+        std::vector<char> active(nc, 0);
+        for (int a = 0; a < nb; ++a)
+          if (!retired[a]) active[big[a]] = 1;
+        retire(active);
+        // captured steps hold the previous operator's launches
+        gstep = std::make_unique<GraphSet>(m, s);
+        gfin = std::make_unique<GraphSet>(m, s);
+      }
+    }
     // thick restart of every chain: its `keep` lowest Ritz vectors and the residual
     std::vector<double> Yk((size_t)nb * m * keep);
     for (int a = 0; a < nb; ++a)
@@ -619,15 +662,22 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
   DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
   double* out = (double*)Tmp.ptr;
   ops.combine(V, m, (const double*)Yb.ptr, nv, out);
-  ops.normalise(out, h, (double*)scl.ptr);
-  if (small.empty()) {
+  int n_retired = 0;
+  for (char x : retired) n_retired += x;
+  ops.normalise(out, h, (double*)scl.ptr, n_retired > 0);
+  if (small.empty() && n_retired == 0) {
     copy_d2d(evec, out, (size_t)nv * nr * 8, s);
   } else {
     for (int q = 0; q < nv; ++q)
-      for (int c : big) copy_d2d(evec + q * nr + cw * off[c], out + q * nr + cw * off[c], len[c] * cw * 8, s);
+      for (int a = 0; a < nb; ++a)
+        if (!retired[a]) {
+          const int c = big[a];
+          copy_d2d(evec + q * nr + cw * off[c], out + q * nr + cw * off[c], len[c] * cw * 8, s);
+        }
   }
   DMRGX_HIP(hipStreamSynchronize(s));
   for (int a = 0; a < nb; ++a) {
+    if (retired[a]) continue;
     EigshResult& r = res[big[a]];
     r.evals.assign(theta[a].begin(), theta[a].begin() + nev);
     r.n_matvec = n_matvec;
