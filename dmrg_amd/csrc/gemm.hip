@@ -446,9 +446,11 @@ __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__
 
 // ---------------------------------------------------------------------------
 // BF16x3 on pre-split operands: A, B are bfloat16 planes (re_hi, re_lo[, im_hi,
-// im_lo]); 128x128 output tiles, each wavefront 64x64 (4x4 MFMA 16x16x32
-// tiles); 16-byte vector loads straight into LDS when a segment is aligned.
+// im_lo]); 128x128 output tiles, eight wavefronts of 32x64 (2x4 MFMA 16x16x32
+// tiles); 16-byte vector loads staged through registers into LDS.
 constexpr int PT = 128, PBK = 32, PLDK = PBK + 8;
+// eight wavefronts per 128x128 tile, each 32 rows x 64 columns (2 x 4 MFMA tiles)
+constexpr int PNT = 512;
 
 // Row of the q-th 64-byte row segment a work-group stages per plane: within
 // each group of 16 rows the 4 x 4 (q % 4, q / 4) order is transposed, so the
@@ -463,7 +465,7 @@ __device__ __forceinline__ uint4 xor_sign(uint4 v) {
 }
 
 template <bool CPLX, int OUT>
-__global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ items, int nitems, int per_xcd,
+__global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ items, int nitems, int per_xcd,
                                                   const GemmSeg* __restrict__ segs,
                                                   const unsigned short* __restrict__ A, int64_t as,
                                                   const unsigned short* __restrict__ B, int64_t bs,
@@ -472,7 +474,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int CPR = PBK / 8;               // 16-byte chunks per row per plane (4)
   constexpr int CH = PT * CPR * NPL;         // chunks per operand per k-step
-  constexpr int CPT = CH / NT;               // chunks per thread (8 complex, 4 real)
+  constexpr int CPT = CH / PNT;              // chunks per thread (4 complex, 2 real)
   const int bid = blockIdx.x;
   const int item_id = (bid & 7) * per_xcd + (bid >> 3);
   if (item_id >= nitems) return;
@@ -481,12 +483,12 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
   __shared__ __attribute__((aligned(16))) unsigned short sA[NPL][PT * PLDK];
   __shared__ __attribute__((aligned(16))) unsigned short sB[NPL][PT * PLDK];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
-  f4v acc[NP][4][4];
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+  f4v acc[NP][2][4];
 #pragma unroll
   for (int p = 0; p < NP; ++p)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j) acc[p][i][j] = f4v{0, 0, 0, 0};
 
@@ -503,7 +505,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
     okA = okB = 0;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int ch = t + NT * j;
+      const int ch = t + PNT * j;
       const int plane = ch / (PT * CPR);
       const int rem = ch % (PT * CPR);
       const int row = stage_row(rem / CPR), kc = (rem % CPR) * 8;
@@ -545,7 +547,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
       const bool tail = c.s.k1 == c.s.K;
 #pragma unroll
       for (int j = 0; j < CPT; ++j) {
-        const int ch = t + NT * j;
+        const int ch = t + PNT * j;
         const int kc = ((ch % (PT * CPR)) % CPR) * 8;
         const int gk = c.k + kc;
         const bool whole = gk + 8 <= s.k1 || (tail && gk < s.k1);
@@ -570,7 +572,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int ch = t + NT * j;
+      const int ch = t + PNT * j;
       const int plane = ch / (PT * CPR);
       const int rem = ch % (PT * CPR);
       const int row = stage_row(rem / CPR), kc = (rem % CPR) * 8;
@@ -585,11 +587,11 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
       gload(cur);
     }
     const int kx = (lane >> 4) * 8;
-    bf8v a[NPL][4];
+    bf8v a[NPL][2];
 #pragma unroll
     for (int p = 0; p < NPL; ++p)
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
         a[p][i] = *reinterpret_cast<const bf8v*>(&sA[p][(wm + i * 16 + (lane & 15)) * PLDK + kx]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
         nbl = __builtin_bit_cast(bf8v, __builtin_bit_cast(u16x8, b[3]) ^ (unsigned short)0x8000);
       }
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
+      for (int i = 0; i < 2; ++i) {
         f4v r = acc[0][i][j];
         r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0], r, 0, 0, 0);
         r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1], r, 0, 0, 0);
@@ -631,7 +633,7 @@ __global__ __launch_bounds__(NT) void k_gemm_bf3p(const GemmItem* __restrict__ i
   const bool partial = it.flags & ITEM_PARTIAL;
   const bool beta = !partial && (it.flags & ITEM_BETA);
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
@@ -943,7 +945,7 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
     auto it = (const GemmItem*)p.d_items->ptr;
     auto sg = (const GemmSeg*)p.d_segs->ptr;
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(p.grid), dim3(NT), 0, s, it, p.n_items, per, sg, (const unsigned short*)A,
+      hipLaunchKernelGGL(kern, dim3(p.grid), dim3(PNT), 0, s, it, p.n_items, per, sg, (const unsigned short*)A,
                          a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp);
     };
     if (p.dtype == DT_C64) {
