@@ -480,8 +480,10 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
   if (item_id >= nitems) return;
   const GemmItem it = items[item_id];
 
-  __shared__ __attribute__((aligned(16))) unsigned short sA[NPL][PT * PLDK];
-  __shared__ __attribute__((aligned(16))) unsigned short sB[NPL][PT * PLDK];
+  // two LDS stages: the next k-step is stored while this one is multiplied,
+  // so that one barrier per k-step suffices
+  __shared__ __attribute__((aligned(16))) unsigned short sA[2][NPL][PT * PLDK];
+  __shared__ __attribute__((aligned(16))) unsigned short sB[2][NPL][PT * PLDK];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
   f4v acc[NP][2][4];
@@ -567,22 +569,30 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
       }
     }
   };
-  if (cur.valid()) gload(cur);
-  while (cur.valid()) {
-    __syncthreads();
+  auto stage = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
       const int ch = t + PNT * j;
       const int plane = ch / (PT * CPR);
       const int rem = ch % (PT * CPR);
       const int row = stage_row(rem / CPR), kc = (rem % CPR) * 8;
-      *reinterpret_cast<uint4*>(&sA[plane][row * PLDK + kc]) = ra[j];
-      *reinterpret_cast<uint4*>(&sB[plane][row * PLDK + kc]) = rb[j];
+      *reinterpret_cast<uint4*>(&sA[buf][plane][row * PLDK + kc]) = ra[j];
+      *reinterpret_cast<uint4*>(&sB[buf][plane][row * PLDK + kc]) = rb[j];
     }
-    __syncthreads();
+  };
+  // the cursor depends on the work item only, so every thread of the block
+  // takes the same number of trips and meets the same barriers
+  if (cur.valid()) {
+    gload(cur);
+    stage(0);
+  }
+  __syncthreads();
+  int buf = 0;
+  while (cur.valid()) {
     const int prev_seg = cur.seg;
     cur.next<PBK>(segs);
-    if (cur.valid()) {
+    const bool more = cur.valid();
+    if (more) {
       if (cur.seg != prev_seg) seg_setup(cur.s);
       gload(cur);
     }
@@ -592,13 +602,13 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
     for (int p = 0; p < NPL; ++p)
 #pragma unroll
       for (int i = 0; i < 2; ++i)
-        a[p][i] = *reinterpret_cast<const bf8v*>(&sA[p][(wm + i * 16 + (lane & 15)) * PLDK + kx]);
+        a[p][i] = *reinterpret_cast<const bf8v*>(&sA[buf][p][(wm + i * 16 + (lane & 15)) * PLDK + kx]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       bf8v b[NPL];
 #pragma unroll
       for (int p = 0; p < NPL; ++p)
-        b[p] = *reinterpret_cast<const bf8v*>(&sB[p][(wn + j * 16 + (lane & 15)) * PLDK + kx]);
+        b[p] = *reinterpret_cast<const bf8v*>(&sB[buf][p][(wn + j * 16 + (lane & 15)) * PLDK + kx]);
       bf8v nbh, nbl;
       if (CPLX) {
         nbh = __builtin_bit_cast(bf8v, __builtin_bit_cast(u16x8, b[2]) ^ (unsigned short)0x8000);
@@ -628,6 +638,10 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
         }
       }
     }
+    // the other stage was last read before the previous barrier
+    if (more) stage(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
   }
 
   const bool partial = it.flags & ITEM_PARTIAL;
