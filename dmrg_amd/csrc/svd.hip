@@ -962,25 +962,30 @@ __global__ __launch_bounds__(256) void k_jfinal(const double* __restrict__ X, co
 // vectors (X = Q R); vector j in [good, k0) becomes Q e_j times the phase of
 // R_jj (the Gram-Schmidt orthonormalisation of the original direction, same
 // phase), and vector j in [k0, k1) becomes the complement vector Q e_j.  The
-// reflectors live in a workspace of L*k0 elements at w_off.  One workgroup of
-// 1024 threads per task; target vectors are built one per wavefront.
+// reflectors live in a workspace of L*k0 elements at w_off, their squared
+// norms and the phases of R_jj in 3*k0 doubles at h_off.  k_complete_qr: one
+// workgroup of 1024 threads per task; k_complete_apply: one wavefront per
+// target vector, spread over the grid.
 struct CompTask {
-  int64_t off, ks, is, w_off;
+  int64_t off, ks, is, w_off, h_off;
   int L, k0, k1, good;  // QR of vectors [0,k0); rewrite vectors [good,k1)
+};
+// four target vectors (one per wavefront) of one task
+struct CompItem {
+  int task, j0;
 };
 
 constexpr int CNT = 1024;
 
 template <bool CPLX>
-__global__ __launch_bounds__(CNT) void k_complete(double* __restrict__ X, double* __restrict__ Wk,
-                                                   const CompTask* __restrict__ tasks) {
+__global__ __launch_bounds__(CNT) void k_complete_qr(const double* __restrict__ X, double* __restrict__ Wk,
+                                                      double* __restrict__ Hk, const CompTask* __restrict__ tasks) {
   const CompTask t = tasks[blockIdx.x];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = CNT / 64;
   constexpr int C = CPLX ? 2 : 1;
-  __shared__ double red[2][CNT / 64];
-  __shared__ double vn2_s;
+  __shared__ double red[CNT / 64];
   double* W = Wk + C * t.w_off;  // column-major L x k0 (vector j at W + C*j*L)
-  auto xel = [&](int k, int i) -> double* { return X + C * (t.off + (int64_t)k * t.ks + (int64_t)i * t.is); };
+  auto xel = [&](int k, int i) -> const double* { return X + C * (t.off + (int64_t)k * t.ks + (int64_t)i * t.is); };
   // copy the kept vectors
   for (int64_t e = tid; e < (int64_t)t.L * t.k0; e += CNT) {
     int j = (int)(e / t.L), i = (int)(e % t.L);
@@ -989,7 +994,8 @@ __global__ __launch_bounds__(CNT) void k_complete(double* __restrict__ X, double
     if (CPLX) W[C * e + 1] = x[1];
   }
   __syncthreads();
-  // Householder QR; reflector i stored in W[i:, i] with its squared norm in vn2[i] (LDS-free: recomputed)
+  // Householder QR; reflector i stored in W[i:, i], its squared norm in
+  // vn2_all[i] and the phase of R_ii in vn2_all[k0 + 2i, k0 + 2i + 1]
   extern __shared__ double vn2_all[];
   for (int i = 0; i < t.k0; ++i) {
     double* col = W + C * (int64_t)i * t.L;
@@ -997,11 +1003,11 @@ __global__ __launch_bounds__(CNT) void k_complete(double* __restrict__ X, double
     double a = 0;
     for (int r = i + tid; r < t.L; r += CNT) a += CPLX ? col[2 * r] * col[2 * r] + col[2 * r + 1] * col[2 * r + 1] : col[r] * col[r];
     for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-    if (lane == 0) red[0][wave] = a;
+    if (lane == 0) red[wave] = a;
     __syncthreads();
     if (tid == 0) {
       double nrm2 = 0;
-      for (int q = 0; q < nw; ++q) nrm2 += red[0][q];
+      for (int q = 0; q < nw; ++q) nrm2 += red[q];
       double nrm = sqrt(nrm2);
       double x0r = col[C * i], x0i = CPLX ? col[C * i + 1] : 0;
       double ax0 = CPLX ? hypot(x0r, x0i) : fabs(x0r);
@@ -1042,54 +1048,67 @@ __global__ __launch_bounds__(CNT) void k_complete(double* __restrict__ X, double
     }
     __syncthreads();
   }
-  // Q e_j = H_0 H_1 ... H_{k0-1} e_j for j = good .. k1-1
-  for (int j = t.good + wave; j < t.k1; j += nw) {
-    for (int r = lane; r < t.L; r += 64) {
+  for (int e = tid; e < 3 * t.k0; e += CNT) Hk[t.h_off + e] = vn2_all[e];
+}
+
+// Q e_j = H_0 H_1 ... H_{k0-1} e_j for j = j0 + wave of item blockIdx.x
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_complete_apply(double* __restrict__ X, const double* __restrict__ Wk,
+                                                        const double* __restrict__ Hk,
+                                                        const CompTask* __restrict__ tasks,
+                                                        const CompItem* __restrict__ items) {
+  const CompItem it = items[blockIdx.x];
+  const CompTask t = tasks[it.task];
+  const int lane = threadIdx.x & 63, j = it.j0 + (threadIdx.x >> 6);
+  if (j >= t.k1) return;
+  constexpr int C = CPLX ? 2 : 1;
+  const double* W = Wk + C * t.w_off;
+  const double* H = Hk + t.h_off;
+  auto xel = [&](int k, int i) -> double* { return X + C * (t.off + (int64_t)k * t.ks + (int64_t)i * t.is); };
+  for (int r = lane; r < t.L; r += 64) {
+    double* x = xel(j, r);
+    x[0] = r == j ? 1.0 : 0.0;
+    if (CPLX) x[1] = 0.0;
+  }
+  for (int i = t.k0 - 1; i >= 0; --i) {
+    const double vn2 = H[i];
+    if (vn2 <= 0) continue;
+    const double* col = W + C * (int64_t)i * t.L;
+    double dr = 0, di = 0;
+    for (int r = i + lane; r < t.L; r += 64) {
+      const double* x = xel(j, r);
+      if (CPLX) {
+        dr += col[2 * r] * x[0] + col[2 * r + 1] * x[1];
+        di += col[2 * r] * x[1] - col[2 * r + 1] * x[0];
+      } else {
+        dr += col[r] * x[0];
+      }
+    }
+    for (int o = 32; o > 0; o >>= 1) { dr += __shfl_xor(dr, o, 64); if (CPLX) di += __shfl_xor(di, o, 64); }
+    const double fr = 2 * dr / vn2, fi = 2 * di / vn2;
+    for (int r = i + lane; r < t.L; r += 64) {
       double* x = xel(j, r);
-      x[0] = r == j ? 1.0 : 0.0;
-      if (CPLX) x[1] = 0.0;
-    }
-    for (int i = t.k0 - 1; i >= 0; --i) {
-      const double vn2 = vn2_all[i];
-      if (vn2 <= 0) continue;
-      const double* col = W + C * (int64_t)i * t.L;
-      double dr = 0, di = 0;
-      for (int r = i + lane; r < t.L; r += 64) {
-        const double* x = xel(j, r);
-        if (CPLX) {
-          dr += col[2 * r] * x[0] + col[2 * r + 1] * x[1];
-          di += col[2 * r] * x[1] - col[2 * r + 1] * x[0];
-        } else {
-          dr += col[r] * x[0];
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) { dr += __shfl_xor(dr, o, 64); if (CPLX) di += __shfl_xor(di, o, 64); }
-      const double fr = 2 * dr / vn2, fi = 2 * di / vn2;
-      for (int r = i + lane; r < t.L; r += 64) {
-        double* x = xel(j, r);
-        if (CPLX) {
-          x[0] -= col[2 * r] * fr - col[2 * r + 1] * fi;
-          x[1] -= col[2 * r] * fi + col[2 * r + 1] * fr;
-        } else {
-          x[0] -= col[r] * fr;
-        }
-      }
-    }
-    if (j < t.k0) {  // keep the phase of the original direction: multiply by R_jj/|R_jj|
-      const double qr = vn2_all[t.k0 + 2 * j], qi = vn2_all[t.k0 + 2 * j + 1];
-      for (int r = lane; r < t.L; r += 64) {
-        double* x = xel(j, r);
-        if (CPLX) {
-          double a = x[0], b = x[1];
-          x[0] = a * qr - b * qi;
-          x[1] = a * qi + b * qr;
-        } else {
-          x[0] *= qr;
-        }
+      if (CPLX) {
+        x[0] -= col[2 * r] * fr - col[2 * r + 1] * fi;
+        x[1] -= col[2 * r] * fi + col[2 * r + 1] * fr;
+      } else {
+        x[0] -= col[r] * fr;
       }
     }
   }
-  (void)vn2_s;
+  if (j < t.k0) {  // keep the phase of the original direction: multiply by R_jj/|R_jj|
+    const double qr = H[t.k0 + 2 * j], qi = H[t.k0 + 2 * j + 1];
+    for (int r = lane; r < t.L; r += 64) {
+      double* x = xel(j, r);
+      if (CPLX) {
+        double a = x[0], b = x[1];
+        x[0] = a * qr - b * qi;
+        x[1] = a * qi + b * qr;
+      } else {
+        x[0] *= qr;
+      }
+    }
+  }
 }
 
 template <bool CPLX>
@@ -1436,7 +1455,7 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
   // norms, so X / sig is orthonormal there; the negligible ones are replaced
   // by the Householder complement of the others.
   std::vector<CompTask> tu, tv;
-  int64_t wu = 0, wv = 0;
+  int64_t wu = 0, wv = 0, hu = 0, hv = 0;
   int maxk = 0;
   for (int b = 0; b < nb; ++b) {
     const JBlk& j = jb[b];
@@ -1452,30 +1471,47 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
     if (need > good) {
       const int k0 = good;
       if (j.transposed) {
-        tu.push_back(CompTask{blocks[b].u_off, 1, j.rows, wu, j.rows, k0, need, good});
+        tu.push_back(CompTask{blocks[b].u_off, 1, j.rows, wu, hu, j.rows, k0, need, good});
         wu += (int64_t)j.rows * std::max(k0, 1);
+        hu += 3 * (int64_t)std::max(k0, 1);
       } else {
-        tv.push_back(CompTask{blocks[b].v_off, j.cols, 1, wv, j.cols, k0, need, good});
+        tv.push_back(CompTask{blocks[b].v_off, j.cols, 1, wv, hv, j.cols, k0, need, good});
         wv += (int64_t)j.cols * std::max(k0, 1);
+        hv += 3 * (int64_t)std::max(k0, 1);
       }
       maxk = std::max(maxk, k0);
     }
   }
-  auto launch = [&](std::vector<CompTask>& v, int64_t wlen, void* Xo) {
+  auto launch = [&](std::vector<CompTask>& v, int64_t wlen, int64_t hlen, void* Xo) {
     if (v.empty()) return;
-    DevBuf d(v.size() * sizeof(CompTask)), wk(std::max<int64_t>(wlen, 1) * 8 * cw);
+    std::vector<CompItem> items;
+    for (int q = 0; q < (int)v.size(); ++q)
+      for (int j = v[q].good; j < v[q].k1; j += 4) items.push_back(CompItem{q, j});
+    DevBuf d(v.size() * sizeof(CompTask)), di(std::max<size_t>(items.size(), 1) * sizeof(CompItem));
+    DevBuf wk(std::max<int64_t>(wlen, 1) * 8 * cw), hk(std::max<int64_t>(hlen, 1) * 8);
     DMRGX_HIP(hipMemcpyAsync(d.ptr, v.data(), v.size() * sizeof(CompTask), hipMemcpyHostToDevice, s));
+    if (!items.empty())
+      DMRGX_HIP(hipMemcpyAsync(di.ptr, items.data(), items.size() * sizeof(CompItem), hipMemcpyHostToDevice, s));
     size_t shm = 3 * (size_t)std::max(maxk, 1) * sizeof(double);
     DMRGX_CHECK(shm <= 64 * 1024, "block_svd: too many kept vectors for the completion kernel");
-    if (cplx) hipLaunchKernelGGL(k_complete<true>, dim3((unsigned)v.size()), dim3(CNT), shm, s, (double*)Xo,
-                                 (double*)wk.ptr, (const CompTask*)d.ptr);
-    else hipLaunchKernelGGL(k_complete<false>, dim3((unsigned)v.size()), dim3(CNT), shm, s, (double*)Xo,
-                            (double*)wk.ptr, (const CompTask*)d.ptr);
+    if (cplx) hipLaunchKernelGGL(k_complete_qr<true>, dim3((unsigned)v.size()), dim3(CNT), shm, s,
+                                 (const double*)Xo, (double*)wk.ptr, (double*)hk.ptr, (const CompTask*)d.ptr);
+    else hipLaunchKernelGGL(k_complete_qr<false>, dim3((unsigned)v.size()), dim3(CNT), shm, s,
+                            (const double*)Xo, (double*)wk.ptr, (double*)hk.ptr, (const CompTask*)d.ptr);
     DMRGX_HIP(hipGetLastError());
+    if (!items.empty()) {
+      if (cplx) hipLaunchKernelGGL(k_complete_apply<true>, dim3((unsigned)items.size()), dim3(256), 0, s,
+                                   (double*)Xo, (const double*)wk.ptr, (const double*)hk.ptr,
+                                   (const CompTask*)d.ptr, (const CompItem*)di.ptr);
+      else hipLaunchKernelGGL(k_complete_apply<false>, dim3((unsigned)items.size()), dim3(256), 0, s,
+                              (double*)Xo, (const double*)wk.ptr, (const double*)hk.ptr,
+                              (const CompTask*)d.ptr, (const CompItem*)di.ptr);
+      DMRGX_HIP(hipGetLastError());
+    }
     DMRGX_HIP(hipStreamSynchronize(s));
   };
-  launch(tu, wu, U);
-  launch(tv, wv, V);
+  launch(tu, wu, hu, U);
+  launch(tv, wv, hv, V);
   DMRGX_HIP(hipStreamSynchronize(s));
   return sweeps;
 }
