@@ -495,89 +495,45 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
       for (int j = 0; j < 4; ++j) acc[p][i][j] = f4v{0, 0, 0, 0};
 
   uint4 ra[CPT], rb[CPT];
-  // per-chunk element offsets, recomputed once per segment
-  int64_t offA[CPT], offB[CPT];
-  unsigned okA = 0, okB = 0;  // bit j: chunk j's row is inside the block
+  // chunk j of thread t is chunk t of plane j (PNT = PT * CPR): one row and
+  // k offset per thread, the plane stride added per chunk
+  static_assert(PNT == PT * CPR && CPT == NPL, "one chunk per plane and thread");
+  const int srow = stage_row(t / CPR), skc = (t % CPR) * 8;
+  // element offsets of this thread's chunk in plane 0, recomputed once per segment
+  int64_t offA = 0, offB = 0;
+  bool okA = false, okB = false;  // the chunk's row is inside the block
   Cursor cur;
   cur.start(segs, it.seg_begin, it.seg_end);
-  bool avec = false, bvec = false;
   auto seg_setup = [&](const GemmSeg& s) {
-    avec = ((s.a_off | s.lda | s.k0) & 7) == 0;
-    bvec = ((s.b_off | s.ldb | s.k0) & 7) == 0;
-    okA = okB = 0;
-#pragma unroll
-    for (int j = 0; j < CPT; ++j) {
-      const int ch = t + PNT * j;
-      const int plane = ch / (PT * CPR);
-      const int rem = ch % (PT * CPR);
-      const int row = stage_row(rem / CPR), kc = (rem % CPR) * 8;
-      const int gr = it.m0 + row, gn = it.n0 + row;
-      offA[j] = plane * as + s.a_off + (int64_t)gr * s.lda + kc;
-      offB[j] = plane * bs + s.b_off + (int64_t)gn * s.ldb + kc;
-      if (gr < it.M) okA |= 1u << j;
-      if (gn < it.N) okB |= 1u << j;
-    }
+    const int gr = it.m0 + srow, gn = it.n0 + srow;
+    offA = s.a_off + (int64_t)gr * s.lda + skc;
+    offB = s.b_off + (int64_t)gn * s.ldb + skc;
+    okA = gr < it.M;
+    okB = gn < it.N;
   };
   if (cur.valid()) seg_setup(cur.s);
-  auto load8 = [&](const unsigned short* src, int gk, int k1) {
-    unsigned short h[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) h[e] = (gk + e < k1) ? src[e] : (unsigned short)0;
-    uint4 v;
-    v.x = h[0] | ((unsigned)h[1] << 16); v.y = h[2] | ((unsigned)h[3] << 16);
-    v.z = h[4] | ((unsigned)h[5] << 16); v.w = h[6] | ((unsigned)h[7] << 16);
-    return v;
-  };
+  // plane plans are 8-element aligned (operand offsets, row strides and K
+  // cuts) and their rows are zero-padded to a multiple of 8, so a 16-byte
+  // chunk is either inside [k0, k1) or wholly past k1
   auto gload = [&](const Cursor& c) {
-    const GemmSeg& s = c.s;
-    const bool neg = s.alpha < 0;
-    const bool full = c.k + PBK <= s.k1;
-    if (full && avec && bvec) {
+    const bool neg = c.s.alpha < 0;
+    const bool in = c.k + skc < c.s.k1;
+    const bool la = okA && in, lb = okB && in;
 #pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (okA & (1u << j)) v = *reinterpret_cast<const uint4*>(A + offA[j] + c.k);
-        ra[j] = neg ? xor_sign(v) : v;
-        uint4 w = make_uint4(0, 0, 0, 0);
-        if (okB & (1u << j)) w = *reinterpret_cast<const uint4*>(B + offB[j] + c.k);
-        rb[j] = w;
-      }
-    } else {
-      // a segment that ends at its row end may read the whole last 8-chunk: an
-      // aligned row stride (lda, ldb multiples of 8) is either K itself or the
-      // zero-padded round-up of K
-      const bool tail = c.s.k1 == c.s.K;
-#pragma unroll
-      for (int j = 0; j < CPT; ++j) {
-        const int ch = t + PNT * j;
-        const int kc = ((ch % (PT * CPR)) % CPR) * 8;
-        const int gk = c.k + kc;
-        const bool whole = gk + 8 <= s.k1 || (tail && gk < s.k1);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (okA & (1u << j)) {
-          const unsigned short* src = A + offA[j] + c.k;
-          v = (avec && whole) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
-          if (neg) v = xor_sign(v);
-        }
-        ra[j] = v;
-        uint4 w = make_uint4(0, 0, 0, 0);
-        if (okB & (1u << j)) {
-          const unsigned short* src = B + offB[j] + c.k;
-          w = (bvec && whole) ? *reinterpret_cast<const uint4*>(src) : load8(src, gk, s.k1);
-        }
-        rb[j] = w;
-      }
+    for (int j = 0; j < CPT; ++j) {
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (la) v = *reinterpret_cast<const uint4*>(A + j * as + offA + c.k);
+      ra[j] = neg ? xor_sign(v) : v;
+      uint4 w = make_uint4(0, 0, 0, 0);
+      if (lb) w = *reinterpret_cast<const uint4*>(B + j * bs + offB + c.k);
+      rb[j] = w;
     }
   };
   auto stage = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
-      const int ch = t + PNT * j;
-      const int plane = ch / (PT * CPR);
-      const int rem = ch % (PT * CPR);
-      const int row = stage_row(rem / CPR), kc = (rem % CPR) * 8;
-      *reinterpret_cast<uint4*>(&sA[buf][plane][row * PLDK + kc]) = ra[j];
-      *reinterpret_cast<uint4*>(&sB[buf][plane][row * PLDK + kc]) = rb[j];
+      *reinterpret_cast<uint4*>(&sA[buf][j][srow * PLDK + skc]) = ra[j];
+      *reinterpret_cast<uint4*>(&sB[buf][j][srow * PLDK + skc]) = rb[j];
     }
   };
   // the cursor depends on the work item only, so every thread of the block
@@ -805,6 +761,10 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     }
     Group& g = groups[f->second];
     DMRGX_CHECK(g.M == k.M && g.N == k.N, "gemm plan: tasks into one output block disagree on shape");
+    // the pre-split kernel loads whole 16-byte chunks: operand offsets and
+    // row strides on 8-element boundaries, rows zero-padded to a multiple of 8
+    DMRGX_CHECK(!planes || ((k.a_off | k.b_off | (k.lda > 0 ? k.lda : k.K) | (k.ldb > 0 ? k.ldb : k.K)) & 7) == 0,
+                "pre-split gemm plan: operand offsets and row strides must be multiples of 8");
     g.t.push_back(i);
     g.ksum += k.K;
     plan->flops += cf * (double)k.M * k.N * k.K;
