@@ -71,6 +71,10 @@ void fill_zero(void* p, size_t bytes, hipStream_t s);
 // n bfloat16 / u16 zeros at a 2-byte aligned address
 void fill_zero_u16(uint16_t* p, size_t n, hipStream_t s);
 void copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Host-to-device copy on stream s that returns without waiting for the
+// stream: up to 1 MiB is first copied into a pinned ring (src may be reused at
+// once), larger copies go straight.
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 inline size_t elem_bytes(int dtype) { return dtype == DT_C64 ? 16 : 8; }
 

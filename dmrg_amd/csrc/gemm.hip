@@ -854,11 +854,9 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
   if (plan->n_items) {
     plan->d_items = new DevBuf(items.size() * sizeof(GemmItem));
     plan->d_segs = new DevBuf(std::max<size_t>(segs.size(), 1) * sizeof(GemmSeg));
-    DMRGX_HIP(hipMemcpyAsync(plan->d_items->ptr, items.data(), items.size() * sizeof(GemmItem),
-                             hipMemcpyHostToDevice, rt().stream));
+    upload(plan->d_items->ptr, items.data(), items.size() * sizeof(GemmItem), rt().stream);
     if (!segs.empty())
-      DMRGX_HIP(hipMemcpyAsync(plan->d_segs->ptr, segs.data(), segs.size() * sizeof(GemmSeg),
-                               hipMemcpyHostToDevice, rt().stream));
+      upload(plan->d_segs->ptr, segs.data(), segs.size() * sizeof(GemmSeg), rt().stream);
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   if (!red.empty()) {
@@ -874,10 +872,8 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     plan->red_blocks = (int)acc;
     plan->d_red = new DevBuf(red.size() * sizeof(RedTask));
     plan->d_red_start = new DevBuf(red.size() * 8);
-    DMRGX_HIP(hipMemcpyAsync(plan->d_red->ptr, red.data(), red.size() * sizeof(RedTask), hipMemcpyHostToDevice,
-                             rt().stream));
-    DMRGX_HIP(hipMemcpyAsync(plan->d_red_start->ptr, start.data(), start.size() * 8, hipMemcpyHostToDevice,
-                             rt().stream));
+    upload(plan->d_red->ptr, red.data(), red.size() * sizeof(RedTask), rt().stream);
+    upload(plan->d_red_start->ptr, start.data(), start.size() * 8, rt().stream);
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   return plan;
@@ -989,8 +985,7 @@ std::unique_ptr<PadLayout> make_pad_layout(std::vector<PadBlk> blocks) {
   L->total = pos;
   if (!L->blk.empty()) {
     L->d_blk = std::make_unique<DevBuf>(L->blk.size() * sizeof(PadBlk));
-    DMRGX_HIP(hipMemcpyAsync(L->d_blk->ptr, L->blk.data(), L->blk.size() * sizeof(PadBlk), hipMemcpyHostToDevice,
-                             rt().stream));
+    upload(L->d_blk->ptr, L->blk.data(), L->blk.size() * sizeof(PadBlk), rt().stream);
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   return L;

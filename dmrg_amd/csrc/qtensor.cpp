@@ -180,7 +180,7 @@ void qtensor<T>::update(const tensor<T>& val, int i0, int i1, int i2, int i3, in
   layout();
   old.insert(old.end(), val.val_.begin(), val.val_.end());
   allocate();
-  DMRGX_HIP(hipMemcpyAsync(data(), old.data(), old.size() * sizeof(T), hipMemcpyHostToDevice, rt().stream));
+  upload(data(), old.data(), old.size() * sizeof(T), rt().stream);
   DMRGX_HIP(hipStreamSynchronize(rt().stream));
 }
 
@@ -219,7 +219,7 @@ qtensor<cplx> qtensor<T>::comp() const {
     // real -> complex: copy every real value into the real slot (stride 2)
     vector<T> h = values();
     vector<cplx> hc(h.begin(), h.end());
-    DMRGX_HIP(hipMemcpyAsync(r.data(), hc.data(), hc.size() * sizeof(cplx), hipMemcpyHostToDevice, rt().stream));
+    upload(r.data(), hc.data(), hc.size() * sizeof(cplx), rt().stream);
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   return r;
@@ -494,7 +494,7 @@ qtensor<T> qtensor<T>::remove_symmetry() const {
     }
   }
   r.allocate();
-  DMRGX_HIP(hipMemcpyAsync(r.data(), out.data(), out.size() * sizeof(T), hipMemcpyHostToDevice, rt().stream));
+  upload(r.data(), out.data(), out.size() * sizeof(T), rt().stream);
   DMRGX_HIP(hipStreamSynchronize(rt().stream));
   return r;
 }
@@ -927,7 +927,7 @@ vector<double> svd_finish(const vector<vector<int>>& dim, const SvdJob<T>& j, co
     for (int k = 0; k < n; ++k) hs[S.off_[i] + (int64_t)k * scols[i] + k] = Sv[i][k];
   }
   if (S.total_) {
-    DMRGX_HIP(hipMemcpyAsync(S.data(), hs.data(), hs.size() * 8, hipMemcpyHostToDevice, rt().stream));
+    upload(S.data(), hs.data(), hs.size() * 8, rt().stream);
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   vector<vector<int>> dl(dim.begin(), dim.begin() + num), dr(dim.begin() + num, dim.end());

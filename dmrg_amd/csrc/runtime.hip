@@ -2,6 +2,7 @@
 // caching allocator and the error string behind dmrgx_last_error().
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <unordered_map>
@@ -125,6 +126,36 @@ void fill_zero_u16(uint16_t* p, size_t n, hipStream_t s) {
 
 void copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
   if (bytes) DMRGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s));
+}
+
+namespace {
+// pinned ring for small host-to-device uploads: a copy from pageable memory
+// returns only after the stream has drained up to it, one from pinned memory
+// is enqueued and returns
+constexpr size_t UP_RING = 32u << 20, UP_MAX = 1u << 20;
+std::mutex g_up_mu;
+char* g_up_ring = nullptr;
+size_t g_up_head = 0;
+}  // namespace
+
+void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  if (!bytes) return;
+  if (bytes > UP_MAX) {
+    DMRGX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s));
+    return;
+  }
+  std::lock_guard<std::mutex> lk(g_up_mu);
+  if (!g_up_ring) DMRGX_HIP(hipHostMalloc((void**)&g_up_ring, UP_RING, hipHostMallocDefault));
+  size_t a = (g_up_head + 255) & ~(size_t)255;
+  if (a + bytes > UP_RING) {
+    // the next lap overwrites staged bytes: every copy that reads them must
+    // have completed
+    DMRGX_HIP(hipDeviceSynchronize());
+    a = 0;
+  }
+  std::memcpy(g_up_ring + a, src, bytes);
+  DMRGX_HIP(hipMemcpyAsync(dst, g_up_ring + a, bytes, hipMemcpyHostToDevice, s));
+  g_up_head = a + bytes;
 }
 
 }  // namespace dmrgx

@@ -1133,8 +1133,8 @@ std::vector<double> block_maxabs(bool cplx, const void* A, const std::vector<Svd
   std::vector<int64_t> off(nb), len(nb);
   for (int b = 0; b < nb; ++b) { off[b] = blocks[b].a_off; len[b] = (int64_t)blocks[b].rows * blocks[b].cols; }
   DevBuf doff(nb * 8), dlen(nb * 8), dout(nb * 8);
-  DMRGX_HIP(hipMemcpyAsync(doff.ptr, off.data(), nb * 8, hipMemcpyHostToDevice, s));
-  DMRGX_HIP(hipMemcpyAsync(dlen.ptr, len.data(), nb * 8, hipMemcpyHostToDevice, s));
+  upload(doff.ptr, off.data(), nb * 8, s);
+  upload(dlen.ptr, len.data(), nb * 8, s);
   if (cplx) hipLaunchKernelGGL(k_blk_maxabs<true>, dim3(nb), dim3(256), 0, s, (const double*)A,
                                (const int64_t*)doff.ptr, (const int64_t*)dlen.ptr, (double*)dout.ptr);
   else hipLaunchKernelGGL(k_blk_maxabs<false>, dim3(nb), dim3(256), 0, s, (const double*)A,
@@ -1245,8 +1245,8 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
       dPIV(std::max<int64_t>(coff, 1) * 4), dPINV(std::max<int64_t>(coff, 1) * 4), dCB(nb * 8);
   DevBuf dqb(nb * sizeof(JBlk)), djb(nb * sizeof(JBlk)), dflag(nb * 8), dGP(std::max<int64_t>(gpoff, 1) * 8 * cw);
   DevBuf dZW(std::max<int64_t>(pbase, 1) * PJ * PJ * 8 * cw), dZF(std::max<int64_t>(pbase, 1) * 4);
-  DMRGX_HIP(hipMemcpyAsync(dqb.ptr, qb.data(), nb * sizeof(JBlk), hipMemcpyHostToDevice, s));
-  DMRGX_HIP(hipMemcpyAsync(djb.ptr, jb.data(), nb * sizeof(JBlk), hipMemcpyHostToDevice, s));
+  upload(dqb.ptr, qb.data(), nb * sizeof(JBlk), s);
+  upload(djb.ptr, jb.data(), nb * sizeof(JBlk), s);
   const JBlk* QB = (const JBlk*)dqb.ptr;
   const JBlk* JB = (const JBlk*)djb.ptr;
   if (ebase > 0) {
@@ -1261,7 +1261,7 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
   if (coff > 0) {
     std::vector<int64_t> cb(nb);
     for (int b = 0; b < nb; ++b) cb[b] = qb[b].c_off;
-    DMRGX_HIP(hipMemcpyAsync(dCB.ptr, cb.data(), nb * 8, hipMemcpyHostToDevice, s));
+    upload(dCB.ptr, cb.data(), nb * 8, s);
     unsigned g = (unsigned)((coff + 3) / 4);
     if (cplx) hipLaunchKernelGGL(k_qr_colnorm<true>, dim3(g), dim3(256), 0, s, (const double*)dW.ptr, QB,
                                  (const int64_t*)dCB.ptr, nb, coff, (double*)dCN.ptr, (int*)dPIV.ptr);
@@ -1308,7 +1308,7 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
         pinv[o + p] = j;
       }
     }
-    DMRGX_HIP(hipMemcpyAsync(dPINV.ptr, pinv.data(), coff * 4, hipMemcpyHostToDevice, s));
+    upload(dPINV.ptr, pinv.data(), coff * 4, s);
   }
   // 2. X = R^H, Qj = I
   if (xbase > 0) {
@@ -1365,7 +1365,7 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
   std::vector<double> sig(std::max<int64_t>(ncol, 1), 0.0);
   DevBuf dsig(std::max<int64_t>(ncol, 1) * 8), dcb(nb * 8), dperm(std::max<int64_t>(ncol, 1) * 4);
   if (ncol > 0) {
-    DMRGX_HIP(hipMemcpyAsync(dcb.ptr, col_base.data(), nb * 8, hipMemcpyHostToDevice, s));
+    upload(dcb.ptr, col_base.data(), nb * 8, s);
     unsigned g = (unsigned)((ncol + 3) / 4);
     if (cplx) hipLaunchKernelGGL(k_jnorms<true>, dim3(g), dim3(256), 0, s, (const double*)dX.ptr, JB, nb, ncol,
                                  (const int64_t*)dcb.ptr, (double*)dsig.ptr);
@@ -1393,7 +1393,7 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
     if (j.special == 2) sv[0] = sig[c0];  // |a| of the 1x1 block
     else if (j.special == 0)
       for (int i = 0; i < mn; ++i) sv[i] = sig[c0 + p[i]];
-    DMRGX_HIP(hipMemcpyAsync(S + blocks[b].s_off, sv.data(), mn * 8, hipMemcpyHostToDevice, s));
+    upload(S + blocks[b].s_off, sv.data(), mn * 8, s);
     FinBlk& f = fb[b];
     f.rows = j.rows;
     f.cols = j.cols;
@@ -1414,11 +1414,11 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
     fbase += (int64_t)j.rows * j.rows + (int64_t)j.cols * j.cols;
     bbase[b] = q.b_off;
   }
-  if (ncol > 0) DMRGX_HIP(hipMemcpyAsync(dperm.ptr, perm.data(), ncol * 4, hipMemcpyHostToDevice, s));
+  if (ncol > 0) upload(dperm.ptr, perm.data(), ncol * 4, s);
   // 5. B = Q_h [Qj[:, perm]; 0 | e_n ...]: reflectors applied last panel first
   if (boff > 0) {
     DevBuf dbb(nb * 8);
-    DMRGX_HIP(hipMemcpyAsync(dbb.ptr, bbase.data(), nb * 8, hipMemcpyHostToDevice, s));
+    upload(dbb.ptr, bbase.data(), nb * 8, s);
     unsigned g = (unsigned)((boff + 255) / 256);
     if (cplx) hipLaunchKernelGGL(k_binit<true>, dim3(g), dim3(256), 0, s, (const double*)dQ.ptr, (double*)dB.ptr,
                                  QB, JB, (const int64_t*)dbb.ptr, (const int64_t*)dcb.ptr, (const int*)dperm.ptr, nb,
@@ -1439,7 +1439,7 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
   }
   // 6. U, V in the reference layout
   DevBuf dF(nb * sizeof(FinBlk));
-  DMRGX_HIP(hipMemcpyAsync(dF.ptr, fb.data(), nb * sizeof(FinBlk), hipMemcpyHostToDevice, s));
+  upload(dF.ptr, fb.data(), nb * sizeof(FinBlk), s);
   if (fbase > 0) {
     unsigned g = (unsigned)((fbase + 255) / 256);
     if (cplx) hipLaunchKernelGGL(k_jfinal<true>, dim3(g), dim3(256), 0, s, (const double*)dX.ptr, (const double*)dB.ptr,
@@ -1489,9 +1489,9 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
       for (int j = v[q].good; j < v[q].k1; j += 4) items.push_back(CompItem{q, j});
     DevBuf d(v.size() * sizeof(CompTask)), di(std::max<size_t>(items.size(), 1) * sizeof(CompItem));
     DevBuf wk(std::max<int64_t>(wlen, 1) * 8 * cw), hk(std::max<int64_t>(hlen, 1) * 8);
-    DMRGX_HIP(hipMemcpyAsync(d.ptr, v.data(), v.size() * sizeof(CompTask), hipMemcpyHostToDevice, s));
+    upload(d.ptr, v.data(), v.size() * sizeof(CompTask), s);
     if (!items.empty())
-      DMRGX_HIP(hipMemcpyAsync(di.ptr, items.data(), items.size() * sizeof(CompItem), hipMemcpyHostToDevice, s));
+      upload(di.ptr, items.data(), items.size() * sizeof(CompItem), s);
     size_t shm = 3 * (size_t)std::max(maxk, 1) * sizeof(double);
     DMRGX_CHECK(shm <= 64 * 1024, "block_svd: too many kept vectors for the completion kernel");
     if (cplx) hipLaunchKernelGGL(k_complete_qr<true>, dim3((unsigned)v.size()), dim3(CNT), shm, s,

@@ -170,8 +170,8 @@ int64_t upload_tasks(const std::vector<T>& t, const std::vector<int64_t>& sizes,
     start[i] = acc;
     acc += std::max<int64_t>(1, (sizes[i] + 255) / 256);
   }
-  DMRGX_HIP(hipMemcpyAsync(dt.ptr, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice, s));
-  DMRGX_HIP(hipMemcpyAsync(ds.ptr, start.data(), start.size() * 8, hipMemcpyHostToDevice, s));
+  upload(dt.ptr, t.data(), t.size() * sizeof(T), s);
+  upload(ds.ptr, start.data(), start.size() * 8, s);
   return acc;
 }
 
@@ -186,7 +186,7 @@ void leg_contract(int dtype, const void* A, const void* B, void* C, const std::v
       ds(outs.size() * 8);
   int64_t nb = upload_tasks(outs, sizes, dt, ds, s);
   if (!pairs.empty())
-    DMRGX_HIP(hipMemcpyAsync(dp.ptr, pairs.data(), pairs.size() * sizeof(LegPair), hipMemcpyHostToDevice, s));
+    upload(dp.ptr, pairs.data(), pairs.size() * sizeof(LegPair), s);
   DMRGX_CHECK(nb < (int64_t)1 << 31, "leg_contract: grid too large");
   if (dtype == DT_C64)
     hipLaunchKernelGGL(k_leg<true>, dim3((unsigned)nb), dim3(256), 0, s, (const double*)A, (const double*)B,
@@ -241,7 +241,7 @@ std::vector<double> box_maxabs(int dtype, const void* src, const std::vector<Box
   std::vector<double> out(t.size(), 0.0);
   if (t.empty()) return out;
   DevBuf dt(t.size() * sizeof(BoxTask)), dout(t.size() * 8);
-  DMRGX_HIP(hipMemcpyAsync(dt.ptr, t.data(), t.size() * sizeof(BoxTask), hipMemcpyHostToDevice, s));
+  upload(dt.ptr, t.data(), t.size() * sizeof(BoxTask), s);
   if (dtype == DT_C64)
     hipLaunchKernelGGL(k_box_maxabs<true>, dim3((unsigned)t.size()), dim3(256), 0, s, (const double*)src,
                        (const BoxTask*)dt.ptr, (double*)dout.ptr);
@@ -288,7 +288,7 @@ void blocks_axpby(bool cplx, const void* A, const void* B, void* out, const std:
                   int64_t total, double ar, double ai, double br, double bi, hipStream_t s) {
   if (total <= 0 || segs.empty()) return;
   DevBuf d(segs.size() * sizeof(AxpbySeg));
-  DMRGX_HIP(hipMemcpyAsync(d.ptr, segs.data(), segs.size() * sizeof(AxpbySeg), hipMemcpyHostToDevice, s));
+  upload(d.ptr, segs.data(), segs.size() * sizeof(AxpbySeg), s);
   const unsigned g = (unsigned)std::min<int64_t>(4096, blocks_for(total));
   if (cplx) hipLaunchKernelGGL(k_axpby_seg<true>, dim3(g), dim3(256), 0, s, (const double*)A, (const double*)B,
                                (double*)out, (const AxpbySeg*)d.ptr, (int)segs.size(), total, ar, ai, br, bi);
