@@ -37,7 +37,7 @@ HBM_PEAK = 8000.0  # GB/s
 # FETCH_SIZE and WRITE_SIZE runs of `bench.py --only bf16x3`; each counter is
 # scaled by what it reads for a plain 256 MiB copy kernel (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0).
-TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 0.5 * (47267.0 + 82155.2) * 2.0, 0.5 * (47246.7 + 65536.0) * 1.0
+TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 0.5 * (47233.0 + 82129.8) * 2.0, 0.5 * (48217.6 + 65536.0) * 1.0
 # algorithmic bytes per launch: env blocks (5 x 1024^2 complex) and the
 # input operand as four bf16 planes (8 B per complex), output FP32 complex
 ALG_BYTES = (5 * 1024 * 1024 + 524288) * 8 + 5 * 1024 * 1024 * 8
