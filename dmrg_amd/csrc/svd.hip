@@ -963,41 +963,78 @@ __global__ __launch_bounds__(256) void k_jfinal(const double* __restrict__ X, co
 // R_jj (the Gram-Schmidt orthonormalisation of the original direction, same
 // phase), and vector j in [k0, k1) becomes the complement vector Q e_j.  The
 // reflectors live in a workspace of L*k0 elements at w_off, their squared
-// norms and the phases of R_jj in 3*k0 doubles at h_off.  k_complete_qr: one
-// workgroup of 1024 threads per task; k_complete_apply: one wavefront per
-// target vector, spread over the grid.
+// norms and the phases of R_jj in 3*k0 doubles at h_off.  The QR runs in
+// panels of CPB columns: k_cqr_panel (one workgroup of 1024 threads per task)
+// forms the panel's reflectors, k_cqr_trail applies them to the later columns
+// (one wavefront per column over the grid); k_complete_apply then builds the
+// target vectors, one wavefront each.
 struct CompTask {
   int64_t off, ks, is, w_off, h_off;
   int L, k0, k1, good;  // QR of vectors [0,k0); rewrite vectors [good,k1)
 };
-// four target vectors (one per wavefront) of one task
+// four columns or target vectors (one per wavefront) of one task
 struct CompItem {
   int task, j0;
 };
 
 constexpr int CNT = 1024;
+constexpr int CPB = CNT / 64;  // panel width: one wavefront per panel column
 
+// W = the task's kept vectors [0, k0), column-major L x k0
 template <bool CPLX>
-__global__ __launch_bounds__(CNT) void k_complete_qr(const double* __restrict__ X, double* __restrict__ Wk,
-                                                      double* __restrict__ Hk, const CompTask* __restrict__ tasks) {
+__global__ __launch_bounds__(256) void k_cqr_copy(const double* __restrict__ X, double* __restrict__ Wk,
+                                                   const CompTask* __restrict__ tasks) {
   const CompTask t = tasks[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = CNT / 64;
   constexpr int C = CPLX ? 2 : 1;
-  __shared__ double red[CNT / 64];
-  double* W = Wk + C * t.w_off;  // column-major L x k0 (vector j at W + C*j*L)
-  auto xel = [&](int k, int i) -> const double* { return X + C * (t.off + (int64_t)k * t.ks + (int64_t)i * t.is); };
-  // copy the kept vectors
-  for (int64_t e = tid; e < (int64_t)t.L * t.k0; e += CNT) {
-    int j = (int)(e / t.L), i = (int)(e % t.L);
-    const double* x = xel(j, i);
+  double* W = Wk + C * t.w_off;
+  for (int64_t e = blockIdx.y * 256 + threadIdx.x; e < (int64_t)t.L * t.k0; e += (int64_t)gridDim.y * 256) {
+    const int j = (int)(e / t.L), i = (int)(e % t.L);
+    const double* x = X + C * (t.off + (int64_t)j * t.ks + (int64_t)i * t.is);
     W[C * e] = x[0];
     if (CPLX) W[C * e + 1] = x[1];
   }
-  __syncthreads();
-  // Householder QR; reflector i stored in W[i:, i], its squared norm in
-  // vn2_all[i] and the phase of R_ii in vn2_all[k0 + 2i, k0 + 2i + 1]
-  extern __shared__ double vn2_all[];
-  for (int i = 0; i < t.k0; ++i) {
+}
+
+// y <- H_i y for the reflector v = W[i:, i] with |v|^2 = vn2, by one wavefront
+template <bool CPLX>
+__device__ __forceinline__ void reflect(const double* __restrict__ col, double* __restrict__ y, int i, int L,
+                                        double vn2, int lane) {
+  double dr = 0, di = 0;  // v^H y
+  for (int r = i + lane; r < L; r += 64) {
+    if (CPLX) {
+      dr += col[2 * r] * y[2 * r] + col[2 * r + 1] * y[2 * r + 1];
+      di += col[2 * r] * y[2 * r + 1] - col[2 * r + 1] * y[2 * r];
+    } else {
+      dr += col[r] * y[r];
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) { dr += __shfl_xor(dr, o, 64); if (CPLX) di += __shfl_xor(di, o, 64); }
+  const double fr = 2 * dr / vn2, fi = 2 * di / vn2;
+  for (int r = i + lane; r < L; r += 64) {
+    if (CPLX) {
+      y[2 * r] -= col[2 * r] * fr - col[2 * r + 1] * fi;
+      y[2 * r + 1] -= col[2 * r] * fi + col[2 * r + 1] * fr;
+    } else {
+      y[r] -= col[r] * fr;
+    }
+  }
+}
+
+// reflectors p0 .. min(p0 + CPB, k0) - 1, each applied at once to the panel's
+// later columns
+template <bool CPLX>
+__global__ __launch_bounds__(CNT) void k_cqr_panel(double* __restrict__ Wk, double* __restrict__ Hk,
+                                                    const CompTask* __restrict__ tasks, int p0) {
+  const CompTask t = tasks[blockIdx.x];
+  const int p1 = min(p0 + CPB, t.k0);
+  if (p0 >= p1) return;  // uniform over the block
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = CNT / 64;
+  constexpr int C = CPLX ? 2 : 1;
+  __shared__ double red[CNT / 64];
+  __shared__ double s_vn2;
+  double* W = Wk + C * t.w_off;
+  double* H = Hk + t.h_off;
+  for (int i = p0; i < p1; ++i) {
     double* col = W + C * (int64_t)i * t.L;
     // |x|^2 over rows i..L-1
     double a = 0;
@@ -1016,39 +1053,40 @@ __global__ __launch_bounds__(CNT) void k_complete_qr(const double* __restrict__ 
       col[C * i] = x0r + pr * nrm;
       if (CPLX) col[C * i + 1] = x0i + pi * nrm;
       // |v|^2 = 2|x|(|x| + |x0|); R_ii = beta = -e^{i arg x0}|x|
-      vn2_all[i] = 2 * nrm * (nrm + ax0);
-      vn2_all[t.k0 + 2 * i] = -pr;
-      vn2_all[t.k0 + 2 * i + 1] = -pi;
+      const double vn2 = 2 * nrm * (nrm + ax0);
+      H[i] = vn2;
+      H[t.k0 + 2 * i] = -pr;
+      H[t.k0 + 2 * i + 1] = -pi;
+      s_vn2 = vn2;
     }
     __syncthreads();
-    const double vn2 = vn2_all[i];
-    if (vn2 <= 0) continue;
-    // apply H_i to the remaining kept vectors, one wavefront per column
-    for (int c = i + 1 + wave; c < t.k0; c += nw) {
-      double* y = W + C * (int64_t)c * t.L;
-      double dr = 0, di = 0;  // v^H y
-      for (int r = i + lane; r < t.L; r += 64) {
-        if (CPLX) {
-          dr += col[2 * r] * y[2 * r] + col[2 * r + 1] * y[2 * r + 1];
-          di += col[2 * r] * y[2 * r + 1] - col[2 * r + 1] * y[2 * r];
-        } else {
-          dr += col[r] * y[r];
-        }
-      }
-      for (int o = 32; o > 0; o >>= 1) { dr += __shfl_xor(dr, o, 64); if (CPLX) di += __shfl_xor(di, o, 64); }
-      const double fr = 2 * dr / vn2, fi = 2 * di / vn2;
-      for (int r = i + lane; r < t.L; r += 64) {
-        if (CPLX) {
-          y[2 * r] -= col[2 * r] * fr - col[2 * r + 1] * fi;
-          y[2 * r + 1] -= col[2 * r] * fi + col[2 * r + 1] * fr;
-        } else {
-          y[r] -= col[r] * fr;
-        }
-      }
-    }
+    const double vn2 = s_vn2;
+    const int c = i + 1 + wave;  // at most CPB - 1 later panel columns, one wavefront each
+    if (vn2 > 0 && c < p1) reflect<CPLX>(col, W + C * (int64_t)c * t.L, i, t.L, vn2, lane);
     __syncthreads();
   }
-  for (int e = tid; e < 3 * t.k0; e += CNT) Hk[t.h_off + e] = vn2_all[e];
+}
+
+// the panel's reflectors applied to column c = j0 + wave of item blockIdx.x,
+// for the columns past the panel
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_cqr_trail(double* __restrict__ Wk, const double* __restrict__ Hk,
+                                                    const CompTask* __restrict__ tasks,
+                                                    const CompItem* __restrict__ items, int p0) {
+  const CompItem it = items[blockIdx.x];
+  const CompTask t = tasks[it.task];
+  const int lane = threadIdx.x & 63, c = it.j0 + (threadIdx.x >> 6);
+  const int p1 = min(p0 + CPB, t.k0);
+  if (c < p1 || c >= t.k0) return;
+  constexpr int C = CPLX ? 2 : 1;
+  double* W = Wk + C * t.w_off;
+  const double* H = Hk + t.h_off;
+  double* y = W + C * (int64_t)c * t.L;
+  for (int i = p0; i < p1; ++i) {
+    const double vn2 = H[i];
+    if (vn2 <= 0) continue;
+    reflect<CPLX>(W + C * (int64_t)i * t.L, y, i, t.L, vn2, lane);
+  }
 }
 
 // Q e_j = H_0 H_1 ... H_{k0-1} e_j for j = j0 + wave of item blockIdx.x
@@ -1456,7 +1494,6 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
   // by the Householder complement of the others.
   std::vector<CompTask> tu, tv;
   int64_t wu = 0, wv = 0, hu = 0, hv = 0;
-  int maxk = 0;
   for (int b = 0; b < nb; ++b) {
     const JBlk& j = jb[b];
     if (j.special) continue;
@@ -1479,33 +1516,53 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
         wv += (int64_t)j.cols * std::max(k0, 1);
         hv += 3 * (int64_t)std::max(k0, 1);
       }
-      maxk = std::max(maxk, k0);
     }
   }
   auto launch = [&](std::vector<CompTask>& v, int64_t wlen, int64_t hlen, void* Xo) {
     if (v.empty()) return;
-    std::vector<CompItem> items;
-    for (int q = 0; q < (int)v.size(); ++q)
+    std::vector<CompItem> items, cols;
+    int k0max = 0;
+    for (int q = 0; q < (int)v.size(); ++q) {
       for (int j = v[q].good; j < v[q].k1; j += 4) items.push_back(CompItem{q, j});
+      for (int c = 0; c < v[q].k0; c += 4) cols.push_back(CompItem{q, c});
+      k0max = std::max(k0max, v[q].k0);
+    }
     DevBuf d(v.size() * sizeof(CompTask)), di(std::max<size_t>(items.size(), 1) * sizeof(CompItem));
+    DevBuf dc(std::max<size_t>(cols.size(), 1) * sizeof(CompItem));
     DevBuf wk(std::max<int64_t>(wlen, 1) * 8 * cw), hk(std::max<int64_t>(hlen, 1) * 8);
     upload(d.ptr, v.data(), v.size() * sizeof(CompTask), s);
-    if (!items.empty())
-      upload(di.ptr, items.data(), items.size() * sizeof(CompItem), s);
-    size_t shm = 3 * (size_t)std::max(maxk, 1) * sizeof(double);
-    DMRGX_CHECK(shm <= 64 * 1024, "block_svd: too many kept vectors for the completion kernel");
-    if (cplx) hipLaunchKernelGGL(k_complete_qr<true>, dim3((unsigned)v.size()), dim3(CNT), shm, s,
-                                 (const double*)Xo, (double*)wk.ptr, (double*)hk.ptr, (const CompTask*)d.ptr);
-    else hipLaunchKernelGGL(k_complete_qr<false>, dim3((unsigned)v.size()), dim3(CNT), shm, s,
-                            (const double*)Xo, (double*)wk.ptr, (double*)hk.ptr, (const CompTask*)d.ptr);
-    DMRGX_HIP(hipGetLastError());
+    if (!items.empty()) upload(di.ptr, items.data(), items.size() * sizeof(CompItem), s);
+    if (!cols.empty()) upload(dc.ptr, cols.data(), cols.size() * sizeof(CompItem), s);
+    const unsigned nt = (unsigned)v.size();
+    auto* T = (const CompTask*)d.ptr;
+    if (k0max > 0) {
+      if (cplx) hipLaunchKernelGGL(k_cqr_copy<true>, dim3(nt, 64), dim3(256), 0, s, (const double*)Xo,
+                                   (double*)wk.ptr, T);
+      else hipLaunchKernelGGL(k_cqr_copy<false>, dim3(nt, 64), dim3(256), 0, s, (const double*)Xo,
+                              (double*)wk.ptr, T);
+      DMRGX_HIP(hipGetLastError());
+    }
+    for (int p0 = 0; p0 < k0max; p0 += CPB) {
+      if (cplx) hipLaunchKernelGGL(k_cqr_panel<true>, dim3(nt), dim3(CNT), 0, s, (double*)wk.ptr, (double*)hk.ptr,
+                                   T, p0);
+      else hipLaunchKernelGGL(k_cqr_panel<false>, dim3(nt), dim3(CNT), 0, s, (double*)wk.ptr, (double*)hk.ptr,
+                              T, p0);
+      DMRGX_HIP(hipGetLastError());
+      if (p0 + CPB < k0max) {
+        if (cplx) hipLaunchKernelGGL(k_cqr_trail<true>, dim3((unsigned)cols.size()), dim3(256), 0, s,
+                                     (double*)wk.ptr, (const double*)hk.ptr, T, (const CompItem*)dc.ptr, p0);
+        else hipLaunchKernelGGL(k_cqr_trail<false>, dim3((unsigned)cols.size()), dim3(256), 0, s,
+                                (double*)wk.ptr, (const double*)hk.ptr, T, (const CompItem*)dc.ptr, p0);
+        DMRGX_HIP(hipGetLastError());
+      }
+    }
     if (!items.empty()) {
       if (cplx) hipLaunchKernelGGL(k_complete_apply<true>, dim3((unsigned)items.size()), dim3(256), 0, s,
-                                   (double*)Xo, (const double*)wk.ptr, (const double*)hk.ptr,
-                                   (const CompTask*)d.ptr, (const CompItem*)di.ptr);
+                                   (double*)Xo, (const double*)wk.ptr, (const double*)hk.ptr, T,
+                                   (const CompItem*)di.ptr);
       else hipLaunchKernelGGL(k_complete_apply<false>, dim3((unsigned)items.size()), dim3(256), 0, s,
-                              (double*)Xo, (const double*)wk.ptr, (const double*)hk.ptr,
-                              (const CompTask*)d.ptr, (const CompItem*)di.ptr);
+                              (double*)Xo, (const double*)wk.ptr, (const double*)hk.ptr, T,
+                              (const CompItem*)di.ptr);
       DMRGX_HIP(hipGetLastError());
     }
     DMRGX_HIP(hipStreamSynchronize(s));
