@@ -14,7 +14,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <map>
+#include <numeric>
+#include <queue>
 
 #include "common.hpp"
 #include "gemm.hpp"
@@ -781,11 +784,75 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
   int64_t kchunk = (int64_t)std::max(8.0 * BKq, work / std::max<double>(target, 1.0));
   kchunk = (kchunk + BKq - 1) / BKq * BKq;
 
-  std::vector<GemmItem> items;
+  // Balanced mode (split_target < 0, -split_target work-groups resident at
+  // once): whole output blocks are cut into two K halves where that shortens
+  // the launch.  Work-groups go round-robin to the 8 XCDs and each XCD runs
+  // a contiguous range of items, so every XCD gets its share of whole tiles
+  // first and then its share of half tiles; a list-scheduling model of those
+  // per-XCD queues picks how many blocks (the ones with the most K steps) to
+  // cut, charging the partial tiles' extra traffic.
+  const bool balanced = split_target < 0;
+  const size_t G = groups.size();
+  std::vector<int> gnch(G, 1);
+  if (balanced && G > 0) {
+    const int slots = std::max(1, -split_target / 8);
+    std::vector<int64_t> tl(G), ks(G);
+    for (size_t q = 0; q < G; ++q) {
+      tl[q] = (int64_t)((groups[q].M + BM - 1) / BM) * ((groups[q].N + BN - 1) / BN);
+      ks[q] = (groups[q].ksum + BKq - 1) / BKq;
+    }
+    std::vector<size_t> ord(G);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return ks[a] > ks[b]; });
+    // partial elements whose write and read-back cost about one tile K step
+    const double red_unit = dtype == DT_C64 ? 6e5 : 1.2e6;
+    auto cost = [&](const std::vector<int>& nc) {
+      std::vector<int64_t> F, H;  // item lengths in K steps, in emission order
+      for (size_t q = 0; q < G; ++q) {
+        if (nc[q] == 1) F.insert(F.end(), tl[q], ks[q]);
+        else H.insert(H.end(), 2 * tl[q], (ks[q] + 1) / 2);
+      }
+      int64_t worst = 0;
+      for (int x = 0; x < 8; ++x) {
+        std::priority_queue<int64_t, std::vector<int64_t>, std::greater<int64_t>> pq;
+        for (int i = 0; i < slots; ++i) pq.push(0);
+        auto run = [&](const std::vector<int64_t>& L) {
+          for (size_t i = L.size() * x / 8; i < L.size() * (x + 1) / 8; ++i) {
+            const int64_t t0 = pq.top();
+            pq.pop();
+            pq.push(t0 + L[i] + 1);  // +1: prologue and epilogue of an item
+          }
+        };
+        run(F);
+        run(H);
+        while (pq.size() > 1) pq.pop();
+        worst = std::max(worst, pq.top());
+      }
+      double extra = 0;
+      for (size_t q = 0; q < G; ++q)
+        if (nc[q] == 2) extra += 2.0 * groups[q].M * groups[q].N;
+      return (double)worst + extra / red_unit;
+    };
+    std::vector<int> nc(G, 1), best_nc = nc;
+    double best = cost(nc);
+    const size_t step = std::max<size_t>(1, G / 64);
+    for (size_t x = step; x <= G; x += step) {
+      for (size_t q = x - step; q < x; ++q) nc[ord[q]] = ks[ord[q]] > 1 ? 2 : 1;
+      const double c = cost(nc);
+      if (c < best) {
+        best = c;
+        best_nc = nc;
+      }
+    }
+    gnch = best_nc;
+  }
+
+  std::vector<GemmItem> items, half;  // balanced mode: whole tiles, then half tiles
   std::vector<GemmSeg> segs;
   std::vector<RedTask> red;
-  for (auto& g : groups) {
-    int nch = (int)std::max<int64_t>(1, (g.ksum + kchunk - 1) / kchunk);
+  for (size_t gi = 0; gi < G; ++gi) {
+    const Group& g = groups[gi];
+    int nch = balanced ? gnch[gi] : (int)std::max<int64_t>(1, (g.ksum + kchunk - 1) / kchunk);
     int64_t ch = (g.ksum + nch - 1) / nch;
     ch = (ch + BKq - 1) / BKq * BKq;
     if (g.beta) plan->any_beta = true;
@@ -843,8 +910,27 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
           itm.seg_begin = chunk_rng[c].first;
           itm.seg_end = chunk_rng[c].second;
           itm.flags = shared ? ITEM_PARTIAL : (g.beta ? ITEM_BETA : 0);
-          items.push_back(itm);
+          (balanced && shared ? half : items).push_back(itm);
         }
+  }
+  if (balanced) {
+    // XCD x runs items [x per, (x+1) per): its eighth of the whole tiles, then
+    // its eighth of the half tiles, padded with empty items (no rows)
+    std::vector<GemmItem> whole;
+    whole.swap(items);
+    auto share = [](size_t n, int x) { return std::make_pair(n * x / 8, n * (x + 1) / 8); };
+    size_t per = 0;
+    for (int x = 0; x < 8; ++x) {
+      auto f = share(whole.size(), x), h = share(half.size(), x);
+      per = std::max(per, (f.second - f.first) + (h.second - h.first));
+    }
+    items.assign(per * 8, GemmItem{});
+    for (int x = 0; x < 8; ++x) {
+      size_t pos = per * x;
+      auto f = share(whole.size(), x), h = share(half.size(), x);
+      for (size_t i = f.first; i < f.second; ++i) items[pos++] = whole[i];
+      for (size_t i = h.first; i < h.second; ++i) items[pos++] = half[i];
+    }
   }
   for (auto& z : extra_zero) plan->zero_ranges.push_back(z);
   plan->n_items = (int)items.size();

@@ -72,6 +72,9 @@ struct GemmPlan {
 };
 
 // Build a plan; `extra_zero` lists C ranges with no task that must read as 0.
+// split_target > 0: cut K so that the launch has about that many work items;
+// 0: two per CU; < 0: balanced over -split_target resident work-groups (whole
+// output blocks cut into two K halves where that shortens the launch).
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,

@@ -232,14 +232,18 @@ void HeffPlan::autotune(hipStream_t s) {
       return;
     }
   }
-  // candidates: 2 (the default, first), 1, 4 and 8 work items per CU.  The
+  // candidates: 2 (the default, first), 1, 4 and 8 work items per CU, and for
+  // BF16x3 the balanced decomposition over the resident work-groups (one
+  // 160 KB-LDS work-group per CU for complex operands, two for real).  The
   // whole matvec (both passes with their split-K reductions and zero fills)
-  // is timed with events, two applies after one warm-up on a zero input: the
-  // R pass's candidates first with the L pass at the default, then the L
+  // is timed with events, three applies after one warm-up on a zero input:
+  // the R pass's candidates first with the L pass at the default, then the L
   // pass's with the R pass at its best.  Another candidate replaces the
-  // default only when the matvec is at least 5 % faster.
+  // default only when the matvec is at least 3 % faster.
   const int cu = std::max(rt().num_cu, 1);
-  const int cand[4] = {2 * cu, cu, 4 * cu, 8 * cu};
+  std::vector<int> cand = {2 * cu, cu, 4 * cu, 8 * cu};
+  if (prec == PREC_BF16X3) cand.push_back(-cu * (dtype == DT_C64 ? 1 : 2));
+  const int nc = (int)cand.size();
   const int c = dtype == DT_C64 ? 2 : 1;
   DevBuf x(std::max<int64_t>(n_psi, 1) * 8 * c), y(std::max<int64_t>(n_psi, 1) * 8 * c);
   fill_zero(x.ptr, n_psi * 8 * c, s);
@@ -249,24 +253,29 @@ void HeffPlan::autotune(hipStream_t s) {
     replan(rt_, lt_);
     apply((const double*)x.ptr, (double*)y.ptr, s);
     DMRGX_HIP(hipEventRecord(ev[0], s));
-    for (int rep = 0; rep < 2; ++rep) apply((const double*)x.ptr, (double*)y.ptr, s);
+    for (int rep = 0; rep < 3; ++rep) apply((const double*)x.ptr, (double*)y.ptr, s);
     DMRGX_HIP(hipEventRecord(ev[1], s));
     DMRGX_HIP(hipEventSynchronize(ev[1]));
     float ms = 0;
     DMRGX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     return (double)ms;
   };
-  int br = cand[0], bl = cand[0];
-  double best = time_apply(br, bl);
-  for (int i = 1; i < 4; ++i) {
-    const double t = time_apply(cand[i], bl);
-    if (t < 0.95 * best) { best = t; br = cand[i]; }
-  }
-  const double base_l = best;
-  for (int i = 1; i < 4; ++i) {
-    const double t = time_apply(br, cand[i]);
-    if (t < 0.95 * best && t < base_l) { best = t; bl = cand[i]; }
-  }
+  // the fastest candidate that beats the default by the margin, else the default
+  auto pick = [&](auto&& time_of) {
+    const double t0 = time_of(cand[0]);
+    double bt = t0;
+    int b = cand[0];
+    for (int i = 1; i < nc; ++i) {
+      const double t = time_of(cand[i]);
+      if (t < 0.97 * t0 && t < bt) {
+        bt = t;
+        b = cand[i];
+      }
+    }
+    return b;
+  };
+  const int br = pick([&](int c) { return time_apply(c, cand[0]); });
+  const int bl = pick([&](int c) { return time_apply(br, c); });
   for (auto& e : ev) (void)hipEventDestroy(e);
   replan(br, bl);
   DMRGX_HIP(hipStreamSynchronize(s));

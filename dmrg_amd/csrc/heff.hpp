@@ -33,7 +33,8 @@ struct HeffPlan {
   std::vector<std::pair<int64_t, int64_t>> rzero, lzero;
   int r_split = 0, l_split = 0;
   // rebuild the two pass plans with the given split-K targets (work items
-  // per launch; 0: two per CU)
+  // per launch; 0: two per CU; negative: balanced over that many resident
+  // work-groups)
   void replan(int r_target, int l_target);
   // rebuild the pass plans over the tasks with keep_r[i] / keep_l[i] set
   // (outputs of the dropped tasks read as zero); empty masks: all tasks

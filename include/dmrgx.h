@@ -128,9 +128,11 @@ int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int 
  * split-K depth and load balance.  The first plan with a given sector-shape
  * histogram (task shapes with M, N, K rounded to powers of two, counts too,
  * per pass, with precision and element type) of at least 1e9 flop times 1, 2,
- * 4 and 8 items per CU with HIP events and keeps the fastest for every later
- * plan with that histogram in the process (the default of 2 per CU is kept
- * unless another is 5 % faster).  Choices change the summation order only.
+ * 4 and 8 items per CU, and for BF16x3 a decomposition balanced over the
+ * resident work-groups (whole output blocks cut into two K halves, items
+ * ordered per XCD), with HIP events and keeps the fastest for every later plan
+ * with that histogram in the process (the default of 2 per CU is kept unless
+ * another is 3 % faster).  Choices change the summation order only.
  * dmrgx_autotune_keys: histograms timed so far.  No counterpart in the
  * reference. */
 int dmrgx_set_autotune(int on);
