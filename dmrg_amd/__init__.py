@@ -28,7 +28,7 @@ EXPORTS = [
     "dmrgx_free", "dmrgx_memcpy_h2d", "dmrgx_memcpy_d2h", "dmrgx_memset_zero", "dmrgx_synchronize",
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
     "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_eigsh_batched", "dmrgx_set_lanczos_graphs", "dmrgx_set_autotune", "dmrgx_autotune_keys", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
-    "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time",
+    "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time", "dmrgx_heff_set_split", "dmrgx_heff_get_split",
 ]
 
 _lib = None
@@ -89,6 +89,8 @@ def _declare(lib):
         "dmrgx_set_lanczos_graphs": ([C.c_int], C.c_int),
         "dmrgx_set_autotune": ([C.c_int], C.c_int),
         "dmrgx_autotune_keys": ([], C.c_int),
+        "dmrgx_heff_set_split": ([vp, C.c_int, C.c_int], C.c_int),
+        "dmrgx_heff_get_split": ([vp, C.POINTER(C.c_int)], C.c_int),
         "dmrgx_block_svd": ([C.c_int, C.c_int, i32p, i32p, vp, i64p, vp, i64p, vp, i64p, vp, i64p], C.c_int),
         "dmrgx_block_svd_sweeps": ([], C.c_int),
         "dmrgx_block_gemm": ([C.c_int, C.c_char, C.c_char, vp, vp, vp, i64p, C.c_int], C.c_int),

@@ -633,6 +633,24 @@ int dmrgx_set_autotune(int on) {
 
 int dmrgx_autotune_keys(void) { return heff_autotune_keys(); }
 
+int dmrgx_heff_set_split(dmrgx_heff* plan, int r_target, int l_target) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p, "heff_set_split: null plan");
+  plan->p->replan(r_target, l_target);
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  CAPI_END
+}
+
+int dmrgx_heff_get_split(const dmrgx_heff* plan, int* targets) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p && targets, "heff_get_split: null argument");
+  targets[0] = plan->p->r_split;
+  targets[1] = plan->p->l_split;
+  targets[2] = plan->p->rplan->n_items;
+  targets[3] = plan->p->lplan->n_items;
+  CAPI_END
+}
+
 int dmrgx_set_lanczos_graphs(int on) {
   set_lanczos_graphs(on != 0);
   return 0;

@@ -334,6 +334,18 @@ class SynthHeff:
         from . import check, lib
         check(lib().dmrgx_heff_apply(self.h, x.ptr, y.ptr), "dmrgx_heff_apply")
 
+    def set_split(self, r_target, l_target):
+        """Pin the work decomposition of both passes (dmrgx_heff_set_split)."""
+        from . import check, lib
+        check(lib().dmrgx_heff_set_split(self.h, int(r_target), int(l_target)), "dmrgx_heff_set_split")
+
+    def split(self):
+        """(R target, L target, R work items, L work items)"""
+        from . import check, lib
+        out = (C.c_int * 4)()
+        check(lib().dmrgx_heff_get_split(self.h, out), "dmrgx_heff_get_split")
+        return tuple(out)
+
     def sizes(self):
         """(n_psi, n_store, renv length, lenv length)"""
         from . import check, lib

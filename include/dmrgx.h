@@ -137,6 +137,14 @@ int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int 
  * reference. */
 int dmrgx_set_autotune(int on);
 int dmrgx_autotune_keys(void);
+/* Pin the work decomposition of a plan's two passes (no counterpart in the
+ * reference): target > 0 cuts K so that the pass has about that many work
+ * items, 0 is the default (two per CU), < 0 balances the pass over -target
+ * resident work-groups (whole output blocks cut into two K halves, items
+ * ordered per XCD).  dmrgx_heff_get_split writes {R target, L target, R work
+ * items, L work items}.  Results change in summation order only. */
+int dmrgx_heff_set_split(dmrgx_heff* plan, int r_target, int l_target);
+int dmrgx_heff_get_split(const dmrgx_heff* plan, int* targets);
 
 /* dmrgx_eigsh_batched (and the batched sweeps) run each Lanczos step -- the
  * matvec and both Gram-Schmidt passes, then the normalisation of the next
