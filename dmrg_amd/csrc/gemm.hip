@@ -18,6 +18,7 @@
 #include <map>
 #include <numeric>
 #include <queue>
+#include <type_traits>
 
 #include "common.hpp"
 #include "gemm.hpp"
@@ -73,13 +74,24 @@ __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* _
   const RedTask r = t[lo];
   const int64_t e = (b - start[lo]) * 256 + threadIdx.x;
   if (e >= r.len) return;
-  constexpr int c = CPLX ? 2 : 1;
-#pragma unroll
-  for (int q = 0; q < c; ++q) {
+  // the parts in chunk order; complex elements read as one vector
+  if (CPLX) {
+    using V2 = typename std::conditional<sizeof(S) == 8, double2, float2>::type;
+    const V2* W2 = reinterpret_cast<const V2*>(W);
+    S ar = 0, ai = 0;
+    for (int k = 0; k < r.nch; ++k) {
+      const V2 v = W2[r.w_off + k * r.len + e];
+      ar += v.x;
+      ai += v.y;
+    }
+    S* o = C + 2 * (r.c_off + e);
+    if (r.beta) { o[0] += ar; o[1] += ai; }
+    else { o[0] = ar; o[1] = ai; }
+  } else {
     S acc = 0;
-    for (int k = 0; k < r.nch; ++k) acc += W[c * (r.w_off + k * r.len + e) + q];
-    if (r.beta) C[c * (r.c_off + e) + q] += acc;
-    else C[c * (r.c_off + e) + q] = acc;
+    for (int k = 0; k < r.nch; ++k) acc += W[r.w_off + k * r.len + e];
+    if (r.beta) C[r.c_off + e] += acc;
+    else C[r.c_off + e] = acc;
   }
 }
 
@@ -433,16 +445,31 @@ __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__
   const int64_t e = (b - start[lo]) * 256 + threadIdx.x;
   if (e >= r.len) return;
   constexpr int c = CPLX ? 2 : 1;
+  // the parts in chunk order; complex elements read as one float2
+  float acc[c];
+  if (CPLX) {
+    const float2* W2 = reinterpret_cast<const float2*>(W);
+    float ar = 0, ai = 0;
+    for (int k = 0; k < r.nch; ++k) {
+      const float2 v = W2[r.w_off + k * r.len + e];
+      ar += v.x;
+      ai += v.y;
+    }
+    acc[0] = ar;
+    acc[c - 1] = ai;
+  } else {
+    float a = 0;
+    for (int k = 0; k < r.nch; ++k) a += W[r.w_off + k * r.len + e];
+    acc[0] = a;
+  }
 #pragma unroll
   for (int q = 0; q < c; ++q) {
-    float acc = 0;
-    for (int k = 0; k < r.nch; ++k) acc += W[c * (r.w_off + k * r.len + e) + q];
     if (OUT == OUT_F64) {
       double* C = (double*)Cv;
-      if (r.beta) C[c * (r.c_off + e) + q] += acc;
-      else C[c * (r.c_off + e) + q] = acc;
+      if (r.beta) C[c * (r.c_off + e) + q] += acc[q];
+      else C[c * (r.c_off + e) + q] = acc[q];
     } else {
-      store_split((unsigned short*)Cv, cps, r.c_off + e, q, acc);
+      store_split((unsigned short*)Cv, cps, r.c_off + e, q, acc[q]);
     }
   }
 }
