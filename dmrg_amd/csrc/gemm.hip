@@ -670,12 +670,16 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
       }
 }
 
+// Eight consecutive padded elements per thread: every block starts on a
+// multiple of 8 and Kp is one, so the eight lie in one row of one block; the
+// hi / lo parts of each go out as one 16-byte store per plane.
 template <typename SRC>
 __global__ void k_split_padded(const SRC* __restrict__ src, unsigned short* __restrict__ dst, int64_t ps,
                                const PadBlk* __restrict__ blk, int nb, int64_t total, int cplx) {
-  int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; e < total; e += st) {
+  for (; g < total / 8; g += st) {
+    const int64_t e = g * 8;
     int lo = 0, hi = nb - 1;
     while (lo < hi) {
       int mid = (lo + hi + 1) >> 1;
@@ -684,15 +688,26 @@ __global__ void k_split_padded(const SRC* __restrict__ src, unsigned short* __re
     }
     const PadBlk b = blk[lo];
     const int64_t loc = e - b.dst_off;
-    const int row = (int)(loc / b.Kp), col = (int)(loc % b.Kp);
-    const bool ok = row < b.rows && col < b.K;
-    const int64_t si = b.src_off + (int64_t)row * b.K + col;
+    const int64_t row = loc / b.Kp;
+    const int col0 = (int)(loc - row * b.Kp);
+    const bool rok = row < b.rows;
+    const int64_t si = b.src_off + row * b.K + col0;
     for (int part = 0; part <= cplx; ++part) {
-      const double x = ok ? (double)src[cplx ? 2 * si + part : si] : 0.0;
-      const unsigned short h = f2bf_rne((float)x);
-      const unsigned short l = f2bf_rne((float)(x - (double)bf2f(h)));
-      dst[(2 * part) * ps + e] = h;
-      dst[(2 * part + 1) * ps + e] = l;
+      unsigned short h[8], l[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool ok = rok && col0 + j < b.K;
+        const double x = ok ? (double)src[cplx ? 2 * (si + j) + part : si + j] : 0.0;
+        h[j] = f2bf_rne((float)x);
+        l[j] = f2bf_rne((float)(x - (double)bf2f(h[j])));
+      }
+      uint4 vh, vl;
+      vh.x = h[0] | ((unsigned)h[1] << 16); vh.y = h[2] | ((unsigned)h[3] << 16);
+      vh.z = h[4] | ((unsigned)h[5] << 16); vh.w = h[6] | ((unsigned)h[7] << 16);
+      vl.x = l[0] | ((unsigned)l[1] << 16); vl.y = l[2] | ((unsigned)l[3] << 16);
+      vl.z = l[4] | ((unsigned)l[5] << 16); vl.w = l[6] | ((unsigned)l[7] << 16);
+      *reinterpret_cast<uint4*>(dst + (2 * part) * ps + e) = vh;
+      *reinterpret_cast<uint4*>(dst + (2 * part + 1) * ps + e) = vl;
     }
   }
 }
@@ -1108,7 +1123,8 @@ template <typename SRC>
 static void split_padded(const SRC* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s) {
   if (L.total <= 0) return;
   DMRGX_CHECK(ps >= L.total, "split_padded: plane stride below the padded length");
-  const int64_t blocks = std::min<int64_t>((L.total + 255) / 256, 8192);
+  DMRGX_CHECK(ps % 8 == 0 && L.total % 8 == 0, "split_padded: plane stride and length must be multiples of 8");
+  const int64_t blocks = std::min<int64_t>((L.total / 8 + 255) / 256, 8192);
   hipLaunchKernelGGL(k_split_padded<SRC>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, ps,
                      (const PadBlk*)L.d_blk->ptr, (int)L.blk.size(), L.total, cplx ? 1 : 0);
   DMRGX_HIP(hipGetLastError());
