@@ -875,11 +875,15 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
         if (nc[q] == 2) extra += 2.0 * groups[q].M * groups[q].N;
       return (double)worst + extra / red_unit;
     };
+    // the model runs at every plan build: cut 0, 1/8, ..., 8/8 of the blocks
     std::vector<int> nc(G, 1), best_nc = nc;
     double best = cost(nc);
-    const size_t step = std::max<size_t>(1, G / 64);
-    for (size_t x = step; x <= G; x += step) {
-      for (size_t q = x - step; q < x; ++q) nc[ord[q]] = ks[ord[q]] > 1 ? 2 : 1;
+    size_t done = 0;
+    for (int f = 1; f <= 8; ++f) {
+      const size_t x = (G * f + 7) / 8;
+      if (x <= done) continue;
+      for (size_t q = done; q < x; ++q) nc[ord[q]] = ks[ord[q]] > 1 ? 2 : 1;
+      done = x;
       const double c = cost(nc);
       if (c < best) {
         best = c;

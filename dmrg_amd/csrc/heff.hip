@@ -236,7 +236,7 @@ void HeffPlan::autotune(hipStream_t s) {
   // BF16x3 the balanced decomposition over the resident work-groups (one
   // 160 KB-LDS work-group per CU for complex operands, two for real).  The
   // whole matvec (both passes with their split-K reductions and zero fills)
-  // is timed with events, three applies after one warm-up on a zero input:
+  // is timed with events, two applies after one warm-up on a zero input:
   // the R pass's candidates first with the L pass at the default, then the L
   // pass's with the R pass at its best.  Another candidate replaces the
   // default only when the matvec is at least 3 % faster.
@@ -249,33 +249,47 @@ void HeffPlan::autotune(hipStream_t s) {
   fill_zero(x.ptr, n_psi * 8 * c, s);
   hipEvent_t ev[2];
   for (auto& e : ev) DMRGX_HIP(hipEventCreate(&e));
-  auto time_apply = [&](int rt_, int lt_) {
-    replan(rt_, lt_);
+  // the matvec with the current pass plans: one warm-up, then two timed applies
+  auto time_now = [&]() {
     apply((const double*)x.ptr, (double*)y.ptr, s);
     DMRGX_HIP(hipEventRecord(ev[0], s));
-    for (int rep = 0; rep < 3; ++rep) apply((const double*)x.ptr, (double*)y.ptr, s);
+    for (int rep = 0; rep < 2; ++rep) apply((const double*)x.ptr, (double*)y.ptr, s);
     DMRGX_HIP(hipEventRecord(ev[1], s));
     DMRGX_HIP(hipEventSynchronize(ev[1]));
     float ms = 0;
     DMRGX_HIP(hipEventElapsedTime(&ms, ev[0], ev[1]));
     return (double)ms;
   };
-  // the fastest candidate that beats the default by the margin, else the default
-  auto pick = [&](auto&& time_of) {
-    const double t0 = time_of(cand[0]);
-    double bt = t0;
+  // the fastest candidate that beats the default by the margin, else the
+  // default; a candidate whose plan repeats one already timed (the same work
+  // items, segments and reductions) reuses that time
+  auto pick = [&](bool r_pass, int other) {
+    std::vector<std::pair<std::array<int64_t, 4>, double>> seen;
+    double t0 = 0, bt = 0;
     int b = cand[0];
-    for (int i = 1; i < nc; ++i) {
-      const double t = time_of(cand[i]);
-      if (t < 0.97 * t0 && t < bt) {
+    for (int i = 0; i < nc; ++i) {
+      if (r_pass) replan(cand[i], other);
+      else replan(other, cand[i]);
+      const GemmPlan& g = r_pass ? *rplan : *lplan;
+      const std::array<int64_t, 4> sig = {g.n_items, g.n_segs, g.n_red, g.ws_elems};
+      double t = -1;
+      for (auto& e : seen)
+        if (e.first == sig) t = e.second;
+      if (t < 0) {
+        t = time_now();
+        seen.push_back({sig, t});
+      }
+      if (i == 0) {
+        t0 = bt = t;
+      } else if (t < 0.97 * t0 && t < bt) {
         bt = t;
         b = cand[i];
       }
     }
     return b;
   };
-  const int br = pick([&](int c) { return time_apply(c, cand[0]); });
-  const int bl = pick([&](int c) { return time_apply(br, c); });
+  const int br = pick(true, cand[0]);
+  const int bl = pick(false, br);
   for (auto& e : ev) (void)hipEventDestroy(e);
   replan(br, bl);
   DMRGX_HIP(hipStreamSynchronize(s));
