@@ -32,12 +32,29 @@ METRIC = "DMRG sweep s + H_eff matvec TFLOP/s, mixed vs FP64 GPU at |ΔE0|<1e-8"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # MI355X dense matrix peaks, TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
 
-# HBM bytes per launch of k_gemm_bf3p<complex> on this workload (mean of the
-# R- and L-pass launches of the autotuned plans), from rocprofv3 --pmc
-# FETCH_SIZE and WRITE_SIZE runs of `bench.py --only bf16x3`; each counter is
-# scaled by what it reads for a plain 256 MiB copy kernel (FETCH_SIZE x 2.0,
-# WRITE_SIZE x 1.0).
-TRAFFIC_FETCH_KIB, TRAFFIC_WRITE_KIB = 0.5 * (47233.0 + 82129.8) * 2.0, 0.5 * (48217.6 + 65536.0) * 1.0
+# HBM bytes per launch of k_gemm_bf3p<complex> on this workload, per work
+# decomposition the autotuner can pick (pass, work items) -> (FETCH_SIZE KiB,
+# WRITE_SIZE KiB per dispatch), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+# runs of `bench.py --only bf16x3` (each candidate is dispatched while the
+# plan is tuned); each counter is scaled by what it reads for a plain 256 MiB
+# copy kernel (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  The roofline's traffic is
+# the mean over the R and L launches of the chosen plan, None if the chosen
+# decomposition was not measured.
+TRAFFIC_KIB = {
+    ("R", 320): (47237.0, 47765.3), ("R", 400): (52062.1, 55940.7), ("R", 640): (61586.8, 81920.0),
+    ("L", 64): (82023.7, 8192.0), ("L", 256): (82041.3, 32768.0), ("L", 512): (82158.1, 65536.0),
+    ("L", 640): (82051.6, 81920.0),
+}
+FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
+
+
+def traffic_bytes(r_items, l_items):
+    r, l = TRAFFIC_KIB.get(("R", r_items)), TRAFFIC_KIB.get(("L", l_items))
+    if r is None or l is None:
+        return None
+    return round(0.5 * sum(f * FETCH_SCALE + w * WRITE_SCALE for f, w in (r, l)) * 1024)
+
+
 # algorithmic bytes per launch: env blocks (5 x 1024^2 complex) and the
 # input operand as four bf16 planes (8 B per complex), output FP32 complex
 ALG_BYTES = (5 * 1024 * 1024 + 524288) * 8 + 5 * 1024 * 1024 * 8
@@ -66,8 +83,9 @@ def gpu_leg(steps, warmup, only=None):
             h.time(xd, yd, warmup)
         ms = h.time(xd, yd, steps)
         flops = h.flops
+        _, _, r_items, l_items = h.split()
         out[prec] = dict(ms=ms[0], ms_r=ms[1], ms_l=ms[2], flops=flops, flops_r=ms[3], flops_l=ms[4],
-                         tflops=flops / (ms[0] * 1e-3) / 1e12, n_psi=h.n_psi)
+                         tflops=flops / (ms[0] * 1e-3) / 1e12, n_psi=h.n_psi, r_items=r_items, l_items=l_items)
         print(f"[bench] {prec}: {ms[0]:.3f} ms/matvec, {out[prec]['tflops']:.1f} TFLOP/s "
               f"(R {ms[1]:.3f} ms, L {ms[2]:.3f} ms)", file=sys.stderr, flush=True)
         h.close()
@@ -341,7 +359,8 @@ def main():
         "fp64_roofline_frac": round(fp64["flops"] / (fp64["ms"] * 1e-3) / 1e12 / PEAK["fp64"], 4),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["bf16x3"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["bf16x3"], 5),
-                     "traffic": round((TRAFFIC_FETCH_KIB + TRAFFIC_WRITE_KIB) * 1024),
+                     "traffic": traffic_bytes(mixed["r_items"], mixed["l_items"]),
+                     "work_items": {"R": mixed["r_items"], "L": mixed["l_items"]},
                      "algorithmic_bytes": ALG_BYTES,
                      "kernel": "k_gemm_bf3p<complex> (R and L pass launches)",
                      "issued_tflops": round(3 * achieved, 3)},
