@@ -712,23 +712,6 @@ __global__ void k_split_padded(const SRC* __restrict__ src, unsigned short* __re
   }
 }
 
-template <typename SRC>
-__global__ void k_split_planes(const SRC* __restrict__ src, unsigned short* __restrict__ dst, int64_t n,
-                               int64_t stride, int cplx) {
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; i < n; i += st) {
-    for (int part = 0; part <= cplx; ++part) {
-      double x = (double)src[cplx ? 2 * i + part : i];
-      float xf = (float)x;
-      unsigned short h = f2bf_rne(xf);
-      unsigned short l = f2bf_rne((float)(x - (double)bf2f(h)));
-      dst[(2 * part) * stride + i] = h;
-      dst[(2 * part + 1) * stride + i] = l;
-    }
-  }
-}
-
 __global__ void k_f64_to_f32(const double* __restrict__ s, float* __restrict__ d, int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -1140,22 +1123,6 @@ void split_padded_f64(const double* src, uint16_t* dst, int64_t ps, const PadLay
 
 void split_padded_f32(const float* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s) {
   split_padded(src, dst, ps, L, cplx, s);
-}
-
-void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s) {
-  if (n <= 0) return;
-  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_split_planes<double>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, n,
-                     stride, cplx ? 1 : 0);
-  DMRGX_HIP(hipGetLastError());
-}
-
-void split_planes_f32(const float* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s) {
-  if (n <= 0) return;
-  int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
-  hipLaunchKernelGGL(k_split_planes<float>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, n,
-                     stride, cplx ? 1 : 0);
-  DMRGX_HIP(hipGetLastError());
 }
 
 void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, hipEvent_t ev_begin,

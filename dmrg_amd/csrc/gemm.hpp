@@ -93,9 +93,7 @@ void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hip
 void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
                      void* C, int out, int64_t c_ps, hipStream_t s, hipEvent_t ev_begin = nullptr,
                      hipEvent_t ev_end = nullptr);
-// Split n elements (complex if cplx) into hi/lo bfloat16 planes of stride `stride`.
-void split_planes_f64(const double* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);
-void split_planes_f32(const float* src, uint16_t* dst, int64_t n, int64_t stride, bool cplx, hipStream_t s);
+// Stride of a bf16 plane of n elements (a multiple of 8: 16-byte rows).
 inline int64_t plane_stride(int64_t n) { return (n + 7) / 8 * 8; }
 
 // Padded plane layout of an operand: its blocks (rows x K, row-major at

@@ -70,16 +70,6 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
 // identical either way).
 bool lanczos_graphs();
 void set_lanczos_graphs(bool on);
-// Rayleigh-Ritz polish of the lowest eigenpair of every chain in FP64 over
-// span{X} + K_k(H, x_0) (X: the p <= 8 lowest approximate eigenvectors of each
-// chain, p columns of the concatenated length, as eigsh writes them; the
-// Krylov vectors H x_0 .. H^k x_0 extend it): X resolves nearly degenerate
-// lowest pairs that a single-vector Krylov polish cannot.  Cycles
-// repeat from the new Ritz vectors until every chain's residual |H x - theta
-// x| <= tol max(|theta|, eps^(2/3)) or max_cycles ran.  X is overwritten
-// (column 0 = the polished lowest vector, normalised); returns the Ritz values.
-std::vector<double> subspace_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int p,
-                                            int k, double* X, double tol, int max_cycles, int* n_matvec);
 // Jacobi-Davidson refinement of the lowest eigenpair of every chain: the FP64
 // Rayleigh-Ritz basis starts as the p approximate eigenvectors X (p columns
 // of the concatenated length, as eigsh writes them) and grows by one vector

@@ -783,115 +783,6 @@ std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<in
 
 namespace dmrgx {
 
-std::vector<double> subspace_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int p,
-                                            int k, double* X, double tol, int max_cycles, int* n_matvec) {
-  ensure_init();
-  hipStream_t s = rt().stream;
-  const int nc = (int)len.size();
-  DMRGX_CHECK(nc > 0, "subspace_polish: no chains");
-  const std::vector<int64_t> off = offsets(len);
-  const int64_t N = off.back() + len.back();
-  const int64_t nr = N * (cplx ? 2 : 1);
-  DMRGX_CHECK(p >= 1 && p <= G && k >= 0, "subspace_polish: 1 <= p <= 8 and k >= 0");
-  const int q = p + k;  // basis: X (p vectors) and the Krylov vectors H x_0, H^2 x_0, ... (k)
-  std::vector<int> all(nc);
-  for (int c = 0; c < nc; ++c) all[c] = c;
-  Segs sg(off, len, all, s);
-  SegOps ops(sg, cplx, nr, q + 1, s, q);
-  const int hs = ops.hs;
-  DevBuf Bb((size_t)q * nr * 8), HBb((size_t)q * nr * 8), tb(nr * 8), gbuf((size_t)q * nc * hs * 8),
-      scl((size_t)nc * 8), Zb((size_t)nc * q * p * 16);
-  double* Bv = (double*)Bb.ptr;
-  double* HB = (double*)HBb.ptr;
-  double* t = (double*)tb.ptr;
-  double* h = (double*)gbuf.ptr;
-  double* h2 = h + (size_t)nc * hs;
-  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
-  std::vector<double> th0(nc, 0.0), sc(nc);
-  for (int cycle = 0; cycle < std::max(1, max_cycles); ++cycle) {
-    // orthonormal basis B (CGS2 per chain; a vector dependent on the earlier
-    // ones becomes zero and is left out of the projection) and HB = H B
-    std::vector<std::vector<char>> valid(nc, std::vector<char>(q, 0));
-    for (int j = 0; j < q; ++j) {
-      const double* src = j < p ? X + (size_t)j * nr : HB + (size_t)(j == p ? 0 : j - 1) * nr;
-      ops.vdots(Bv, j, src, h);
-      ops.sub(Bv, j, h, src, t, nullptr);
-      ops.vdots(Bv, j, t, h2);
-      const double* hh = ops.fetch(h, 2);
-      for (int c = 0; c < nc; ++c) {
-        const double* a = hh + (size_t)c * hs;
-        const double* b = hh + (size_t)(nc + c) * hs;
-        double hsq = 0;
-        for (int i = 0; i < j; ++i) hsq += b[2 * i] * b[2 * i] + b[2 * i + 1] * b[2 * i + 1];
-        const double w2 = a[2 * j], b2 = b[2 * j] - hsq;
-        const bool ok = w2 > 0 && b2 > 1e-20 * w2 && j < len[c];
-        valid[c][j] = ok;
-        sc[c] = ok ? 1.0 / std::sqrt(b2) : 0.0;
-      }
-      ops.put(sc, (double*)scl.ptr);
-      ops.sub(Bv, j, h2, t, Bv + (size_t)j * nr, (const double*)scl.ptr);
-      op(Bv + (size_t)j * nr, HB + (size_t)j * nr);
-      if (n_matvec) (*n_matvec)++;
-      DMRGX_HIP(hipStreamSynchronize(s));
-    }
-    // G = B^H H B per chain, one column per launch pair
-    for (int j = 0; j < q; ++j) ops.vdots(Bv, q, HB + (size_t)j * nr, h + (size_t)j * nc * hs);
-    const double* gg = ops.fetch(h, q);
-    std::vector<double> Z((size_t)nc * q * p * 2, 0.0);
-    parallel_for(nc, [&](int c) {
-      std::vector<int> idx;
-      for (int j = 0; j < q; ++j)
-        if (valid[c][j]) idx.push_back(j);
-      const int r = (int)idx.size();
-      DMRGX_CHECK(r > 0, "subspace_polish: empty basis");
-      std::vector<dmrgx::cplx> A((size_t)r * r), W;
-      for (int a = 0; a < r; ++a)
-        for (int b = 0; b < r; ++b) {
-          // column idx[b] of G, row idx[a]: <B_a, H B_b>
-          const double* col = gg + ((size_t)idx[b] * nc + c) * hs;
-          A[(size_t)a * r + b] = dmrgx::cplx(col[2 * idx[a]], cplx ? col[2 * idx[a] + 1] : 0.0);
-        }
-      std::vector<double> w;
-      host_hermitian_eig(r, A, !cplx, w, W);
-      th0[c] = w[0];
-      for (int col = 0; col < std::min(p, r); ++col)
-        for (int a = 0; a < r; ++a) {
-          double* z = Z.data() + (((size_t)c * q + idx[a]) * p + col) * 2;
-          z[0] = W[(size_t)a * r + col].real();
-          z[1] = W[(size_t)a * r + col].imag();
-        }
-    });
-    // new X = B Z (p Ritz vectors) and the residual of the lowest, H x - theta x
-    DMRGX_HIP(hipMemcpyAsync(Zb.ptr, Z.data(), Z.size() * 8, hipMemcpyHostToDevice, s));
-    ops.combine_z(Bv, q, (const double*)Zb.ptr, p, X);
-    std::vector<double> z0((size_t)nc * q * 2);
-    for (int c = 0; c < nc; ++c)
-      for (int j = 0; j < q; ++j) {
-        z0[((size_t)c * q + j) * 2] = Z[(((size_t)c * q + j) * p) * 2];
-        z0[((size_t)c * q + j) * 2 + 1] = Z[(((size_t)c * q + j) * p) * 2 + 1];
-      }
-    DMRGX_HIP(hipMemcpyAsync(Zb.ptr, z0.data(), z0.size() * 8, hipMemcpyHostToDevice, s));
-    ops.combine_z(HB, q, (const double*)Zb.ptr, 1, t);
-    std::vector<double> th(nc * hs, 0.0);
-    for (int c = 0; c < nc; ++c) th[(size_t)c * hs] = th0[c];
-    DMRGX_HIP(hipMemcpyAsync(h, th.data(), th.size() * 8, hipMemcpyHostToDevice, s));
-    ops.sub(X, 1, h, t, t, nullptr);
-    ops.vdots(nullptr, 0, t, h2);
-    const double* rr = ops.fetch(h2, 1);
-    bool done = true;
-    for (int c = 0; c < nc; ++c)
-      done = done && std::sqrt(std::max(rr[(size_t)c * hs], 0.0)) <= tol * std::max(std::fabs(th0[c]), eps23);
-    ops.normalise(X, h, (double*)scl.ptr);
-    if (done) break;
-  }
-  DMRGX_HIP(hipStreamSynchronize(s));
-  return th0;
-}
-
-}  // namespace dmrgx
-
-namespace dmrgx {
-
 std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, const std::vector<int64_t>& len,
                                       bool cplx, int p, double* X, double tol, int max_outer, int inner,
                                       int* n64, int* nlo) {
