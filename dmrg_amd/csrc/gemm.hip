@@ -972,7 +972,6 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     upload(plan->d_items->ptr, items.data(), items.size() * sizeof(GemmItem), rt().stream);
     if (!segs.empty())
       upload(plan->d_segs->ptr, segs.data(), segs.size() * sizeof(GemmSeg), rt().stream);
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   if (!red.empty()) {
     const size_t eb = (prec == PREC_FP64 ? 8 : 4) * (dtype == DT_C64 ? 2 : 1);
@@ -989,7 +988,6 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     plan->d_red_start = new DevBuf(red.size() * 8);
     upload(plan->d_red->ptr, red.data(), red.size() * sizeof(RedTask), rt().stream);
     upload(plan->d_red_start->ptr, start.data(), start.size() * 8, rt().stream);
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   return plan;
 }
@@ -1101,7 +1099,6 @@ std::unique_ptr<PadLayout> make_pad_layout(std::vector<PadBlk> blocks) {
   if (!L->blk.empty()) {
     L->d_blk = std::make_unique<DevBuf>(L->blk.size() * sizeof(PadBlk));
     upload(L->d_blk->ptr, L->blk.data(), L->blk.size() * sizeof(PadBlk), rt().stream);
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   return L;
 }

@@ -181,7 +181,6 @@ void qtensor<T>::update(const tensor<T>& val, int i0, int i1, int i2, int i3, in
   old.insert(old.end(), val.val_.begin(), val.val_.end());
   allocate();
   upload(data(), old.data(), old.size() * sizeof(T), rt().stream);
-  DMRGX_HIP(hipStreamSynchronize(rt().stream));
 }
 
 template <typename T>
@@ -220,7 +219,6 @@ qtensor<cplx> qtensor<T>::comp() const {
     vector<T> h = values();
     vector<cplx> hc(h.begin(), h.end());
     upload(r.data(), hc.data(), hc.size() * sizeof(cplx), rt().stream);
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   return r;
 }
@@ -495,7 +493,6 @@ qtensor<T> qtensor<T>::remove_symmetry() const {
   }
   r.allocate();
   upload(r.data(), out.data(), out.size() * sizeof(T), rt().stream);
-  DMRGX_HIP(hipStreamSynchronize(rt().stream));
   return r;
 }
 
@@ -572,7 +569,6 @@ qtensor<T>& qtensor<T>::contract(const qtensor<T>& A, const qtensor<T>& B, char 
   if (!tasks.empty()) {
     auto plan = make_gemm_plan(tasks, dtype_of<T>::value, PREC_FP64, an ? 'N' : 'T', bn ? 'N' : 'T', {});
     run_gemm_plan(*plan, A.data(), B.data(), r.data(), rt().stream);
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   *this = r;
   return *this;
@@ -689,7 +685,6 @@ void qtensor<T>::contract(T* out, const T* in, const vector<vector<int>>& map, c
   }
   auto plan = make_gemm_plan(t, dtype_of<T>::value, PREC_FP64, 'N', 'T', {});
   run_gemm_plan(*plan, data(), in, out, rt().stream);
-  DMRGX_HIP(hipStreamSynchronize(rt().stream));
 }
 
 // src/qtensor.cpp:997-1119
@@ -928,7 +923,6 @@ vector<double> svd_finish(const vector<vector<int>>& dim, const SvdJob<T>& j, co
   }
   if (S.total_) {
     upload(S.data(), hs.data(), hs.size() * 8, rt().stream);
-    DMRGX_HIP(hipStreamSynchronize(rt().stream));
   }
   vector<vector<int>> dl(dim.begin(), dim.begin() + num), dr(dim.begin() + num, dim.end());
   U = U.split(0, dl);

@@ -197,7 +197,6 @@ void leg_contract(int dtype, const void* A, const void* B, void* C, const std::v
                        (double*)C, (const LegOut*)dt.ptr, (int)outs.size(), (const LegPair*)dp.ptr,
                        (const int64_t*)ds.ptr);
   DMRGX_HIP(hipGetLastError());
-  DMRGX_HIP(hipStreamSynchronize(s));
 }
 
 void permute(int dtype, const void* in, void* out, const std::vector<PermTask>& t, hipStream_t s) {
@@ -218,7 +217,6 @@ void permute(int dtype, const void* in, void* out, const std::vector<PermTask>& 
     hipLaunchKernelGGL(k_permute<false>, dim3((unsigned)nb), dim3(256), 0, s, (const double*)in, (double*)out,
                        (const PermTask*)dt.ptr, (int)t.size(), (const int64_t*)ds.ptr);
   DMRGX_HIP(hipGetLastError());
-  DMRGX_HIP(hipStreamSynchronize(s));
 }
 
 void copy_boxes(int dtype, const void* src, void* dst, const std::vector<BoxTask>& t, hipStream_t s) {
@@ -234,7 +232,6 @@ void copy_boxes(int dtype, const void* src, void* dst, const std::vector<BoxTask
     hipLaunchKernelGGL(k_boxes<false>, dim3((unsigned)nb), dim3(256), 0, s, (const double*)src, (double*)dst,
                        (const BoxTask*)dt.ptr, (int)t.size(), (const int64_t*)ds.ptr);
   DMRGX_HIP(hipGetLastError());
-  DMRGX_HIP(hipStreamSynchronize(s));
 }
 
 std::vector<double> box_maxabs(int dtype, const void* src, const std::vector<BoxTask>& t, hipStream_t s) {
@@ -295,7 +292,6 @@ void blocks_axpby(bool cplx, const void* A, const void* B, void* out, const std:
   else hipLaunchKernelGGL(k_axpby_seg<false>, dim3(g), dim3(256), 0, s, (const double*)A, (const double*)B,
                           (double*)out, (const AxpbySeg*)d.ptr, (int)segs.size(), total, ar, ai, br, bi);
   DMRGX_HIP(hipGetLastError());
-  DMRGX_HIP(hipStreamSynchronize(s));  // d is released at scope end
 }
 
 __global__ void k_conj_f32(float* __restrict__ p, int64_t n) {
@@ -365,6 +361,7 @@ int dmrgx_leg_contract(int dtype, const void* A, const void* B, void* C, const i
     outs.back().pair_end = (int)pairs.size();
   }
   leg_contract(dtype, A, B, C, outs, pairs, rt().stream);
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }
 
@@ -383,6 +380,7 @@ int dmrgx_permute(int dtype, const void* in, void* out, const int64_t* tasks, in
     }
   }
   permute(dtype, in, out, t, rt().stream);
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }
 
@@ -395,6 +393,7 @@ int dmrgx_copy_boxes(int dtype, const void* src, void* dst, const int64_t* tasks
     t[i] = BoxTask{q[0], q[1], q[2], q[3], q[4], q[5], (int)q[6], (int)q[7], (int)q[8], (int)q[9]};
   }
   copy_boxes(dtype, src, dst, t, rt().stream);
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }
 
