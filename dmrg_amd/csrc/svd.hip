@@ -1473,7 +1473,6 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
                               (double*)dB.ptr, QB, tiles, kp, (const double*)dT.ptr, 0, (double*)dCN.ptr);
     }
     DMRGX_HIP(hipGetLastError());
-    DMRGX_HIP(hipStreamSynchronize(s));  // dbb is released at scope end
   }
   // 6. U, V in the reference layout
   DevBuf dF(nb * sizeof(FinBlk));
@@ -1565,7 +1564,6 @@ static int block_svd_impl(int dtype, const void* A, void* U, double* S, void* V,
                               (const CompItem*)di.ptr);
       DMRGX_HIP(hipGetLastError());
     }
-    DMRGX_HIP(hipStreamSynchronize(s));
   };
   launch(tu, wu, hu, U);
   launch(tv, wv, hv, V);
