@@ -114,7 +114,6 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   bl.d = P;
   HeffPlan* hp = plan.get();
   MatVec op = [hp, s](const double* x, double* y) { hp->apply(x, y, s); };
-  const int ncv = (cx ? 6 : 4) * cfg.nev;  // src/dsaupd.cpp:76, src/znaupd.cpp:94
   const double tol = cfg.prec == PREC_FP64 ? cfg.tol : cfg.tol_lo;
   vector<uint64_t> seeds(n);
   for (int c = 0; c < n; ++c)
@@ -123,6 +122,8 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   int64_t dmin = d[0];
   for (auto x : d) dmin = std::min(dmin, x);
   const int nev = plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
+  // ARPACK's basis size for the nev this solve asks for (src/dsaupd.cpp:76, src/znaupd.cpp:94)
+  const int ncv = (cx ? 6 : 4) * nev;
   const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, dmin})) : 1;
   DevBuf evec(P * sizeof(T) * pv);
   // converged chains leave the operator at the end of their restart cycle

@@ -103,9 +103,10 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
   HeffPlan* hp = plan.get();
   MatVec op = [hp, s](const double* x, double* y) { hp->apply(x, y, s); };
   const bool cx = dt == DT_C64;
-  const int ncv = (cx ? 6 : 4) * cfg.nev;  // src/dsaupd.cpp:76, src/znaupd.cpp:94
   // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64 polish
   const int nev = plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
+  // ARPACK's basis size for the nev this solve asks for (src/dsaupd.cpp:76, src/znaupd.cpp:94)
+  const int ncv = (cx ? 6 : 4) * nev;
   const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, d})) : 1;
   DevBuf evec(std::max<int64_t>(d, 1) * sizeof(T) * pv);
   const double tol = cfg.prec == PREC_FP64 ? cfg.tol : cfg.tol_lo;
