@@ -41,8 +41,8 @@ HBM_PEAK = 8000.0  # GB/s
 # the mean over the R and L launches of the chosen plan, None if the chosen
 # decomposition was not measured.
 TRAFFIC_KIB = {
-    ("R", 320): (47237.0, 47765.3), ("R", 400): (52062.1, 55940.7), ("R", 640): (61586.8, 81920.0),
-    ("L", 64): (82023.7, 8192.0), ("L", 256): (82041.3, 32768.0), ("L", 512): (82158.1, 65536.0),
+    ("R", 320): (47233.0, 47852.5), ("R", 400): (52067.8, 56371.7), ("R", 640): (61582.8, 81920.0),
+    ("L", 64): (82023.6, 8192.0), ("L", 256): (82220.4, 32768.0), ("L", 512): (82258.0, 65536.0),
     ("L", 640): (82051.6, 81920.0),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
