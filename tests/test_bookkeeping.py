@@ -55,7 +55,7 @@ def test_get_map_two_site_passes_bit_exact(S, seed):
             assert m3 == lmap and dr3 == dim_b and sr3 == sym
 
 
-def test_get_map_invariants_config2_shape():
+def test_get_map_invariants_xyz_m512_shape():
     """Every output block gets exactly one beta=0 task and it comes first;
     offsets are prefix sums; the flop count matches 4*D*S^2*m^3*(c/2)."""
     qt.set_symmetry(2)

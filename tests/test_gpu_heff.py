@@ -64,7 +64,7 @@ def test_heff_real_envs_kitaev(gpu):
     assert err < TOL["fp64"]
 
 
-def test_heff_config2_shape(gpu):
+def test_heff_xyz_m512_shape(gpu):
     """XYZ m=512 two-site shapes: 32+32 tasks of (1280,256,256)/(256,256,1280)."""
     print("case z2 m=512 D=5 complex", flush=True)
     case = synthetic_case(2, 5, 512, dtype=np.complex128)
