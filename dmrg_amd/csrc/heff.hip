@@ -539,6 +539,7 @@ int dmrgx_heff_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, voi
   DMRGX_CHECK(plan && plan->p && psi && g_out, "heff_adjoint: null arguments");
   plan->p->adjoint((const double*)psi, (const double*)g_out, (double*)g_psi, (double*)g_renv, (double*)g_lenv,
                    rt().stream);
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }
 
@@ -603,6 +604,7 @@ int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
     info[2] = r.converged ? 1 : 0;
     info[3] = polish_mv;
   }
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }
 
@@ -637,6 +639,7 @@ int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int 
       info[4 * c + 2] = r[c].converged ? 1 : 0;
       info[4 * c + 3] = polish_mv;
     }
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }
 

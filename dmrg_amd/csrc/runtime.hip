@@ -252,6 +252,7 @@ int dmrgx_memset_zero(void* dst, size_t bytes) {
   CAPI_BEGIN
   ensure_init();
   fill_zero(dst, bytes, g_rt.stream);
+  DMRGX_HIP(hipStreamSynchronize(g_rt.stream));  // complete on return
   CAPI_END
 }
 

@@ -12,7 +12,10 @@
  *   - every function returns 0 on success and a negative code on failure;
  *     dmrgx_last_error() describes the failure (per host thread);
  *   - one device per process, one in-order stream; calls are not re-entrant
- *     on one plan.
+ *     on one plan;
+ *   - a call that writes device memory returns with that work complete, so
+ *     results may be read at once from any stream (dmrgx_stream() gives the
+ *     library's stream to callers that order their own work after it).
  */
 #ifndef DMRGX_H
 #define DMRGX_H
