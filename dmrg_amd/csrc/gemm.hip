@@ -799,10 +799,8 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
   }
   // split-K so the launch has enough work items for every CU
   double work = 0;
-  int64_t tiles_total = 0;
   for (auto& g : groups) {
     int64_t tl = (int64_t)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
-    tiles_total += tl;
     work += (double)tl * g.ksum;
   }
   int target = split_target > 0 ? split_target : 2 * std::max(rt().num_cu, 1);

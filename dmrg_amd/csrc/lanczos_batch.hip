@@ -860,7 +860,7 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
   int q = 0;
   for (int j = 0; j < p; ++j) add_vector(q++, X + (size_t)j * nr);
   std::vector<char> done(nc, 0);
-  for (int outer = 0;; ++outer) {
+  for (;;) {  // ends when every chain converged or the basis is full
     // Rayleigh-Ritz in span V: the lowest pair, u = V z, r = H u - theta u
     std::vector<double> Z((size_t)nc * q * 2, 0.0);
     parallel_for(nc, [&](int c) {
