@@ -74,11 +74,13 @@ __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* _
   const RedTask r = t[lo];
   const int64_t e = (b - start[lo]) * 256 + threadIdx.x;
   if (e >= r.len) return;
-  // the parts in chunk order; complex elements read as one vector
+  // the parts in chunk order (unrolled so that four loads are in flight);
+  // complex elements read as one vector
   if (CPLX) {
     using V2 = typename std::conditional<sizeof(S) == 8, double2, float2>::type;
     const V2* W2 = reinterpret_cast<const V2*>(W);
     S ar = 0, ai = 0;
+#pragma unroll 4
     for (int k = 0; k < r.nch; ++k) {
       const V2 v = W2[r.w_off + k * r.len + e];
       ar += v.x;
@@ -89,6 +91,7 @@ __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* _
     else { o[0] = ar; o[1] = ai; }
   } else {
     S acc = 0;
+#pragma unroll 4
     for (int k = 0; k < r.nch; ++k) acc += W[r.w_off + k * r.len + e];
     if (r.beta) C[r.c_off + e] += acc;
     else C[r.c_off + e] = acc;
@@ -445,11 +448,13 @@ __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__
   const int64_t e = (b - start[lo]) * 256 + threadIdx.x;
   if (e >= r.len) return;
   constexpr int c = CPLX ? 2 : 1;
-  // the parts in chunk order; complex elements read as one float2
+  // the parts in chunk order (unrolled so that four loads are in flight);
+  // complex elements read as one float2
   float acc[c];
   if (CPLX) {
     const float2* W2 = reinterpret_cast<const float2*>(W);
     float ar = 0, ai = 0;
+#pragma unroll 4
     for (int k = 0; k < r.nch; ++k) {
       const float2 v = W2[r.w_off + k * r.len + e];
       ar += v.x;
@@ -459,6 +464,7 @@ __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__
     acc[c - 1] = ai;
   } else {
     float a = 0;
+#pragma unroll 4
     for (int k = 0; k < r.nch; ++k) a += W[r.w_off + k * r.len + e];
     acc[0] = a;
   }
