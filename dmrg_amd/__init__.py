@@ -82,10 +82,10 @@ def _declare(lib):
         "dmrgx_heff_destroy": ([vp], C.c_int),
         "dmrgx_heff_adjoint": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "dmrgx_heff_sizes": ([vp, i64p], C.c_int),
-        "dmrgx_eigsh": ([vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, vp, dp, vp, vp, C.c_int,
-                         C.POINTER(C.c_int)], C.c_int),
+        "dmrgx_eigsh": ([vp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_uint64, vp, dp, vp, vp, C.c_double,
+                         C.POINTER(C.c_int), dp], C.c_int),
         "dmrgx_eigsh_batched": ([vp, C.c_int, i64p, C.c_int, C.c_int, C.c_double, C.c_int,
-                                 C.POINTER(C.c_uint64), dp, vp, vp, C.c_int, C.POINTER(C.c_int)], C.c_int),
+                                 C.POINTER(C.c_uint64), dp, vp, vp, C.c_double, C.POINTER(C.c_int), dp], C.c_int),
         "dmrgx_set_lanczos_graphs": ([C.c_int], C.c_int),
         "dmrgx_set_autotune": ([C.c_int], C.c_int),
         "dmrgx_autotune_keys": ([], C.c_int),
@@ -219,22 +219,26 @@ class Heff:
         return ms
 
     def eigsh(self, nev=6, ncv=None, tol=1e-10, max_restart=1000, seed=1234, v0=None, refine=None,
-              refine_k=6):
+              refine_tol=1e-10):
+        """dmrgx_eigsh: (evals, lowest vector, info[6], FP64 residual of the returned pair)."""
         if ncv is None:
             ncv = (6 if self.dtype == C64 else 4) * nev
         npd = np.complex128 if self.dtype == C64 else np.float64
         vec = DeviceArray(self.n_psi, npd)
         ev = np.zeros(max(nev, 1))
-        info = (C.c_int * 4)()
+        info = (C.c_int * 6)()
+        res = np.zeros(1)
         check(lib().dmrgx_eigsh(self.h, nev, ncv, tol, max_restart, seed, v0.ptr if v0 is not None else None,
                                 ev.ctypes.data_as(C.POINTER(C.c_double)), vec.ptr,
-                                refine.h if refine is not None else None, refine_k, info), "dmrgx_eigsh")
-        return ev, vec, list(info)
+                                refine.h if refine is not None else None, refine_tol, info,
+                                res.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_eigsh")
+        return ev, vec, list(info), float(res[0])
 
     def eigsh_batched(self, lens, nev=6, ncv=None, tol=1e-10, max_restart=1000, seeds=None, refine=None,
-                      refine_k=6):
+                      refine_tol=1e-10):
         """dmrgx_eigsh_batched over chains of lengths `lens` concatenated in
-        this plan: (evals n_chains x nev, the concatenated vectors, info n_chains x 4)."""
+        this plan: (evals n_chains x nev, the concatenated vectors, info
+        n_chains x 6, FP64 residuals n_chains)."""
         if ncv is None:
             ncv = (6 if self.dtype == C64 else 4) * nev
         n = len(lens)
@@ -243,13 +247,15 @@ class Heff:
         npd = np.complex128 if self.dtype == C64 else np.float64
         vec = DeviceArray(self.n_psi, npd)
         ev = np.zeros((n, max(nev, 1)))
-        info = np.zeros((n, 4), dtype=np.int32)
+        info = np.zeros((n, 6), dtype=np.int32)
+        res = np.zeros(n)
         check(lib().dmrgx_eigsh_batched(self.h, n, ln.ctypes.data_as(C.POINTER(C.c_int64)), nev, ncv, tol,
                                         max_restart, sd.ctypes.data_as(C.POINTER(C.c_uint64)),
                                         ev.ctypes.data_as(C.POINTER(C.c_double)), vec.ptr,
-                                        refine.h if refine is not None else None, refine_k,
-                                        info.ctypes.data_as(C.POINTER(C.c_int))), "dmrgx_eigsh_batched")
-        return ev, vec, info
+                                        refine.h if refine is not None else None, refine_tol,
+                                        info.ctypes.data_as(C.POINTER(C.c_int)),
+                                        res.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_eigsh_batched")
+        return ev, vec, info, res
 
     def close(self):
         if self.h:

@@ -218,6 +218,22 @@ void run_batch(BatchEngine<T>& b, const dmrgx_run_config* c, double* energy, dou
       write_rows(b.chains[i]->log, chain_log + (size_t)i * cap * DMRGX_STEP_LOG_WIDTH, cap);
 }
 
+// Host-side checks of a run configuration, made before the device is touched.
+// The single-site schedule under the reference's sign rule is refused for
+// bosonic models: there get_map's alpha (src/qtensor.cpp:1078-1085) makes the
+// move2right operator non-Hermitian (||H - H^H|| / ||H|| up to 0.6 on an open
+// XYZ chain), which the reference hands to complex Arnoldi (znaupd, which="SR",
+// src/znaupd.cpp:78-197) and this library's Hermitian Lanczos cannot solve.
+void validate_cfg(const dmrgx_run_config* c) {
+  DMRGX_CHECK(c != nullptr, "run: null configuration");
+  DMRGX_CHECK(c->schedule == DMRGX_SCHEDULE_DMRG || c->schedule == DMRGX_SCHEDULE_DMRG2, "run: unknown schedule");
+  DMRGX_CHECK(c->sign_rule == DMRGX_SIGNS_PHYSICAL || c->sign_rule == DMRGX_SIGNS_REFERENCE, "run: unknown sign rule");
+  DMRGX_CHECK(c->prec >= PREC_FP64 && c->prec <= PREC_BF16X3, "run: unknown precision");
+  DMRGX_CHECK(!(c->schedule == DMRGX_SCHEDULE_DMRG && c->sign_rule == DMRGX_SIGNS_REFERENCE && !model_fermion(c->model)),
+              "run: the single-site schedule (dmrg) with the reference sign rule makes the operator of a bosonic "
+              "model non-Hermitian; use schedule dmrg2 or the physical sign rule");
+}
+
 RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
   RunConfig r;
   r.prec = c->prec;
@@ -290,9 +306,10 @@ int dmrgx_run_dmrg(const dmrgx_run_config* c, dmrgx_run_stats* st, double* step_
 int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double* step_log, int log_cap,
                          int* n_steps, dmrgx_state** state) {
   CAPI_BEGIN
-  ensure_init();
   DMRGX_CHECK(c && st, "run_dmrg: null arguments");
+  validate_cfg(c);
   DMRGX_CHECK(c->L >= 4, "run_dmrg: L must be at least 4");
+  ensure_init();
   if (state) *state = nullptr;
   std::unique_ptr<dmrgx_state> out;
   if (state) {
@@ -345,10 +362,11 @@ int dmrgx_parse_args(int argc, const char* const* argv, int n_para, const char* 
 int dmrgx_run_batch(const dmrgx_run_config* c, int n_chains, const double* p, double* energy, double* trunc,
                     dmrgx_run_stats* st, double* step_log, int log_cap, int* n_steps, double* chain_log) {
   CAPI_BEGIN
-  ensure_init();
   DMRGX_CHECK(c && st && p && n_chains > 0, "run_batch: null arguments or no chains");
+  validate_cfg(c);
   DMRGX_CHECK(c->L >= 6, "run_batch: L must be at least 6");
   DMRGX_CHECK(c->schedule == DMRGX_SCHEDULE_DMRG2, "run_batch: only the two-site schedule is batched");
+  ensure_init();
   const RunConfig cfg = make_cfg(c, model_fermion(c->model));
   if (model_complex(c->model)) {
     vector<mpo<cplx>> m;

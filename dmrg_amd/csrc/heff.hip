@@ -19,13 +19,17 @@
 namespace dmrgx {
 
 std::vector<GemmTask> tasks_from_map(const int32_t* map, int n_task, const int64_t* blk_off,
-                                     const int64_t* blk_size) {
+                                     const int64_t* blk_size, int n_blk) {
+  DMRGX_CHECK(n_task >= 0 && (n_task == 0 || map != nullptr), "task list: null map");
   std::vector<GemmTask> t;
   t.reserve(n_task);
   for (int i = 0; i < n_task; ++i) {
     const int32_t* m = map + 7 * i;
     GemmTask g{};
     int col = m[4], row = m[3];
+    DMRGX_CHECK(m[0] >= 0 && m[0] < n_blk, "task block index out of range");
+    DMRGX_CHECK(m[1] >= 0 && m[2] >= 0, "task with a negative input or output offset");
+    DMRGX_CHECK(blk_off[m[0]] >= 0 && blk_size[m[0]] >= 0, "task block with a negative offset or size");
     DMRGX_CHECK(col > 0 && row > 0, "task with empty row/col");
     DMRGX_CHECK(blk_size[m[0]] % col == 0, "task col does not divide the block size");
     g.a_off = blk_off[m[0]];
@@ -69,22 +73,16 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
                                     const int64_t* lblk_off, const int64_t* lblk_size, int n_lblk,
                                     int64_t lenv_len, const int32_t* lmap, int n_ltask, int64_t n_psi,
                                     int64_t n_store) {
-  ensure_init();
+  // every argument is validated before the device is touched
   DMRGX_CHECK(dtype == DT_R64 || dtype == DT_C64, "heff: dtype must be R64 or C64");
   DMRGX_CHECK(prec >= PREC_FP64 && prec <= PREC_BF16X3, "heff: unknown precision");
+  DMRGX_CHECK(n_psi >= 0 && n_store >= 0, "heff: negative psi or store length");
   for (int i = 0; i < n_rtask; ++i)
     DMRGX_CHECK(rmap[7 * i] >= 0 && rmap[7 * i] < n_rblk, "heff: rmap block index out of range");
   for (int i = 0; i < n_ltask; ++i)
     DMRGX_CHECK(lmap[7 * i] >= 0 && lmap[7 * i] < n_lblk, "heff: lmap block index out of range");
-  auto h = std::make_unique<HeffPlan>();
-  h->dtype = dtype;
-  h->prec = prec;
-  h->n_psi = n_psi;
-  h->n_store = n_store;
-  h->renv = renv;
-  h->lenv = lenv;
-  auto rt_ = tasks_from_map(rmap, n_rtask, rblk_off, rblk_size);
-  auto lt_ = tasks_from_map(lmap, n_ltask, lblk_off, lblk_size);
+  auto rt_ = tasks_from_map(rmap, n_rtask, rblk_off, rblk_size, n_rblk);
+  auto lt_ = tasks_from_map(lmap, n_ltask, lblk_off, lblk_size, n_lblk);
   for (auto& g : rt_) {
     DMRGX_CHECK(g.b_off + (int64_t)g.N * g.K <= n_psi, "heff: rmap reads past the end of psi");
     DMRGX_CHECK(g.c_off + (int64_t)g.M * g.N <= n_store, "heff: rmap writes past the end of store");
@@ -95,6 +93,14 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     DMRGX_CHECK(g.c_off + (int64_t)g.M * g.N <= n_psi, "heff: lmap writes past the end of psi");
     DMRGX_CHECK(g.a_off + (int64_t)g.M * g.K <= lenv_len, "heff: lmap block past the end of lenv");
   }
+  ensure_init();
+  auto h = std::make_unique<HeffPlan>();
+  h->dtype = dtype;
+  h->prec = prec;
+  h->n_psi = n_psi;
+  h->n_store = n_store;
+  h->renv = renv;
+  h->lenv = lenv;
   const bool planes = prec == PREC_BF16X3;
   h->rtasks = rt_;
   h->ltasks = lt_;
@@ -466,16 +472,16 @@ extern "C" {
 int dmrgx_task_gemm(int dtype, int prec, const void* A, const int64_t* blk_off, const int64_t* blk_size,
                     int n_blk, const int32_t* map, int n_task, const void* in, void* out, int64_t n_out) {
   CAPI_BEGIN
-  ensure_init();
-  hipStream_t s = rt().stream;
-  for (int i = 0; i < n_task; ++i)
-    DMRGX_CHECK(map[7 * i] >= 0 && map[7 * i] < n_blk, "task_gemm: block index out of range");
-  auto t = tasks_from_map(map, n_task, blk_off, blk_size);
+  DMRGX_CHECK(dtype == DT_R64 || dtype == DT_C64, "task_gemm: dtype must be R64 or C64");
+  DMRGX_CHECK(prec >= PREC_FP64 && prec <= PREC_BF16X3, "task_gemm: unknown precision");
+  auto t = tasks_from_map(map, n_task, blk_off, blk_size, n_blk);
   int64_t n_in = 0;
   for (auto& g : t) {
     n_in = std::max(n_in, g.b_off + (int64_t)g.N * g.K);
     DMRGX_CHECK(g.c_off + (int64_t)g.M * g.N <= n_out, "task_gemm: output past n_out");
   }
+  ensure_init();
+  hipStream_t s = rt().stream;
   int64_t alen = blocks_len(blk_off, blk_size, n_blk);
   // outputs without any task are left untouched, as the reference's loop does
   auto plan = make_gemm_plan(t, dtype, prec, 'N', 'T', {});
@@ -581,64 +587,133 @@ int dmrgx_heff_destroy(dmrgx_heff* plan) {
   CAPI_END
 }
 
+namespace {
+
+// FP64 refinement of a reduced-precision solve: Jacobi-Davidson from the pv
+// lowest low-precision Ritz vectors of every chain (in `vecs`, pv columns of
+// the concatenated length), the correction equations solved with the
+// low-precision operator.  The refined lowest vector of every chain is left
+// in column 0.
+struct Refined {
+  std::vector<double> e0, resid;
+  std::vector<char> conv;
+  int n64 = 0, nlo = 0;
+};
+Refined refine_lowest(HeffPlan* lo, HeffPlan* f, const std::vector<int64_t>& len, int pv, double* vecs, double tol) {
+  hipStream_t s = rt().stream;
+  MatVec op = [lo, s](const double* x, double* y) { lo->apply(x, y, s); };
+  MatVec op64 = [f, s](const double* x, double* y) { f->apply(x, y, s); };
+  Refined r;
+  // the sweep engine's settings: at most 12 outer steps of at most 16 CG steps
+  r.e0 = jd_polish_batched(op, op64, len, f->dtype == DT_C64, pv, vecs, tol, 12, 16, &r.n64, &r.nlo, &r.resid,
+                           &r.conv);
+  return r;
+}
+
+void check_refine(const HeffPlan* h, const HeffPlan* f, const char* who) {
+  DMRGX_CHECK(f->n_psi == h->n_psi && f->dtype == h->dtype,
+              std::string(who) + ": refine plan of another psi length or element type");
+  DMRGX_CHECK(f->prec == PREC_FP64, std::string(who) + ": the refine plan must be an FP64 plan");
+}
+
+}  // namespace
+
 int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart, uint64_t seed,
-                const void* v0, double* evals, void* evec, dmrgx_heff* refine, int refine_k, int* info) {
+                const void* v0, double* evals, void* evec, dmrgx_heff* refine, double refine_tol, int* info,
+                double* resid) {
   CAPI_BEGIN
-  DMRGX_CHECK(plan && plan->p, "eigsh: null plan");
+  DMRGX_CHECK(plan && plan->p && evals && evec, "eigsh: null arguments");
   HeffPlan* h = plan->p.get();
+  HeffPlan* f = refine && refine->p ? refine->p.get() : nullptr;
+  if (f) check_refine(h, f, "eigsh");
+  const bool use_ref = f && h->prec != PREC_FP64;
+  const bool cx = h->dtype == DT_C64;
   hipStream_t s = rt().stream;
   MatVec op = [h, s](const double* x, double* y) { h->apply(x, y, s); };
-  EigshResult r = eigsh(op, h->n_psi, h->dtype == DT_C64, nev, ncv, tol, max_restart, seed,
-                        (const double*)v0, (double*)evec);
+  // a refined solve hands its pv lowest Ritz vectors to the FP64 refinement
+  const int pv = use_ref ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, h->n_psi})) : 1;
+  std::unique_ptr<DevBuf> vecs;
+  double* ev = (double*)evec;
+  if (pv > 1) {
+    vecs = std::make_unique<DevBuf>((size_t)pv * h->n_psi * (cx ? 16 : 8));
+    ev = (double*)vecs->ptr;
+  }
+  EigshResult r = eigsh(op, h->n_psi, cx, nev, ncv, tol, max_restart, seed, (const double*)v0, ev, pv);
   for (int i = 0; i < (int)r.evals.size(); ++i) evals[i] = r.evals[i];
-  int polish_mv = 0;
-  if (refine && refine->p && h->prec != PREC_FP64) {
-    HeffPlan* f = refine->p.get();
-    MatVec op64 = [f, s](const double* x, double* y) { f->apply(x, y, s); };
-    evals[0] = krylov_polish(op64, f->n_psi, f->dtype == DT_C64, std::max(refine_k, 1), (double*)evec,
-                             &polish_mv);
+  Refined rf;
+  if (use_ref) {
+    rf = refine_lowest(h, f, {h->n_psi}, pv, ev, refine_tol > 0 ? refine_tol : 1e-10);
+    evals[0] = rf.e0[0];
+    if (pv > 1) copy_d2d(evec, ev, (size_t)h->n_psi * (cx ? 16 : 8), s);
+  } else if (resid) {
+    rf.resid = residual_batched(op, {h->n_psi}, cx, (const double*)evec, {evals[0]});
   }
   if (info) {
     info[0] = r.n_matvec;
     info[1] = r.n_restart;
     info[2] = r.converged ? 1 : 0;
-    info[3] = polish_mv;
+    info[3] = rf.n64;
+    info[4] = use_ref ? (rf.conv[0] ? 1 : 0) : info[2];
+    info[5] = rf.nlo;
   }
+  if (resid) resid[0] = rf.resid[0];
   DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }
 
 int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int nev, int ncv, double tol,
                         int max_restart, const uint64_t* seeds, double* evals, void* evec, dmrgx_heff* refine,
-                        int refine_k, int* info) {
+                        double refine_tol, int* info, double* resid) {
   CAPI_BEGIN
   DMRGX_CHECK(plan && plan->p && len && seeds && evals && evec && n_chains > 0, "eigsh_batched: null arguments");
   HeffPlan* h = plan->p.get();
   std::vector<int64_t> ln(len, len + n_chains);
-  int64_t tot = 0;
-  for (auto n : ln) tot += n;
+  int64_t tot = 0, nmin = ln[0];
+  for (auto n : ln) {
+    DMRGX_CHECK(n > 0, "eigsh_batched: empty chain");
+    tot += n;
+    nmin = std::min(nmin, n);
+  }
   DMRGX_CHECK(tot == h->n_psi, "eigsh_batched: chain lengths do not add up to the plan's psi length");
+  HeffPlan* f = refine && refine->p ? refine->p.get() : nullptr;
+  if (f) check_refine(h, f, "eigsh_batched");
+  const bool use_ref = f && h->prec != PREC_FP64;
+  const bool cx = h->dtype == DT_C64;
   hipStream_t s = rt().stream;
   MatVec op = [h, s](const double* x, double* y) { h->apply(x, y, s); };
+  const int pv = use_ref ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, nmin})) : 1;
+  std::unique_ptr<DevBuf> vecs;
+  double* ev = (double*)evec;
+  if (pv > 1) {
+    vecs = std::make_unique<DevBuf>((size_t)pv * h->n_psi * (cx ? 16 : 8));
+    ev = (double*)vecs->ptr;
+  }
   std::vector<uint64_t> sd(seeds, seeds + n_chains);
-  auto r = eigsh_batched(op, ln, h->dtype == DT_C64, nev, ncv, tol, max_restart, sd, nullptr, (double*)evec);
+  auto r = eigsh_batched(op, ln, cx, nev, ncv, tol, max_restart, sd, nullptr, ev, pv);
   for (int c = 0; c < n_chains; ++c)
     for (int i = 0; i < nev; ++i) evals[(size_t)c * nev + i] = i < (int)r[c].evals.size() ? r[c].evals[i] : 0.0;
-  int polish_mv = 0;
-  if (refine && refine->p && h->prec != PREC_FP64) {
-    HeffPlan* f = refine->p.get();
-    DMRGX_CHECK(f->n_psi == h->n_psi, "eigsh_batched: refine plan of another size");
-    MatVec op64 = [f, s](const double* x, double* y) { f->apply(x, y, s); };
-    auto e0 = krylov_polish_batched(op64, ln, f->dtype == DT_C64, std::max(refine_k, 1), (double*)evec, &polish_mv);
-    for (int c = 0; c < n_chains; ++c) evals[(size_t)c * nev] = e0[c];
+  Refined rf;
+  if (use_ref) {
+    rf = refine_lowest(h, f, ln, pv, ev, refine_tol > 0 ? refine_tol : 1e-10);
+    for (int c = 0; c < n_chains; ++c) evals[(size_t)c * nev] = rf.e0[c];
+    if (pv > 1) copy_d2d(evec, ev, (size_t)h->n_psi * (cx ? 16 : 8), s);
+  } else if (resid) {
+    std::vector<double> th(n_chains);
+    for (int c = 0; c < n_chains; ++c) th[c] = evals[(size_t)c * nev];
+    rf.resid = residual_batched(op, ln, cx, (const double*)evec, th);
   }
   if (info)
     for (int c = 0; c < n_chains; ++c) {
-      info[4 * c] = r[c].n_matvec;
-      info[4 * c + 1] = r[c].n_restart;
-      info[4 * c + 2] = r[c].converged ? 1 : 0;
-      info[4 * c + 3] = polish_mv;
+      int* q = info + 6 * c;
+      q[0] = r[c].n_matvec;
+      q[1] = r[c].n_restart;
+      q[2] = r[c].converged ? 1 : 0;
+      q[3] = rf.n64;
+      q[4] = use_ref ? (rf.conv[c] ? 1 : 0) : q[2];
+      q[5] = rf.nlo;
     }
+  if (resid)
+    for (int c = 0; c < n_chains; ++c) resid[c] = rf.resid[c];
   DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
   CAPI_END
 }

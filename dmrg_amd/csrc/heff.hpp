@@ -57,9 +57,10 @@ struct HeffPlan {
                hipStream_t s);
 };
 
-// get_map tuples -> GEMM tasks (M = blk_size / col, N = row, K = col).
+// get_map tuples -> GEMM tasks (M = blk_size / col, N = row, K = col); throws
+// on a block index outside [0, n_blk), negative offsets or empty rows / cols.
 std::vector<GemmTask> tasks_from_map(const int32_t* map, int n_task, const int64_t* blk_off,
-                                     const int64_t* blk_size);
+                                     const int64_t* blk_size, int n_blk);
 // C ranges in [0, n_out) that no task writes (they must read as zero).
 std::vector<std::pair<int64_t, int64_t>> uncovered(const std::vector<GemmTask>& t, int64_t n_out);
 

@@ -426,73 +426,4 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
   return res;
 }
 
-namespace {
-
-// one Rayleigh-Ritz polish cycle in K_k(H, x); *resid = |beta_k y_k[0]|, the
-// residual norm of the returned Ritz pair (0 when the space became invariant)
-double polish_cycle(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec, double* resid) {
-  hipStream_t s = rt().stream;
-  const int64_t nr = n * (cplx ? 2 : 1);
-  k = (int)std::max<int64_t>(1, std::min<int64_t>(k, n));
-  DevBuf Vb((size_t)(k + 1) * nr * 8), wb(nr * 8);
-  DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
-  DevBuf hbuf(2 * (k + 2) * 8), h2buf(2 * (k + 2) * 8), Yb((size_t)k * 8);
-  double* V = (double*)Vb.ptr;
-  double* w = (double*)wb.ptr;
-  double nx = dev_norm2(x, n, cplx, s);
-  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, x, V, nr, 1.0 / nx);
-  std::vector<double> T((size_t)k * k, 0.0), hh(2 * (k + 2)), hh2(2 * (k + 2));
-  int kk = k;
-  double beta_k = 0;
-  for (int j = 0; j < k; ++j) {
-    op(V + (int64_t)j * nr, w);
-    if (n_matvec) (*n_matvec)++;
-    vdots(V, nr, j + 1, w, n, cplx, (double*)hbuf.ptr, (double*)partial.ptr, s);
-    gemv_sub(V, nr, j + 1, (double*)hbuf.ptr, w, n, cplx, s);
-    vdots(V, nr, j + 1, w, n, cplx, (double*)h2buf.ptr, (double*)partial.ptr, s);
-    gemv_sub(V, nr, j + 1, (double*)h2buf.ptr, w, n, cplx, s);
-    vdots(w, 0, 1, w, n, cplx, (double*)h2buf.ptr + 2 * (j + 1), (double*)partial.ptr, s);
-    DMRGX_HIP(hipMemcpyAsync(hh.data(), hbuf.ptr, 2 * (j + 1) * 8, hipMemcpyDeviceToHost, s));
-    DMRGX_HIP(hipMemcpyAsync(hh2.data(), h2buf.ptr, 2 * (j + 2) * 8, hipMemcpyDeviceToHost, s));
-    DMRGX_HIP(hipStreamSynchronize(s));
-    T[(size_t)j * k + j] = hh[2 * j] + hh2[2 * j];
-    double beta = std::sqrt(std::max(hh2[2 * (j + 1)], 0.0));
-    if (j + 1 == k) beta_k = beta;
-    if (j + 1 < k) {
-      if (beta < 1e-14 * std::fabs(T[(size_t)j * k + j]) || beta < 1e-300) { kk = j + 1; break; }
-      T[(size_t)j * k + j + 1] = T[(size_t)(j + 1) * k + j] = beta;
-      hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, w, V + (int64_t)(j + 1) * nr, nr,
-                         1.0 / beta);
-    }
-  }
-  std::vector<double> A((size_t)kk * kk), th, Y;
-  for (int i = 0; i < kk; ++i)
-    for (int j = 0; j < kk; ++j) A[(size_t)i * kk + j] = T[(size_t)i * k + j];
-  host_symeig(kk, A, th, Y);
-  std::vector<double> y0(kk);
-  for (int r = 0; r < kk; ++r) y0[r] = Y[(size_t)r * kk];
-  DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), kk * 8, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, nr, kk, (const double*)Yb.ptr, 1, 0, 1, w,
-                     nr);
-  double z = dev_norm2(w, n, cplx, s);
-  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, w, x, nr, 1.0 / z);
-  DMRGX_HIP(hipStreamSynchronize(s));
-  *resid = kk < k ? 0.0 : std::fabs(beta_k * Y[(size_t)(kk - 1) * kk]);
-  return th[0];
-}
-
-}  // namespace
-
-double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec, double tol,
-                     int max_cycles) {
-  ensure_init();
-  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
-  double theta = 0, resid = 0;
-  for (int c = 0; c < std::max(1, max_cycles); ++c) {
-    theta = polish_cycle(op, n, cplx, k, x, n_matvec, &resid);
-    if (resid <= tol * std::max(std::fabs(theta), eps23)) break;
-  }
-  return theta;
-}
-
 }  // namespace dmrgx

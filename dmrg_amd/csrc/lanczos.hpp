@@ -30,14 +30,6 @@ struct EigshResult {
 EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
                   int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec = 1);
 
-// Rayleigh-Ritz polish of an approximate lowest eigenvector in a small Krylov
-// space K_k(H, x) built with `op`; x is replaced by the polished vector and
-// the Ritz value returned.  Cycles repeat (each from the previous polished
-// vector) until the Ritz pair's residual |beta_k y_k| <= tol max(|theta|,
-// eps^(2/3)) (eigsh's test) or max_cycles cycles ran.
-double krylov_polish(const MatVec& op, int64_t n, bool cplx, int k, double* x, int* n_matvec, double tol = 0,
-                     int max_cycles = 1);
-
 // device vector helpers (n = number of elements, complex if cplx)
 void dev_random(double* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s);
 double dev_norm2(const double* v, int64_t n, bool cplx, hipStream_t s);
@@ -80,12 +72,14 @@ void set_lanczos_graphs(bool on);
 // chain's FP64 residual passes tol max(|theta|, eps^(2/3)) or after max_outer
 // steps.  X column 0 receives the refined lowest vector (normalised); returns
 // the Ritz values; *n64 / *nlo count the matvecs of each operator.
+// res_out / conv_out (optional) receive each chain's final FP64 residual norm
+// |H64 u - theta u| and whether it passed the test.
 std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, const std::vector<int64_t>& len,
                                       bool cplx, int p, double* X, double tol, int max_outer, int inner, int* n64,
-                                      int* nlo);
-// krylov_polish of every chain's segment of x at once (cycles until every
-// chain passes the test); returns the Ritz values
-std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int k,
-                                          double* x, int* n_matvec, double tol = 0, int max_cycles = 1);
-
+                                      int* nlo, std::vector<double>* res_out = nullptr,
+                                      std::vector<char>* conv_out = nullptr);
+// |H x_c - theta_c x_c| for every chain c of the concatenated vector x (one
+// call of `op`, computed explicitly, not from Lanczos coefficients).
+std::vector<double> residual_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, const double* x,
+                                     const std::vector<double>& theta);
 }  // namespace dmrgx

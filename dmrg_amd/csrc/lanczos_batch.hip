@@ -689,103 +689,14 @@ std::vector<EigshResult> eigsh_batched(const MatVec& op, const std::vector<int64
   return res;
 }
 
-std::vector<double> krylov_polish_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, int k,
-                                          double* x, int* n_matvec, double tol, int max_cycles) {
-  ensure_init();
-  hipStream_t s = rt().stream;
-  const int nc = (int)len.size();
-  DMRGX_CHECK(nc > 0, "krylov_polish_batched: no chains");
-  const std::vector<int64_t> off = offsets(len);
-  const int64_t N = off.back() + len.back();
-  const int cw = cplx ? 2 : 1;
-  const int64_t nr = N * cw;
-  int64_t nmax = 0;
-  for (auto n : len) nmax = std::max(nmax, n);
-  k = (int)std::max<int64_t>(1, std::min<int64_t>(k, nmax));
-  std::vector<int> all(nc);
-  for (int c = 0; c < nc; ++c) all[c] = c;
-  Segs sg(off, len, all, s);
-  SegOps ops(sg, cplx, nr, k + 1, s);
-  const int hs = ops.hs;
-  DevBuf Vb((size_t)(k + 1) * nr * 8), wb(nr * 8), hbuf((size_t)2 * nc * hs * 8), scl((size_t)nc * 8),
-      Yb((size_t)nc * k * 8);
-  double* V = (double*)Vb.ptr;
-  double* w = (double*)wb.ptr;
-  double* h = (double*)hbuf.ptr;
-  double* h2 = h + (size_t)nc * hs;
-  const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
-  std::vector<double> th0(nc), resid(nc);
-  for (int cycle = 0; cycle < std::max(1, max_cycles); ++cycle) {
-    copy_d2d(V, x, nr * 8, s);
-    ops.normalise(V, h, (double*)scl.ptr);
-    // chain c's Krylov space stops growing at kk[c]; its later vectors are zero
-    std::vector<std::vector<double>> T(nc, std::vector<double>((size_t)k * k, 0.0));
-    std::vector<int> kk(nc, k);
-    std::vector<double> sc(nc), beta_k(nc, 0.0);
-    for (int j = 0; j < k; ++j) {
-      op(V + (int64_t)j * nr, w);
-      if (n_matvec) (*n_matvec)++;
-      // two full Gram-Schmidt passes (as in krylov_polish); |w''|^2 = |w'|^2 - |h2|^2
-      ops.vdots(V, j + 1, w, h);
-      ops.sub(V, j + 1, h, w, w, nullptr);
-      ops.vdots(V, j + 1, w, h2);
-      const double* hh = ops.fetch(h, 2);  // h and h2 are adjacent
-      for (int c = 0; c < nc; ++c) {
-        const double* q = hh + (size_t)c * hs;
-        const double* q2 = hh + (size_t)(nc + c) * hs;
-        sc[c] = 0;
-        if (j >= kk[c]) continue;
-        const double a = q[2 * j] + q2[2 * j];
-        T[c][(size_t)j * k + j] = a;
-        double hsq = 0;
-        for (int i = 0; i <= j; ++i) hsq += q2[2 * i] * q2[2 * i] + q2[2 * i + 1] * q2[2 * i + 1];
-        const double beta = std::sqrt(std::max(q2[2 * (j + 1)] - hsq, 0.0));
-        if (j + 1 == k) beta_k[c] = beta;
-        if (j + 1 < k) {
-          if (beta < 1e-14 * std::fabs(a) || beta < 1e-300 || j + 1 >= len[c]) {
-            kk[c] = j + 1;
-          } else {
-            T[c][(size_t)j * k + j + 1] = T[c][(size_t)(j + 1) * k + j] = beta;
-            sc[c] = 1.0 / beta;
-          }
-        }
-      }
-      if (j + 1 < k) {
-        ops.put(sc, (double*)scl.ptr);
-        ops.sub(V, j + 1, h2, w, V + (int64_t)(j + 1) * nr, (const double*)scl.ptr);
-        DMRGX_HIP(hipStreamSynchronize(s));
-      }
-    }
-    std::vector<double> y0((size_t)nc * k, 0.0);
-    parallel_for(nc, [&](int c) {
-      const int n = kk[c];
-      std::vector<double> A((size_t)n * n), th, Y;
-      for (int i = 0; i < n; ++i)
-        for (int j = 0; j < n; ++j) A[(size_t)i * n + j] = T[c][(size_t)i * k + j];
-      host_symeig(n, A, th, Y);
-      th0[c] = th[0];
-      resid[c] = kk[c] < k ? 0.0 : std::fabs(beta_k[c] * Y[(size_t)(n - 1) * n]);
-      for (int r = 0; r < n; ++r) y0[(size_t)c * k + r] = Y[(size_t)r * n];
-    });
-    DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
-    ops.combine(V, k, (const double*)Yb.ptr, 1, w);
-    ops.normalise(w, h, (double*)scl.ptr);
-    copy_d2d(x, w, nr * 8, s);
-    DMRGX_HIP(hipStreamSynchronize(s));
-    bool done = true;
-    for (int c = 0; c < nc; ++c) done = done && resid[c] <= tol * std::max(std::fabs(th0[c]), eps23);
-    if (done) break;
-  }
-  return th0;
-}
-
 }  // namespace dmrgx
 
 namespace dmrgx {
 
 std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, const std::vector<int64_t>& len,
                                       bool cplx, int p, double* X, double tol, int max_outer, int inner,
-                                      int* n64, int* nlo) {
+                                      int* n64, int* nlo, std::vector<double>* res_out,
+                                      std::vector<char>* conv_out) {
   ensure_init();
   hipStream_t s = rt().stream;
   const int nc = (int)len.size();
@@ -860,6 +771,7 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
   int q = 0;
   for (int j = 0; j < p; ++j) add_vector(q++, X + (size_t)j * nr);
   std::vector<char> done(nc, 0);
+  std::vector<double> rr(nc, 0.0);
   for (;;) {  // ends when every chain converged or the basis is full
     // Rayleigh-Ritz in span V: the lowest pair, u = V z, r = H u - theta u
     std::vector<double> Z((size_t)nc * q * 2, 0.0);
@@ -888,7 +800,6 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
     ops.vdots(nullptr, 0, r, h2);
     const double* rr0 = ops.fetch(h2, 1);
     bool all_done = true;
-    std::vector<double> rr(nc);
     for (int c = 0; c < nc; ++c) {
       rr[c] = std::max(rr0[(size_t)c * hs], 0.0);
       done[c] = std::sqrt(rr[c]) <= tol * std::max(std::fabs(th0[c]), eps23);
@@ -956,7 +867,40 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
   copy_d2d(X, u, vb, s);
   ops.normalise(X, h, (double*)scl.ptr);
   DMRGX_HIP(hipStreamSynchronize(s));
+  if (res_out) {
+    res_out->resize(nc);
+    for (int c = 0; c < nc; ++c) (*res_out)[c] = std::sqrt(rr[c]);
+  }
+  if (conv_out) *conv_out = done;
   return th0;
+}
+
+std::vector<double> residual_batched(const MatVec& op, const std::vector<int64_t>& len, bool cplx, const double* x,
+                                     const std::vector<double>& theta) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  const int nc = (int)len.size();
+  DMRGX_CHECK(nc > 0 && (int)theta.size() == nc, "residual_batched: one Ritz value per chain");
+  const std::vector<int64_t> off = offsets(len);
+  const int64_t nr = (off.back() + len.back()) * (cplx ? 2 : 1);
+  std::vector<int> all(nc);
+  for (int c = 0; c < nc; ++c) all[c] = c;
+  Segs sg(off, len, all, s);
+  SegOps ops(sg, cplx, nr, 2, s);
+  const int hs = ops.hs;
+  DevBuf yb((size_t)nr * 8), hb((size_t)nc * hs * 8);
+  double* y = (double*)yb.ptr;
+  double* h = (double*)hb.ptr;
+  op(x, y);
+  std::vector<double> coef((size_t)nc * hs, 0.0);
+  for (int c = 0; c < nc; ++c) coef[(size_t)c * hs] = theta[c];
+  DMRGX_HIP(hipMemcpyAsync(h, coef.data(), coef.size() * 8, hipMemcpyHostToDevice, s));
+  ops.sub(x, 1, h, y, y, nullptr);  // y = H x - theta x
+  ops.vdots(nullptr, 0, y, h);
+  const double* rr = ops.fetch(h, 1);
+  std::vector<double> out(nc);
+  for (int c = 0; c < nc; ++c) out[c] = std::sqrt(std::max(rr[(size_t)c * hs], 0.0));
+  return out;
 }
 
 }  // namespace dmrgx

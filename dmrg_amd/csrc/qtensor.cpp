@@ -877,14 +877,20 @@ vector<double> svd_finish(const vector<vector<int>>& dim, const SvdJob<T>& j, co
     scols[i] = cols[i];
   }
   if (cutoff != 0) {
+    // zero-weight padding up to min(dimension, cutoff) columns of U / rows of
+    // V.  Each block's bound is the leg dimension of the sector numbered like
+    // the block, comb.dim_[.][i] (src/qtensor.cpp:1225,1233), not the block's
+    // own sector; the box copies below zero-pad a U wider than its rows.
+    DMRGX_CHECK(nsym <= (int)comb.dim_[0].size() && nsym <= (int)comb.dim_[1].size(),
+                "svd: more blocks than sectors in the padding");
     int diff = (int)std::min<int64_t>(comb.dimension(0), cutoff) - keep;
     while (diff > 0)
       for (int i = 0; i < nsym; ++i)
-        if (u_num[i] < comb.dim_[0][comb.sym_[i][0]] && diff > 0) { u_num[i] += 1; --diff; }
+        if (u_num[i] < comb.dim_[0][i] && diff > 0) { u_num[i] += 1; --diff; }
     diff = (int)std::min<int64_t>(comb.dimension(1), cutoff) - keep;
     while (diff > 0)
       for (int i = 0; i < nsym; ++i)
-        if (v_num[i] < comb.dim_[1][comb.sym_[i][1]] && diff > 0) { v_num[i] += 1; --diff; }
+        if (v_num[i] < comb.dim_[1][i] && diff > 0) { v_num[i] += 1; --diff; }
     for (int i = 0; i < nsym; ++i) {
       ucols[i] = u_num[i];
       U.dim_[1][U.sym_[i][1]] = u_num[i];

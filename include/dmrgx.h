@@ -99,14 +99,26 @@ int dmrgx_heff_time(dmrgx_heff* plan, const void* in, void* out, int iters, doub
  * thick-restart Lanczos, full reorthogonalisation, ARPACK convergence test
  * |beta*y_last| <= tol*max(|theta|, eps^(2/3)).  evec (device, n_psi elements)
  * receives the vector; v0 (device, may be NULL) is the start vector, else a
- * vector seeded by `seed`.  When `refine` is non-NULL (an FP64 plan of the same
- * operator) and `plan` is a reduced-precision plan, the lowest pair is polished
- * with `refine_k` FP64 Krylov steps (Rayleigh-Ritz) and evals[0] comes from it.
- * info[0] = matvecs, info[1] = restarts, info[2] = converged (0/1),
- * info[3] = FP64 polish matvecs. */
+ * vector seeded by `seed`.
+ * Mixed precision: when `plan` is an FP32 / BF16X3 plan and `refine` is an FP64
+ * plan of the same operator (same psi length and element type, else an error),
+ * the Lanczos solve runs on `plan` to `tol` (e.g. 1e-4) and its lowest
+ * min(4, nev) Ritz vectors start an FP64 Jacobi-Davidson refinement: the
+ * Rayleigh-Ritz basis grows by one vector per step (one FP64 matvec), the
+ * correction from at most 16 conjugate-gradient steps on the projected
+ * correction equation with the low-precision operator, at most 12 steps, until
+ * the FP64 residual passes refine_tol*max(|theta|, eps^(2/3)) (refine_tol <= 0:
+ * 1e-10).  Starting from several Ritz vectors keeps near-degenerate lowest
+ * pairs apart.  evals[0] and evec then come from the refinement.
+ * info (6 ints, may be NULL): {Lanczos matvecs, restarts, Lanczos converged
+ * (0/1), FP64 refinement matvecs, refined pair converged (0/1; the Lanczos flag
+ * when not refined), low-precision matvecs of the refinement}.
+ * resid (may be NULL): |H u - theta u| of the returned pair, computed from
+ * FP64 matvecs of the refinement; without refinement from one extra matvec of
+ * `plan`. */
 int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
                 uint64_t seed, const void* v0, double* evals, void* evec,
-                dmrgx_heff* refine, int refine_k, int* info);
+                dmrgx_heff* refine, double refine_tol, int* info, double* resid);
 
 /* dmrgx_eigsh on many independent operators at once: `plan` maps the
  * concatenation of n_chains vectors (chain c: len[c] elements, in order) to
@@ -117,13 +129,14 @@ int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
  * and every matvec is one call of `plan` for all chains; all chains take the
  * same number of steps until the last one has converged.  Chains of at most
  * 64 elements are diagonalised densely.  evals: n_chains x nev; evec: the
- * concatenated lowest vectors; refine / refine_k as in dmrgx_eigsh (one
- * batched polish); info: n_chains x {matvecs, restarts, converged, polish
- * matvecs}.  No counterpart in the reference (one ARPACK call per chain,
- * src/dsaupd.cpp:65-155, src/znaupd.cpp:78-197). */
+ * concatenated lowest vectors; refine / refine_tol as in dmrgx_eigsh (one
+ * batched refinement); info: n_chains x 6 as in dmrgx_eigsh (the refinement
+ * counts are for the whole batch); resid: n_chains.  No counterpart in the
+ * reference (one ARPACK call per chain, src/dsaupd.cpp:65-155,
+ * src/znaupd.cpp:78-197). */
 int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int nev, int ncv, double tol,
                         int max_restart, const uint64_t* seeds, double* evals, void* evec, dmrgx_heff* refine,
-                        int refine_k, int* info);
+                        double refine_tol, int* info, double* resid);
 
 /* Work-decomposition autotuner of the H_eff plans (on by default).  Each
  * pass plan (R and L) cuts its output tiles and the concatenated K of every

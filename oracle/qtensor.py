@@ -692,6 +692,34 @@ def _resize(a: np.ndarray, k: int, n: int) -> np.ndarray:
     return out
 
 
+# Test knob (None in every restatement run): an integer seed makes tensor_svd
+# replace LAPACK's orthonormal complement -- the columns of U and rows of V
+# beyond min(rows, cols), which carry no singular value -- by another
+# orthonormal basis of the same complement (a seeded random rotation).  The
+# reference pads truncated U / V with these vectors (src/qtensor.cpp:1221-1252),
+# so a run's sensitivity to this LAPACK-specific choice can be measured.
+COMPLEMENT_ROTATION_SEED = None
+
+
+def _rotate_complement(U, V, mn, seed):
+    rng = np.random.default_rng(seed)
+
+    def unitary(n, dtype):
+        z = rng.standard_normal((n, n))
+        if np.dtype(dtype).kind == "c":
+            z = z + 1j * rng.standard_normal((n, n))
+        q, r = np.linalg.qr(z)
+        return (q * (np.diag(r) / np.abs(np.diag(r)))).astype(dtype)
+
+    if U.shape[1] > mn:
+        U = U.copy()
+        U[:, mn:] = U[:, mn:] @ unitary(U.shape[1] - mn, U.dtype)
+    if V.shape[0] > mn:
+        V = V.copy()
+        V[mn:, :] = unitary(V.shape[0] - mn, V.dtype) @ V[mn:, :]
+    return U, V
+
+
 def tensor_svd(a: np.ndarray):
     """tensor::svd(U,S,V,1) on a matrix block, src/tensor.cpp:736-785, with the
     row-major ?gesvd wrappers (src/dgesvd.cpp:32-97, src/zgesvd.cpp:36-104):
@@ -705,7 +733,10 @@ def tensor_svd(a: np.ndarray):
         return np.ones((1, 1), dtype=a.dtype), np.array([abs(x)]), np.array([[x / abs(x)]], dtype=a.dtype)
     # LAPACK sees the row-major matrix as its column-major transpose
     Up, s, VTp = scipy.linalg.svd(a.T, full_matrices=True, lapack_driver="gesvd")
-    return np.ascontiguousarray(VTp.T), s, np.ascontiguousarray(Up.T)
+    U, V = np.ascontiguousarray(VTp.T), np.ascontiguousarray(Up.T)
+    if COMPLEMENT_ROTATION_SEED is not None:
+        U, V = _rotate_complement(U, V, mn, COMPLEMENT_ROTATION_SEED + r_ * 7919 + c_)
+    return U, s, V
 
 
 def dense_eig(h: np.ndarray):

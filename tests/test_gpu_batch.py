@@ -64,7 +64,7 @@ def test_eigsh_batched_matches_dense(gpu):
     cases = _cases()
     h = _merged_plan(gpu, cases, "fp64")
     lens = [c.n_psi for c in cases]
-    ev, vec, info = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[11, 12, 13, 14])
+    ev, vec, info, res = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[11, 12, 13, 14])
     x = vec.numpy()
     off = np.cumsum([0] + lens)
     for k, c in enumerate(cases):
@@ -77,6 +77,7 @@ def test_eigsh_batched_matches_dense(gpu):
         assert abs(np.linalg.norm(xk) - 1) < 1e-12
         r = H @ xk - ev[k, 0] * xk
         assert np.linalg.norm(r) < 1e-7 * max(1.0, abs(ev[k, 0])), k
+        assert abs(res[k] - np.linalg.norm(r)) < 1e-9 * max(1.0, abs(ev[k, 0])), k
 
 
 def test_eigsh_batched_equals_single(gpu):
@@ -84,10 +85,10 @@ def test_eigsh_batched_equals_single(gpu):
     print("case eigsh_batched vs single", flush=True)
     cases = _cases()
     h = _merged_plan(gpu, cases, "fp64")
-    ev, _, _ = h.eigsh_batched([c.n_psi for c in cases], nev=6, tol=1e-10)
+    ev, _, _, _ = h.eigsh_batched([c.n_psi for c in cases], nev=6, tol=1e-10)
     for k, c in enumerate(cases):
         hk = _merged_plan(gpu, [c], "fp64")
-        e1, _, _ = hk.eigsh(nev=6, tol=1e-10)
+        e1, _, _, _ = hk.eigsh(nev=6, tol=1e-10)
         ne = min(6, c.n_psi)
         assert np.max(np.abs(ev[k, :ne] - e1[:ne])) < 1e-9 * max(1.0, abs(e1[0]))
 
@@ -98,12 +99,13 @@ def test_eigsh_batched_mixed_polish(gpu, prec):
     cases = _cases()
     lo = _merged_plan(gpu, cases, prec)
     hi = _merged_plan(gpu, cases, "fp64")
-    ev, _, info = lo.eigsh_batched([c.n_psi for c in cases], nev=6, tol=1e-6 if prec == "fp32" else 1e-4,
-                                   refine=hi, refine_k=8)
-    assert info[0, 3] > 0
+    ev, _, info, res = lo.eigsh_batched([c.n_psi for c in cases], nev=6, tol=1e-6 if prec == "fp32" else 1e-4,
+                                        refine=hi, refine_tol=1e-10)
+    assert info[0, 3] > 0 and np.all(info[:, 4] == 1)
     for k, c in enumerate(cases):
         w = scipy.linalg.eigvalsh(dense_heff(c.ctx, c.n_psi, c.dtype))
         assert abs(ev[k, 0] - w[0]) < 1e-8 * max(1.0, abs(w[0])), (k, ev[k, 0], w[0])
+        assert res[k] <= 1e-10 * max(1.0, abs(w[0])), (k, res[k])
 
 
 def test_run_batch_matches_individual_runs(gpu):
@@ -161,10 +163,10 @@ def test_eigsh_batched_graphs_equal_eager(gpu):
     lib = gpu.lib()
     try:
         lib.dmrgx_set_lanczos_graphs(0)
-        e0, v0, i0 = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[5, 6, 7, 8])
+        e0, v0, i0, _ = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[5, 6, 7, 8])
         lib.dmrgx_set_lanczos_graphs(1)
-        e1, v1, i1 = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[5, 6, 7, 8])
-        e2, v2, i2 = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[5, 6, 7, 8])
+        e1, v1, i1, _ = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[5, 6, 7, 8])
+        e2, v2, i2, _ = h.eigsh_batched(lens, nev=6, tol=1e-10, seeds=[5, 6, 7, 8])
     finally:
         lib.dmrgx_set_lanczos_graphs(1)
     assert i0[:, 1].max() > 0  # restart cycles happened: graphs were replayed

@@ -67,17 +67,39 @@ def test_xyz_matches_oracle_step_by_step(gpu):
     assert abs(res.energy - e) < 1e-8
 
 
-def test_xyz_reference_sign_rule_final_energy(gpu):
-    """The reference's unconditional sign rule on the bosonic chain: the
-    schedule pads U/V with complement vectors whenever fewer singular values
-    than the cutoff exist (src/qtensor.cpp:1221-1252); those vectors are
-    LAPACK-specific, so steps right after such a padding may differ.  The
-    converged final energy is fixed by the operator and must agree."""
+def _oracle_steps(model, L, m, sweeps, sign_rule, params, rotation_seed=None):
+    from oracle import qtensor as qt
+    old = qt.COMPLEMENT_ROTATION_SEED
+    qt.COMPLEMENT_ROTATION_SEED = rotation_seed
+    try:
+        e, R = _oracle(model, L, m, sweeps, "dmrg2", sign_rule, params)
+    finally:
+        qt.COMPLEMENT_ROTATION_SEED = old
+    return e, np.array([o.energies[0] for o in R.log]), R
+
+
+def test_xyz_reference_sign_rule_step_by_step(gpu):
+    """The reference's unconditional sign rule on the bosonic chain, step by
+    step.  When fewer singular values than the cutoff exist, the schedule pads
+    U / V with orthonormal complement vectors (src/qtensor.cpp:1221-1252) that
+    carry no weight and depend on the SVD routine (LAPACK's here, a Householder
+    complement on the GPU).  The oracle itself, rerun with two other orthonormal
+    complements, marks the steps whose energy depends on that choice; every
+    other step and the final energy must agree with the oracle to 1e-8."""
     L, m = 10, 16
     print("case xyz oracle-vs-gpu reference rule", flush=True)
     res = driver.run_dmrg("xyz", L, m, 1, XYZ, schedule="dmrg2", sign_rule="reference")
-    e, R = _oracle("xyz", L, m, 1, "dmrg2", "reference", XYZ)
+    e, ref, R = _oracle_steps("xyz", L, m, 1, "reference", XYZ)
+    _, alt1, _ = _oracle_steps("xyz", L, m, 1, "reference", XYZ, rotation_seed=5)
+    _, alt2, _ = _oracle_steps("xyz", L, m, 1, "reference", XYZ, rotation_seed=6)
+    sensitive = (np.abs(alt1 - ref) > 1e-9) | (np.abs(alt2 - ref) > 1e-9)
+    assert sensitive.any()  # the reference rule makes these energies LAPACK-specific
     assert len(R.log) == len(res.steps)
+    gpu_e = res.steps[:, 2]
+    for k in range(len(ref)):
+        assert (int(res.steps[k, 0]), int(res.steps[k, 1])) == (R.log[k].l, R.log[k].r)
+        if not sensitive[k]:
+            assert abs(gpu_e[k] - ref[k]) < 1e-8 * max(1, abs(ref[k])), (k, gpu_e[k], ref[k])
     assert abs(res.energy - e) < 1e-8, (res.energy, e)
     assert abs(e - exact.xyz_e0(L, *XYZ)[0]) > 1e-3  # not the physical ground state
 

@@ -157,3 +157,33 @@ def test_oracle_calc_fermionic_norm():
     R.dmrg2(16, 2)
     st = prepared(R.mps)
     assert abs(calc_overlap(st, st, True) - 1) < 1e-10
+
+
+def test_reference_rule_depends_on_svd_complement():
+    """Under the reference's sign rule the energies of some two-site steps of
+    an open XYZ chain depend on which orthonormal complement the SVD pads U / V
+    with (src/qtensor.cpp:1221-1252): another valid complement moves them,
+    while the converged final energy stays.  Under the physical rule the same
+    change moves no step by more than rounding."""
+    from oracle import qtensor as qt
+    from oracle.dmrg import Runner
+
+    def run(rule, seed):
+        old = qt.COMPLEMENT_ROTATION_SEED
+        qt.COMPLEMENT_ROTATION_SEED = seed
+        try:
+            mpo, n, ferm = models.make_model("xyz", 10, jx=1.0, jy=0.8, jz=0.5)
+            R = Runner(mpo, n, ferm, sign_rule=rule)
+            e = R.dmrg2(16, 1)
+        finally:
+            qt.COMPLEMENT_ROTATION_SEED = old
+        return e, np.array([o.energies[0] for o in R.log])
+
+    e0, a = run("reference", None)
+    e1, b = run("reference", 5)
+    assert np.max(np.abs(a - b)) > 1e-6
+    assert abs(e0 - e1) < 1e-10
+    p0, c = run("physical", None)
+    p1, d = run("physical", 5)
+    assert np.max(np.abs(c - d)) < 1e-11
+    assert abs(p0 - p1) < 1e-11
