@@ -29,6 +29,7 @@ EXPORTS = [
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
     "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_eigsh_batched", "dmrgx_set_lanczos_graphs", "dmrgx_set_autotune", "dmrgx_autotune_keys", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time", "dmrgx_heff_set_split", "dmrgx_heff_get_split",
+    "dmrgx_task_gemm_noalloc",
 ]
 
 _lib = None
@@ -74,6 +75,8 @@ def _declare(lib):
         "dmrgx_stream": ([], vp),
         "dmrgx_task_gemm": ([C.c_int, C.c_int, vp, i64p, i64p, C.c_int, i32p, C.c_int, vp, vp, C.c_int64],
                             C.c_int),
+        "dmrgx_task_gemm_noalloc": ([C.c_int, vp, i64p, i64p, C.c_int, i32p, C.c_int, vp, vp, C.c_int64],
+                                    C.c_int),
         "dmrgx_heff_create": ([C.POINTER(vp), C.c_int, C.c_int, vp, i64p, i64p, C.c_int, i32p, C.c_int,
                                vp, i64p, i64p, C.c_int, i32p, C.c_int, C.c_int64, C.c_int64], C.c_int),
         "dmrgx_heff_apply": ([vp, vp, vp], C.c_int),

@@ -75,6 +75,11 @@ void copy_d2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 // stream: up to 1 MiB is first copied into a pinned ring (src may be reused at
 // once), larger copies go straight.
 void upload(void* dst, const void* src, size_t bytes, hipStream_t s);
+// The pinned staging ring of upload(), allocated by dmrgx_init.
+void upload_ring_init();
+// Device address of the no-allocation entry point's scratch array on
+// `device` (current), looked up once by dmrgx_init.
+void noalloc_setup(int device);
 
 inline size_t elem_bytes(int dtype) { return dtype == DT_C64 ? 16 : 8; }
 

@@ -25,14 +25,6 @@
 
 namespace dmrgx {
 
-GemmPlan::~GemmPlan() {
-  delete d_items;
-  delete d_segs;
-  delete d_ws;
-  delete d_red;
-  delete d_red_start;
-}
-
 namespace {
 
 constexpr int BM = 64, BN = 64, NT = 256;
@@ -731,12 +723,12 @@ __global__ void k_f32_to_f64(const float* __restrict__ s, double* __restrict__ d
 
 template <typename S, bool CPLX>
 void launch4(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, int per) {
-  auto it = (const GemmItem*)p.d_items->ptr;
-  auto sg = (const GemmSeg*)p.d_segs->ptr;
+  auto it = p.items;
+  auto sg = p.segs;
   const bool akc = (p.ta == 'N' || p.ta == 'n'), bkc = (p.tb == 'T' || p.tb == 't');
   dim3 g(p.grid), b(NT);
 #define L4(AK, BKc) hipLaunchKernelGGL((k_gemm4<S, CPLX, AK, BKc>), g, b, 0, s, it, p.n_items, per, sg, \
-                                       (const S*)A, (const S*)B, (S*)C, (S*)(p.d_ws ? p.d_ws->ptr : nullptr))
+                                       (const S*)A, (const S*)B, (S*)C, (S*)p.ws)
   if (akc && bkc) L4(true, true);
   else if (akc) L4(true, false);
   else if (bkc) L4(false, true);
@@ -746,13 +738,13 @@ void launch4(const GemmPlan& p, const void* A, const void* B, void* C, hipStream
 
 template <bool CPLX>
 void launch_bf(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, int per) {
-  auto it = (const GemmItem*)p.d_items->ptr;
-  auto sg = (const GemmSeg*)p.d_segs->ptr;
+  auto it = p.items;
+  auto sg = p.segs;
   const bool akc = (p.ta == 'N' || p.ta == 'n'), bkc = (p.tb == 'T' || p.tb == 't');
   dim3 g(p.grid), b(NT);
 #define LB(AK, BKc) hipLaunchKernelGGL((k_gemm_bf16x3<CPLX, AK, BKc>), g, b, 0, s, it, p.n_items, per, sg, \
                                        (const float*)A, (const float*)B, (float*)C, \
-                                       (float*)(p.d_ws ? p.d_ws->ptr : nullptr))
+                                       (float*)p.ws)
   if (akc && bkc) LB(true, true);
   else if (akc) LB(true, false);
   else if (bkc) LB(false, true);
@@ -765,8 +757,8 @@ void launch_bf(const GemmPlan& p, const void* A, const void* B, void* C, hipStre
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
-                                         int split_target, bool planes) {
-  ensure_init();
+                                         int split_target, bool planes, GemmTables* host_only) {
+  if (!host_only) ensure_init();
   auto plan = std::make_unique<GemmPlan>();
   plan->planes = planes;
   plan->tile = planes ? 128 : 64;
@@ -970,44 +962,55 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
   plan->n_segs = (int)segs.size();
   int per = (plan->n_items + 7) / 8;
   plan->grid = per * 8;
+  std::vector<int64_t> start(red.size());
+  int64_t acc = 0;
+  for (size_t i = 0; i < red.size(); ++i) {
+    start[i] = acc;
+    acc += (red[i].len + 255) / 256;
+  }
+  plan->n_red = (int)red.size();
+  plan->red_blocks = (int)acc;
+  if (host_only) {
+    host_only->items = std::move(items);
+    host_only->segs = std::move(segs);
+    host_only->red = std::move(red);
+    host_only->red_start = std::move(start);
+    return plan;
+  }
+  hipStream_t s = rt().stream;
   if (plan->n_items) {
-    plan->d_items = new DevBuf(items.size() * sizeof(GemmItem));
-    plan->d_segs = new DevBuf(std::max<size_t>(segs.size(), 1) * sizeof(GemmSeg));
-    upload(plan->d_items->ptr, items.data(), items.size() * sizeof(GemmItem), rt().stream);
-    if (!segs.empty())
-      upload(plan->d_segs->ptr, segs.data(), segs.size() * sizeof(GemmSeg), rt().stream);
+    plan->own_items = std::make_unique<DevBuf>(items.size() * sizeof(GemmItem));
+    plan->own_segs = std::make_unique<DevBuf>(std::max<size_t>(segs.size(), 1) * sizeof(GemmSeg));
+    upload(plan->own_items->ptr, items.data(), items.size() * sizeof(GemmItem), s);
+    if (!segs.empty()) upload(plan->own_segs->ptr, segs.data(), segs.size() * sizeof(GemmSeg), s);
+    plan->items = (const GemmItem*)plan->own_items->ptr;
+    plan->segs = (const GemmSeg*)plan->own_segs->ptr;
   }
   if (!red.empty()) {
-    const size_t eb = (prec == PREC_FP64 ? 8 : 4) * (dtype == DT_C64 ? 2 : 1);
-    plan->d_ws = new DevBuf(plan->ws_elems * eb);
-    std::vector<int64_t> start(red.size());
-    int64_t acc = 0;
-    for (size_t i = 0; i < red.size(); ++i) {
-      start[i] = acc;
-      acc += (red[i].len + 255) / 256;
-    }
-    plan->n_red = (int)red.size();
-    plan->red_blocks = (int)acc;
-    plan->d_red = new DevBuf(red.size() * sizeof(RedTask));
-    plan->d_red_start = new DevBuf(red.size() * 8);
-    upload(plan->d_red->ptr, red.data(), red.size() * sizeof(RedTask), rt().stream);
-    upload(plan->d_red_start->ptr, start.data(), start.size() * 8, rt().stream);
+    plan->own_ws = std::make_unique<DevBuf>(gemm_ws_bytes(*plan));
+    plan->own_red = std::make_unique<DevBuf>(red.size() * sizeof(RedTask));
+    plan->own_red_start = std::make_unique<DevBuf>(red.size() * 8);
+    upload(plan->own_red->ptr, red.data(), red.size() * sizeof(RedTask), s);
+    upload(plan->own_red_start->ptr, start.data(), start.size() * 8, s);
+    plan->ws = plan->own_ws->ptr;
+    plan->red = (const RedTask*)plan->own_red->ptr;
+    plan->red_start = (const int64_t*)plan->own_red_start->ptr;
   }
   return plan;
 }
 
 void reduce_partials(const GemmPlan& p, void* C, hipStream_t s) {
   if (!p.n_red) return;
-  auto t = (const RedTask*)p.d_red->ptr;
-  auto st = (const int64_t*)p.d_red_start->ptr;
+  auto t = p.red;
+  auto st = p.red_start;
   dim3 g(p.red_blocks), b(256);
   const bool cx = p.dtype == DT_C64;
   if (p.prec == PREC_FP64) {
-    if (cx) hipLaunchKernelGGL((k_reduce_ws<double, true>), g, b, 0, s, (const double*)p.d_ws->ptr, (double*)C, t, p.n_red, st);
-    else hipLaunchKernelGGL((k_reduce_ws<double, false>), g, b, 0, s, (const double*)p.d_ws->ptr, (double*)C, t, p.n_red, st);
+    if (cx) hipLaunchKernelGGL((k_reduce_ws<double, true>), g, b, 0, s, (const double*)p.ws, (double*)C, t, p.n_red, st);
+    else hipLaunchKernelGGL((k_reduce_ws<double, false>), g, b, 0, s, (const double*)p.ws, (double*)C, t, p.n_red, st);
   } else {
-    if (cx) hipLaunchKernelGGL((k_reduce_ws<float, true>), g, b, 0, s, (const float*)p.d_ws->ptr, (float*)C, t, p.n_red, st);
-    else hipLaunchKernelGGL((k_reduce_ws<float, false>), g, b, 0, s, (const float*)p.d_ws->ptr, (float*)C, t, p.n_red, st);
+    if (cx) hipLaunchKernelGGL((k_reduce_ws<float, true>), g, b, 0, s, (const float*)p.ws, (float*)C, t, p.n_red, st);
+    else hipLaunchKernelGGL((k_reduce_ws<float, false>), g, b, 0, s, (const float*)p.ws, (float*)C, t, p.n_red, st);
   }
   DMRGX_HIP(hipGetLastError());
 }
@@ -1026,11 +1029,11 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
     else fill_zero((char*)C + z.first * (out == OUT_F64 ? 8 : 4) * cw, (size_t)z.second * (out == OUT_F64 ? 8 : 4) * cw, s);
   }
   if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
-  float* wsp = (float*)(p.d_ws ? p.d_ws->ptr : nullptr);
+  float* wsp = (float*)p.ws;
   if (p.n_items) {
     const int per = p.grid / 8;
-    auto it = (const GemmItem*)p.d_items->ptr;
-    auto sg = (const GemmSeg*)p.d_segs->ptr;
+    auto it = p.items;
+    auto sg = p.segs;
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(p.grid), dim3(PNT), 0, s, it, p.n_items, per, sg, (const unsigned short*)A,
                          a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp);
@@ -1050,10 +1053,10 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
   if (out == OUT_F32) {
     reduce_partials(p, C, s);
   } else if (p.n_red) {
-    auto t = (const RedTask*)p.d_red->ptr;
-    auto st = (const int64_t*)p.d_red_start->ptr;
+    auto t = p.red;
+    auto st = p.red_start;
     auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(p.red_blocks), dim3(256), 0, s, (const float*)p.d_ws->ptr, C, c_ps, t, p.n_red,
+      hipLaunchKernelGGL(kern, dim3(p.red_blocks), dim3(256), 0, s, (const float*)p.ws, C, c_ps, t, p.n_red,
                          st);
     };
     if (p.dtype == DT_C64) {

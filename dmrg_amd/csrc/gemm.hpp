@@ -51,6 +51,14 @@ struct RedTask {
 // (re_hi, re_lo[, im_hi, im_lo], stride c_ps; outputs may not accumulate), or FP64.
 enum { OUT_F32 = 0, OUT_PLANES = 1, OUT_F64 = 2 };
 
+// The work decomposition of a plan as host arrays (what the plan uploads).
+struct GemmTables {
+  std::vector<GemmItem> items;
+  std::vector<GemmSeg> segs;
+  std::vector<RedTask> red;
+  std::vector<int64_t> red_start;  // first reduction block of each RedTask
+};
+
 struct GemmPlan {
   int dtype = DT_R64;
   int prec = PREC_FP64;
@@ -58,27 +66,38 @@ struct GemmPlan {
   int tile = 64;    // output tile edge (64, or 128 for the pre-split BF16x3 kernel)
   bool planes = false;  // operands are pre-split bfloat16 planes
   int n_items = 0, n_segs = 0, grid = 0;
-  DevBuf* d_items = nullptr;
-  DevBuf* d_segs = nullptr;
   std::vector<std::pair<int64_t, int64_t>> zero_ranges;  // [off, len) in C elements
   int64_t ws_elems = 0;                 // split-K workspace, C elements
-  DevBuf* d_ws = nullptr;
   int n_red = 0, red_blocks = 0;
-  DevBuf* d_red = nullptr;              // RedTask array
-  DevBuf* d_red_start = nullptr;        // first block of each task
+  // device addresses the kernels read: the work items, their K segments, the
+  // split-K workspace, the reduction tasks and their first blocks; they point
+  // into the owned buffers below, or into caller-provided memory for a plan
+  // built with host tables only
+  const GemmItem* items = nullptr;
+  const GemmSeg* segs = nullptr;
+  void* ws = nullptr;
+  const RedTask* red = nullptr;
+  const int64_t* red_start = nullptr;
+  std::unique_ptr<DevBuf> own_items, own_segs, own_ws, own_red, own_red_start;
   double flops = 0;        // algorithmic real flops (c*M*N*K summed)
   bool any_beta = false;   // some output block accumulates onto existing C
-  ~GemmPlan();
 };
 
 // Build a plan; `extra_zero` lists C ranges with no task that must read as 0.
 // split_target > 0: cut K so that the launch has about that many work items;
 // 0: two per CU; < 0: balanced over -split_target resident work-groups (whole
 // output blocks cut into two K halves where that shortens the launch).
+// With `host_only` the plan allocates and uploads nothing: its decomposition
+// is returned in *host_only and the caller sets the device addresses.
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
-                                         int split_target = 0, bool planes = false);
+                                         int split_target = 0, bool planes = false,
+                                         GemmTables* host_only = nullptr);
+// Bytes of the workspace of a plan (split-K partial tiles).
+inline size_t gemm_ws_bytes(const GemmPlan& p) {
+  return (size_t)p.ws_elems * (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);
+}
 
 // Execute: A, B, C are device pointers with storage precision of the plan
 // (double for PREC_FP64, float for PREC_FP32 / PREC_BF16X3; complex

@@ -138,6 +138,10 @@ char* g_up_ring = nullptr;
 size_t g_up_head = 0;
 }  // namespace
 
+void upload_ring_init() {
+  if (!g_up_ring) DMRGX_HIP(hipHostMalloc((void**)&g_up_ring, UP_RING, hipHostMallocDefault));
+}
+
 void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
   if (!bytes) return;
   if (bytes > UP_MAX) {
@@ -145,7 +149,7 @@ void upload(void* dst, const void* src, size_t bytes, hipStream_t s) {
     return;
   }
   std::lock_guard<std::mutex> lk(g_up_mu);
-  if (!g_up_ring) DMRGX_HIP(hipHostMalloc((void**)&g_up_ring, UP_RING, hipHostMallocDefault));
+  if (!g_up_ring) upload_ring_init();
   size_t a = (g_up_head + 255) & ~(size_t)255;
   if (a + bytes > UP_RING) {
     // the next lap overwrites staged bytes: every copy that reads them must
@@ -192,6 +196,11 @@ int dmrgx_init(int device) {
   DMRGX_HIP(hipGetDeviceProperties(&prop, device));
   g_rt.num_cu = prop.multiProcessorCount;
   DMRGX_HIP(hipStreamCreateWithFlags(&g_rt.stream, hipStreamNonBlocking));
+  {
+    std::lock_guard<std::mutex> lk(g_up_mu);
+    upload_ring_init();
+  }
+  noalloc_setup(device);
   g_rt.device = device;
   g_rt.ready = true;
   CAPI_END

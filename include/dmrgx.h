@@ -58,6 +58,21 @@ int dmrgx_task_gemm(int dtype, int prec, const void* A, const int64_t* blk_off,
                     const int64_t* blk_size, int n_blk, const int32_t* map, int n_task,
                     const void* in, void* out, int64_t n_out);
 
+/* dmrgx_task_gemm for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  FP64 arithmetic; same arguments and
+ * results as dmrgx_task_gemm with prec = DMRGX_FP64 (outputs without a task
+ * are left untouched).  It allocates nothing: the work decomposition and the
+ * split-K partial tiles go to a fixed 64 MiB device array per device, set up
+ * by dmrgx_init (call it first: initialisation allocates).  The split-K
+ * decomposition is the default one when it fits, else one with fewer K cuts;
+ * a task list whose tables alone exceed 64 MiB is an error.  Every call on a
+ * device shares that array, so calls that overlap on one device (several host
+ * threads) race; calls return with their work complete. */
+int dmrgx_task_gemm_noalloc(int dtype, const void* A, const int64_t* blk_off,
+                            const int64_t* blk_size, int n_blk, const int32_t* map, int n_task,
+                            const void* in, void* out, int64_t n_out);
+
 /* ---- effective Hamiltonian ------------------------------------------------
  * Replaces av(in, out, lanczos<T>&) (src/core.cpp:22-31): store = R * in over
  * rmap (r_num contracted legs), out = L * store over lmap.  renv / lenv are
