@@ -1,22 +1,33 @@
 #!/usr/bin/env python3
-"""Benchmark of the DMRG two-site H_eff matvec (and optionally full sweeps)
-on one MI355X.
+"""Benchmark of the DMRG two-site H_eff matvec and of full DMRG sweeps on one
+MI355X.
 
-Workload: the XYZ spin-1/2 chain at L=100, m=512 (Z2 sectors of 256, MPO bond
-D=5, complex): one step = one H_eff matvec over the exact block structure of
-that step (32 R-pass + 32 L-pass tasks of (1280,256,256) / (256,256,1280),
-4.29e10 algorithmic flop), synthetic uniform(-1,1) environments built by the
-library on the device.  The mixed-precision path (BF16x3 products, FP32
-accumulate) is the headline value; FP64 and FP32 run on the same shapes.
+Workload (the JSON line's value): the XYZ spin-1/2 chain at L=100, m=512 (Z2
+sectors of 256, MPO bond D=5, complex): one step = one H_eff matvec over the
+exact block structure of that step (32 R-pass + 32 L-pass tasks of
+(1280,256,256) / (256,256,1280), 4.29e10 algorithmic flop), synthetic
+uniform(-1,1) environments built by the library on the device.  The
+mixed-precision path (BF16x3 products, FP32 accumulate) is the headline value;
+FP64 and FP32 run on the same shapes.
 
-Prints one JSON line.  Its ``sweep`` object is a full two-site DMRG of the
-same model at m=1024 on a shorter chain (``--sweep-L``, default 24; 2 sweeps,
-the last one timed; 0 sweeps skips it), FP64 against BF16x3 with the FP64
-refinement: sweep seconds, E0 of both and their difference, fidelity.  Its ``batch`` object is the 64-chain
-batched workload (open XYZ chains L=32, m=128, couplings U[0.5, 1.5) from a
-seeded splitmix64 stream) swept in lockstep, FP64 and BF16x3, beside the
-same chains run one at a time.  ``--part svd`` / ``--part copy`` time the
-step SVD alone / the 256 MiB copy used to calibrate the HBM byte counters.
+Beside it, in the same JSON line:
+* ``sweep``: full two-site DMRG of that chain (XYZ L=100, m=512, the
+  reference's dmrg2 schedule: growth at cutoff 10, then sweeps at m; the
+  cutoff ladder is off unless --ladder), FP64 against BF16x3 with the FP64
+  refinement: seconds of the last full sweep (steady state at m) and of every
+  sweep, E0 of both and |dE0|, truncation error, fidelity of the final states.
+  --sweep-model potts3 --sweep-L 64 --sweep-m 1024 runs the chiral Z3 Potts
+  chain instead.
+* ``batch`` (--batch N): the 64-chain batched workload (open XYZ chains L=32, m=128,
+  couplings U[0.5, 1.5) from a seeded splitmix64 stream) swept in lockstep,
+  FP64 and BF16x3, beside the same chains run one at a time.
+* ``cpu_baseline``: the oracle's av() (numpy, one OpenBLAS GEMM per get_map
+  task) at the same m=512 shapes on the host, all threads and one thread,
+  with the host CPU model; ``cpu_sweep`` (--cpu-sweep): the Kitaev chain
+  L=40, m=64, 5 two-site sweeps (Z2, real FP64) run by the oracle on the host
+  and by the FP64 GPU path.
+``--part svd`` / ``--part copy`` time the step SVD alone / the 256 MiB copy
+used to calibrate the HBM byte counters.
 """
 from __future__ import annotations
 
@@ -62,6 +73,7 @@ ALG_BYTES = (5 * 1024 * 1024 + 524288) * 8 + 5 * 1024 * 1024 * 8
 S, D, M = 2, 5, 512
 L_CHAIN = 100
 JXYZ = (1.0, 0.8, 0.5)
+POTTS = (1.0, 1.0, 0.0, 3.14159265359 / 8)  # f, j, p, theta of H_Potts3 (the reference's truncated pi)
 
 
 def gpu_leg(steps, warmup, only=None):
@@ -185,53 +197,114 @@ def svd_leg(reps=5):
             "max_sv_err": float(np.max(np.abs(s - ref)))}
 
 
+def cpu_info():
+    """Host CPU model and logical CPU count (from /proc/cpuinfo)."""
+    model, n = "unknown", os.cpu_count() or 1
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        pass
+    return model, n
+
+
 def cpu_leg(seconds):
     """The oracle's av() (numpy, one BLAS GEMM per get_map task) on the same
-    shapes, timed on this host's cores for about `seconds`."""
+    shapes, timed on this host's cores for about `seconds`, with all BLAS
+    threads and with one."""
     from oracle.synth import synthetic_case
+    from threadpoolctl import threadpool_info, threadpool_limits
 
-    try:
-        from threadpoolctl import threadpool_info
-        threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [os.cpu_count() or 1])
-    except Exception:
-        threads = os.cpu_count() or 1
+    threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [os.cpu_count() or 1])
+    model, ncpu = cpu_info()
     case = synthetic_case(S, D, M, dtype=np.complex128)
     rng = np.random.default_rng(2)
     x = rng.uniform(-1, 1, case.n_psi) + 1j * rng.uniform(-1, 1, case.n_psi)
-    case.oracle_apply(x)  # warm
-    n, t0 = 0, time.perf_counter()
-    while True:
-        case.oracle_apply(x)
-        n += 1
-        el = time.perf_counter() - t0
-        if el > seconds or n >= 200:
-            break
     flops = case.flops()
-    return {"value": flops * n / el / 1e12, "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
-            "sample": f"{n} H_eff matvecs of the oracle's av() (numpy/OpenBLAS per-task zgemm) at the "
-                      f"same m=512 Z2 D=5 shapes, {el:.1f} s"}
+
+    def rate(limit):
+        case.oracle_apply(x)  # warm
+        n, t0 = 0, time.perf_counter()
+        while True:
+            case.oracle_apply(x)
+            n += 1
+            el = time.perf_counter() - t0
+            if el > limit or n >= 200:
+                return flops * n / el / 1e12, n, el
+
+    all_r, all_n, all_el = rate(seconds)
+    with threadpool_limits(limits=1):
+        one_r, one_n, one_el = rate(seconds / 2)
+    print(f"[bench] cpu av(): {all_r:.3f} TFLOP/s on {threads} threads, {one_r:.4f} on 1 ({model})", file=sys.stderr,
+          flush=True)
+    return {"value": all_r, "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
+            "sample": f"{all_n} H_eff matvecs of the oracle's av() (numpy/OpenBLAS per-task zgemm) at the "
+                      f"same m=512 Z2 D=5 shapes, {all_el:.1f} s",
+            "one_core": {"value": one_r, "matvecs": one_n, "seconds": round(one_el, 2)},
+            "cpu_model": model, "host_cpus": ncpu}
 
 
-def sweep_leg(L, sweeps, m=1024):
-    """Full two-site DMRG (the reference's dmrg2 schedule: growth at cutoff 10,
-    then full sweeps at m; the pre-sweep cutoff ladder is skipped) of the XYZ
-    chain at m=1024, FP64 against BF16x3 with the FP64 Jacobi-Davidson
-    refinement; sweep_s is the wall time of the last full sweep (steady state
-    at m); fidelity is |<psi_bf16x3|psi_fp64>| of the final states."""
+def cpu_sweep_leg(L=40, m=64, sweeps=5):
+    """The Kitaev chain (t = Delta = 1, mu = 0.5; the reference's t=-1 p=-1
+    u=0.25), open, L=40, Z2 parity, real FP64, two-site DMRG at m=64 with 5
+    sweeps: the oracle on the host (numpy/scipy: OpenBLAS GEMMs, ARPACK,
+    LAPACK gesvd) and the FP64 GPU path, both checked against the BdG closed
+    form.  The reference's own timing sites are kitaev.cpp:19-20,56-57."""
     from dmrg_amd import driver
-    res = {"L": L, "m": m, "sweeps": sweeps, "timed": "last full sweep"}
+    from oracle import exact
+    from oracle.dmrg import Runner
+    from oracle.models import make_model
+    from threadpoolctl import threadpool_info
+
+    p = (-1.0, -1.0, 0.25)
+    e_bdg = exact.kitaev_bdg_e0(L, *p)[0]
+    t0 = time.perf_counter()
+    g = driver.run_dmrg("kitaev", L, m, sweeps, p, schedule="dmrg2")
+    gpu_s = time.perf_counter() - t0
+    mpo, n, ferm = make_model("kitaev", L, t=p[0], p=p[1], u=p[2])
+    R = Runner(mpo, n, ferm)
+    t0 = time.perf_counter()
+    e_cpu = R.dmrg2(m, sweeps)
+    cpu_s = time.perf_counter() - t0
+    threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
+    print(f"[bench] kitaev L={L} m={m} {sweeps} sweeps: GPU {gpu_s:.2f} s, CPU oracle {cpu_s:.1f} s", file=sys.stderr,
+          flush=True)
+    return {"workload": f"Kitaev L={L} m={m}, {sweeps} two-site sweeps (Z2, real FP64)", "steps": len(R.log),
+            "gpu_fp64_s": round(gpu_s, 3), "cpu_s": round(cpu_s, 3), "cpu_threads": int(threads), "kind": "port",
+            "gpu_E0": g.energy, "cpu_E0": e_cpu, "bdg_E0": e_bdg,
+            "gpu_err": abs(g.energy - e_bdg), "cpu_err": abs(e_cpu - e_bdg)}
+
+
+def sweep_leg(model, L, m, sweeps, ladder):
+    """Full two-site DMRG (the reference's dmrg2 schedule: growth at cutoff 10,
+    the optional pre-sweep cutoff ladder, then full sweeps at m), FP64 against
+    BF16x3 with the FP64 Jacobi-Davidson refinement; sweep_s is the wall time
+    of the last full sweep (steady state at m), sweeps_s every full sweep;
+    fidelity is |<psi_bf16x3|psi_fp64>| of the final states."""
+    from dmrg_amd import driver
+    params = {"xyz": JXYZ, "potts3": POTTS}[model]
+    res = {"model": model, "L": L, "m": m, "sweeps": sweeps, "ladder": bool(ladder), "params": list(params),
+           "timed": "last full sweep"}
     states = {}
     for prec in ("fp64", "bf16x3"):
         t0 = time.perf_counter()
-        r = driver.run_dmrg("xyz", L, m, sweeps, JXYZ, schedule="dmrg2", prec=prec, tol_lo=1e-4, refine_k=8,
-                            ladder=False, keep_state=True)
+        r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=1e-4, refine_k=8,
+                            ladder=ladder, keep_state=True, verbose="stderr")
         states[prec] = r.state
         st = r.stats
+        per = sweep_seconds(r.steps, L, sweeps)
         res[prec] = dict(energy=r.energy, sweep_s=round(st["last_sweep_seconds"], 4),
-                         total_s=round(time.perf_counter() - t0, 3), matvecs=st["matvecs"],
-                         polish=st["polish_matvecs"], trunc_err=st["trunc_err"], t_eig=round(st["t_eig"], 3),
-                         t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3))
-        print(f"[bench] sweep {prec} L={L} m={m}: E0={r.energy:.12f} last sweep {st['last_sweep_seconds']:.2f} s "
+                         sweeps_s=[round(x, 3) for x in per], total_s=round(time.perf_counter() - t0, 3),
+                         matvecs=st["matvecs"], polish=st["polish_matvecs"], trunc_err=st["trunc_err"],
+                         t_eig=round(st["t_eig"], 3), t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3))
+        print(f"[bench] sweep {prec} {model} L={L} m={m}: E0={r.energy:.12f} last sweep {st['last_sweep_seconds']:.2f} s "
               f"(eig {st['t_eig']:.2f} svd {st['t_svd']:.2f} env {st['t_env']:.2f} s in all steps)", file=sys.stderr,
               flush=True)
     res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
@@ -241,6 +314,20 @@ def sweep_leg(L, sweeps, m=1024):
         st.close()
     res["speedup"] = round(res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12), 3)
     return res
+
+
+def sweep_seconds(steps, L, sweeps):
+    """Wall seconds of each full sweep of the dmrg2 schedule: the last
+    sweeps * 2 (L - 4) steps before the final half sweep (L - 3 - L/2 + 1
+    steps); column 12 of the step log is the step's wall time."""
+    per, fin = 2 * (L - 4), (L - 3) - (L // 2) + 1
+    end = len(steps) - max(fin, 0)
+    out = []
+    for k in range(sweeps):
+        a = end - per * (sweeps - k)
+        if a >= 0:
+            out.append(float(np.sum(steps[a:a + per, 12])))
+    return out
 
 
 def batch_leg(n_chains=64, L=32, m=128, sweeps=1):
@@ -307,15 +394,21 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-configs", action="store_true", help="skip the Potts m=1024 / XYZ m=2048 matvec shapes")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the host matvec baseline")
     ap.add_argument("--sweep", type=int, default=2, help="full sweeps in the sweep comparison (0 skips it)")
-    ap.add_argument("--sweep-L", type=int, default=24, help="chain length of the sweep comparison at m=1024")
-    ap.add_argument("--batch", type=int, default=2, help="sweeps of the 64-chain batched workload (0 skips it)")
+    ap.add_argument("--sweep-model", default="xyz", choices=["xyz", "potts3"])
+    ap.add_argument("--sweep-L", type=int, default=L_CHAIN, help="chain length of the sweep comparison")
+    ap.add_argument("--sweep-m", type=int, default=M, help="bond dimension of the sweep comparison")
+    ap.add_argument("--ladder", action="store_true", help="run the reference's pre-sweep cutoff ladder 100, 200, ..")
+    ap.add_argument("--batch", type=int, default=0, help="sweeps of the 64-chain batched workload (0 skips it)")
+    ap.add_argument("--cpu-sweep", action="store_true",
+                    help="also run the host Kitaev L=40 m=64 5-sweep baseline beside the GPU (about 1-2 minutes)")
     ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
-    ap.add_argument("--part", default="", choices=["", "svd", "copy"],
-                    help="time one part alone: the step SVD, or the copy kernel used to calibrate PMC counters")
+    ap.add_argument("--part", default="", choices=["", "svd", "copy", "sweep", "cpusweep"],
+                    help="run one part alone: the step SVD, the copy kernel used to calibrate PMC counters, the "
+                         "sweep comparison or the host Kitaev sweep (JSON of that part only)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
@@ -326,6 +419,12 @@ def main():
         return
     if a.part == "copy":
         print(json.dumps(copy_leg(a.steps)), flush=True)
+        return
+    if a.part == "sweep":
+        print(json.dumps(sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder)), flush=True)
+        return
+    if a.part == "cpusweep":
+        print(json.dumps(cpu_sweep_leg()), flush=True)
         return
     if a.only:
         gpu_leg(a.steps, a.warmup, a.only.split(","))
@@ -368,15 +467,18 @@ def main():
         "configs": None,
         "sweep": None,
         "batch": None,
+        "cpu_sweep": None,
     }
     if not a.no_configs:
         line["configs"] = configs_leg()
     if a.sweep:
-        line["sweep"] = sweep_leg(a.sweep_L, a.sweep)
+        line["sweep"] = sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder)
     if a.batch:
         line["batch"] = batch_leg(sweeps=a.batch)
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_leg(a.cpu_seconds)
+    if a.cpu_sweep:
+        line["cpu_sweep"] = cpu_sweep_leg()
     print(json.dumps(line), flush=True)
 
 

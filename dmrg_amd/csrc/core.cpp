@@ -147,10 +147,13 @@ void Engine<T>::finish(StepLog& lg, const vector<double>& s, int cutoff) {
     f_singular_->flush();
   }
   log.push_back(lg);
-  if (cfg.verbose)
-    std::printf("l=%d r=%d d=%lld E0=%.12f trunc=%.3e matvec=%d t=%.3fs\n", lg.l, lg.r, (long long)lg.d, lg.e[0],
-                lg.trunc, lg.n_matvec, lg.t_total);
-  if (cfg.verbose) std::fflush(stdout);
+  if (cfg.verbose) {
+    // 1: one line per step on stdout; 2: on stderr (stdout stays free for results)
+    FILE* f = cfg.verbose == 2 ? stderr : stdout;
+    std::fprintf(f, "l=%d r=%d d=%lld E0=%.12f trunc=%.3e matvec=%d t=%.3fs\n", lg.l, lg.r, (long long)lg.d,
+                 lg.e[0], lg.trunc, lg.n_matvec, lg.t_total);
+    std::fflush(f);
+  }
   if (on_step) on_step(lg);
 }
 

@@ -27,7 +27,7 @@ struct RunConfig {
   int nev = NEV;
   int nev_lo = 4;              // eigenpairs converged by a reduced-precision solve
   int max_restart = 5000;
-  bool verbose = false;
+  int verbose = 0;             // 1: a line per step on stdout, 2: on stderr
   // when set, every step appends to "energy" + output_suffix and "singular" +
   // output_suffix in the working directory, as the reference's drivers do
   std::string output_suffix;

@@ -246,7 +246,7 @@ RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
   r.refine_k = c->refine_k > 0 ? c->refine_k : 8;
   r.nev_lo = c->nev_lo > 0 ? c->nev_lo : 4;
   if (c->output) r.output_suffix = c->output;
-  r.verbose = c->verbose != 0;
+  r.verbose = c->verbose == 2 ? 2 : (c->verbose != 0 ? 1 : 0);
   return r;
 }
 

@@ -144,7 +144,8 @@ def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladd
     cfg.sign_rule = 0 if sign_rule == "physical" else 1
     cfg.ladder = 1 if ladder else 0
     cfg.seed = seed
-    cfg.tol, cfg.tol_lo, cfg.refine_k, cfg.verbose = tol, tol_lo, refine_k, 1 if verbose else 0
+    cfg.tol, cfg.tol_lo, cfg.refine_k = tol, tol_lo, refine_k
+    cfg.verbose = 2 if verbose == "stderr" else (1 if verbose else 0)
     cfg.nev_lo = int(nev_lo)
     cfg.output = output.encode() if output else None
     return cfg

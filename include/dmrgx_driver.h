@@ -32,7 +32,7 @@ typedef struct dmrgx_run_config {
   double tol;       /* FP64 Lanczos tolerance (reference: 1e-10) */
   double tol_lo;    /* reduced-precision Lanczos tolerance */
   int refine_k;     /* FP64 Krylov polish steps after a reduced-precision solve */
-  int verbose;
+  int verbose;      /* 0: quiet; 1: one line per step on stdout; 2: the same on stderr */
   int nev_lo;       /* eigenpairs converged by a reduced-precision solve (0: 4); FP64 solves use 6 */
   const char* output;  /* NULL: no files; else every step appends to "energy" + output and
                           "singular" + output in the working directory in the reference's
