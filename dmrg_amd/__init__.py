@@ -29,7 +29,7 @@ EXPORTS = [
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
     "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_eigsh_batched", "dmrgx_set_lanczos_graphs", "dmrgx_set_autotune", "dmrgx_autotune_keys", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time", "dmrgx_heff_set_split", "dmrgx_heff_get_split",
-    "dmrgx_task_gemm_noalloc",
+    "dmrgx_task_gemm_noalloc", "dmrgx_set_skip_zero_slices", "dmrgx_heff_flops_exec",
 ]
 
 _lib = None
@@ -81,6 +81,8 @@ def _declare(lib):
                                vp, i64p, i64p, C.c_int, i32p, C.c_int, C.c_int64, C.c_int64], C.c_int),
         "dmrgx_heff_apply": ([vp, vp, vp], C.c_int),
         "dmrgx_heff_flops": ([vp], C.c_double),
+        "dmrgx_heff_flops_exec": ([vp], C.c_double),
+        "dmrgx_set_skip_zero_slices": ([C.c_int], C.c_int),
         "dmrgx_heff_time": ([vp, vp, vp, C.c_int, dp], C.c_int),
         "dmrgx_heff_destroy": ([vp], C.c_int),
         "dmrgx_heff_adjoint": ([vp, vp, vp, vp, vp, vp], C.c_int),
@@ -214,9 +216,14 @@ class Heff:
         check(lib().dmrgx_heff_adjoint(self.h, psi.ptr, g_out.ptr, ptr(g_psi), ptr(g_renv), ptr(g_lenv)),
               "dmrgx_heff_adjoint")
 
+    @property
+    def flops_exec(self):
+        return lib().dmrgx_heff_flops_exec(self.h)
+
     def time(self, x: DeviceArray, y: DeviceArray, iters: int):
-        """(ms per apply, ms R-pass kernel, ms L-pass kernel, R flops, L flops)"""
-        ms = np.zeros(5)
+        """(ms per apply, ms R-pass kernel, ms L-pass kernel, R flops, L flops,
+        R executed flops, L executed flops)"""
+        ms = np.zeros(7)
         check(lib().dmrgx_heff_time(self.h, x.ptr, y.ptr, int(iters), ms.ctypes.data_as(C.POINTER(C.c_double))),
               "dmrgx_heff_time")
         return ms

@@ -52,10 +52,19 @@ struct Mfma4<float> {
 };
 
 // Sum the K-chunk partial tiles of the split output blocks into C.
+// Element e (row-major M x N, compact) of an output block at its place in C:
+// the row goes through the block's row map when it has one.
+__device__ __forceinline__ int64_t out_index(const RedTask& r, const int32_t* __restrict__ RM, int64_t e) {
+  if (r.row_map < 0) return r.c_off + e;
+  const int64_t row = e / r.ncol;
+  return r.c_off + (int64_t)RM[r.row_map + row] * r.ncol + (e - row * r.ncol);
+}
+
 template <typename S, bool CPLX>
 __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* __restrict__ C,
                                                    const RedTask* __restrict__ t, int n,
-                                                   const int64_t* __restrict__ start) {
+                                                   const int64_t* __restrict__ start,
+                                                   const int32_t* __restrict__ RM) {
   int lo = 0, hi = n - 1;
   const int64_t b = blockIdx.x;
   while (lo < hi) {
@@ -78,15 +87,16 @@ __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* _
       ar += v.x;
       ai += v.y;
     }
-    S* o = C + 2 * (r.c_off + e);
+    S* o = C + 2 * out_index(r, RM, e);
     if (r.beta) { o[0] += ar; o[1] += ai; }
     else { o[0] = ar; o[1] = ai; }
   } else {
     S acc = 0;
 #pragma unroll 4
     for (int k = 0; k < r.nch; ++k) acc += W[r.w_off + k * r.len + e];
-    if (r.beta) C[r.c_off + e] += acc;
-    else C[r.c_off + e] = acc;
+    const int64_t o = out_index(r, RM, e);
+    if (r.beta) C[o] += acc;
+    else C[o] = acc;
   }
 }
 
@@ -94,7 +104,8 @@ __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* _
 template <bool CPLX, int OUT>
 __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__ W, void* __restrict__ Cv,
                                                        int64_t cps, const RedTask* __restrict__ t, int n,
-                                                       const int64_t* __restrict__ start);
+                                                       const int64_t* __restrict__ start,
+                                                       const int32_t* __restrict__ RM);
 
 // Walks the (segment, k-tile) sequence of one work item.
 struct Cursor {
@@ -129,7 +140,8 @@ template <typename S, bool CPLX, bool AKC, bool BKC>
 __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items, int nitems,
                                               int per_xcd, const GemmSeg* __restrict__ segs,
                                               const S* __restrict__ A, const S* __restrict__ B,
-                                              S* __restrict__ C, S* __restrict__ W) {
+                                              S* __restrict__ C, S* __restrict__ W,
+                                              const int32_t* __restrict__ RM) {
   constexpr int BK = 16;
   constexpr int LDK = BK + 1;
   constexpr int NP = CPLX ? 2 : 1;
@@ -155,12 +167,15 @@ __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items
   S ra[NP][EPT], rb[NP][EPT];
   int lr[EPT], lk[EPT];  // LDS coordinates of A loads
   int mr[EPT], mk[EPT];  // LDS coordinates of B loads
+  int ar[EPT];           // A row of each load (through the row map), -1 outside the tile
 #pragma unroll
   for (int j = 0; j < EPT; ++j) {
     if (AKC) { lr[j] = t / BK + (NT / BK) * j; lk[j] = t % BK; }
     else     { lr[j] = t % BM; lk[j] = t / BM + (NT / BM) * j; }
     if (BKC) { mr[j] = t / BK + (NT / BK) * j; mk[j] = t % BK; }
     else     { mr[j] = t % BN; mk[j] = t / BN + (NT / BN) * j; }
+    const int gr = it.m0 + lr[j];
+    ar[j] = gr < it.M ? (it.row_map >= 0 ? RM[it.row_map + gr] : gr) : -1;
   }
 
   Cursor cur;
@@ -171,8 +186,8 @@ __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items
     const S al = (S)s.alpha;
 #pragma unroll
     for (int j = 0; j < EPT; ++j) {
-      int gr = it.m0 + lr[j], gk = c.k + lk[j];
-      bool ok = gr < it.M && gk < s.k1;
+      int gr = ar[j], gk = c.k + lk[j];
+      bool ok = gr >= 0 && gk < s.k1;
       int64_t e = AKC ? s.a_off + (int64_t)gr * s.K + gk : s.a_off + (int64_t)gk * it.M + gr;
       if (CPLX) {
         ra[0][j] = ok ? A[2 * e] * al : (S)0;
@@ -247,6 +262,7 @@ __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items
         int row = it.m0 + wm + i * 16 + (F64 ? r * 4 + (lane >> 4) : (lane >> 4) * 4 + r);
         int col = it.n0 + wn + j * 16 + (lane & 15);
         if (row < it.M && col < it.N) {
+          if (it.row_map >= 0) row = RM[it.row_map + row];
           int64_t e = it.c_off + (int64_t)row * it.N + col;
           if (CPLX) {
             S vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
@@ -428,7 +444,8 @@ __device__ __forceinline__ void store_split(unsigned short* P, int64_t ps, int64
 template <bool CPLX, int OUT>
 __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__ W, void* __restrict__ Cv,
                                                        int64_t cps, const RedTask* __restrict__ t, int n,
-                                                       const int64_t* __restrict__ start) {
+                                                       const int64_t* __restrict__ start,
+                                                       const int32_t* __restrict__ RM) {
   int lo = 0, hi = n - 1;
   const int64_t b = blockIdx.x;
   while (lo < hi) {
@@ -460,14 +477,15 @@ __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__
     for (int k = 0; k < r.nch; ++k) a += W[r.w_off + k * r.len + e];
     acc[0] = a;
   }
+  const int64_t o = out_index(r, RM, e);
 #pragma unroll
   for (int q = 0; q < c; ++q) {
     if (OUT == OUT_F64) {
       double* C = (double*)Cv;
-      if (r.beta) C[c * (r.c_off + e) + q] += acc[q];
-      else C[c * (r.c_off + e) + q] = acc[q];
+      if (r.beta) C[c * o + q] += acc[q];
+      else C[c * o + q] = acc[q];
     } else {
-      store_split((unsigned short*)Cv, cps, r.c_off + e, q, acc[q]);
+      store_split((unsigned short*)Cv, cps, o, q, acc[q]);
     }
   }
 }
@@ -497,7 +515,8 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
                                                   const GemmSeg* __restrict__ segs,
                                                   const unsigned short* __restrict__ A, int64_t as,
                                                   const unsigned short* __restrict__ B, int64_t bs,
-                                                  void* __restrict__ Cv, int64_t cps, float* __restrict__ W) {
+                                                  void* __restrict__ Cv, int64_t cps, float* __restrict__ W,
+                                                  const int32_t* __restrict__ RM) {
   constexpr int NPL = CPLX ? 4 : 2;
   constexpr int NP = CPLX ? 2 : 1;
   constexpr int CPR = PBK / 8;               // 16-byte chunks per row per plane (4)
@@ -532,11 +551,13 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
   bool okA = false, okB = false;  // the chunk's row is inside the block
   Cursor cur;
   cur.start(segs, it.seg_begin, it.seg_end);
+  // this thread's A row (through the row map), -1 outside the tile
+  const int arow = it.m0 + srow < it.M ? (it.row_map >= 0 ? RM[it.row_map + it.m0 + srow] : it.m0 + srow) : -1;
   auto seg_setup = [&](const GemmSeg& s) {
-    const int gr = it.m0 + srow, gn = it.n0 + srow;
-    offA = s.a_off + (int64_t)gr * s.lda + skc;
+    const int gn = it.n0 + srow;
+    offA = s.a_off + (int64_t)max(arow, 0) * s.lda + skc;
     offB = s.b_off + (int64_t)gn * s.ldb + skc;
-    okA = gr < it.M;
+    okA = arow >= 0;
     okB = gn < it.N;
   };
   if (cur.valid()) seg_setup(cur.s);
@@ -639,6 +660,7 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
         int row = it.m0 + wm + i * 16 + (lane >> 4) * 4 + r;
         int col = it.n0 + wn + j * 16 + (lane & 15);
         if (row < it.M && col < it.N) {
+          if (it.row_map >= 0) row = RM[it.row_map + row];
           int64_t e = it.c_off + (int64_t)row * it.N + col;
           const float vr = acc[0][i][j][r], vi = CPLX ? acc[NP - 1][i][j][r] : 0.0f;
           if (partial || OUT == OUT_F32) {
@@ -728,7 +750,7 @@ void launch4(const GemmPlan& p, const void* A, const void* B, void* C, hipStream
   const bool akc = (p.ta == 'N' || p.ta == 'n'), bkc = (p.tb == 'T' || p.tb == 't');
   dim3 g(p.grid), b(NT);
 #define L4(AK, BKc) hipLaunchKernelGGL((k_gemm4<S, CPLX, AK, BKc>), g, b, 0, s, it, p.n_items, per, sg, \
-                                       (const S*)A, (const S*)B, (S*)C, (S*)p.ws)
+                                       (const S*)A, (const S*)B, (S*)C, (S*)p.ws, p.row_maps)
   if (akc && bkc) L4(true, true);
   else if (akc) L4(true, false);
   else if (bkc) L4(false, true);
@@ -757,7 +779,8 @@ void launch_bf(const GemmPlan& p, const void* A, const void* B, void* C, hipStre
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
-                                         int split_target, bool planes, GemmTables* host_only) {
+                                         int split_target, bool planes, GemmTables* host_only,
+                                         const std::vector<int32_t>* row_maps) {
   if (!host_only) ensure_init();
   auto plan = std::make_unique<GemmPlan>();
   plan->planes = planes;
@@ -772,28 +795,37 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
   const int BKq = prec == PREC_BF16X3 ? 32 : 16;
   const double cf = dtype == DT_C64 ? 8.0 : 2.0;
 
-  // group valid tasks by output block, first-appearance order
-  struct Group { int64_t c_off; int M, N; int beta; std::vector<int> t; int64_t ksum = 0; };
+  // group valid tasks by output block (and row map), first-appearance order
+  struct Group { int64_t c_off; int64_t row_map; int M, N; int beta; std::vector<int> t; int64_t ksum = 0; };
   std::vector<Group> groups;
-  std::map<int64_t, int> gidx;
+  std::map<std::pair<int64_t, int64_t>, int> gidx;
+  const bool akc_ = (ta == 'N' || ta == 'n');
   for (int i = 0; i < (int)tasks_in.size(); ++i) {
     const GemmTask& k = tasks_in[i];
-    if (k.M <= 0 || k.N <= 0 || k.K <= 0) continue;  // BLAS wrapper skips empty GEMMs
-    auto f = gidx.find(k.c_off);
+    const int kb = k.k0, ke = k.k1 > 0 ? k.k1 : k.K;
+    if (k.M <= 0 || k.N <= 0 || k.K <= 0 || ke <= kb) continue;  // BLAS wrapper skips empty GEMMs
+    DMRGX_CHECK(kb >= 0 && ke <= k.K, "gemm plan: K range outside the task");
+    if (k.row_map >= 0) {
+      DMRGX_CHECK(row_maps && k.row_map + k.M <= (int64_t)row_maps->size(), "gemm plan: row map out of range");
+      DMRGX_CHECK(akc_ && (planes || prec != PREC_BF16X3), "gemm plan: row maps need a k-contiguous A");
+    }
+    const std::pair<int64_t, int64_t> key{k.c_off, k.row_map};
+    auto f = gidx.find(key);
     if (f == gidx.end()) {
-      gidx[k.c_off] = (int)groups.size();
-      groups.push_back(Group{k.c_off, k.M, k.N, k.beta, {}, 0});
-      f = gidx.find(k.c_off);
+      gidx[key] = (int)groups.size();
+      groups.push_back(Group{k.c_off, k.row_map, k.M, k.N, k.beta, {}, 0});
+      f = gidx.find(key);
     }
     Group& g = groups[f->second];
     DMRGX_CHECK(g.M == k.M && g.N == k.N, "gemm plan: tasks into one output block disagree on shape");
-    // the pre-split kernel loads whole 16-byte chunks: operand offsets and
-    // row strides on 8-element boundaries, rows zero-padded to a multiple of 8
-    DMRGX_CHECK(!planes || ((k.a_off | k.b_off | (k.lda > 0 ? k.lda : k.K) | (k.ldb > 0 ? k.ldb : k.K)) & 7) == 0,
-                "pre-split gemm plan: operand offsets and row strides must be multiples of 8");
+    // the pre-split kernel loads whole 16-byte chunks: operand offsets, row
+    // strides and K ranges on 8-element boundaries, rows zero-padded to a
+    // multiple of 8
+    DMRGX_CHECK(!planes || ((k.a_off | k.b_off | (k.lda > 0 ? k.lda : k.K) | (k.ldb > 0 ? k.ldb : k.K) | kb) & 7) == 0,
+                "pre-split gemm plan: operand offsets, row strides and K ranges must be multiples of 8");
     g.t.push_back(i);
-    g.ksum += k.K;
-    plan->flops += cf * (double)k.M * k.N * k.K;
+    g.ksum += ke - kb;
+    plan->flops += cf * (double)k.M * k.N * (ke - kb);
   }
   // split-K so the launch has enough work items for every CU
   double work = 0;
@@ -886,14 +918,15 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     // on a 16-byte boundary of an aligned operand row
     std::vector<std::pair<int, int>> chunk_rng;  // seg range per chunk
     size_t ti = 0;
-    int64_t kin = 0;  // position inside current task
+    int64_t kin = g.t.empty() ? 0 : tasks_in[g.t[0]].k0;  // position inside current task
     while (ti < g.t.size()) {
       int sb = (int)segs.size();
       int64_t need = ch;
       while (need > 0 && ti < g.t.size()) {
         const GemmTask& k = tasks_in[g.t[ti]];
-        int64_t take = std::min<int64_t>(need, k.K - kin);
-        if (take < k.K - kin) {
+        const int64_t kend = k.k1 > 0 ? k.k1 : k.K;
+        int64_t take = std::min<int64_t>(need, kend - kin);
+        if (take < kend - kin) {
           take = take / 8 * 8;
           if (take == 0) break;
         }
@@ -909,7 +942,10 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
         segs.push_back(s);
         need -= take;
         kin += take;
-        if (kin >= k.K) { ++ti; kin = 0; }
+        if (kin >= kend) {
+          ++ti;
+          if (ti < g.t.size()) kin = tasks_in[g.t[ti]].k0;
+        }
       }
       DMRGX_CHECK((int)segs.size() > sb, "gemm plan: empty K chunk");
       chunk_rng.push_back({sb, (int)segs.size()});
@@ -921,13 +957,15 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     if (shared) {
       w_off = plan->ws_elems;
       plan->ws_elems += len * nch;
-      red.push_back(RedTask{g.c_off, w_off, len, nch, g.beta});
+      red.push_back(RedTask{g.c_off, w_off, len, g.row_map, nch, g.beta, g.N, 0});
     }
     for (int c = 0; c < nch; ++c)
       for (int n0 = 0; n0 < g.N; n0 += BN)
         for (int m0 = 0; m0 < g.M; m0 += BM) {
           GemmItem itm{};
           itm.c_off = shared ? w_off + (int64_t)c * len : g.c_off;
+          // partial tiles are compact in the workspace; the reduction maps rows
+          itm.row_map = shared ? -1 : g.row_map;
           itm.M = g.M;
           itm.N = g.N;
           itm.m0 = m0;
@@ -949,7 +987,9 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
       auto f = share(whole.size(), x), h = share(half.size(), x);
       per = std::max(per, (f.second - f.first) + (h.second - h.first));
     }
-    items.assign(per * 8, GemmItem{});
+    GemmItem empty{};
+    empty.row_map = -1;
+    items.assign(per * 8, empty);
     for (int x = 0; x < 8; ++x) {
       size_t pos = per * x;
       auto f = share(whole.size(), x), h = share(half.size(), x);
@@ -970,7 +1010,9 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
   }
   plan->n_red = (int)red.size();
   plan->red_blocks = (int)acc;
+  const bool any_map = std::any_of(groups.begin(), groups.end(), [](const Group& g) { return g.row_map >= 0; });
   if (host_only) {
+    DMRGX_CHECK(!any_map, "gemm plan: row maps need device tables");
     host_only->items = std::move(items);
     host_only->segs = std::move(segs);
     host_only->red = std::move(red);
@@ -978,6 +1020,11 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     return plan;
   }
   hipStream_t s = rt().stream;
+  if (any_map) {
+    plan->own_row_maps = std::make_unique<DevBuf>(row_maps->size() * sizeof(int32_t));
+    upload(plan->own_row_maps->ptr, row_maps->data(), row_maps->size() * sizeof(int32_t), s);
+    plan->row_maps = (const int32_t*)plan->own_row_maps->ptr;
+  }
   if (plan->n_items) {
     plan->own_items = std::make_unique<DevBuf>(items.size() * sizeof(GemmItem));
     plan->own_segs = std::make_unique<DevBuf>(std::max<size_t>(segs.size(), 1) * sizeof(GemmSeg));
@@ -1006,11 +1053,15 @@ void reduce_partials(const GemmPlan& p, void* C, hipStream_t s) {
   dim3 g(p.red_blocks), b(256);
   const bool cx = p.dtype == DT_C64;
   if (p.prec == PREC_FP64) {
-    if (cx) hipLaunchKernelGGL((k_reduce_ws<double, true>), g, b, 0, s, (const double*)p.ws, (double*)C, t, p.n_red, st);
-    else hipLaunchKernelGGL((k_reduce_ws<double, false>), g, b, 0, s, (const double*)p.ws, (double*)C, t, p.n_red, st);
+    if (cx) hipLaunchKernelGGL((k_reduce_ws<double, true>), g, b, 0, s, (const double*)p.ws, (double*)C, t, p.n_red, st,
+                               p.row_maps);
+    else hipLaunchKernelGGL((k_reduce_ws<double, false>), g, b, 0, s, (const double*)p.ws, (double*)C, t, p.n_red, st,
+                            p.row_maps);
   } else {
-    if (cx) hipLaunchKernelGGL((k_reduce_ws<float, true>), g, b, 0, s, (const float*)p.ws, (float*)C, t, p.n_red, st);
-    else hipLaunchKernelGGL((k_reduce_ws<float, false>), g, b, 0, s, (const float*)p.ws, (float*)C, t, p.n_red, st);
+    if (cx) hipLaunchKernelGGL((k_reduce_ws<float, true>), g, b, 0, s, (const float*)p.ws, (float*)C, t, p.n_red, st,
+                               p.row_maps);
+    else hipLaunchKernelGGL((k_reduce_ws<float, false>), g, b, 0, s, (const float*)p.ws, (float*)C, t, p.n_red, st,
+                            p.row_maps);
   }
   DMRGX_HIP(hipGetLastError());
 }
@@ -1036,7 +1087,7 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
     auto sg = p.segs;
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(p.grid), dim3(PNT), 0, s, it, p.n_items, per, sg, (const unsigned short*)A,
-                         a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp);
+                         a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp, p.row_maps);
     };
     if (p.dtype == DT_C64) {
       if (out == OUT_F32) go(k_gemm_bf3p<true, OUT_F32>);
@@ -1057,7 +1108,7 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
     auto st = p.red_start;
     auto go = [&](auto kern) {
       hipLaunchKernelGGL(kern, dim3(p.red_blocks), dim3(256), 0, s, (const float*)p.ws, C, c_ps, t, p.n_red,
-                         st);
+                         st, p.row_maps);
     };
     if (p.dtype == DT_C64) {
       if (out == OUT_PLANES) go(k_reduce_ws_out<true, OUT_PLANES>);

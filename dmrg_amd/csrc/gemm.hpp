@@ -17,6 +17,13 @@ struct GemmTask {
   int alpha;  // +1 / -1 (the reference's fermion sign)
   int beta;   // 0: first write overwrites, 1: accumulate onto C
   int lda = 0, ldb = 0;  // row strides of A and B ('N','T' pre-split plans); 0 means K
+  // only the K range [k0, k1) is contracted (k1 == 0: the whole K); K stays
+  // the row stride of k-contiguous operands
+  int k0 = 0, k1 = 0;
+  // >= 0: the task's M rows are rows row_map[0..M) (offset into the plan's
+  // row-map table) of A and of the output block, whose full row count is not
+  // needed ('N' A only); -1: rows 0..M
+  int64_t row_map = -1;
 };
 
 // Device work decomposition (shared with the kernels).
@@ -30,6 +37,7 @@ struct GemmSeg {
 };
 struct GemmItem {
   int64_t c_off;
+  int64_t row_map;  // offset of the output block's row indices in the row-map table, or -1
   int M, N;
   int m0, n0;
   int seg_begin, seg_end;
@@ -44,7 +52,9 @@ enum : int { ITEM_PARTIAL = 1, ITEM_BETA = 2 };
 
 struct RedTask {
   int64_t c_off, w_off, len;  // len = M*N elements
+  int64_t row_map;            // row-map offset of the output rows (-1: rows 0..M)
   int nch, beta;
+  int ncol, pad;              // N: output elements per row
 };
 
 // Output format of the pre-split BF16x3 kernel: FP32, bf16 hi/lo planes
@@ -78,7 +88,8 @@ struct GemmPlan {
   void* ws = nullptr;
   const RedTask* red = nullptr;
   const int64_t* red_start = nullptr;
-  std::unique_ptr<DevBuf> own_items, own_segs, own_ws, own_red, own_red_start;
+  const int32_t* row_maps = nullptr;  // the row-map table of row-mapped tasks
+  std::unique_ptr<DevBuf> own_items, own_segs, own_ws, own_red, own_red_start, own_row_maps;
   double flops = 0;        // algorithmic real flops (c*M*N*K summed)
   bool any_beta = false;   // some output block accumulates onto existing C
 };
@@ -89,11 +100,14 @@ struct GemmPlan {
 // output blocks cut into two K halves where that shortens the launch).
 // With `host_only` the plan allocates and uploads nothing: its decomposition
 // is returned in *host_only and the caller sets the device addresses.
+// `row_maps` holds the row indices that row-mapped tasks point into (tasks of
+// one output block with different row maps form separate groups).
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
                                          int split_target = 0, bool planes = false,
-                                         GemmTables* host_only = nullptr);
+                                         GemmTables* host_only = nullptr,
+                                         const std::vector<int32_t>* row_maps = nullptr);
 // Bytes of the workspace of a plan (split-K partial tiles).
 inline size_t gemm_ws_bytes(const GemmPlan& p) {
   return (size_t)p.ws_elems * (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);

@@ -60,6 +60,268 @@ std::vector<std::pair<int64_t, int64_t>> uncovered(const std::vector<GemmTask>& 
 }
 
 static std::atomic<bool> g_autotune{true};
+static std::atomic<bool> g_skip_zeros{true};
+
+void set_heff_skip_zeros(bool on) { g_skip_zeros.store(on); }
+
+namespace {
+
+// A block scanned for zero rows or columns: rows x cols elements row-major at
+// `off`; its flags start at flag_off, its scan units (rows or columns) at
+// `first` in the launch's numbering.
+struct NzBlk {
+  int64_t off, flag_off, first;
+  int rows, cols;
+};
+
+__device__ __forceinline__ int find_blk(const NzBlk* __restrict__ b, int nb, int64_t u) {
+  int lo = 0, hi = nb - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b[mid].first <= u) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// flags[row] = 1 when the row holds a nonzero element: one wavefront per row
+__global__ __launch_bounds__(256) void k_nonzero_rows(const double* __restrict__ A, const NzBlk* __restrict__ b,
+                                                      int nb, int64_t units, int cplx, int* __restrict__ flags) {
+  const int64_t u = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (u >= units) return;  // uniform over the wavefront
+  const int lane = threadIdx.x & 63;
+  const NzBlk q = b[find_blk(b, nb, u)];
+  const int64_t row = u - q.first;
+  const int64_t len = (int64_t)q.cols << cplx;
+  const double* p = A + ((q.off + row * q.cols) << cplx);
+  bool nz = false;
+  for (int64_t i = lane; i < len; i += 64) nz = nz || p[i] != 0.0;
+  const unsigned long long any = __ballot(nz);
+  if (lane == 0) flags[q.flag_off + row] = any != 0ull;
+}
+
+// flags[col] = 1 when the column holds a nonzero element: one thread per column
+__global__ __launch_bounds__(256) void k_nonzero_cols(const double* __restrict__ A, const NzBlk* __restrict__ b,
+                                                      int nb, int64_t units, int cplx, int* __restrict__ flags) {
+  const int64_t u = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (u >= units) return;
+  const NzBlk q = b[find_blk(b, nb, u)];
+  const int64_t col = u - q.first;
+  bool nz = false;
+  for (int r = 0; r < q.rows && !nz; ++r) {
+    const double* p = A + ((q.off + (int64_t)r * q.cols + col) << cplx);
+    nz = p[0] != 0.0 || (cplx && p[1] != 0.0);
+  }
+  flags[q.flag_off + col] = nz;
+}
+
+// Zero rows (rows = true) or columns of the distinct A blocks of a task list
+// (keyed by a_off): one flag per row / column, in the order of `blocks`.
+std::vector<int> scan_nonzero(const void* A, const std::vector<GemmTask>& blocks, bool rows, bool cplx,
+                              hipStream_t s) {
+  std::vector<NzBlk> nb;
+  int64_t nflags = 0;
+  for (auto& g : blocks) {
+    NzBlk q{g.a_off, nflags, nflags, g.M, g.K};
+    nflags += rows ? g.M : g.K;
+    nb.push_back(q);
+  }
+  std::vector<int> out(nflags, 1);
+  if (nflags == 0) return out;
+  DevBuf db(nb.size() * sizeof(NzBlk)), df(nflags * sizeof(int));
+  upload(db.ptr, nb.data(), nb.size() * sizeof(NzBlk), s);
+  if (rows)
+    hipLaunchKernelGGL(k_nonzero_rows, dim3((unsigned)((nflags + 3) / 4)), dim3(256), 0, s, (const double*)A,
+                       (const NzBlk*)db.ptr, (int)nb.size(), nflags, cplx ? 1 : 0, (int*)df.ptr);
+  else
+    hipLaunchKernelGGL(k_nonzero_cols, dim3((unsigned)((nflags + 255) / 256)), dim3(256), 0, s, (const double*)A,
+                       (const NzBlk*)db.ptr, (int)nb.size(), nflags, cplx ? 1 : 0, (int*)df.ptr);
+  DMRGX_HIP(hipGetLastError());
+  DMRGX_HIP(hipMemcpyAsync(out.data(), df.ptr, nflags * sizeof(int), hipMemcpyDeviceToHost, s));
+  DMRGX_HIP(hipStreamSynchronize(s));
+  return out;
+}
+
+// The executed task lists of a two-site H_eff without the environments'
+// structurally zero slices (exact zeros: the MPO bond index w of an
+// environment block carries only the operators its sectors allow).
+//   R pass (store = renv x psi, renv block rows (s, b, w)): the rows of an
+//   output block are grouped by the set of tasks in which they are nonzero;
+//   each group becomes row-mapped tasks over those rows and tasks only, and
+//   rows that are zero in every task are not computed (their store elements
+//   are zero and stay so: the plan zero-fills its store once).
+//   L pass (out = lenv x store, lenv block columns (w, a', s')): each task
+//   contracts only the K columns that are nonzero in lenv and computed in
+//   store, as contiguous K ranges (8-aligned for the bf16 plane kernels).
+// The sums keep their order over the remaining terms.
+void skip_zero_slices(HeffPlan& h, std::vector<GemmTask>& rx, std::vector<GemmTask>& lx, bool align8,
+                      hipStream_t s) {
+  const bool cx = h.dtype == DT_C64;
+  const std::vector<GemmTask> rt = rx, lt = lx;
+  // distinct A blocks of each pass
+  auto distinct = [](const std::vector<GemmTask>& t) {
+    std::vector<GemmTask> b;
+    std::map<int64_t, int> idx;
+    std::vector<int> of(t.size());
+    for (size_t i = 0; i < t.size(); ++i) {
+      auto f = idx.find(t[i].a_off);
+      if (f == idx.end()) {
+        idx[t[i].a_off] = (int)b.size();
+        b.push_back(t[i]);
+        of[i] = (int)b.size() - 1;
+      } else {
+        DMRGX_CHECK(b[f->second].M == t[i].M && b[f->second].K == t[i].K, "skip: one block with two shapes");
+        of[i] = f->second;
+      }
+    }
+    return std::make_pair(b, of);
+  };
+  auto [rb, rof] = distinct(rt);
+  auto [lb, lof] = distinct(lt);
+  const std::vector<int> rflag = scan_nonzero(h.renv, rb, true, cx, s);
+  const std::vector<int> lflag = scan_nonzero(h.lenv, lb, false, cx, s);
+  std::vector<int64_t> rbase(rb.size()), lbase(lb.size());
+  for (size_t i = 0, a = 0; i < rb.size(); a += rb[i].M, ++i) rbase[i] = (int64_t)a;
+  for (size_t i = 0, a = 0; i < lb.size(); a += lb[i].K, ++i) lbase[i] = (int64_t)a;
+
+  // ---- R pass: output blocks (first-appearance order) and their rows
+  struct OutBlk { int64_t c_off; int M, N; std::vector<int> t; std::vector<char> comp; };
+  std::vector<OutBlk> ob;
+  std::map<int64_t, int> oidx;
+  for (int i = 0; i < (int)rt.size(); ++i) {
+    auto f = oidx.find(rt[i].c_off);
+    if (f == oidx.end()) {
+      oidx[rt[i].c_off] = (int)ob.size();
+      ob.push_back(OutBlk{rt[i].c_off, rt[i].M, rt[i].N, {}, {}});
+      f = oidx.find(rt[i].c_off);
+    }
+    DMRGX_CHECK(rt[i].M == ob[f->second].M && rt[i].N == ob[f->second].N,
+                "skip: tasks into one output block disagree on shape");
+    ob[f->second].t.push_back(i);
+  }
+  rx.clear();
+  h.rexec_src.clear();
+  h.row_maps.clear();
+  for (auto& o : ob) {
+    const int nt = (int)o.t.size();
+    // row signature: which of the block's tasks are nonzero on the row
+    std::map<std::vector<char>, std::vector<int32_t>> sig;
+    std::vector<std::vector<char>> order;
+    o.comp.assign(o.M, 0);
+    for (int r = 0; r < o.M; ++r) {
+      std::vector<char> key(nt);
+      bool any = false;
+      for (int j = 0; j < nt; ++j) {
+        const int i = o.t[j];
+        key[j] = (char)rflag[rbase[rof[i]] + r];
+        any = any || key[j];
+      }
+      if (!any) continue;
+      o.comp[r] = 1;
+      auto f = sig.find(key);
+      if (f == sig.end()) {
+        order.push_back(key);
+        sig[key].push_back(r);
+      } else {
+        f->second.push_back(r);
+      }
+    }
+    const int beta0 = rt[o.t[0]].beta;
+    const bool dense = sig.size() == 1 && (int)sig.begin()->second.size() == o.M &&
+                       std::all_of(order[0].begin(), order[0].end(), [](char c) { return c != 0; });
+    // every row zero: nothing to compute (the store block stays zero)
+    if (sig.empty()) continue;
+    // dense, or many small row groups (which would cost more in tile
+    // efficiency than they save): the block's tasks as they are
+    if (dense || sig.size() > 8) {
+      for (int j = 0; j < nt; ++j) {
+        rx.push_back(rt[o.t[j]]);
+        h.rexec_src.push_back(o.t[j]);
+      }
+      std::fill(o.comp.begin(), o.comp.end(), 1);
+      continue;
+    }
+    for (auto& key : order) {
+      const std::vector<int32_t>& rows = sig[key];
+      const int64_t off = (int64_t)h.row_maps.size();
+      h.row_maps.insert(h.row_maps.end(), rows.begin(), rows.end());
+      bool first = true;
+      for (int j = 0; j < nt; ++j) {
+        if (!key[j]) continue;
+        GemmTask g = rt[o.t[j]];
+        g.M = (int)rows.size();
+        g.row_map = off;
+        g.beta = first ? beta0 : 1;
+        first = false;
+        rx.push_back(g);
+        h.rexec_src.push_back(o.t[j]);
+      }
+    }
+  }
+
+  // ---- L pass: per task, the K columns worth contracting
+  lx.clear();
+  h.lexec_src.clear();
+  std::map<int64_t, int> lfirst;  // output block -> index in lx of its first executed task
+  for (int i = 0; i < (int)lt.size(); ++i) {
+    const GemmTask& t = lt[i];
+    const int K = t.K;
+    std::vector<char> col(K);
+    const int64_t lb0 = lbase[lof[i]];
+    for (int k = 0; k < K; ++k) col[k] = (char)lflag[lb0 + k];
+    // store columns: the B block (N x K at b_off) inside one R output block
+    // whose rows hold N_R = an exact divisor of K elements each
+    auto ob_it = std::find_if(ob.begin(), ob.end(), [&](const OutBlk& o) {
+      return o.c_off <= t.b_off && t.b_off + (int64_t)t.N * K <= o.c_off + (int64_t)o.M * o.N;
+    });
+    if (ob_it != ob.end() && K % ob_it->N == 0 && (t.b_off - ob_it->c_off) % ob_it->N == 0) {
+      const int q = K / ob_it->N;
+      const int64_t r0 = (t.b_off - ob_it->c_off) / ob_it->N;
+      for (int c = 0; c < q; ++c) {
+        bool any = false;
+        for (int n = 0; n < t.N && !any; ++n) any = ob_it->comp[r0 + (int64_t)n * q + c] != 0;
+        if (!any)
+          for (int k = c * ob_it->N; k < (c + 1) * ob_it->N; ++k) col[k] = 0;
+      }
+    }
+    // contiguous runs of columns (8-aligned for the plane kernels); gaps
+    // shorter than 16 columns are contracted rather than cut
+    std::vector<std::pair<int, int>> runs;
+    for (int k = 0; k < K;) {
+      if (!col[k]) { ++k; continue; }
+      int e = k;
+      while (e < K && col[e]) ++e;
+      int b0 = k, e0 = e;
+      if (align8) {
+        b0 = b0 / 8 * 8;
+        e0 = std::min(K, (e0 + 7) / 8 * 8);
+      }
+      if (!runs.empty() && b0 - runs.back().second < 16) runs.back().second = std::max(runs.back().second, e0);
+      else runs.push_back({b0, e0});
+      k = e;
+    }
+    for (auto& r : runs) {
+      GemmTask g = t;
+      g.k0 = r.first;
+      g.k1 = r.second == K && r.first == 0 ? 0 : r.second;
+      auto f = lfirst.find(t.c_off);
+      if (f == lfirst.end()) {
+        lfirst[t.c_off] = (int)lx.size();
+        // the block's first write overwrites, whichever task it comes from
+        int b0 = 0;
+        for (auto& u : lt)
+          if (u.c_off == t.c_off) { b0 = u.beta; break; }
+        g.beta = b0;
+      } else {
+        g.beta = 1;
+      }
+      lx.push_back(g);
+      h.lexec_src.push_back(i);
+    }
+  }
+}
+
+}  // namespace
 
 static std::unique_ptr<DevBuf> to_f32(const void* src, int64_t nreal, hipStream_t s) {
   auto b = std::make_unique<DevBuf>(std::max<int64_t>(nreal, 1) * 4);
@@ -106,7 +368,20 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
   h->ltasks = lt_;
   h->renv_len = renv_len;
   h->lenv_len = lenv_len;
-  const auto rzero = uncovered(rt_, n_store), lzero = uncovered(lt_, n_psi);
+  const double cf = dtype == DT_C64 ? 8.0 : 2.0;
+  for (auto* v : {&rt_, &lt_})
+    for (auto& g : *v) h->flops += cf * (double)g.M * g.N * g.K;
+  // store ranges no task writes read as zero at every apply; with the zero
+  // slices skipped, rows that no executed task writes are zero-filled once
+  const auto rzero = uncovered(rt_, n_store);
+  std::vector<GemmTask> rx = rt_, lx = lt_;
+  h->rexec_src.resize(rx.size());
+  h->lexec_src.resize(lx.size());
+  for (size_t i = 0; i < rx.size(); ++i) h->rexec_src[i] = (int)i;
+  for (size_t i = 0; i < lx.size(); ++i) h->lexec_src[i] = (int)i;
+  hipStream_t s = rt().stream;
+  if (g_skip_zeros.load() && renv && lenv) skip_zero_slices(*h, rx, lx, planes, s);
+  const auto lzero = uncovered(lx, n_psi);
   if (planes) {
     // every operand block gets rows of a multiple of 8 bf16 elements
     auto lay = [](const std::vector<GemmTask>& t, bool a) {
@@ -126,18 +401,17 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
         g.b_off = B.map(g.b_off);
       }
     };
-    remap(rt_, *h->renv_lay, *h->psi_lay);
-    remap(lt_, *h->lenv_lay, *h->store_lay);
+    remap(rx, *h->renv_lay, *h->psi_lay);
+    remap(lx, *h->lenv_lay, *h->store_lay);
   }
-  h->rexec = rt_;
-  h->lexec = lt_;
+  h->rexec = rx;
+  h->lexec = lx;
   h->rzero = rzero;
   h->lzero = lzero;
-  h->rplan = make_gemm_plan(rt_, dtype, prec, 'N', 'T', rzero, 0, planes);
-  h->lplan = make_gemm_plan(lt_, dtype, prec, 'N', 'T', lzero, 0, planes);
-  h->flops = h->rplan->flops + h->lplan->flops;
+  h->rplan = make_gemm_plan(rx, dtype, prec, 'N', 'T', rzero, 0, planes, nullptr, h->maps());
+  h->lplan = make_gemm_plan(lx, dtype, prec, 'N', 'T', lzero, 0, planes, nullptr, h->maps());
+  h->flops_exec = h->rplan->flops + h->lplan->flops;
   const int c = dtype == DT_C64 ? 2 : 1;
-  hipStream_t s = rt().stream;
   if (prec == PREC_FP64) {
     h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 8 * c);
   } else if (planes) {
@@ -164,6 +438,9 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     h->in_lo = std::make_unique<DevBuf>(std::max<int64_t>(n_psi, 1) * 4 * c);
     h->out_lo = std::make_unique<DevBuf>(std::max<int64_t>(n_psi, 1) * 4 * c);
   }
+  // rows of store that no executed task writes hold zeros from here on
+  if (h->store) fill_zero(h->store->ptr, h->store->bytes, s);
+  if (h->store_pl) fill_zero(h->store_pl->ptr, h->store_pl->bytes, s);
   DMRGX_HIP(hipStreamSynchronize(s));
   if (g_autotune.load()) h->autotune(s);
   return h;
@@ -171,8 +448,8 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
 
 void HeffPlan::replan(int r_target, int l_target) {
   const bool planes = prec == PREC_BF16X3;
-  rplan = make_gemm_plan(rexec, dtype, prec, 'N', 'T', rzero, r_target, planes);
-  lplan = make_gemm_plan(lexec, dtype, prec, 'N', 'T', lzero, l_target, planes);
+  rplan = make_gemm_plan(rexec, dtype, prec, 'N', 'T', rzero, r_target, planes, nullptr, maps());
+  lplan = make_gemm_plan(lexec, dtype, prec, 'N', 'T', lzero, l_target, planes, nullptr, maps());
   r_split = r_target;
   l_split = l_target;
 }
@@ -182,15 +459,21 @@ void HeffPlan::restrict_tasks(const std::vector<char>& keep_r, const std::vector
     replan(r_split, l_split);
     return;
   }
-  DMRGX_CHECK(keep_r.size() == rexec.size() && keep_l.size() == lexec.size(), "restrict_tasks: mask sizes");
-  std::vector<GemmTask> r, l;
+  DMRGX_CHECK(keep_r.size() == rtasks.size() && keep_l.size() == ltasks.size(), "restrict_tasks: mask sizes");
+  // the masks select original tasks; the executed tasks follow their origin
+  std::vector<GemmTask> r, l, ro;
   for (size_t i = 0; i < rexec.size(); ++i)
-    if (keep_r[i]) r.push_back(rexec[i]);
+    if (keep_r[rexec_src[i]]) r.push_back(rexec[i]);
   for (size_t i = 0; i < lexec.size(); ++i)
-    if (keep_l[i]) l.push_back(lexec[i]);
+    if (keep_l[lexec_src[i]]) l.push_back(lexec[i]);
+  for (size_t i = 0; i < rtasks.size(); ++i)
+    if (keep_r[i]) ro.push_back(rtasks[i]);
   const bool planes = prec == PREC_BF16X3;
-  rplan = make_gemm_plan(r, dtype, prec, 'N', 'T', uncovered(r, n_store), r_split, planes);
-  lplan = make_gemm_plan(l, dtype, prec, 'N', 'T', uncovered(l, n_psi), l_split, planes);
+  // dropped outputs read as zero: the ranges that no kept original R task
+  // writes (rows skipped inside kept blocks stay zero) and that no kept
+  // executed L task writes
+  rplan = make_gemm_plan(r, dtype, prec, 'N', 'T', uncovered(ro, n_store), r_split, planes, nullptr, maps());
+  lplan = make_gemm_plan(l, dtype, prec, 'N', 'T', uncovered(l, n_psi), l_split, planes, nullptr, maps());
 }
 
 namespace {
@@ -355,8 +638,8 @@ void HeffPlan::adjoint(const double* psi, const double* g_out, double* g_psi, do
     ad_gp = make_gemm_plan(tb, dtype, aprec, 'T', 'N', uncovered(tb, n_psi));
     ad_gl = make_gemm_plan(la, dtype, aprec, 'N', 'N', uncovered(la, lenv_len));
     ad_gs = make_gemm_plan(lb, dtype, aprec, 'T', 'N', uncovered(lb, n_store));
-    if (aprec != PREC_FP64 && prec == PREC_BF16X3)
-      ad_fr = make_gemm_plan(rtasks, dtype, PREC_FP32, 'N', 'T', uncovered(rtasks, n_store));
+    // store is recomputed over the original tasks (every row written)
+    ad_fr = make_gemm_plan(rtasks, dtype, aprec, 'N', 'T', uncovered(rtasks, n_store));
   }
   // operands in the adjoint precision
   auto as_prec = [&](const double* x, int64_t n, std::unique_ptr<DevBuf>& keep) -> const void* {
@@ -372,7 +655,7 @@ void HeffPlan::adjoint(const double* psi, const double* g_out, double* g_psi, do
   const void* Lv = aprec == PREC_FP64 ? lenv : (lenv_lo ? lenv_lo->ptr : as_prec((const double*)lenv, lenv_len, kl));
   // store = R pass of psi
   DevBuf st(std::max<int64_t>(n_store, 1) * eb);
-  run_gemm_plan(ad_fr ? *ad_fr : *rplan, R, P, st.ptr, s);
+  run_gemm_plan(*ad_fr, R, P, st.ptr, s);
   // conjugated copies of the B operands (complex only)
   auto conj_copy = [&](const void* x, int64_t n) {
     auto b = std::make_unique<DevBuf>(std::max<int64_t>(n, 1) * eb);
@@ -568,13 +851,27 @@ int dmrgx_heff_time(dmrgx_heff* plan, const void* in, void* out, int iters, doub
     l += x;
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
+  const HeffPlan& h = *plan->p;
+  const double cf = h.dtype == DT_C64 ? 8.0 : 2.0;
+  double ar = 0, al = 0;
+  for (auto& g : h.rtasks) ar += cf * (double)g.M * g.N * g.K;
+  for (auto& g : h.ltasks) al += cf * (double)g.M * g.N * g.K;
   ms[0] = tot / iters;
   ms[1] = r / iters;
   ms[2] = l / iters;
-  ms[3] = plan->p->rplan->flops;
-  ms[4] = plan->p->lplan->flops;
+  ms[3] = ar;
+  ms[4] = al;
+  ms[5] = h.rplan->flops;
+  ms[6] = h.lplan->flops;
   CAPI_END
 }
+
+int dmrgx_set_skip_zero_slices(int on) {
+  set_heff_skip_zeros(on != 0);
+  return 0;
+}
+
+double dmrgx_heff_flops_exec(const dmrgx_heff* plan) { return plan && plan->p ? plan->p->flops_exec : 0.0; }
 
 double dmrgx_heff_flops(const dmrgx_heff* plan) { return plan && plan->p ? plan->p->flops : 0.0; }
 

@@ -24,12 +24,18 @@ struct HeffPlan {
   std::unique_ptr<PadLayout> renv_lay, lenv_lay, psi_lay, store_lay;
   bool store_direct = false;  // the R pass writes store's planes itself
   std::unique_ptr<GemmPlan> rplan, lplan;
-  double flops = 0;
+  double flops = 0;       // algorithmic flops of one apply (every task in full, as the reference)
+  double flops_exec = 0;  // flops the pass plans execute (structurally zero slices skipped)
   int64_t renv_len = 0, lenv_len = 0;
   std::vector<GemmTask> rtasks, ltasks;  // the get_map tasks (element offsets of the plain layout)
   // the tasks as the pass plans execute them (padded-plane offsets for
   // BF16x3), the output ranges no task writes, and the chosen split targets
   std::vector<GemmTask> rexec, lexec;
+  // the original task each executed task comes from, and the row indices
+  // that the row-mapped R-pass tasks point into
+  std::vector<int> rexec_src, lexec_src;
+  std::vector<int32_t> row_maps;
+  const std::vector<int32_t>* maps() const { return row_maps.empty() ? nullptr : &row_maps; }
   std::vector<std::pair<int64_t, int64_t>> rzero, lzero;
   int r_split = 0, l_split = 0;
   // rebuild the two pass plans with the given split-K targets (work items
@@ -68,6 +74,10 @@ std::vector<std::pair<int64_t, int64_t>> uncovered(const std::vector<GemmTask>& 
 // histogram keys the tuner has timed in this process.
 void set_heff_autotune(bool on);
 int heff_autotune_keys();
+// Whether make_heff skips the environments' structurally zero slices (default
+// on): R-pass rows that are zero in every task of an output block, and L-pass
+// K columns that are zero in the environment or in the R pass's output.
+void set_heff_skip_zeros(bool on);
 
 std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const int64_t* rblk_off,
                                     const int64_t* rblk_size, int n_rblk, int64_t renv_len,

@@ -325,8 +325,10 @@ class SynthHeff:
         return lib().dmrgx_heff_flops(self.h)
 
     def time(self, x, y, iters):
+        """(ms per apply, ms R-pass kernel, ms L-pass kernel, R flops, L flops,
+        R executed flops, L executed flops)"""
         from . import check, lib
-        ms = np.zeros(5)
+        ms = np.zeros(7)
         check(lib().dmrgx_heff_time(self.h, x.ptr, y.ptr, int(iters), ms.ctypes.data_as(C.POINTER(C.c_double))),
               "dmrgx_heff_time")
         return ms

@@ -93,3 +93,35 @@ def test_task_gemm_direct(gpu):
                                   m.ctypes.data_as(C.POINTER(C.c_int32)), len(m), xd.ptr, yd.ptr, case.n_store))
     y = yd.numpy()
     assert np.max(np.abs(y - ref)) / np.max(np.abs(ref)) < 1e-12
+
+
+@pytest.mark.parametrize("model,kw,L,m,ratio", [
+    ("xyz", dict(jx=1.0, jy=0.8, jz=0.5), 12, 32, 0.30),
+    ("potts3", dict(f=1.0, j=1.0, p=0.0, t=3.14159265359 / 8), 8, 54, 1 / 6),
+    ("kitaev", dict(t=-1.0, p=-1.0, u=0.25), 12, 32, 0.25),
+])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_heff_skips_zero_slices(gpu, model, kw, L, m, ratio, prec):
+    """Real environments carry structurally zero MPO-bond slices (the sectors
+    of a block allow only some operators on w).  The plan skips them: it
+    executes about `ratio` of the algorithmic flops (the reference's count,
+    which dmrgx_heff_flops keeps reporting), and the matvec still equals the
+    oracle's full av(), with the skipping on and off."""
+    print(f"case skip zeros {model} L={L} m={m} {prec}", flush=True)
+    case, _ = dmrg_case(model, L, m, 1, **kw)
+    lib = gpu.lib()
+    err_on, h_on = _run(gpu, case, prec)
+    assert err_on < TOL[prec], err_on
+    assert abs(h_on.flops - case.flops()) < 1e-6 * case.flops()
+    frac = h_on.flops_exec / h_on.flops
+    print(f"  executed / algorithmic flops {frac:.3f}, rel err {err_on:.2e}", flush=True)
+    # the bf16 plane kernels contract 8-aligned K ranges, so a few zero
+    # columns next to each range are contracted too
+    assert ratio - 0.02 < frac < ratio + (0.15 if prec == "bf16x3" else 0.02), frac
+    try:
+        lib.dmrgx_set_skip_zero_slices(0)
+        err_off, h_off = _run(gpu, case, prec)
+    finally:
+        lib.dmrgx_set_skip_zero_slices(1)
+    assert err_off < TOL[prec]
+    assert h_off.flops_exec == h_off.flops
