@@ -43,24 +43,31 @@ METRIC = "DMRG sweep s + H_eff matvec TFLOP/s, mixed vs FP64 GPU at |ΔE0|<1e-8"
 PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0}  # MI355X dense matrix peaks, TFLOP/s
 HBM_PEAK = 8000.0  # GB/s
 
-# HBM bytes per launch of k_gemm_bf3p<complex> on this workload, per work
-# decomposition the autotuner can pick (pass, work items) -> (FETCH_SIZE KiB,
-# WRITE_SIZE KiB per dispatch), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-# runs of `bench.py --only bf16x3` (each candidate is dispatched while the
-# plan is tuned); each counter is scaled by what it reads for a plain 256 MiB
-# copy kernel (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  The roofline's traffic is
-# the mean over the R and L launches of the chosen plan, None if the chosen
-# decomposition was not measured.
+# HBM bytes per launch of k_gemm_bf3p<complex>, per workload and work
+# decomposition the autotuner can pick: (workload, pass, work items) ->
+# (FETCH_SIZE KiB, WRITE_SIZE KiB per dispatch), from rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE runs of `bench.py --only bf16x3` (each candidate is
+# dispatched while the plan is tuned); each counter is scaled by what it reads
+# for a plain 256 MiB copy kernel (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0).  The
+# roofline's traffic is the mean over the R and L launches of the chosen plan,
+# None if the chosen decomposition was not measured.
 TRAFFIC_KIB = {
-    ("R", 320): (47233.0, 47852.5), ("R", 400): (52067.8, 56371.7), ("R", 640): (61582.8, 81920.0),
-    ("L", 64): (82023.6, 8192.0), ("L", 256): (82220.4, 32768.0), ("L", 512): (82258.0, 65536.0),
-    ("L", 640): (82051.6, 81920.0),
+    ("synthetic", "R", 320): (47233.0, 47852.5), ("synthetic", "R", 400): (52067.8, 56371.7),
+    ("synthetic", "R", 640): (61582.8, 81920.0), ("synthetic", "L", 64): (82023.6, 8192.0),
+    ("synthetic", "L", 256): (82220.4, 32768.0), ("synthetic", "L", 512): (82258.0, 65536.0),
+    ("synthetic", "L", 640): (82051.6, 81920.0),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
-def traffic_bytes(r_items, l_items):
-    r, l = TRAFFIC_KIB.get(("R", r_items)), TRAFFIC_KIB.get(("L", l_items))
+def workload_key(real, a):
+    if not real:
+        return "synthetic"
+    return f"real_{a.sweep_model}_L{a.sweep_L}_m{a.sweep_m}"
+
+
+def traffic_bytes(workload, r_items, l_items):
+    r, l = TRAFFIC_KIB.get((workload, "R", r_items)), TRAFFIC_KIB.get((workload, "L", l_items))
     if r is None or l is None:
         return None
     return round(0.5 * sum(f * FETCH_SCALE + w * WRITE_SCALE for f, w in (r, l)) * 1024)
@@ -76,7 +83,30 @@ JXYZ = (1.0, 0.8, 0.5)
 POTTS = (1.0, 1.0, 0.0, 3.14159265359 / 8)  # f, j, p, theta of H_Potts3 (the reference's truncated pi)
 
 
+def time_plan(h, prec, x, steps, warmup):
+    """HIP-event timing of `steps` applies of plan h (after `warmup`)."""
+    import dmrg_amd
+    xd = dmrg_amd.DeviceArray.from_numpy(x)
+    yd = dmrg_amd.DeviceArray(h.n_psi, np.complex128)
+    if warmup:
+        h.time(xd, yd, warmup)
+    ms = h.time(xd, yd, steps)
+    flops = h.flops
+    _, _, r_items, l_items = h.split()
+    out = dict(ms=ms[0], ms_r=ms[1], ms_l=ms[2], flops=flops, flops_r=ms[3], flops_l=ms[4],
+               flops_exec=h.flops_exec, exec_r=ms[5], exec_l=ms[6],
+               tflops=flops / (ms[0] * 1e-3) / 1e12, exec_tflops=h.flops_exec / (ms[0] * 1e-3) / 1e12,
+               n_psi=h.n_psi, r_items=r_items, l_items=l_items)
+    print(f"[bench] {prec}: {ms[0]:.3f} ms/matvec, {out['tflops']:.1f} TFLOP/s algorithmic, "
+          f"{out['exec_tflops']:.1f} executed (R {ms[1]:.3f} ms, L {ms[2]:.3f} ms)", file=sys.stderr, flush=True)
+    xd.free()
+    yd.free()
+    return out
+
+
 def gpu_leg(steps, warmup, only=None):
+    """The matvec on dense synthetic environments of the m=512 shapes (every
+    MPO-bond slice nonzero, so nothing is skipped)."""
     import dmrg_amd
     from dmrg_amd.driver import SynthHeff
 
@@ -89,19 +119,39 @@ def gpu_leg(steps, warmup, only=None):
         h = SynthHeff(S, D, bond, complex_=True, prec=prec, seed=20261203)
         rng = np.random.default_rng(1)
         x = rng.uniform(-1, 1, h.n_psi) + 1j * rng.uniform(-1, 1, h.n_psi)
-        xd = dmrg_amd.DeviceArray.from_numpy(x)
-        yd = dmrg_amd.DeviceArray(h.n_psi, np.complex128)
-        if warmup:
-            h.time(xd, yd, warmup)
-        ms = h.time(xd, yd, steps)
-        flops = h.flops
-        _, _, r_items, l_items = h.split()
-        out[prec] = dict(ms=ms[0], ms_r=ms[1], ms_l=ms[2], flops=flops, flops_r=ms[3], flops_l=ms[4],
-                         tflops=flops / (ms[0] * 1e-3) / 1e12, n_psi=h.n_psi, r_items=r_items, l_items=l_items)
-        print(f"[bench] {prec}: {ms[0]:.3f} ms/matvec, {out[prec]['tflops']:.1f} TFLOP/s "
-              f"(R {ms[1]:.3f} ms, L {ms[2]:.3f} ms)", file=sys.stderr, flush=True)
+        out[prec] = time_plan(h, prec, x, steps, warmup)
         h.close()
     return out
+
+
+def state_leg(state, steps, warmup, only=None):
+    """The matvec of the real operator: the two-site H_eff at the centre bond
+    of a converged state (environments rebuilt from its MPS), applied to that
+    state's two-site vector."""
+    import ctypes as C
+    import dmrg_amd
+
+    out = {}
+    for prec in ("fp64", "fp32", "bf16x3"):
+        if only and prec not in only:
+            continue
+        h = state.centre_heff(prec)
+        x = np.zeros(h.n_psi, np.complex128)
+        dmrg_amd.check(dmrg_amd.lib().dmrgx_memcpy_d2h(x.ctypes.data_as(C.c_void_p), C.c_void_p(h.psi_ptr),
+                                                       x.nbytes), "d2h")
+        out[prec] = time_plan(h, prec, x, steps, warmup)
+        h.close()
+    return out
+
+
+def quick_state(model, L, m):
+    """A converged-enough state for profiling the real operator without the
+    full sweep comparison: BF16x3 growth and one sweep at m."""
+    from dmrg_amd import driver
+    params = {"xyz": JXYZ, "potts3": POTTS}[model]
+    r = driver.run_dmrg(model, L, m, 1, params, schedule="dmrg2", prec="bf16x3", tol_lo=1e-4, refine_k=8,
+                        ladder=False, keep_state=True)
+    return r.state
 
 
 def configs_leg(reps=5, warmup=2):
@@ -282,7 +332,7 @@ def cpu_sweep_leg(L=40, m=64, sweeps=5):
             "gpu_err": abs(g.energy - e_bdg), "cpu_err": abs(e_cpu - e_bdg)}
 
 
-def sweep_leg(model, L, m, sweeps, ladder):
+def sweep_leg(model, L, m, sweeps, ladder, keep=None):
     """Full two-site DMRG (the reference's dmrg2 schedule: growth at cutoff 10,
     the optional pre-sweep cutoff ladder, then full sweeps at m), FP64 against
     BF16x3 with the FP64 Jacobi-Davidson refinement; sweep_s is the wall time
@@ -310,9 +360,12 @@ def sweep_leg(model, L, m, sweeps, ladder):
     res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
     # |<psi_bf16x3|psi_fp64>| of the two final MPSs (the reference's calc(mps, mps))
     res["fidelity"] = states["bf16x3"].fidelity(states["fp64"])
-    for st in states.values():
-        st.close()
+    for p, st in states.items():
+        if p != keep:
+            st.close()
     res["speedup"] = round(res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12), 3)
+    if keep:
+        return res, states[keep]
     return res
 
 
@@ -406,6 +459,10 @@ def main():
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="also run the host Kitaev L=40 m=64 5-sweep baseline beside the GPU (about 1-2 minutes)")
     ap.add_argument("--only", default="", help="profiling aid: comma-separated precisions to run (no JSON line)")
+    ap.add_argument("--matvec-from", default="sweep", choices=["sweep", "quick", "synthetic"],
+                    help="operator of the matvec timing: the centre of the sweep comparison's FP64 state "
+                         "(default; synthetic when --sweep 0), of a quick BF16x3 run (growth + one sweep), or dense "
+                         "synthetic environments")
     ap.add_argument("--part", default="", choices=["", "svd", "copy", "sweep", "cpusweep"],
                     help="run one part alone: the step SVD, the copy kernel used to calibrate PMC counters, the "
                          "sweep comparison or the host Kitaev sweep (JSON of that part only)")
@@ -427,14 +484,44 @@ def main():
         print(json.dumps(cpu_sweep_leg()), flush=True)
         return
     if a.only:
-        gpu_leg(a.steps, a.warmup, a.only.split(","))
+        if a.matvec_from == "quick":
+            st = quick_state(a.sweep_model, a.sweep_L, a.sweep_m)
+            state_leg(st, a.steps, a.warmup, a.only.split(","))
+            st.close()
+        else:
+            gpu_leg(a.steps, a.warmup, a.only.split(","))
         return
-    g = gpu_leg(a.steps, a.warmup)
+    # the sweep comparison first: the matvec is then timed on the real operator
+    # at the centre of its converged FP64 state
+    sweep, state = None, None
+    if a.sweep:
+        sweep, state = sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder, keep="fp64")
+        if a.matvec_from != "sweep":
+            state.close()
+            state = None
+    if a.matvec_from == "quick":
+        state = quick_state(a.sweep_model, a.sweep_L, a.sweep_m)
+    if state is not None:
+        g = state_leg(state, a.steps, a.warmup)
+        state.close()
+        name = {"xyz": "XYZ", "potts3": "chiral Z3 Potts"}[a.sweep_model]
+        workload = (f"two-site H_eff matvec at the centre bond of the converged open {name} chain "
+                    f"L={a.sweep_L} m={a.sweep_m} ({'FP64 sweep state' if a.sweep else 'BF16x3 state after one sweep'}"
+                    f"; its real environments, whose structurally zero MPO-bond slices the plan skips)")
+        data = "real operator of a DMRG run on the model (no external data)"
+    else:
+        g = gpu_leg(a.steps, a.warmup)
+        workload = ("two-site H_eff matvec with the block structure of the open XYZ chain L=100 m=512 (Z2 sectors "
+                    "256+256, MPO bond 5, complex; 32+32 get_map tasks, 4.295e10 flop) on dense synthetic "
+                    "environments")
+        data = "synthetic"
     mixed, fp64 = g["bf16x3"], g["fp64"]
     # dominant kernel: the BF16x3 grouped GEMM (R and L pass launches)
     per_launch_ms = 0.5 * (mixed["ms_r"] + mixed["ms_l"])
     per_launch_flop = 0.5 * (mixed["flops_r"] + mixed["flops_l"])
+    per_launch_exec = 0.5 * (mixed["exec_r"] + mixed["exec_l"])
     achieved = per_launch_flop / (per_launch_ms * 1e-3) / 1e12
+    achieved_exec = per_launch_exec / (per_launch_ms * 1e-3) / 1e12
     line = {
         "metric": METRIC,
         "value": round(mixed["tflops"], 3),
@@ -447,32 +534,34 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "complex128 vectors; bf16x3 split products, fp32 accumulate",
-        "data": "synthetic",
-        "config": {"workload": "two-site H_eff matvec of the open XYZ chain L=100 m=512 (Z2 sectors 256+256, "
-                               "MPO bond 5, complex; 32+32 get_map tasks, 4.295e10 flop)",
-                   "model": "XYZ", "L": L_CHAIN, "m": M, "jxyz": list(JXYZ), "prec": "bf16x3"},
+        "data": data,
+        "config": {"workload": workload, "model": a.sweep_model, "L": a.sweep_L, "m": a.sweep_m,
+                   "params": list(JXYZ if a.sweep_model == "xyz" else POTTS), "prec": "bf16x3",
+                   "flop_algorithmic": mixed["flops"], "flop_executed": mixed["flops_exec"]},
+        "exec_tflops": round(mixed["exec_tflops"], 3),
         "fp64_tflops": round(fp64["tflops"], 3),
+        "fp64_exec_tflops": round(fp64["exec_tflops"], 3),
         "fp32_tflops": round(g["fp32"]["tflops"], 3),
         "fp64_ms_per_matvec": round(fp64["ms"], 4),
         "mixed_vs_fp64_matvec": round(fp64["ms"] / mixed["ms"], 3),
-        "fp64_roofline_frac": round(fp64["flops"] / (fp64["ms"] * 1e-3) / 1e12 / PEAK["fp64"], 4),
+        "fp64_roofline_frac_executed": round(fp64["exec_tflops"] / PEAK["fp64"], 4),
         "roofline": {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK["bf16x3"], "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK["bf16x3"], 5),
-                     "traffic": traffic_bytes(mixed["r_items"], mixed["l_items"]),
+                     "achieved_executed": round(achieved_exec, 3),
+                     "frac_executed": round(achieved_exec / PEAK["bf16x3"], 5),
+                     "traffic": traffic_bytes(workload_key(state is not None, a), mixed["r_items"], mixed["l_items"]),
                      "work_items": {"R": mixed["r_items"], "L": mixed["l_items"]},
                      "algorithmic_bytes": ALG_BYTES,
                      "kernel": "k_gemm_bf3p<complex> (R and L pass launches)",
-                     "issued_tflops": round(3 * achieved, 3)},
+                     "issued_tflops_executed": round(3 * achieved_exec, 3)},
         "cpu_baseline": None,
         "configs": None,
-        "sweep": None,
+        "sweep": sweep,
         "batch": None,
         "cpu_sweep": None,
     }
     if not a.no_configs:
         line["configs"] = configs_leg()
-    if a.sweep:
-        line["sweep"] = sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder)
     if a.batch:
         line["batch"] = batch_leg(sweeps=a.batch)
     if not a.no_cpu:

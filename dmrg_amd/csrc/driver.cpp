@@ -17,6 +17,7 @@ using namespace dmrgx;
 // a finished run's MPS (centre absorbed), one of r / c set
 struct dmrgx_state {
   int model = 0, L = 0, sector = 0;
+  double p[4] = {0, 0, 0, 0};  // the model parameters of the run
   bool signs = false;  // graded signs on: fermionic model, or the reference's sign rule
   std::unique_ptr<mps<double>> r, raw_r;  // prepared (centre absorbed) and as left by the run
   std::unique_ptr<mps<cplx>> c, raw_c;
@@ -316,6 +317,7 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double*
     out = std::make_unique<dmrgx_state>();
     out->model = c->model;
     out->L = c->L;
+    for (int i = 0; i < 4; ++i) out->p[i] = c->p[i];
     out->signs = model_fermion(c->model) || c->sign_rule == DMRGX_SIGNS_REFERENCE;
     out->sector = c->sector;
   }
@@ -448,6 +450,35 @@ int dmrgx_state_derivative(const dmrgx_state* a, const double* p, const double* 
     out[0] = centre_derivative(*a->raw_r, W, dW, a->sector, a->signs, prec, &e);
   }
   out[1] = e;
+  CAPI_END
+}
+
+int dmrgx_state_heff_create(const dmrgx_state* a, int prec, dmrgx_heff** plan, int64_t* n_psi, const void** psi) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && plan && n_psi, "state_heff_create: null arguments");
+  DMRGX_CHECK(prec >= PREC_FP64 && prec <= PREC_BF16X3, "state_heff_create: unknown precision");
+  DMRGX_CHECK(a->raw_c || a->raw_r, "state_heff_create: the state holds no MPS");
+  ensure_init();
+  auto h = std::make_unique<dmrgx_heff>();
+  auto take = [&](auto& c) {
+    *n_psi = get_dimension(c.psi.dim_, c.psi.sym_);
+    h->p = std::move(c.plan);
+    h->keep.push_back(c.lenv.buf_);
+    h->keep.push_back(c.renv.buf_);
+    h->keep.push_back(c.psi.buf_);
+    if (psi) *psi = c.psi.data();
+  };
+  if (a->raw_c) {
+    mpo<cplx> W = model_mpo_c(a->model, a->p, a->L);
+    auto c = centre_operator(*a->raw_c, W, a->sector, a->signs, prec);
+    take(c);
+  } else {
+    mpo<double> W = model_mpo_r(a->model, a->p, a->L);
+    auto c = centre_operator(*a->raw_r, W, a->sector, a->signs, prec);
+    take(c);
+  }
+  DMRGX_HIP(hipStreamSynchronize(rt().stream));
+  *plan = h.release();
   CAPI_END
 }
 

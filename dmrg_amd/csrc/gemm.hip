@@ -262,7 +262,7 @@ __global__ __launch_bounds__(NT) void k_gemm4(const GemmItem* __restrict__ items
         int row = it.m0 + wm + i * 16 + (F64 ? r * 4 + (lane >> 4) : (lane >> 4) * 4 + r);
         int col = it.n0 + wn + j * 16 + (lane & 15);
         if (row < it.M && col < it.N) {
-          if (it.row_map >= 0) row = RM[it.row_map + row];
+          if (!partial && it.row_map >= 0) row = RM[it.row_map + row];
           int64_t e = it.c_off + (int64_t)row * it.N + col;
           if (CPLX) {
             S vr = acc[0][i][j][r], vi = acc[NP - 1][i][j][r];
@@ -660,7 +660,7 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
         int row = it.m0 + wm + i * 16 + (lane >> 4) * 4 + r;
         int col = it.n0 + wn + j * 16 + (lane & 15);
         if (row < it.M && col < it.N) {
-          if (it.row_map >= 0) row = RM[it.row_map + row];
+          if (!partial && it.row_map >= 0) row = RM[it.row_map + row];
           int64_t e = it.c_off + (int64_t)row * it.N + col;
           const float vr = acc[0][i][j][r], vi = CPLX ? acc[NP - 1][i][j][r] : 0.0f;
           if (partial || OUT == OUT_F32) {
@@ -964,8 +964,9 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
         for (int m0 = 0; m0 < g.M; m0 += BM) {
           GemmItem itm{};
           itm.c_off = shared ? w_off + (int64_t)c * len : g.c_off;
-          // partial tiles are compact in the workspace; the reduction maps rows
-          itm.row_map = shared ? -1 : g.row_map;
+          // A rows always go through the row map; a partial tile is written
+          // compact into the workspace and the reduction maps its rows
+          itm.row_map = g.row_map;
           itm.M = g.M;
           itm.N = g.N;
           itm.m0 = m0;

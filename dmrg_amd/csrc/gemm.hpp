@@ -37,7 +37,8 @@ struct GemmSeg {
 };
 struct GemmItem {
   int64_t c_off;
-  int64_t row_map;  // offset of the output block's row indices in the row-map table, or -1
+  int64_t row_map;  // offset of the block's row indices in the row-map table (A rows, and output rows
+                    // unless the item writes a compact partial tile), or -1
   int M, N;
   int m0, n0;
   int seg_begin, seg_end;

@@ -159,6 +159,76 @@ double block_dot(const qtensor<T>& ref, const void* g, const qtensor<T>& x) {
 
 }  // namespace
 
+// the two-site state U_l S V_r of a finished MPS in the block order
+// (dim, sym) of the eigensolver
+template <typename T>
+qtensor<T> centre_psi(const mps<T>& m, const vector<vector<int>>& dim, const vector<vector<int>>& sym) {
+  const int l = m.cut_ - 1, r = m.cut_;
+  qtensor<T> us, two;
+  us.contract(m.state_[l], centre_as<T>(m.singular_));
+  two.contract(us, m.state_[r]);
+  qtensor<T> psi;
+  psi.dim_ = dim;
+  psi.sym_ = sym;
+  psi.dir_.assign(dim.size(), 0);
+  psi.layout();
+  psi.allocate();
+  auto code = [&](const vector<int>& s) {
+    int64_t c = 0;
+    for (size_t j = 0; j < dim.size(); ++j) c = c * dim[j].size() + s[j];
+    return c;
+  };
+  std::map<int64_t, int> at;
+  for (int i = 0; i < two.nblocks(); ++i) at[code(two.sym_[i])] = i;
+  for (int b = 0; b < psi.nblocks(); ++b) {
+    auto it = at.find(code(psi.sym_[b]));
+    if (it == at.end()) fill_zero(psi.block(b), psi.bsize(b) * sizeof(T), rt().stream);
+    else {
+      DMRGX_CHECK(two.bsize(it->second) == psi.bsize(b), "centre state: two-site block shapes differ");
+      copy_d2d(psi.block(b), two.block(it->second), psi.bsize(b) * sizeof(T), rt().stream);
+    }
+  }
+  return psi;
+}
+
+template <typename T>
+std::unique_ptr<HeffPlan> make_centre_plan(const qtensor<T>& Lv, const qtensor<T>& Rv, const vector<vector<int>>& dim,
+                                           const vector<vector<int>>& sym, bool alpha, int prec) {
+  vector<vector<int>> rmap, lmap, dim_ret, sym_ret, dim2;
+  DMRGX_CHECK(Rv.get_map(rmap, dim_ret, sym_ret, dim, sym, 'N', 'T', 2, 0, false, alpha), "get_map (rmap)");
+  const int64_t d_store = get_dimension(dim_ret, sym_ret);
+  vector<vector<int>> psym = sym;
+  DMRGX_CHECK(Lv.get_map(lmap, dim2, psym, dim_ret, sym_ret, 'N', 'T', 3, 0, true, alpha), "get_map (lmap)");
+  vector<int32_t> rm, lm;
+  for (auto& t : rmap) rm.insert(rm.end(), t.begin(), t.end());
+  for (auto& t : lmap) lm.insert(lm.end(), t.begin(), t.end());
+  vector<int64_t> rsz(Rv.nblocks()), lsz(Lv.nblocks());
+  for (int b = 0; b < Rv.nblocks(); ++b) rsz[b] = Rv.bsize(b);
+  for (int b = 0; b < Lv.nblocks(); ++b) lsz[b] = Lv.bsize(b);
+  return make_heff(dtype_of<T>::value, prec, Rv.data(), Rv.off_.data(), rsz.data(), Rv.nblocks(), Rv.total_,
+                   rm.data(), (int)rmap.size(), Lv.data(), Lv.off_.data(), lsz.data(), Lv.nblocks(), Lv.total_,
+                   lm.data(), (int)lmap.size(), get_dimension(dim, sym), d_store);
+}
+
+template <typename T>
+CentreOp<T> centre_operator(const mps<T>& m, mpo<T>& W, int sector, bool alpha, int prec) {
+  const int n = m.size();
+  const int l = m.cut_ - 1, r = m.cut_;
+  DMRGX_CHECK(l >= 1 && r <= n - 2, "centre operator: the centre must have a site on each side");
+  CentreOp<T> c;
+  qtensor<T> S = left_sandwich(W[0], m.state_[0], 1);
+  for (int k = 1; k < l; ++k) S = left_sandwich(lenv_of(S, W[k]), m.state_[k], 2);
+  c.lenv = lenv_of(S, W[l]);
+  qtensor<T> Q = right_sandwich(W[n - 1], m.state_[n - 1], 1);
+  for (int k = n - 2; k > r; --k) Q = right_sandwich(renv_of(Q, W[k]), m.state_[k], 2);
+  c.renv = renv_of(Q, W[r]);
+  vector<vector<int>> dim, sym;
+  generate_dim_sym(dim, sym, c.lenv, 2, c.renv, 2, sector);
+  c.psi = centre_psi(m, dim, sym);
+  c.plan = make_centre_plan(c.lenv, c.renv, dim, sym, alpha, prec);
+  return c;
+}
+
 template <typename T>
 double centre_derivative(const mps<T>& m, mpo<T>& W, mpo<T>& dW, int sector, bool alpha, int prec, double* energy) {
   const int n = m.size();
@@ -181,54 +251,14 @@ double centre_derivative(const mps<T>& m, mpo<T>& W, mpo<T>& dW, int sector, boo
     dQ = right_sandwich(de, m.state_[k], 2);
   }
   qtensor<T> Rv = renv_of(Q, W[r]), dR = add(renv_of(dQ, W[r]), renv_of(Q, dW[r]));
-  // the two-site state U_l S V_r in the block order of the eigensolver
   vector<vector<int>> dim, sym;
   generate_dim_sym(dim, sym, Lv, 2, Rv, 2, sector);
   const int64_t d = get_dimension(dim, sym);
-  qtensor<T> us, two;
-  us.contract(m.state_[l], centre_as<T>(m.singular_));
-  two.contract(us, m.state_[r]);
-  qtensor<T> psi;
-  psi.dim_ = dim;
-  psi.sym_ = sym;
-  psi.dir_.assign(dim.size(), 0);
-  psi.layout();
-  psi.allocate();
-  {
-    auto code = [&](const vector<int>& s) {
-      int64_t c = 0;
-      for (size_t j = 0; j < dim.size(); ++j) c = c * dim[j].size() + s[j];
-      return c;
-    };
-    std::map<int64_t, int> at;
-    for (int i = 0; i < two.nblocks(); ++i) at[code(two.sym_[i])] = i;
-    for (int b = 0; b < psi.nblocks(); ++b) {
-      auto it = at.find(code(psi.sym_[b]));
-      if (it == at.end()) fill_zero(psi.block(b), psi.bsize(b) * sizeof(T), rt().stream);
-      else {
-        DMRGX_CHECK(two.bsize(it->second) == psi.bsize(b), "centre_derivative: two-site block shapes differ");
-        copy_d2d(psi.block(b), two.block(it->second), psi.bsize(b) * sizeof(T), rt().stream);
-      }
-    }
-  }
+  qtensor<T> psi = centre_psi(m, dim, sym);
   // H_eff plan of (Lv, Rv) and the adjoint for the output cotangent psi
-  vector<vector<int>> rmap, lmap, dim_ret, sym_ret, dim2;
-  DMRGX_CHECK(Rv.get_map(rmap, dim_ret, sym_ret, dim, sym, 'N', 'T', 2, 0, false, alpha), "get_map (rmap)");
-  const int64_t d_store = get_dimension(dim_ret, sym_ret);
-  vector<vector<int>> psym = sym;
-  DMRGX_CHECK(Lv.get_map(lmap, dim2, psym, dim_ret, sym_ret, 'N', 'T', 3, 0, true, alpha), "get_map (lmap)");
-  vector<int32_t> rm, lm;
-  for (auto& t : rmap) rm.insert(rm.end(), t.begin(), t.end());
-  for (auto& t : lmap) lm.insert(lm.end(), t.begin(), t.end());
-  vector<int64_t> rsz(Rv.nblocks()), lsz(Lv.nblocks());
-  for (int b = 0; b < Rv.nblocks(); ++b) rsz[b] = Rv.bsize(b);
-  for (int b = 0; b < Lv.nblocks(); ++b) lsz[b] = Lv.bsize(b);
-  const int dt = dtype_of<T>::value;
-  auto plan = make_heff(dt, prec, Rv.data(), Rv.off_.data(), rsz.data(), Rv.nblocks(), Rv.total_, rm.data(),
-                        (int)rmap.size(), Lv.data(), Lv.off_.data(), lsz.data(), Lv.nblocks(), Lv.total_, lm.data(),
-                        (int)lmap.size(), d, d_store);
+  auto plan = make_centre_plan(Lv, Rv, dim, sym, alpha, prec);
   hipStream_t s = rt().stream;
-  const bool cx = dt == DT_C64;
+  const bool cx = dtype_of<T>::value == DT_C64;
   DevBuf out(std::max<int64_t>(d, 1) * sizeof(T)), gR(std::max<int64_t>(Rv.total_, 1) * sizeof(T)),
       gL(std::max<int64_t>(Lv.total_, 1) * sizeof(T));
   plan->apply((const double*)psi.data(), (double*)out.ptr, s);
@@ -241,6 +271,8 @@ double centre_derivative(const mps<T>& m, mpo<T>& W, mpo<T>& dW, int sector, boo
   return der / nn[0];
 }
 
+template CentreOp<double> centre_operator(const mps<double>&, mpo<double>&, int, bool, int);
+template CentreOp<cplx> centre_operator(const mps<cplx>&, mpo<cplx>&, int, bool, int);
 template double centre_derivative(const mps<double>&, mpo<double>&, mpo<double>&, int, bool, int, double*);
 template double centre_derivative(const mps<cplx>&, mpo<cplx>&, mpo<cplx>&, int, bool, int, double*);
 

@@ -1,6 +1,9 @@
 // Observables of a finished MPS (src/functions.cpp:394-511, src/mps.cpp:33-54).
 #pragma once
 
+#include <memory>
+
+#include "heff.hpp"
 #include "qtensor.hpp"
 
 namespace dmrgx {
@@ -37,5 +40,17 @@ T calc(const mps<T>& m, mpo<T>& o, bool fermion = true);
 template <typename T>
 double centre_derivative(const mps<T>& m, mpo<T>& W, mpo<T>& dW, int sector, bool alpha, int prec,
                          double* energy);
+
+// The two-site operator at the centre of a finished (unprepared) MPS: both
+// environments recomputed from the MPS site by site, the two-site state U S V
+// in the eigensolver's block order, and the H_eff plan (arithmetic `prec`)
+// over them; the plan reads lenv / renv, which must outlive it.
+template <typename T>
+struct CentreOp {
+  qtensor<T> lenv, renv, psi;
+  std::unique_ptr<HeffPlan> plan;
+};
+template <typename T>
+CentreOp<T> centre_operator(const mps<T>& m, mpo<T>& W, int sector, bool alpha, int prec);
 
 }  // namespace dmrgx

@@ -11,7 +11,8 @@ MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
 STEP_LOG_WIDTH = 19
 DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym",
                   "dmrgx_run_dmrg_state", "dmrgx_state_norm", "dmrgx_state_overlap", "dmrgx_state_expect",
-                  "dmrgx_state_destroy", "dmrgx_state_derivative", "dmrgx_run_batch", "dmrgx_parse_args"]
+                  "dmrgx_state_destroy", "dmrgx_state_derivative", "dmrgx_run_batch", "dmrgx_parse_args",
+                  "dmrgx_state_heff_create"]
 BATCH_LOG_WIDTH = 11
 
 
@@ -54,6 +55,9 @@ def declare(lib):
     lib.dmrgx_state_expect.restype = C.c_int
     lib.dmrgx_state_derivative.argtypes = [C.c_void_p, dp, dp, C.c_int, dp]
     lib.dmrgx_state_derivative.restype = C.c_int
+    lib.dmrgx_state_heff_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_void_p), C.POINTER(C.c_int64),
+                                            C.POINTER(C.c_void_p)]
+    lib.dmrgx_state_heff_create.restype = C.c_int
     lib.dmrgx_run_batch.argtypes = [C.POINTER(RunConfig), C.c_int, dp, dp, dp, C.POINTER(RunStats), dp, C.c_int,
                                     C.POINTER(C.c_int), dp]
     lib.dmrgx_run_batch.restype = C.c_int
@@ -105,6 +109,16 @@ class State:
         check(lib().dmrgx_state_derivative(self.h, pad(params), pad(dparams), PREC_NAMES[prec],
                                            out.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_state_derivative")
         return float(out[0]), float(out[1])
+
+    def centre_heff(self, prec="fp64"):
+        """The H_eff plan at the centre bond of this state (environments
+        recomputed from the MPS); its .psi_ptr is the two-site state."""
+        from . import C64, PREC_NAMES, R64, check, lib
+        h, n, psi = C.c_void_p(), C.c_int64(), C.c_void_p()
+        check(lib().dmrgx_state_heff_create(self.h, PREC_NAMES[prec], C.byref(h), C.byref(n), C.byref(psi)),
+              "dmrgx_state_heff_create")
+        dt = C64 if self.model in ("xyz", "potts3") else R64
+        return SynthHeff.from_handle(h, n.value, dt, psi.value)
 
     def fidelity(self, other: "State") -> float:
         """|<b|a>| / sqrt(<a|a> <b|b>)"""
@@ -306,6 +320,18 @@ def generate_dim_sym(symmetry, ldim, lsym, ldir, lnum, rdim, rsym, rdir, rnum, s
 
 class SynthHeff:
     """A config-shaped synthetic two-site operator built by the library."""
+
+    @classmethod
+    def from_handle(cls, h, n_psi, dtype, psi_ptr=None):
+        """Wrap an existing plan handle (e.g. State.centre_heff)."""
+        o = cls.__new__(cls)
+        o.h, o.n_psi, o.dtype, o.psi_ptr = h, n_psi, dtype, psi_ptr
+        return o
+
+    @property
+    def flops_exec(self):
+        from . import lib
+        return lib().dmrgx_heff_flops_exec(self.h)
 
     def __init__(self, S, D, bond, complex_=True, prec="fp64", seed=20261203, fermion=False):
         from . import C64, PREC_NAMES, R64, check, init, lib

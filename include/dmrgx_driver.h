@@ -112,6 +112,14 @@ int dmrgx_state_expect(const dmrgx_state* a, int model, const double* p, double*
  * cotangents for the output cotangent psi (Hellmann-Feynman at fixed MPS
  * basis).  out = {dE0/dt, E0 of the centre}; prec DMRGX_FP64 or DMRGX_FP32. */
 int dmrgx_state_derivative(const dmrgx_state* a, const double* p, const double* dp, int prec, double* out);
+/* The two-site H_eff at the centre bond of a finished run's state (the
+ * operator its last step solved, rebuilt): both environments recomputed from
+ * the MPS site by site (src/core.cpp:188-192, 266-271), the get_map task lists
+ * (src/qtensor.cpp:997-1119) and a plan of arithmetic `prec` that owns the
+ * environments.  *psi (may be NULL) receives the device address of the
+ * state's two-site vector in the plan's psi layout (n_psi elements, owned by
+ * the plan).  No counterpart in the reference beyond av() itself. */
+int dmrgx_state_heff_create(const dmrgx_state* a, int prec, dmrgx_heff** plan, int64_t* n_psi, const void** psi);
 int dmrgx_state_destroy(dmrgx_state* a);
 
 /* Synthetic two-site operator with the block structure of a Z_S model at bond

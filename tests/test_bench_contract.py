@@ -19,8 +19,8 @@ def test_traffic_lookup():
     sys.path.insert(0, ROOT)
     import bench
 
-    t = bench.traffic_bytes(400, 256)
-    r, l = bench.TRAFFIC_KIB[("R", 400)], bench.TRAFFIC_KIB[("L", 256)]
+    t = bench.traffic_bytes("synthetic", 400, 256)
+    r, l = bench.TRAFFIC_KIB[("synthetic", "R", 400)], bench.TRAFFIC_KIB[("synthetic", "L", 256)]
     expect = 0.5 * ((r[0] + l[0]) * bench.FETCH_SCALE + (r[1] + l[1]) * bench.WRITE_SCALE) * 1024
     assert t == round(expect)
-    assert bench.traffic_bytes(401, 256) is None
+    assert bench.traffic_bytes("synthetic", 401, 256) is None

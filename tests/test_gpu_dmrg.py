@@ -112,3 +112,23 @@ def test_mixed_precision_energy(gpu, prec):
     mix = driver.run_dmrg("xyz", L, m, 2, XYZ, ladder=False, prec=prec,
                           tol_lo=1e-7 if prec == "fp32" else 1e-5, refine_k=10)
     assert abs(mix.energy - ref.energy) < 1e-8, (mix.energy, ref.energy)
+
+
+@pytest.mark.parametrize("model,params", [("xyz", XYZ), ("potts3", POTTS)])
+def test_zero_slice_skipping_sweep(gpu, model, params):
+    """A sweep whose operators are large enough for split-K partial tiles in
+    the row-mapped R-pass groups: every step's energy with the zero MPO-bond
+    slices skipped equals the one with every slice contracted."""
+    L, m = (20, 96) if model == "xyz" else (12, 108)
+    print(f"case skip zeros sweep {model} L={L} m={m}", flush=True)
+    lib = gpu.lib()
+    on = driver.run_dmrg(model, L, m, 1, params, schedule="dmrg2", ladder=False)
+    try:
+        lib.dmrgx_set_skip_zero_slices(0)
+        off = driver.run_dmrg(model, L, m, 1, params, schedule="dmrg2", ladder=False)
+    finally:
+        lib.dmrgx_set_skip_zero_slices(1)
+    assert len(on.steps) == len(off.steps)
+    d = np.abs(on.steps[:, 2] - off.steps[:, 2])
+    print(f"  max step |dE| {d.max():.2e}, E0 {on.energy:.12f}", flush=True)
+    assert d.max() < 1e-8 * abs(on.energy), d.max()

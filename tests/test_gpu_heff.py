@@ -125,3 +125,24 @@ def test_heff_skips_zero_slices(gpu, model, kw, L, m, ratio, prec):
         lib.dmrgx_set_skip_zero_slices(1)
     assert err_off < TOL[prec]
     assert h_off.flops_exec == h_off.flops
+    # split-K decompositions: partial tiles of row-mapped output groups (their
+    # A rows through the row map, the tile compact, the reduction mapping rows)
+    import ctypes as C
+    n = C.c_int(0)
+    gpu.check(lib.dmrgx_device_info(C.byref(n), None, 0), "dmrgx_device_info")
+    rng = np.random.default_rng(9)
+    x = rng.standard_normal(case.n_psi) + (1j * rng.standard_normal(case.n_psi) if h_on.dtype == gpu.C64 else 0)
+    x = x.astype(case.dtype)
+    ref = case.oracle_apply(x)
+    xd = gpu.DeviceArray.from_numpy(x)
+    yd = gpu.DeviceArray(case.n_psi, case.dtype)
+    items = set()
+    for rt, lt in [(8 * n.value, 8 * n.value), (32 * n.value, 16 * n.value), (-n.value, -n.value)]:
+        gpu.check(lib.dmrgx_heff_set_split(h_on.h, rt, lt), "dmrgx_heff_set_split")
+        out = (C.c_int * 4)()
+        gpu.check(lib.dmrgx_heff_get_split(h_on.h, out), "dmrgx_heff_get_split")
+        items.add((out[2], out[3]))
+        h_on.apply(xd, yd)
+        err = np.max(np.abs(yd.numpy() - ref)) / np.max(np.abs(ref))
+        assert err < TOL[prec], (rt, lt, err)
+    assert len(items) > 1
