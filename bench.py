@@ -332,7 +332,7 @@ def cpu_sweep_leg(L=40, m=64, sweeps=5):
             "gpu_err": abs(g.energy - e_bdg), "cpu_err": abs(e_cpu - e_bdg)}
 
 
-def sweep_leg(model, L, m, sweeps, ladder, keep=None):
+def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
     """Full two-site DMRG (the reference's dmrg2 schedule: growth at cutoff 10,
     the optional pre-sweep cutoff ladder, then full sweeps at m), FP64 against
     BF16x3 with the FP64 Jacobi-Davidson refinement; sweep_s is the wall time
@@ -343,7 +343,7 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None):
     res = {"model": model, "L": L, "m": m, "sweeps": sweeps, "ladder": bool(ladder), "params": list(params),
            "timed": "last full sweep"}
     states = {}
-    for prec in ("fp64", "bf16x3"):
+    for prec in precs:
         t0 = time.perf_counter()
         r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=1e-4, refine_k=8,
                             ladder=ladder, keep_state=True, verbose="stderr")
@@ -357,13 +357,14 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None):
         print(f"[bench] sweep {prec} {model} L={L} m={m}: E0={r.energy:.12f} last sweep {st['last_sweep_seconds']:.2f} s "
               f"(eig {st['t_eig']:.2f} svd {st['t_svd']:.2f} env {st['t_env']:.2f} s in all steps)", file=sys.stderr,
               flush=True)
-    res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
-    # |<psi_bf16x3|psi_fp64>| of the two final MPSs (the reference's calc(mps, mps))
-    res["fidelity"] = states["bf16x3"].fidelity(states["fp64"])
+    if len(precs) == 2:
+        res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
+        # |<psi_bf16x3|psi_fp64>| of the two final MPSs (the reference's calc(mps, mps))
+        res["fidelity"] = states["bf16x3"].fidelity(states["fp64"])
+        res["speedup"] = round(res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12), 3)
     for p, st in states.items():
         if p != keep:
             st.close()
-    res["speedup"] = round(res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12), 3)
     if keep:
         return res, states[keep]
     return res
@@ -455,6 +456,7 @@ def main():
     ap.add_argument("--sweep-L", type=int, default=L_CHAIN, help="chain length of the sweep comparison")
     ap.add_argument("--sweep-m", type=int, default=M, help="bond dimension of the sweep comparison")
     ap.add_argument("--ladder", action="store_true", help="run the reference's pre-sweep cutoff ladder 100, 200, ..")
+    ap.add_argument("--sweep-precs", default="fp64,bf16x3", help="--part sweep: precisions to run")
     ap.add_argument("--batch", type=int, default=0, help="sweeps of the 64-chain batched workload (0 skips it)")
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="also run the host Kitaev L=40 m=64 5-sweep baseline beside the GPU (about 1-2 minutes)")
@@ -478,7 +480,8 @@ def main():
         print(json.dumps(copy_leg(a.steps)), flush=True)
         return
     if a.part == "sweep":
-        print(json.dumps(sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder)), flush=True)
+        print(json.dumps(sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder,
+                                   precs=tuple(a.sweep_precs.split(",")))), flush=True)
         return
     if a.part == "cpusweep":
         print(json.dumps(cpu_sweep_leg()), flush=True)
