@@ -337,31 +337,42 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
     the optional pre-sweep cutoff ladder, then full sweeps at m), FP64 against
     BF16x3 with the FP64 Jacobi-Davidson refinement; sweep_s is the wall time
     of the last full sweep (steady state at m), sweeps_s every full sweep;
-    fidelity is |<psi_bf16x3|psi_fp64>| of the final states."""
+    fidelity is |<psi|psi_fp64>| of the final states.  A run named
+    "<prec>+warm" starts every sweep step from the previous step's state
+    (White's prediction) and converges the lowest pair only; the plain runs
+    start from random vectors and converge six pairs, as the reference's
+    ARPACK calls do."""
     from dmrg_amd import driver
     params = {"xyz": JXYZ, "potts3": POTTS}[model]
     res = {"model": model, "L": L, "m": m, "sweeps": sweeps, "ladder": bool(ladder), "params": list(params),
            "timed": "last full sweep"}
     states = {}
-    for prec in precs:
+    for spec in precs:
+        prec, warm = spec.split("+")[0], spec.endswith("+warm")
         t0 = time.perf_counter()
         r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=1e-4, refine_k=8,
-                            ladder=ladder, keep_state=True, verbose="stderr")
-        states[prec] = r.state
+                            ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm)
+        states[spec] = r.state
         st = r.stats
         per = sweep_seconds(r.steps, L, sweeps)
-        res[prec] = dict(energy=r.energy, sweep_s=round(st["last_sweep_seconds"], 4),
+        res[spec] = dict(energy=r.energy, sweep_s=round(st["last_sweep_seconds"], 4),
                          sweeps_s=[round(x, 3) for x in per], total_s=round(time.perf_counter() - t0, 3),
                          matvecs=st["matvecs"], polish=st["polish_matvecs"], trunc_err=st["trunc_err"],
                          t_eig=round(st["t_eig"], 3), t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3))
-        print(f"[bench] sweep {prec} {model} L={L} m={m}: E0={r.energy:.12f} last sweep {st['last_sweep_seconds']:.2f} s "
+        print(f"[bench] sweep {spec} {model} L={L} m={m}: E0={r.energy:.12f} last sweep {st['last_sweep_seconds']:.2f} s "
               f"(eig {st['t_eig']:.2f} svd {st['t_svd']:.2f} env {st['t_env']:.2f} s in all steps)", file=sys.stderr,
               flush=True)
-    if len(precs) == 2:
-        res["dE0"] = abs(res["bf16x3"]["energy"] - res["fp64"]["energy"])
-        # |<psi_bf16x3|psi_fp64>| of the two final MPSs (the reference's calc(mps, mps))
-        res["fidelity"] = states["bf16x3"].fidelity(states["fp64"])
-        res["speedup"] = round(res["fp64"]["sweep_s"] / max(res["bf16x3"]["sweep_s"], 1e-12), 3)
+    if "fp64" in states:
+        for spec in states:
+            if spec == "fp64":
+                continue
+            res[spec]["dE0"] = abs(res[spec]["energy"] - res["fp64"]["energy"])
+            # |<psi|psi_fp64>| of the two final MPSs (the reference's calc(mps, mps))
+            res[spec]["fidelity"] = states[spec].fidelity(states["fp64"])
+            res[spec]["speedup"] = round(res["fp64"]["sweep_s"] / max(res[spec]["sweep_s"], 1e-12), 3)
+        if "bf16x3" in states:
+            for k in ("dE0", "fidelity", "speedup"):
+                res[k] = res["bf16x3"][k]
     for p, st in states.items():
         if p != keep:
             st.close()
@@ -456,7 +467,8 @@ def main():
     ap.add_argument("--sweep-L", type=int, default=L_CHAIN, help="chain length of the sweep comparison")
     ap.add_argument("--sweep-m", type=int, default=M, help="bond dimension of the sweep comparison")
     ap.add_argument("--ladder", action="store_true", help="run the reference's pre-sweep cutoff ladder 100, 200, ..")
-    ap.add_argument("--sweep-precs", default="fp64,bf16x3", help="--part sweep: precisions to run")
+    ap.add_argument("--sweep-precs", default="fp64,bf16x3",
+                    help="--part sweep: runs to compare (fp64, fp32, bf16x3, each optionally +warm)")
     ap.add_argument("--batch", type=int, default=0, help="sweeps of the 64-chain batched workload (0 skips it)")
     ap.add_argument("--cpu-sweep", action="store_true",
                     help="also run the host Kitaev L=40 m=64 5-sweep baseline beside the GPU (about 1-2 minutes)")

@@ -45,7 +45,7 @@ int fail(const char* what) {
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s xyz|kitaev|potts3|potts2 l=<sites> m=<cutoff> n=<sweeps> s=<sector> "
-                 "<model parameters> [prec=fp64|fp32|bf16x3] [schedule=dmrg|dmrg2] [signs=physical|reference]\n",
+                 "<model parameters> [prec=fp64|fp32|bf16x3] [schedule=dmrg|dmrg2] [signs=physical|reference] [warm=0|1]\n",
                  argv[0]);
     return 2;
   }
@@ -82,6 +82,7 @@ int main(int argc, char** argv) {
   cfg.tol_lo = 1e-4;
   cfg.refine_k = 8;
   cfg.output = suffix;
+  cfg.warm_start = word(argc, argv, "warm", "0") == "1" ? 1 : 0;
   if (dmrgx_init(0)) return fail("dmrgx_init");
   dmrgx_run_stats st;
   int n_steps = 0;

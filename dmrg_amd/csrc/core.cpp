@@ -10,6 +10,7 @@
 #include <cstdio>
 
 #include "lanczos.hpp"
+#include "observables.hpp"
 #include "output.hpp"
 #include "svd.hpp"
 
@@ -53,7 +54,7 @@ static void flatten_map(const vector<vector<int>>& m, vector<int32_t>& out) {
 template <typename T>
 void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, int rnum, int r_num, int l_num,
                       vector<vector<int>>& dim, vector<vector<int>>& sym, double* vals, qtensor<T>& psi,
-                      StepLog& lg, bool allow_ed, int ed_which) {
+                      StepLog& lg, bool allow_ed, int ed_which, const qtensor<T>* guess) {
   generate_dim_sym(dim, sym, lenv, lnum, renv, rnum, cfg.sector);
   const int64_t d = get_dimension(dim, sym);
   lg.d = d;
@@ -77,6 +78,7 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
     DMRGX_HIP(hipStreamSynchronize(rt().stream));
     return;
   }
+  auto tp0 = clk::now();
   vector<vector<int>> rmap, lmap, dim_ret, sym_ret, dim2;
   DMRGX_CHECK(renv.get_map(rmap, dim_ret, sym_ret, dim, sym, 'N', 'T', r_num, 0, false, alpha_on()),
               "get_map (rmap) failed");
@@ -103,15 +105,31 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
   HeffPlan* hp = plan.get();
   MatVec op = [hp, s](const double* x, double* y) { hp->apply(x, y, s); };
   const bool cx = dt == DT_C64;
-  // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64 polish
-  const int nev = plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
-  // ARPACK's basis size for the nev this solve asks for (src/dsaupd.cpp:76, src/znaupd.cpp:94)
-  const int ncv = (cx ? 6 : 4) * nev;
+  // start vector: the predicted state in this solve's block order, when it
+  // has weight there (a truncated U S V keeps norm close to 1); else random
+  qtensor<T> start;
+  const double* v0 = nullptr;
+  if (guess) {
+    int matched = 0;
+    start = embed_state(*guess, dim, sym, &matched);
+    if (matched > 0 && dev_norm2((const double*)start.data(), d, cx, s) > 0.1) v0 = (const double*)start.data();
+  }
+  lg.warm = v0 != nullptr;
+  // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64
+  // polish; a warm start converges the lowest pair only
+  const int nev = lg.warm ? 1 : plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
+  // ARPACK's basis size for the nev this solve asks for (src/dsaupd.cpp:76,
+  // src/znaupd.cpp:94); a warm start keeps a 20-vector basis for its one pair
+  const int ncv = lg.warm ? 20 : (cx ? 6 : 4) * nev;
   const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, d})) : 1;
   DevBuf evec(std::max<int64_t>(d, 1) * sizeof(T) * pv);
   const double tol = cfg.prec == PREC_FP64 ? cfg.tol : cfg.tol_lo;
   uint64_t seed = cfg.seed * 0x9E3779B97F4A7C15ull + (uint64_t)log.size() + 1;
-  EigshResult r = eigsh(op, d, cx, nev, ncv, tol, cfg.max_restart, seed, nullptr, (double*)evec.ptr, pv);
+  auto tp1 = clk::now();
+  EigshResult r = eigsh(op, d, cx, nev, ncv, tol, cfg.max_restart, seed, v0, (double*)evec.ptr, pv, lg.warm);
+  auto tp2 = clk::now();
+  lg.t_plan = secs(tp0, tp1);
+  lg.t_lanczos = secs(tp1, tp2);
   for (int i = 0; i < std::min<int>(NEV, (int)r.evals.size()); ++i) vals[i] = r.evals[i];
   lg.n_matvec = r.n_matvec;
   lg.flops = plan->flops * r.n_matvec;
@@ -124,6 +142,7 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
     lg.n_polish = pm;
     lg.n_matvec += plo;
     lg.flops += plan64->flops * pm + plan->flops * plo;
+    lg.t_polish = secs(tp2, clk::now());
   }
   psi = qtensor<T>((const T*)evec.ptr, dim, sym);
   DMRGX_HIP(hipStreamSynchronize(s));
@@ -150,8 +169,11 @@ void Engine<T>::finish(StepLog& lg, const vector<double>& s, int cutoff) {
   if (cfg.verbose) {
     // 1: one line per step on stdout; 2: on stderr (stdout stays free for results)
     FILE* f = cfg.verbose == 2 ? stderr : stdout;
-    std::fprintf(f, "l=%d r=%d d=%lld E0=%.12f trunc=%.3e matvec=%d t=%.3fs\n", lg.l, lg.r, (long long)lg.d,
-                 lg.e[0], lg.trunc, lg.n_matvec, lg.t_total);
+    std::fprintf(f,
+                 "l=%d r=%d d=%lld E0=%.12f trunc=%.3e matvec=%d polish=%d t=%.3fs (plan %.3f lanczos %.3f polish "
+                 "%.3f svd %.3f env %.3f)\n",
+                 lg.l, lg.r, (long long)lg.d, lg.e[0], lg.trunc, lg.n_matvec, lg.n_polish, lg.t_total, lg.t_plan,
+                 lg.t_lanczos, lg.t_polish, lg.t_svd, lg.t_env);
     std::fflush(f);
   }
   if (on_step) on_step(lg);
@@ -195,6 +217,29 @@ double Engine<T>::two_sites(int l, int r, int cutoff) {
   return lg.e[0];
 }
 
+// White's prediction (S. R. White, PRL 77, 3633 (1996)): after the step on
+// (l-1, l) the state is A_{l-1} S V_l B_{l+1}..., so the two-site state on
+// (l, l+1) is (S V_l) B_{l+1}; after (r, r+1) it is A_l (U_r S).  The
+// environments of the new step are built from the same A / B tensors, so
+// the prediction is already in the new step's basis.
+template <typename T>
+bool Engine<T>::predict(int l, int r, qtensor<T>& guess) {
+  if (!cfg.warm_start || log.empty() || mps_.cut_ <= 0) return false;
+  const StepLog& p = log.back();
+  const qtensor<T> S = centre_as<T>(mps_.singular_);
+  qtensor<T> half;
+  if (p.l == l - 1 && p.r == r - 1 && mps_.cut_ == l) {
+    half.contract(S, mps_[l]);
+    guess.contract(half, mps_[r]);
+  } else if (p.l == l + 1 && p.r == r + 1 && mps_.cut_ == r + 1) {
+    half.contract(mps_[r], S);
+    guess.contract(mps_[l], half);
+  } else {
+    return false;
+  }
+  return true;
+}
+
 // src/core.cpp:182-283
 template <typename T>
 double Engine<T>::update_two(int l, int r, int cutoff) {
@@ -204,9 +249,10 @@ double Engine<T>::update_two(int l, int r, int cutoff) {
   lg.r = r;
   qtensor<T> lenv = make_lenv(l), renv = make_renv(r);
   vector<vector<int>> dim, sym;
-  qtensor<T> psi;
+  qtensor<T> psi, guess;
+  const bool warm = predict(l, r, guess);
   auto t1 = clk::now();
-  solve(lenv, renv, 2, 2, 2, 3, dim, sym, lg.e, psi, lg, false, 0);
+  solve(lenv, renv, 2, 2, 2, 3, dim, sym, lg.e, psi, lg, false, 0, warm ? &guess : nullptr);
   auto t2 = clk::now();
   qtensor<T> U, V;
   qtensor<double> S;

@@ -23,7 +23,9 @@ struct RunConfig {
   double tol_lo = 1e-6;        // Lanczos tolerance of a reduced-precision eigensolve
   int refine_k = 8;            // refinement of a reduced-precision eigensolve: 2 refine_k inner steps
   int polish_cycles = 12;      // FP64 refinement steps at most, until the FP64 residual passes `tol`
-  bool warm_start = false;     // start Lanczos from the previous two-site state
+  // start each sweep step's eigensolve from the state the previous step left
+  // (White's prediction) and converge the lowest pair only (E1..E5 read 0)
+  bool warm_start = false;
   int nev = NEV;
   int nev_lo = 4;              // eigenpairs converged by a reduced-precision solve
   int max_restart = 5000;
@@ -39,8 +41,10 @@ struct StepLog {
   double entropy = 0, trunc = 0;
   int64_t d = 0;
   int n_matvec = 0, n_polish = 0;
+  bool warm = false;  // the eigensolve started from the predicted state
   double flops = 0;  // algorithmic matvec flops of this step (all matvecs)
   double t_total = 0, t_eig = 0, t_svd = 0, t_env = 0;
+  double t_plan = 0, t_lanczos = 0, t_polish = 0;  // inside t_eig: maps and plans, eigsh, FP64 refinement
   int svd_sweeps = 0;     // Jacobi sweeps of the block SVD
   double t_block_svd = 0;  // the block SVD alone (inside t_svd)
   std::vector<double> s;  // leading singular values
@@ -79,14 +83,16 @@ class Engine {
   bool alpha_on() const { return !cfg.physical_signs || cfg.fermion; }
   qtensor<T> make_lenv(int l);
   qtensor<T> make_renv(int r);
-  // Lanczos on (lenv, renv) with r_num / l_num contracted legs; fills vals and psi
+  // Lanczos on (lenv, renv) with r_num / l_num contracted legs; fills vals
+  // and psi.  `guess` (may be null): a two-site state to start from.
   void solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, int rnum, int r_num, int l_num,
              vector<vector<int>>& dim, vector<vector<int>>& sym, double* vals, qtensor<T>& psi, StepLog& lg,
-             bool allow_ed, int ed_which);
+             bool allow_ed, int ed_which, const qtensor<T>* guess = nullptr);
+  // the two-site state on (l, r) predicted from the previous step's
+  // U S V when that step was the neighbouring bond; false otherwise
+  bool predict(int l, int r, qtensor<T>& guess);
   void finish(StepLog& lg, const vector<double>& s, int cutoff);
-  std::shared_ptr<DevBuf> last_psi_;
   std::unique_ptr<std::ofstream> f_energy_, f_singular_;
-  vector<vector<int>> last_dim_, last_sym_;
 };
 
 }  // namespace dmrgx

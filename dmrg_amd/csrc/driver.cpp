@@ -247,6 +247,7 @@ RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
   r.refine_k = c->refine_k > 0 ? c->refine_k : 8;
   r.nev_lo = c->nev_lo > 0 ? c->nev_lo : 4;
   if (c->output) r.output_suffix = c->output;
+  r.warm_start = c->warm_start != 0;
   r.verbose = c->verbose == 2 ? 2 : (c->verbose != 0 ? 1 : 0);
   return r;
 }
@@ -368,6 +369,7 @@ int dmrgx_run_batch(const dmrgx_run_config* c, int n_chains, const double* p, do
   validate_cfg(c);
   DMRGX_CHECK(c->L >= 6, "run_batch: L must be at least 6");
   DMRGX_CHECK(c->schedule == DMRGX_SCHEDULE_DMRG2, "run_batch: only the two-site schedule is batched");
+  DMRGX_CHECK(c->warm_start == 0, "run_batch: warm starts are not batched (warm_start must be 0)");
   ensure_init();
   const RunConfig cfg = make_cfg(c, model_fermion(c->model));
   if (model_complex(c->model)) {

@@ -251,7 +251,7 @@ void dev_orthogonalise(const double* V, int64_t ld, int k, double* v, int64_t n,
 }
 
 EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
-                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec) {
+                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec, bool early) {
   ensure_init();
   hipStream_t s = rt().stream;
   EigshResult res;
@@ -317,6 +317,8 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
   int j0 = 0;
   std::vector<double> theta, Y;
   double beta_last = 0;
+  int mm = m;  // basis vectors behind theta / Y (fewer when an early test passed)
+  bool done = false;
   std::vector<double> hh(2 * (m + 2)), hh2(2 * (m + 2));
   const int keep = std::min(m - 2, nev + std::max(2, (m - nev) / 2));
 
@@ -373,7 +375,32 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
         if (j + 1 < m) T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = 0;
         beta_last = 0;
       }
+      // early test: Rayleigh-Ritz on the leading (j+1) x (j+1) projected
+      // matrix after every step (the same test as at the end of a cycle)
+      if (early && j + 1 < m && j + 1 > nev) {
+        const int k = j + 1;
+        std::vector<double> A((size_t)k * k), th, Yk;
+        for (int r = 0; r < k; ++r)
+          for (int c = 0; c < k; ++c) A[(size_t)r * k + c] = T[(size_t)r * m + c];
+        host_symeig(k, A, th, Yk);
+        bool ok = true;
+        double mr = 0;
+        for (int i = 0; i < nev; ++i) {
+          double r = std::fabs(beta_last * Yk[(size_t)(k - 1) * k + i]);
+          mr = std::max(mr, r);
+          if (r > tol * std::max(std::fabs(th[i]), eps23)) ok = false;
+        }
+        if (ok) {
+          theta = th;
+          Y = Yk;
+          mm = k;
+          res.max_resid = mr;
+          res.converged = done = true;
+          break;
+        }
+      }
     }
+    if (done) break;
     // Rayleigh-Ritz on the projected matrix
     std::vector<double> A = T;
     host_symeig(m, A, theta, Y);
@@ -410,13 +437,13 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
   }
   res.evals.assign(theta.begin(), theta.begin() + nev);
   // the n_vec lowest Ritz vectors, the lowest normalised
-  const int nvec = std::max(1, std::min(n_vec, m));
-  std::vector<double> y0((size_t)m * nvec);
-  for (int r = 0; r < m; ++r)
-    for (int c = 0; c < nvec; ++c) y0[(size_t)r * nvec + c] = Y[(size_t)r * m + c];
+  const int nvec = std::max(1, std::min(n_vec, mm));
+  std::vector<double> y0((size_t)mm * nvec);
+  for (int r = 0; r < mm; ++r)
+    for (int c = 0; c < nvec; ++c) y0[(size_t)r * nvec + c] = Y[(size_t)r * mm + c];
   DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
   for (int c0 = 0; c0 < nvec; c0 += G)
-    hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, m, (const double*)Yb.ptr, nvec, c0,
+    hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, mm, (const double*)Yb.ptr, nvec, c0,
                        std::min(G, nvec - c0), (double*)Tmp.ptr, nr);
   double z = dev_norm2((double*)Tmp.ptr, n, cplx, s);
   hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, (const double*)Tmp.ptr, evec, nr, 1.0 / z);

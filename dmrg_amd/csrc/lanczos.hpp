@@ -26,9 +26,13 @@ struct EigshResult {
 // |beta * y_last| <= tol * max(|theta|, eps^(2/3)) (the ARPACK test).  `v0`
 // (device, may be null) is the start vector; otherwise a seeded random one.
 // `evec` (device, n_vec x n elements) receives the n_vec lowest Ritz vectors
-// (n_vec <= ncv), the lowest normalised.
+// (n_vec <= ncv), the lowest normalised.  `early`: the test also runs after
+// every step, not only when the basis is full (ARPACK tests at the end of a
+// cycle only), so a start vector close to an eigenvector stops within a few
+// matvecs.
 EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
-                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec = 1);
+                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec = 1,
+                  bool early = false);
 
 // device vector helpers (n = number of elements, complex if cplx)
 void dev_random(double* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s);

@@ -159,6 +159,40 @@ double block_dot(const qtensor<T>& ref, const void* g, const qtensor<T>& x) {
 
 }  // namespace
 
+// `two` (a two-site state [a, s, s', b]) copied into the block order
+// (dim, sym) of the eigensolver: blocks matched by their sectors, blocks
+// `two` lacks are zero.  *matched (optional) counts the copied blocks.
+template <typename T>
+qtensor<T> embed_state(const qtensor<T>& two, const vector<vector<int>>& dim, const vector<vector<int>>& sym,
+                       int* matched) {
+  qtensor<T> psi;
+  psi.dim_ = dim;
+  psi.sym_ = sym;
+  psi.dir_.assign(dim.size(), 0);
+  psi.layout();
+  psi.allocate();
+  DMRGX_CHECK(two.dim_.size() == dim.size(), "embed_state: leg counts differ");
+  auto code = [&](const vector<int>& s) {
+    int64_t c = 0;
+    for (size_t j = 0; j < dim.size(); ++j) c = c * dim[j].size() + s[j];
+    return c;
+  };
+  std::map<int64_t, int> at;
+  for (int i = 0; i < two.nblocks(); ++i) at[code(two.sym_[i])] = i;
+  int n = 0;
+  for (int b = 0; b < psi.nblocks(); ++b) {
+    auto it = at.find(code(psi.sym_[b]));
+    if (it == at.end()) fill_zero(psi.block(b), psi.bsize(b) * sizeof(T), rt().stream);
+    else {
+      DMRGX_CHECK(two.bsize(it->second) == psi.bsize(b), "embed_state: two-site block shapes differ");
+      copy_d2d(psi.block(b), two.block(it->second), psi.bsize(b) * sizeof(T), rt().stream);
+      ++n;
+    }
+  }
+  if (matched) *matched = n;
+  return psi;
+}
+
 // the two-site state U_l S V_r of a finished MPS in the block order
 // (dim, sym) of the eigensolver
 template <typename T>
@@ -167,28 +201,7 @@ qtensor<T> centre_psi(const mps<T>& m, const vector<vector<int>>& dim, const vec
   qtensor<T> us, two;
   us.contract(m.state_[l], centre_as<T>(m.singular_));
   two.contract(us, m.state_[r]);
-  qtensor<T> psi;
-  psi.dim_ = dim;
-  psi.sym_ = sym;
-  psi.dir_.assign(dim.size(), 0);
-  psi.layout();
-  psi.allocate();
-  auto code = [&](const vector<int>& s) {
-    int64_t c = 0;
-    for (size_t j = 0; j < dim.size(); ++j) c = c * dim[j].size() + s[j];
-    return c;
-  };
-  std::map<int64_t, int> at;
-  for (int i = 0; i < two.nblocks(); ++i) at[code(two.sym_[i])] = i;
-  for (int b = 0; b < psi.nblocks(); ++b) {
-    auto it = at.find(code(psi.sym_[b]));
-    if (it == at.end()) fill_zero(psi.block(b), psi.bsize(b) * sizeof(T), rt().stream);
-    else {
-      DMRGX_CHECK(two.bsize(it->second) == psi.bsize(b), "centre state: two-site block shapes differ");
-      copy_d2d(psi.block(b), two.block(it->second), psi.bsize(b) * sizeof(T), rt().stream);
-    }
-  }
-  return psi;
+  return embed_state(two, dim, sym, nullptr);
 }
 
 template <typename T>
@@ -271,6 +284,9 @@ double centre_derivative(const mps<T>& m, mpo<T>& W, mpo<T>& dW, int sector, boo
   return der / nn[0];
 }
 
+template qtensor<double> embed_state(const qtensor<double>&, const vector<vector<int>>&, const vector<vector<int>>&,
+                                     int*);
+template qtensor<cplx> embed_state(const qtensor<cplx>&, const vector<vector<int>>&, const vector<vector<int>>&, int*);
 template CentreOp<double> centre_operator(const mps<double>&, mpo<double>&, int, bool, int);
 template CentreOp<cplx> centre_operator(const mps<cplx>&, mpo<cplx>&, int, bool, int);
 template double centre_derivative(const mps<double>&, mpo<double>&, mpo<double>&, int, bool, int, double*);

@@ -12,6 +12,13 @@ namespace dmrgx {
 template <typename T>
 qtensor<T> centre_as(const qtensor<double>& s);
 
+// `two` (a two-site state [a, s, s', b]) copied into the eigensolver's block
+// order (dim, sym): blocks matched by their sectors, the others zero;
+// *matched (optional) counts the copied blocks
+template <typename T>
+qtensor<T> embed_state(const qtensor<T>& two, const vector<vector<int>>& dim, const vector<vector<int>>& sym,
+                       int* matched = nullptr);
+
 // copy of m with the centre absorbed into site cut_ (mps::prep_calc)
 template <typename T>
 mps<T> prepared(const mps<T>& m);

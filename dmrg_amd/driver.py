@@ -20,7 +20,8 @@ class RunConfig(C.Structure):
     _fields_ = [("model", C.c_int), ("p", C.c_double * 4), ("L", C.c_int), ("m", C.c_int), ("sweeps", C.c_int),
                 ("sector", C.c_int), ("schedule", C.c_int), ("prec", C.c_int), ("sign_rule", C.c_int),
                 ("ladder", C.c_int), ("seed", C.c_uint64), ("tol", C.c_double), ("tol_lo", C.c_double),
-                ("refine_k", C.c_int), ("verbose", C.c_int), ("nev_lo", C.c_int), ("output", C.c_char_p)]
+                ("refine_k", C.c_int), ("verbose", C.c_int), ("nev_lo", C.c_int), ("output", C.c_char_p),
+                ("warm_start", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -146,7 +147,7 @@ class RunResult:
 
 
 def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo, refine_k,
-            verbose, nev_lo=0, output=None):
+            verbose, nev_lo=0, output=None, warm_start=False):
     from . import PREC_NAMES
     cfg = RunConfig()
     cfg.model = MODELS[model]
@@ -162,16 +163,17 @@ def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladd
     cfg.verbose = 2 if verbose == "stderr" else (1 if verbose else 0)
     cfg.nev_lo = int(nev_lo)
     cfg.output = output.encode() if output else None
+    cfg.warm_start = 1 if warm_start else 0
     return cfg
 
 
 def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
              sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
-             verbose=False, keep_state=False, nev_lo=0, output=None) -> RunResult:
+             verbose=False, keep_state=False, nev_lo=0, output=None, warm_start=False) -> RunResult:
     from . import check, init, lib
     init()
     cfg = _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo,
-                  refine_k, verbose, nev_lo, output)
+                  refine_k, verbose, nev_lo, output, warm_start)
     st = RunStats()
     cap = 20000
     log = np.zeros((cap, STEP_LOG_WIDTH))

@@ -37,6 +37,11 @@ typedef struct dmrgx_run_config {
   const char* output;  /* NULL: no files; else every step appends to "energy" + output and
                           "singular" + output in the working directory in the reference's
                           formats (print_energy / print_singular, src/useful.cpp:54-102) */
+  int warm_start;   /* 0: every two-site eigensolve starts from a seeded random vector, as the
+                       reference's ARPACK calls (info=0, src/dsaupd.cpp:97) and converges the
+                       reference's six pairs; 1: a sweep step starts from the state the previous
+                       step left (White's prediction) and converges the lowest pair only
+                       (E1..E5 read 0).  dmrgx_run_batch accepts 0 only. */
 } dmrgx_run_config;
 
 typedef struct dmrgx_run_stats {

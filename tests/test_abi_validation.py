@@ -94,3 +94,16 @@ def test_task_gemm_rejects_bad_tuples():
     rc = _lib().dmrgx_task_gemm(0, 0, None, off.ctypes.data_as(i64p), size.ctypes.data_as(i64p), 1,
                                 m.ctypes.data_as(i32p), 1, None, None, 64)
     assert rc != 0 and "negative" in _err()
+
+
+def test_batched_warm_start_refused():
+    cfg = driver._config("xyz", 8, 16, 1, (), 0, "dmrg2", "fp64", "physical", True, 1, 1e-10, 1e-6, 8, False,
+                         warm_start=True)
+    p = np.zeros((2, 4))
+    e, tr = np.zeros(2), np.zeros(2)
+    st = driver.RunStats()
+    n = C.c_int()
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    rc = _lib().dmrgx_run_batch(C.byref(cfg), 2, dp(p), dp(e), dp(tr), C.byref(st), None, 0, C.byref(n), None)
+    assert rc != 0
+    assert "warm" in _err()

@@ -132,3 +132,30 @@ def test_zero_slice_skipping_sweep(gpu, model, params):
     d = np.abs(on.steps[:, 2] - off.steps[:, 2])
     print(f"  max step |dE| {d.max():.2e}, E0 {on.energy:.12f}", flush=True)
     assert d.max() < 1e-8 * abs(on.energy), d.max()
+
+
+@pytest.mark.parametrize("model,params,prec,L,m", [("xyz", XYZ, "fp64", 16, 32), ("xyz", XYZ, "bf16x3", 16, 32),
+                                                   ("potts3", POTTS, "bf16x3", 10, 54),
+                                                   ("kitaev", KITAEV, "fp64", 16, 32)])
+def test_warm_start(gpu, model, params, prec, L, m):
+    """Sweep steps started from White's prediction of the previous step's
+    state reach the energies of random starts (FP64, six pairs) with fewer
+    eigensolver matvecs; the warm steps converge and report the lowest pair
+    only."""
+    print(f"case warm start {model} {prec} L={L} m={m}", flush=True)
+    plain = driver.run_dmrg(model, L, m, 2, params, ladder=False)
+    warm = driver.run_dmrg(model, L, m, 2, params, ladder=False, prec=prec, warm_start=True, tol_lo=1e-4,
+                           refine_k=8)
+    assert len(warm.steps) == len(plain.steps)
+    assert (warm.steps[:, :2] == plain.steps[:, :2]).all()
+    grow = L // 2  # two_sites plus the growth steps: random starts, six pairs
+    assert (warm.steps[:grow, 3] != 0).any()
+    assert (warm.steps[grow:, 3:8] == 0).all()  # E1..E5 of the warm steps
+    tail = 2 * (L - 4)  # the last full sweep and the final half sweep
+    d = np.abs(warm.steps[-tail:, 2] - plain.steps[-tail:, 2])
+    mv_w, mv_p = warm.steps[grow:, 10].sum(), plain.steps[grow:, 10].sum()
+    print(f"  E0 {warm.energy:.12f} vs {plain.energy:.12f}, max step |dE| {d.max():.2e}, "
+          f"sweep matvecs {mv_w:.0f} vs {mv_p:.0f}", flush=True)
+    assert abs(warm.energy - plain.energy) < 1e-8 * max(1, abs(plain.energy))
+    assert d.max() < 1e-7 * max(1, abs(plain.energy))
+    assert mv_w < mv_p
