@@ -183,14 +183,11 @@ def configs_leg(reps=5, warmup=2):
                 # the adjoint (cotangents of psi and both environments): one R pass
                 # plus four gradient GEMM passes = 2.5 matvecs of algorithmic flop
                 gp = dmrg_amd.DeviceArray(h.n_psi, np.complex128)
-                h.adjoint_synth(xd, yd, gp)
-                t0 = time.perf_counter()
-                for _ in range(reps):
-                    h.adjoint_synth(xd, yd, gp)
-                ams = (time.perf_counter() - t0) / reps * 1e3
+                h.time_adjoint(xd, yd, gp, warmup)
+                ams = h.time_adjoint(xd, yd, gp, reps)
                 res["fp32_adjoint"] = {"ms": round(ams, 3), "tflops": round(2.5 * h.flops / (ams * 1e-3) / 1e12, 2),
-                                       "note": "cotangents of psi, renv, lenv; host-timed, includes the operand "
-                                               "conjugation and FP32 conversions"}
+                                       "note": "cotangents of psi, renv, lenv; HIP events around the whole call, "
+                                               "including the operand conjugation and FP32 conversions"}
                 print(f"[bench] {name} fp32 adjoint: {ams:.3f} ms, {res['fp32_adjoint']['tflops']:.1f} TFLOP/s",
                       file=sys.stderr, flush=True)
                 gp.free()
@@ -330,6 +327,68 @@ def cpu_sweep_leg(L=40, m=64, sweeps=5):
             "gpu_fp64_s": round(gpu_s, 3), "cpu_s": round(cpu_s, 3), "cpu_threads": int(threads), "kind": "port",
             "gpu_E0": g.energy, "cpu_E0": e_cpu, "bdg_E0": e_bdg,
             "gpu_err": abs(g.energy - e_bdg), "cpu_err": abs(e_cpu - e_bdg)}
+
+
+DERIV_J = (1.0, 0.9, 0.7)
+
+
+def deriv_leg(L=128, m=1024, sweeps=3, delta=1e-4, reps=3, fd=True):
+    """dE0/dJz of the open XYZ chain (Jx, Jy, Jz) = (1, 0.9, 0.7): the adjoint
+    H_eff kernels with environment tangents recomputed from the MPS, at the
+    centre of the FP64 DMRG state (FP64 and FP32 adjoints), against
+    Hellmann-Feynman <psi|H_XYZ(0,0,1)|psi> (calc(mps, mpo), the reference's
+    src/functions.cpp:469-511) and against the central difference
+    (E0(Jz + delta) - E0(Jz - delta)) / (2 delta) of two more runs.  The runs
+    are warm-started FP64 two-site DMRG.  The FP32 adjoint of the centre
+    operator is timed with HIP events."""
+    import dmrg_amd
+    from dmrg_amd import driver
+
+    def run(jz):
+        t0 = time.perf_counter()
+        r = driver.run_dmrg("xyz", L, m, sweeps, (DERIV_J[0], DERIV_J[1], jz), schedule="dmrg2", ladder=False,
+                            keep_state=True, warm_start=True, verbose="stderr")
+        print(f"[bench] deriv run Jz={jz}: E0={r.energy:.13f} in {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+              flush=True)
+        return r
+
+    base = run(DERIV_J[2])
+    st = base.state
+    res = {"model": "xyz", "L": L, "m": m, "sweeps": sweeps, "J": list(DERIV_J), "energy": base.energy,
+           "trunc_err": base.stats["trunc_err"], "delta": delta}
+    hf = st.expect((0.0, 0.0, 1.0)).real / st.norm2().real
+    res["hellmann_feynman"] = hf
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        der, e = st.derivative(DERIV_J, (0.0, 0.0, 1.0), prec=prec)
+        res[f"adjoint_{prec}"] = {"dE_dJz": der, "centre_energy": e, "vs_hf": abs(der - hf),
+                                  "vs_hf_rel": abs(der - hf) / abs(hf), "host_s": round(time.perf_counter() - t0, 3)}
+        print(f"[bench] deriv {prec}: {der:.12f} (HF {hf:.12f}, |diff| {abs(der - hf):.2e})", file=sys.stderr,
+              flush=True)
+    h = st.centre_heff("fp32")
+    rng = np.random.default_rng(5)
+    x = dmrg_amd.DeviceArray.from_numpy(rng.uniform(-1, 1, h.n_psi) + 1j * rng.uniform(-1, 1, h.n_psi))
+    g = dmrg_amd.DeviceArray.from_numpy(rng.uniform(-1, 1, h.n_psi) + 1j * rng.uniform(-1, 1, h.n_psi))
+    gp = dmrg_amd.DeviceArray(h.n_psi, np.complex128)
+    h.time_adjoint(x, g, gp, 1)
+    ams = h.time_adjoint(x, g, gp, reps)
+    res["fp32_adjoint_kernel"] = {"ms": round(ams, 3), "n_psi": h.n_psi, "flop": 2.5 * h.flops,
+                                  "tflops": round(2.5 * h.flops / (ams * 1e-3) / 1e12, 2),
+                                  "timing": "HIP events around dmrgx_heff_adjoint on the library stream"}
+    for a in (x, g, gp):
+        a.free()
+    h.close()
+    st.close()
+    if fd:
+        ep = run(DERIV_J[2] + delta)
+        em = run(DERIV_J[2] - delta)
+        ep.state.close()
+        em.state.close()
+        fdv = (ep.energy - em.energy) / (2 * delta)
+        res["central_difference"] = fdv
+        res["adjoint_fp64"]["vs_fd"] = abs(res["adjoint_fp64"]["dE_dJz"] - fdv)
+        res["hf_vs_fd"] = abs(hf - fdv)
+    return res
 
 
 def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
@@ -477,7 +536,11 @@ def main():
                     help="operator of the matvec timing: the centre of the sweep comparison's FP64 state "
                          "(default; synthetic when --sweep 0), of a quick BF16x3 run (growth + one sweep), or dense "
                          "synthetic environments")
-    ap.add_argument("--part", default="", choices=["", "svd", "copy", "sweep", "cpusweep"],
+    ap.add_argument("--deriv-L", type=int, default=128, help="--part deriv: chain length")
+    ap.add_argument("--deriv-m", type=int, default=1024, help="--part deriv: bond dimension")
+    ap.add_argument("--deriv-sweeps", type=int, default=3, help="--part deriv: full sweeps per run")
+    ap.add_argument("--no-fd", action="store_true", help="--part deriv: skip the two finite-difference runs")
+    ap.add_argument("--part", default="", choices=["", "svd", "copy", "sweep", "cpusweep", "deriv"],
                     help="run one part alone: the step SVD, the copy kernel used to calibrate PMC counters, the "
                          "sweep comparison or the host Kitaev sweep (JSON of that part only)")
     a = ap.parse_args()
@@ -494,6 +557,9 @@ def main():
     if a.part == "sweep":
         print(json.dumps(sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder,
                                    precs=tuple(a.sweep_precs.split(",")))), flush=True)
+        return
+    if a.part == "deriv":
+        print(json.dumps(deriv_leg(a.deriv_L, a.deriv_m, a.deriv_sweeps, fd=not a.no_fd)), flush=True)
         return
     if a.part == "cpusweep":
         print(json.dumps(cpu_sweep_leg()), flush=True)

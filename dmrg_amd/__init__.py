@@ -28,7 +28,7 @@ EXPORTS = [
     "dmrgx_free", "dmrgx_memcpy_h2d", "dmrgx_memcpy_d2h", "dmrgx_memset_zero", "dmrgx_synchronize",
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
     "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_eigsh_batched", "dmrgx_set_lanczos_graphs", "dmrgx_set_autotune", "dmrgx_autotune_keys", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
-    "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time", "dmrgx_heff_set_split", "dmrgx_heff_get_split",
+    "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time", "dmrgx_heff_time_adjoint", "dmrgx_heff_set_split", "dmrgx_heff_get_split",
     "dmrgx_task_gemm_noalloc", "dmrgx_set_skip_zero_slices", "dmrgx_heff_flops_exec",
 ]
 
@@ -84,6 +84,7 @@ def _declare(lib):
         "dmrgx_heff_flops_exec": ([vp], C.c_double),
         "dmrgx_set_skip_zero_slices": ([C.c_int], C.c_int),
         "dmrgx_heff_time": ([vp, vp, vp, C.c_int, dp], C.c_int),
+        "dmrgx_heff_time_adjoint": ([vp, vp, vp, vp, vp, vp, C.c_int, dp], C.c_int),
         "dmrgx_heff_destroy": ([vp], C.c_int),
         "dmrgx_heff_adjoint": ([vp, vp, vp, vp, vp, vp], C.c_int),
         "dmrgx_heff_sizes": ([vp, i64p], C.c_int),

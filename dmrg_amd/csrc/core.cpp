@@ -8,6 +8,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <set>
 
 #include "lanczos.hpp"
 #include "observables.hpp"
@@ -46,26 +47,91 @@ qtensor<T> Engine<T>::make_renv(int r) {
   return t.exchange(0, 1);
 }
 
+template <typename T>
+qtensor<T> Engine<T>::make_ledge(int l) {
+  // src/core.cpp:193-196
+  qtensor<T> ope = mps_.edge(l - 1).id(mpo_[l]);
+  qtensor<T> t;
+  t.contract(mps_.edge(l - 1), 1, ope, 0);
+  return t.exchange(3, 4);
+}
+
+template <typename T>
+qtensor<T> Engine<T>::make_redge(int r, int from) {
+  // src/core.cpp:193-199 (the identity built from edge(l-1)), :417-420 (from edge(r+1))
+  qtensor<T> ope = mps_.edge(from).id(mpo_[r]);
+  qtensor<T> t;
+  t.contract(mps_.edge(r + 1), 1, ope, 0, true);
+  return t.exchange(0, 1);
+}
+
+template <typename T>
+void Engine<T>::update_ledge(int l, const qtensor<T>& ledge, qtensor<T> U, int num) {
+  // src/core.cpp:273-276
+  qtensor<T> vec;
+  vec.contract(ledge, U, 'N', 'N', num);
+  U.conjugate();
+  mps_.edge(l).contract(U, vec, 'T', 'N', num);
+}
+
+template <typename T>
+void Engine<T>::update_redge(int r, const qtensor<T>& redge, qtensor<T> V, int num) {
+  // src/core.cpp:277-281
+  qtensor<T> vec;
+  vec.contract(redge, V, 'N', 'T', num);
+  V.conjugate();
+  mps_.edge(r).contract(V, vec, 'N', 'N', num);
+}
+
 static void flatten_map(const vector<vector<int>>& m, vector<int32_t>& out) {
   out.clear();
   for (auto& t : m) out.insert(out.end(), t.begin(), t.end());
 }
 
+// one H_eff plan over the two environments' task lists
+template <typename T>
+static std::unique_ptr<HeffPlan> plan_of(const qtensor<T>& renv, const vector<vector<int>>& rmap,
+                                         const qtensor<T>& lenv, const vector<vector<int>>& lmap, int64_t d,
+                                         int64_t d_store, int prec) {
+  vector<int32_t> rm, lm;
+  flatten_map(rmap, rm);
+  flatten_map(lmap, lm);
+  vector<int64_t> rsz(renv.nblocks()), lsz(lenv.nblocks());
+  for (int b = 0; b < renv.nblocks(); ++b) rsz[b] = renv.bsize(b);
+  for (int b = 0; b < lenv.nblocks(); ++b) lsz[b] = lenv.bsize(b);
+  return make_heff(dtype_of<T>::value, prec, renv.data(), renv.off_.data(), rsz.data(), renv.nblocks(), renv.total_,
+                   rm.data(), (int)rmap.size(), lenv.data(), lenv.off_.data(), lsz.data(), lenv.nblocks(),
+                   lenv.total_, lm.data(), (int)lmap.size(), d, d_store);
+}
+
 template <typename T>
 void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, int rnum, int r_num, int l_num,
                       vector<vector<int>>& dim, vector<vector<int>>& sym, double* vals, qtensor<T>& psi,
-                      StepLog& lg, bool allow_ed, int ed_which, const qtensor<T>* guess) {
+                      StepLog& lg, bool allow_ed, int ed_which, const qtensor<T>* guess, const qtensor<T>* ledge,
+                      const qtensor<T>* redge) {
+  DMRGX_CHECK((ledge == nullptr) == (redge == nullptr), "solve: one edge environment without the other");
   generate_dim_sym(dim, sym, lenv, lnum, renv, rnum, cfg.sector);
   const int64_t d = get_dimension(dim, sym);
   lg.d = d;
   for (int i = 0; i < NEV; ++i) vals[i] = 0;
   if (allow_ed && d < NEV) {
     // exact diagonalisation of the tiny superblock (src/core.cpp:106-124, 314-336, 410-433)
-    qtensor<T> whole;
-    if (ed_which == 0) whole.contract(lenv, 1, renv, 1);
-    else if (ed_which == 1) whole.contract(lenv.combine(3, 4).combine(0, 1), 1, renv, 1);
-    else whole.contract(lenv, 1, renv.combine(3, 4).combine(0, 1), 1);
-    whole = whole.simplify().exchange(2, 3).combine(2, 3).combine(0, 1);
+    auto pair = [ed_which](const qtensor<T>& le, const qtensor<T>& re) {
+      qtensor<T> w;
+      if (ed_which == 0) w.contract(le, 1, re, 1);
+      else if (ed_which == 1) w.contract(le.combine(3, 4).combine(0, 1), 1, re, 1);
+      else w.contract(le, 1, re.combine(3, 4).combine(0, 1), 1);
+      return w.simplify();
+    };
+    qtensor<T> whole = pair(lenv, renv);
+    if (ledge) {
+      // bulk + edge (src/core.cpp:111-116, 323-331, 419-427)
+      qtensor<T> edge = pair(*ledge, *redge), all;
+      if (ed_which == 0) all.plus(edge, whole);
+      else all.plus(whole, edge);
+      whole = all.simplify();
+    }
+    whole = whole.exchange(2, 3).combine(2, 3).combine(0, 1);
     vector<T> v;
     std::vector<double> w(std::max<int64_t>(d, NEV) + whole.dimension(0), 0.0);
     int n = whole.eig(w.data(), v, cfg.sector);
@@ -86,21 +152,27 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
   vector<vector<int>> psym = sym;
   DMRGX_CHECK(lenv.get_map(lmap, dim2, psym, dim_ret, sym_ret, 'N', 'T', l_num, 0, true, alpha_on()),
               "get_map (lmap) failed");
-  vector<int32_t> rm, lm;
-  flatten_map(rmap, rm);
-  flatten_map(lmap, lm);
-  vector<int64_t> roff(renv.off_), rsz(renv.nblocks()), loff(lenv.off_), lsz(lenv.nblocks());
-  for (int b = 0; b < renv.nblocks(); ++b) rsz[b] = renv.bsize(b);
-  for (int b = 0; b < lenv.nblocks(); ++b) lsz[b] = lenv.bsize(b);
   const int dt = dtype_of<T>::value;
-  auto plan = make_heff(dt, cfg.prec, renv.data(), roff.data(), rsz.data(), renv.nblocks(), renv.total_, rm.data(),
-                        (int)rmap.size(), lenv.data(), loff.data(), lsz.data(), lenv.nblocks(), lenv.total_, lm.data(),
-                        (int)lmap.size(), d, d_store);
+  auto plan = plan_of(renv, rmap, lenv, lmap, d, d_store, cfg.prec);
   std::unique_ptr<HeffPlan> plan64;
-  if (cfg.prec != PREC_FP64)
-    plan64 = make_heff(dt, PREC_FP64, renv.data(), roff.data(), rsz.data(), renv.nblocks(), renv.total_, rm.data(),
-                       (int)rmap.size(), lenv.data(), loff.data(), lsz.data(), lenv.nblocks(), lenv.total_, lm.data(),
-                       (int)lmap.size(), d, d_store);
+  if (cfg.prec != PREC_FP64) plan64 = plan_of(renv, rmap, lenv, lmap, d, d_store, PREC_FP64);
+  if (ledge) {
+    // the edge term's own task lists over its own store layout
+    // (src/core.cpp:241-247); the reference accumulates every edge L task
+    // onto the bulk result (beta = 1), the edge plan writes its first task
+    // per output block and apply() adds the result
+    vector<vector<int>> remap, lemap, edim_ret, esym_ret, edim2;
+    DMRGX_CHECK(redge->get_map(remap, edim_ret, esym_ret, dim, sym, 'N', 'T', r_num, 0, false, alpha_on()),
+                "get_map (edge rmap) failed");
+    const int64_t e_store = get_dimension(edim_ret, esym_ret);
+    vector<vector<int>> esym = sym;
+    DMRGX_CHECK(ledge->get_map(lemap, edim2, esym, edim_ret, esym_ret, 'N', 'T', l_num, 1, true, alpha_on()),
+                "get_map (edge lmap) failed");
+    std::set<int> written;
+    for (auto& t : lemap) t[5] = written.insert(t[2]).second ? 0 : 1;
+    attach_edge(*plan, plan_of(*redge, remap, *ledge, lemap, d, e_store, cfg.prec));
+    if (plan64) attach_edge(*plan64, plan_of(*redge, remap, *ledge, lemap, d, e_store, PREC_FP64));
+  }
   hipStream_t s = rt().stream;
   HeffPlan* hp = plan.get();
   MatVec op = [hp, s](const double* x, double* y) { hp->apply(x, y, s); };
@@ -187,10 +259,18 @@ double Engine<T>::two_sites(int l, int r, int cutoff) {
   lg.l = l;
   lg.r = r;
   qtensor<T> lenv = mpo_[l], renv = mpo_[r];
+  const bool pbc = mps_.pbc();
+  qtensor<T> ledge, redge;
+  if (pbc) {
+    // src/core.cpp:96-99
+    ledge = mps_.edge(l);
+    redge = mps_.edge(r);
+  }
   vector<vector<int>> dim, sym;
   qtensor<T> psi;
   auto t1 = clk::now();
-  solve(lenv, renv, 1, 1, 1, 2, dim, sym, lg.e, psi, lg, true, 0);
+  solve(lenv, renv, 1, 1, 1, 2, dim, sym, lg.e, psi, lg, true, 0, nullptr, pbc ? &ledge : nullptr,
+        pbc ? &redge : nullptr);
   auto t2 = clk::now();
   qtensor<T> U, V;
   qtensor<double> S;
@@ -199,6 +279,7 @@ double Engine<T>::two_sites(int l, int r, int cutoff) {
   mps_[l] = U;
   mps_.center(l + 1) = S;
   mps_[r] = V;
+  const qtensor<T> U0 = U, V0 = V;
   qtensor<T> vec;
   vec.contract(lenv, U, 'N', 'N');
   U.conjugate();
@@ -206,6 +287,11 @@ double Engine<T>::two_sites(int l, int r, int cutoff) {
   vec.contract(renv, V, 'N', 'T');
   V.conjugate();
   mps_(r).contract(V, vec, 'N', 'N');
+  if (pbc) {
+    // src/core.cpp:161-170
+    update_ledge(l, ledge, U0, 1);
+    update_redge(r, redge, V0, 1);
+  }
   auto t4 = clk::now();
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);
@@ -248,11 +334,18 @@ double Engine<T>::update_two(int l, int r, int cutoff) {
   lg.l = l;
   lg.r = r;
   qtensor<T> lenv = make_lenv(l), renv = make_renv(r);
+  const bool pbc = mps_.pbc();
+  qtensor<T> ledge, redge;
+  if (pbc) {
+    ledge = make_ledge(l);
+    redge = make_redge(r, l - 1);
+  }
   vector<vector<int>> dim, sym;
   qtensor<T> psi, guess;
   const bool warm = predict(l, r, guess);
   auto t1 = clk::now();
-  solve(lenv, renv, 2, 2, 2, 3, dim, sym, lg.e, psi, lg, false, 0, warm ? &guess : nullptr);
+  solve(lenv, renv, 2, 2, 2, 3, dim, sym, lg.e, psi, lg, false, 0, warm ? &guess : nullptr,
+        pbc ? &ledge : nullptr, pbc ? &redge : nullptr);
   auto t2 = clk::now();
   qtensor<T> U, V;
   qtensor<double> S;
@@ -261,6 +354,7 @@ double Engine<T>::update_two(int l, int r, int cutoff) {
   mps_[l] = U;
   mps_.center(l + 1) = S;
   mps_[r] = V;
+  const qtensor<T> U0 = U, V0 = V;
   qtensor<T> vec;
   vec.contract(lenv, U, 'N', 'N', 2);
   U.conjugate();
@@ -268,6 +362,10 @@ double Engine<T>::update_two(int l, int r, int cutoff) {
   vec.contract(renv, V, 'N', 'T', 2);
   V.conjugate();
   mps_(r).contract(V, vec, 'N', 'N', 2);
+  if (pbc) {
+    update_ledge(l, ledge, U0, 2);
+    update_redge(r, redge, V0, 2);
+  }
   auto t4 = clk::now();
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);
@@ -287,10 +385,18 @@ double Engine<T>::move2right(int l, int r, int cutoff) {
   lg.l = l;
   lg.r = r;
   qtensor<T> lenv = make_lenv(l), renv = mps_(r);
+  const bool pbc = mps_.pbc();
+  qtensor<T> ledge, redge;
+  if (pbc) {
+    // src/core.cpp:302-308
+    ledge = make_ledge(l);
+    redge = mps_.edge(r);
+  }
   vector<vector<int>> dim, sym;
   qtensor<T> psi;
   auto t1 = clk::now();
-  solve(lenv, renv, 2, 1, 1, 3, dim, sym, lg.e, psi, lg, true, 1);
+  solve(lenv, renv, 2, 1, 1, 3, dim, sym, lg.e, psi, lg, true, 1, nullptr, pbc ? &ledge : nullptr,
+        pbc ? &redge : nullptr);
   auto t2 = clk::now();
   qtensor<T> U, V;
   qtensor<double> S;
@@ -298,10 +404,12 @@ double Engine<T>::move2right(int l, int r, int cutoff) {
   auto t3 = clk::now();
   mps_[l] = U;
   mps_.center(l + 1) = S;
+  const qtensor<T> U0 = U;
   qtensor<T> vec;
   vec.contract(lenv, U, 'N', 'N', 2);
   U.conjugate();
   mps_(l).contract(U, vec, 'T', 'N', 2);
+  if (pbc) update_ledge(l, ledge, U0, 2);  // src/core.cpp:371-376
   auto t4 = clk::now();
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);
@@ -321,10 +429,18 @@ double Engine<T>::move2left(int l, int r, int cutoff) {
   lg.l = l;
   lg.r = r;
   qtensor<T> lenv = mps_(l), renv = make_renv(r);
+  const bool pbc = mps_.pbc();
+  qtensor<T> ledge, redge;
+  if (pbc) {
+    // src/core.cpp:398-404
+    ledge = mps_.edge(l);
+    redge = make_redge(r, r + 1);
+  }
   vector<vector<int>> dim, sym;
   qtensor<T> psi;
   auto t1 = clk::now();
-  solve(lenv, renv, 1, 2, 2, 2, dim, sym, lg.e, psi, lg, true, 2);
+  solve(lenv, renv, 1, 2, 2, 2, dim, sym, lg.e, psi, lg, true, 2, nullptr, pbc ? &ledge : nullptr,
+        pbc ? &redge : nullptr);
   auto t2 = clk::now();
   qtensor<T> U, V;
   qtensor<double> S;
@@ -332,10 +448,12 @@ double Engine<T>::move2left(int l, int r, int cutoff) {
   auto t3 = clk::now();
   mps_.center(l + 1) = S;
   mps_[r] = V;
+  const qtensor<T> V0 = V;
   qtensor<T> vec;
   vec.contract(renv, V, 'N', 'T', 2);
   V.conjugate();
   mps_(r).contract(V, vec, 'N', 'N', 2);
+  if (pbc) update_redge(r, redge, V0, 2);  // src/core.cpp:467-472
   auto t4 = clk::now();
   lg.t_env = secs(t0, t1) + secs(t3, t4);
   lg.t_eig = secs(t1, t2);

@@ -85,9 +85,19 @@ class Engine {
   qtensor<T> make_renv(int r);
   // Lanczos on (lenv, renv) with r_num / l_num contracted legs; fills vals
   // and psi.  `guess` (may be null): a two-site state to start from.
+  // With ledge / redge (periodic chains) the operator is the bulk one plus
+  // the edge term L_edge W_id W_id R_edge (src/core.cpp:27-30).
   void solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, int rnum, int r_num, int l_num,
              vector<vector<int>>& dim, vector<vector<int>>& sym, double* vals, qtensor<T>& psi, StepLog& lg,
-             bool allow_ed, int ed_which, const qtensor<T>* guess = nullptr);
+             bool allow_ed, int ed_which, const qtensor<T>* guess = nullptr, const qtensor<T>* ledge = nullptr,
+             const qtensor<T>* redge = nullptr);
+  // the edge environments of a step: from the stored ones through the
+  // identity operator (src/core.cpp:193-199, 302-308, 398-404)
+  qtensor<T> make_ledge(int l);
+  qtensor<T> make_redge(int r, int from);
+  // the edge environment updates after the SVD (src/core.cpp:272-281)
+  void update_ledge(int l, const qtensor<T>& ledge, qtensor<T> U, int num);
+  void update_redge(int r, const qtensor<T>& redge, qtensor<T> V, int num);
   // the two-site state on (l, r) predicted from the previous step's
   // U S V when that step was the neighbouring bond; false otherwise
   bool predict(int l, int r, qtensor<T>& guess);

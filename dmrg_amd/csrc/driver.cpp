@@ -2,12 +2,14 @@
 // benchmarking, and structure-only get_map / generate_dim_sym.
 #include <chrono>
 #include <cstring>
+#include <type_traits>
 
 #include "dmrgx_driver.h"
 #include "batch.hpp"
 #include "core.hpp"
 #include "heff.hpp"
 #include "lanczos.hpp"
+#include "model_4e.hpp"
 #include "observables.hpp"
 #include "output.hpp"
 #include "qtensor.hpp"
@@ -17,8 +19,9 @@ using namespace dmrgx;
 // a finished run's MPS (centre absorbed), one of r / c set
 struct dmrgx_state {
   int model = 0, L = 0, sector = 0;
-  double p[4] = {0, 0, 0, 0};  // the model parameters of the run
+  double p[DMRGX_NPARAM] = {};  // the model parameters of the run
   bool signs = false;  // graded signs on: fermionic model, or the reference's sign rule
+  bool pbc = false;    // the run carried periodic edge terms
   std::unique_ptr<mps<double>> r, raw_r;  // prepared (centre absorbed) and as left by the run
   std::unique_ptr<mps<cplx>> c, raw_c;
 };
@@ -38,14 +41,22 @@ struct dmrgx_state {
 
 namespace {
 
-bool model_complex(int model) { return model == DMRGX_MODEL_XYZ || model == DMRGX_MODEL_POTTS3; }
-bool model_fermion(int model) { return model == DMRGX_MODEL_KITAEV; }
-int model_symmetry(int model) { return model == DMRGX_MODEL_POTTS3 ? 3 : 2; }
+bool model_complex(int model) {
+  return model == DMRGX_MODEL_XYZ || model == DMRGX_MODEL_POTTS3 || model == DMRGX_MODEL_4E;
+}
+bool model_fermion(int model) { return model == DMRGX_MODEL_KITAEV || model == DMRGX_MODEL_4E; }
+int model_symmetry(int model) {
+  return model == DMRGX_MODEL_POTTS3 ? 3 : model == DMRGX_MODEL_4E ? SYM_S4E : 2;
+}
+// MPO tensors of a chain of L physical sites: the 4e model puts two sites in
+// each (4e.cpp:46-47, which asks for L % 4 == 0)
+int model_tensors(int model, int L) { return model == DMRGX_MODEL_4E ? L / 2 : L; }
 
 // the model MPOs of src/operators.cpp with their edges (include/mpo.h:45-59);
 // sets the charge group of the model
 mpo<cplx> model_mpo_c(int model, const double* p, int L) {
   set_symmetry(model_symmetry(model));
+  if (model == DMRGX_MODEL_4E) return mpo_4e(model_tensors(model, L), p);
   qtensor<cplx> H;
   if (model == DMRGX_MODEL_XYZ) H = H_XYZ(p[0], p[1], p[2]);
   else if (model == DMRGX_MODEL_POTTS3) H = H_Potts3(p[0], p[1], p[2], p[3]);
@@ -68,6 +79,32 @@ mpo<double> model_mpo_r(int model, const double* p, int L) {
   mpo<double> m(L, H);
   m.edge();
   return m;
+}
+
+// the periodic edge terms the reference's mains add (xyz.cpp:39-43,
+// kitaev.cpp:43-46, potts.cpp:45-48): the two halves of the boundary term
+template <typename T>
+void add_model_edge(mps<T>& s, int model, const double* p);
+template <>
+void add_model_edge(mps<cplx>& s, int model, const double* p) {
+  if (model == DMRGX_MODEL_4E) {
+    qtensor<cplx> a, b;
+    edges_4e(s.size(), p, a, b);
+    s.add_edge(a, b);
+  } else if (model == DMRGX_MODEL_XYZ) {
+    qtensor<cplx> H = H_XYZ(p[0], p[1], p[2]);
+    s.add_edge(H.left(), H.right());
+  } else {
+    DMRGX_CHECK(model == DMRGX_MODEL_POTTS3, "pbc: no edge terms for this model");
+    qtensor<cplx> H = H_Potts3(0.0, p[1], p[2], p[3]);
+    s.add_edge(H.left(), H.right());
+  }
+}
+template <>
+void add_model_edge(mps<double>& s, int model, const double* p) {
+  DMRGX_CHECK(model == DMRGX_MODEL_KITAEV, "pbc: the reference has no periodic edge terms for this model");
+  qtensor<double> H = H_Kitaev(p[0], -p[1], 0.0);
+  s.add_edge(H.left(), H.right());
 }
 
 template <typename T>
@@ -119,7 +156,7 @@ void run_engine(Engine<T>& eng, const dmrgx_run_config* c, dmrgx_run_stats* st, 
   using clk = std::chrono::steady_clock;
   auto t0 = clk::now();
   // sweep boundaries: remember the step index at which each full sweep starts
-  const int L = c->L;
+  const int L = eng.mps_.size();
   double e = c->schedule == DMRGX_SCHEDULE_DMRG ? eng.dmrg(c->m, c->sweeps) : eng.dmrg2(c->m, c->sweeps, c->ladder != 0);
   auto t1 = clk::now();
   std::memset(st, 0, sizeof(*st));
@@ -230,6 +267,10 @@ void validate_cfg(const dmrgx_run_config* c) {
   DMRGX_CHECK(c->schedule == DMRGX_SCHEDULE_DMRG || c->schedule == DMRGX_SCHEDULE_DMRG2, "run: unknown schedule");
   DMRGX_CHECK(c->sign_rule == DMRGX_SIGNS_PHYSICAL || c->sign_rule == DMRGX_SIGNS_REFERENCE, "run: unknown sign rule");
   DMRGX_CHECK(c->prec >= PREC_FP64 && c->prec <= PREC_BF16X3, "run: unknown precision");
+  DMRGX_CHECK(c->model >= DMRGX_MODEL_XYZ && c->model <= DMRGX_MODEL_4E, "run: unknown model");
+  DMRGX_CHECK(c->pbc == 0 || c->model != DMRGX_MODEL_POTTS2, "run: the reference has no periodic edge terms for potts2");
+  DMRGX_CHECK(c->model != DMRGX_MODEL_4E || (c->L % 4 == 0 && c->L >= 8),
+              "run: the 4e chain needs L a multiple of 4 and at least 8 (two sites per MPO tensor)");
   DMRGX_CHECK(!(c->schedule == DMRGX_SCHEDULE_DMRG && c->sign_rule == DMRGX_SIGNS_REFERENCE && !model_fermion(c->model)),
               "run: the single-site schedule (dmrg) with the reference sign rule makes the operator of a bosonic "
               "model non-Hermitian; use schedule dmrg2 or the physical sign rule");
@@ -318,13 +359,15 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double*
     out = std::make_unique<dmrgx_state>();
     out->model = c->model;
     out->L = c->L;
-    for (int i = 0; i < 4; ++i) out->p[i] = c->p[i];
+    for (int i = 0; i < DMRGX_NPARAM; ++i) out->p[i] = c->p[i];
+    out->pbc = c->pbc != 0;
     out->signs = model_fermion(c->model) || c->sign_rule == DMRGX_SIGNS_REFERENCE;
     out->sector = c->sector;
   }
   if (model_complex(c->model)) {
     mpo<cplx> m = model_mpo_c(c->model, c->p, c->L);
     Engine<cplx> eng(m, make_cfg(c, model_fermion(c->model)));
+    if (c->pbc) add_model_edge(eng.mps_, c->model, c->p);
     run_engine(eng, c, st, step_log, log_cap, n_steps);
     if (out) {
       out->c = std::make_unique<mps<cplx>>(prepared(eng.mps_));
@@ -333,6 +376,7 @@ int dmrgx_run_dmrg_state(const dmrgx_run_config* c, dmrgx_run_stats* st, double*
   } else {
     mpo<double> m = model_mpo_r(c->model, c->p, c->L);
     Engine<double> eng(m, make_cfg(c, model_fermion(c->model)));
+    if (c->pbc) add_model_edge(eng.mps_, c->model, c->p);
     run_engine(eng, c, st, step_log, log_cap, n_steps);
     if (out) {
       out->r = std::make_unique<mps<double>>(prepared(eng.mps_));
@@ -370,6 +414,7 @@ int dmrgx_run_batch(const dmrgx_run_config* c, int n_chains, const double* p, do
   DMRGX_CHECK(c->L >= 6, "run_batch: L must be at least 6");
   DMRGX_CHECK(c->schedule == DMRGX_SCHEDULE_DMRG2, "run_batch: only the two-site schedule is batched");
   DMRGX_CHECK(c->warm_start == 0, "run_batch: warm starts are not batched (warm_start must be 0)");
+  DMRGX_CHECK(c->pbc == 0, "run_batch: periodic chains are not batched (pbc must be 0)");
   ensure_init();
   const RunConfig cfg = make_cfg(c, model_fermion(c->model));
   if (model_complex(c->model)) {
@@ -435,12 +480,80 @@ int dmrgx_state_expect(const dmrgx_state* a, int model, const double* p, double*
   CAPI_END
 }
 
+int dmrgx_state_correlations_4e(const dmrgx_state* a, double* rows, int cap, int* n) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && n && (rows || cap == 0), "state_correlations_4e: null arguments");
+  DMRGX_CHECK(a->model == DMRGX_MODEL_4E && a->c, "state_correlations_4e: not a state of the 4e model");
+  set_symmetry(model_symmetry(a->model));
+  vector<Corr4e> c = correlations_4e(*a->c, true);
+  *n = 0;
+  for (auto& r : c) {
+    if (*n >= cap) break;
+    rows[3 * *n] = r.x;
+    rows[3 * *n + 1] = r.value.real();
+    rows[3 * *n + 2] = r.value.imag();
+    ++*n;
+  }
+  CAPI_END
+}
+
+int dmrgx_state_position(const dmrgx_state* a, int* cut) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && cut, "state_position: null arguments");
+  DMRGX_CHECK(a->raw_c || a->raw_r, "state_position: the state holds no MPS");
+  *cut = a->raw_c ? a->raw_c->position() : a->raw_r->position();
+  CAPI_END
+}
+
+int dmrgx_state_move_cut(dmrgx_state* a, int steps) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a, "state_move_cut: null state");
+  DMRGX_CHECK(a->raw_c || a->raw_r, "state_move_cut: the state holds no MPS");
+  set_symmetry(model_symmetry(a->model));
+  auto move = [&](auto& raw, auto& prep) {
+    const int n = raw->size(), c = raw->position();
+    DMRGX_CHECK(c + steps >= 1 && c + steps <= n - 1,
+                "state_move_cut: the centre would leave bonds 1.." + std::to_string(n - 1));
+    for (int i = 0; i < steps; ++i) raw->move_cut_right();
+    for (int i = 0; i < -steps; ++i) raw->move_cut_left();
+    prep = std::make_unique<std::remove_reference_t<decltype(*raw)>>(prepared(*raw));
+  };
+  if (a->raw_c) move(a->raw_c, a->c);
+  else move(a->raw_r, a->r);
+  CAPI_END
+}
+
+int dmrgx_state_inverse(const dmrgx_state* a, dmrgx_state** out) {
+  CAPI_BEGIN
+  DMRGX_CHECK(a && out, "state_inverse: null arguments");
+  DMRGX_CHECK(a->raw_c || a->raw_r, "state_inverse: the state holds no MPS");
+  set_symmetry(model_symmetry(a->model));
+  auto r = std::make_unique<dmrgx_state>();
+  r->model = a->model;
+  r->L = a->L;
+  r->sector = a->sector;
+  r->signs = a->signs;
+  r->pbc = a->pbc;
+  for (int i = 0; i < DMRGX_NPARAM; ++i) r->p[i] = a->p[i];
+  if (a->raw_c) {
+    r->raw_c = std::make_unique<mps<cplx>>(a->raw_c->inverse());
+    r->c = std::make_unique<mps<cplx>>(prepared(*r->raw_c));
+  } else {
+    r->raw_r = std::make_unique<mps<double>>(a->raw_r->inverse());
+    r->r = std::make_unique<mps<double>>(prepared(*r->raw_r));
+  }
+  *out = r.release();
+  CAPI_END
+}
+
 int dmrgx_state_derivative(const dmrgx_state* a, const double* p, const double* dp, int prec, double* out) {
   CAPI_BEGIN
   DMRGX_CHECK(a && p && dp && out, "state_derivative: null arguments");
   DMRGX_CHECK(prec == DMRGX_FP64 || prec == DMRGX_FP32, "state_derivative: prec must be FP64 or FP32");
-  double q[4];
-  for (int i = 0; i < 4; ++i) q[i] = p[i] + dp[i];
+  DMRGX_CHECK(!a->pbc, "state_derivative: the environments of a periodic run carry edge terms this call omits");
+  DMRGX_CHECK(a->model != DMRGX_MODEL_4E, "state_derivative: the 4e MPO is not one tensor repeated");
+  double q[DMRGX_NPARAM];
+  for (int i = 0; i < DMRGX_NPARAM; ++i) q[i] = p[i] + dp[i];
   double e = 0;
   if (a->raw_c) {
     mpo<cplx> W = model_mpo_c(a->model, p, a->L), Wq = model_mpo_c(a->model, q, a->L);
@@ -460,6 +573,7 @@ int dmrgx_state_heff_create(const dmrgx_state* a, int prec, dmrgx_heff** plan, i
   DMRGX_CHECK(a && plan && n_psi, "state_heff_create: null arguments");
   DMRGX_CHECK(prec >= PREC_FP64 && prec <= PREC_BF16X3, "state_heff_create: unknown precision");
   DMRGX_CHECK(a->raw_c || a->raw_r, "state_heff_create: the state holds no MPS");
+  DMRGX_CHECK(!a->pbc, "state_heff_create: the environments of a periodic run carry edge terms this call omits");
   ensure_init();
   auto h = std::make_unique<dmrgx_heff>();
   auto take = [&](auto& c) {

@@ -628,6 +628,7 @@ std::vector<GemmTask> adj_tasks_b(const std::vector<GemmTask>& t) {
 
 void HeffPlan::adjoint(const double* psi, const double* g_out, double* g_psi, double* g_renv, double* g_lenv,
                        hipStream_t s) {
+  DMRGX_CHECK(!edge, "adjoint: operators with periodic edge terms have no adjoint kernels");
   const bool cx = dtype == DT_C64;
   const int c = cx ? 2 : 1;
   const int aprec = prec == PREC_FP64 ? PREC_FP64 : PREC_FP32;
@@ -698,6 +699,26 @@ void HeffPlan::adjoint(const double* psi, const double* g_out, double* g_psi, do
   DMRGX_HIP(hipStreamSynchronize(s));
 }
 
+namespace {
+
+// y += x over n doubles
+__global__ __launch_bounds__(256) void k_add_into(double* __restrict__ y, const double* __restrict__ x, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+}  // namespace
+
+void attach_edge(HeffPlan& h, std::unique_ptr<HeffPlan> e) {
+  DMRGX_CHECK(e && e->n_psi == h.n_psi && e->dtype == h.dtype && e->prec == h.prec && !e->edge,
+              "attach_edge: the edge operator does not match the bulk one");
+  const int c = h.dtype == DT_C64 ? 2 : 1;
+  h.edge_out = std::make_unique<DevBuf>(std::max<int64_t>(h.n_psi, 1) * 8 * c);
+  h.flops += e->flops;
+  h.flops_exec += e->flops_exec;
+  h.edge = std::move(e);
+}
+
 void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev) {
   const int c = dtype == DT_C64 ? 2 : 1;
   hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr, e2 = ev ? ev[2] : nullptr,
@@ -724,6 +745,16 @@ void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* e
     run_gemm_plan(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s, e0, e1);
     run_gemm_plan(*lplan, lenv_lo->ptr, store->ptr, out_lo->ptr, s, e2, e3);
     convert_f32_to_f64((const float*)out_lo->ptr, out, n_psi * c, s);
+  }
+  if (edge) {
+    // out += L_edge W_id W_id R_edge psi (src/core.cpp:27-30)
+    edge->apply(in, (double*)edge_out->ptr, s);
+    const int64_t n = n_psi * c;
+    if (n > 0) {
+      hipLaunchKernelGGL(k_add_into, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, out,
+                         (const double*)edge_out->ptr, n);
+      DMRGX_HIP(hipGetLastError());
+    }
   }
 }
 
@@ -829,6 +860,27 @@ int dmrgx_heff_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, voi
   plan->p->adjoint((const double*)psi, (const double*)g_out, (double*)g_psi, (double*)g_renv, (double*)g_lenv,
                    rt().stream);
   DMRGX_HIP(hipStreamSynchronize(rt().stream));  // complete on return
+  CAPI_END
+}
+
+int dmrgx_heff_time_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, void* g_psi, void* g_renv,
+                            void* g_lenv, int iters, double* ms) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p && psi && g_out && iters > 0 && ms, "heff_time_adjoint: bad arguments");
+  hipStream_t s = rt().stream;
+  hipEvent_t e0, e1;
+  DMRGX_HIP(hipEventCreate(&e0));
+  DMRGX_HIP(hipEventCreate(&e1));
+  DMRGX_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < iters; ++i)
+    plan->p->adjoint((const double*)psi, (const double*)g_out, (double*)g_psi, (double*)g_renv, (double*)g_lenv, s);
+  DMRGX_HIP(hipEventRecord(e1, s));
+  DMRGX_HIP(hipStreamSynchronize(s));
+  float tot = 0;
+  DMRGX_HIP(hipEventElapsedTime(&tot, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  ms[0] = tot / iters;
   CAPI_END
 }
 

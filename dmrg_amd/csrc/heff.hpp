@@ -52,6 +52,11 @@ struct HeffPlan {
   // blocks (gR, gL) and of the pass inputs (gP for psi, gS for store), plus an
   // FP32 R pass when the forward plan works on bf16 planes
   std::unique_ptr<GemmPlan> ad_gr, ad_gl, ad_gp, ad_gs, ad_fr;
+  // the periodic edge term (the reference's PBC build, src/core.cpp:27-30):
+  // a second operator over the edge environments whose result apply() adds
+  // to the bulk one; edge_out holds its result
+  std::unique_ptr<HeffPlan> edge;
+  std::unique_ptr<DevBuf> edge_out;
   // ev: optional {R begin, R end, L begin, L end} events around the two GEMM launches
   void apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev = nullptr);
   // Adjoint of out = H_eff(psi; renv, lenv): for a cotangent g of out, the
@@ -62,6 +67,10 @@ struct HeffPlan {
   void adjoint(const double* psi, const double* g_out, double* g_psi, double* g_renv, double* g_lenv,
                hipStream_t s);
 };
+
+// Adds the edge operator e to h: h.apply then computes bulk + edge (the
+// flop counts include the edge's).  Throws when the sizes differ.
+void attach_edge(HeffPlan& h, std::unique_ptr<HeffPlan> e);
 
 // get_map tuples -> GEMM tasks (M = blk_size / col, N = row, K = col); throws
 // on a block index outside [0, n_blk), negative offsets or empty rows / cols.

@@ -18,6 +18,48 @@ qtensor<cplx> centre_as<cplx>(const qtensor<double>& s) {
   return s.comp();
 }
 
+// mps::inverse, src/mps.cpp:56-68
+template <typename T>
+mps<T> mps<T>::inverse() const {
+  mps<T> ret;
+  const int n = (int)state_.size();
+  DMRGX_CHECK(n >= 2, "mps inverse: empty chain");
+  ret.cut_ = n - cut_;
+  ret.state_.resize(n);
+  ret.store_.resize(n);
+  for (int i = 1; i < n - 1; ++i) ret.state_[n - 1 - i] = state_[i].exchange(0, 2);
+  ret.state_[0] = state_[n - 1].exchange(0, 1);
+  ret.state_[n - 1] = state_[0].exchange(0, 1);
+  ret.singular_ = singular_;
+  return ret;
+}
+
+// mps::move_cut_left, src/mps.cpp:71-99: A_{c-2} A_{c-1} S split again with
+// every singular value kept (cutoff 0); site 0 has one leg before the bond
+template <typename T>
+void mps<T>::move_cut_left() {
+  DMRGX_CHECK(cut_ >= 2, "mps move_cut_left: cut=" + std::to_string(cut_));
+  qtensor<T> two, temp;
+  two.contract(state_[cut_ - 2], state_[cut_ - 1]);
+  temp.contract(two, centre_as<T>(singular_));
+  temp.svd(state_[cut_ - 2], singular_, state_[cut_ - 1], cut_ > 2 ? 2 : 1);
+  --cut_;
+}
+
+// mps::move_cut_right, src/mps.cpp:101-125: S A_c A_{c+1} split again
+template <typename T>
+void mps<T>::move_cut_right() {
+  DMRGX_CHECK(cut_ <= (int)state_.size() - 2, "mps move_cut_right: cut=" + std::to_string(cut_));
+  qtensor<T> one, temp;
+  one.contract(centre_as<T>(singular_), state_[cut_]);
+  temp.contract(one, state_[cut_ + 1]);
+  temp.svd(state_[cut_], singular_, state_[cut_ + 1], 2);
+  ++cut_;
+}
+
+template class mps<double>;
+template class mps<cplx>;
+
 // mps::prep_calc (src/mps.cpp:33-54) on a copy: the singular values at the
 // cut are absorbed into the site to their right, leaving one tensor network
 // with no centre; the environments are not carried over.

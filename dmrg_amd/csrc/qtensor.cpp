@@ -18,11 +18,19 @@
 namespace dmrgx {
 
 static int g_symmetry = 2;
+static bool g_s4e = false;
 void set_symmetry(int n) {
-  DMRGX_CHECK(n >= 1, "symmetry order must be positive");
-  g_symmetry = n;
+  DMRGX_CHECK(n >= 1 || n == SYM_S4E, "symmetry: a positive Z_N order or the Z2^3 group");
+  g_s4e = n == SYM_S4E;
+  g_symmetry = g_s4e ? 8 : n;
 }
 int symmetry() { return g_symmetry; }
+bool s4e() { return g_s4e; }
+int s4e_parity(int a) {
+  static const int f[8] = {0, 0, 1, 1, 1, 1, 0, 0};
+  DMRGX_CHECK(a >= 0 && a < 8, "Z2^3 sector out of range");
+  return f[a];
+}
 
 void generate_map(vector<vector<int>>& map, const vector<int>& index) {
   int64_t dim = 1;
@@ -66,20 +74,26 @@ tensor<T>::tensor(int i0, int i1, int i2, int i3, int i4, int i5, int i6, int i7
   val_.assign(n, T(0));
 }
 
+// src/tensor.cpp:68-102: adds v at the index; |v| < TOL is ignored
 template <typename T>
 void tensor<T>::update(T v, int i0, int i1, int i2, int i3, int i4, int i5, int i6, int i7) {
+  if (std::abs(v) < TOL) return;
   vector<int> loc = pack_args(i0, i1, i2, i3, i4, i5, i6, i7, 0);
   DMRGX_CHECK(loc.size() == index_.size(), "tensor::update: wrong number of indices");
   int64_t e = 0;
-  for (size_t i = 0; i < loc.size(); ++i) e = e * index_[i] + loc[i];
-  val_[e] = v;
+  for (size_t i = 0; i < loc.size(); ++i) {
+    DMRGX_CHECK(loc[i] >= 0 && loc[i] < index_[i], "tensor::update: index out of range");
+    e = e * index_[i] + loc[i];
+  }
+  val_[e] += v;
 }
 
+// src/tensor.cpp:176-182: the sum of |values| is below TOL
 template <typename T>
 bool tensor<T>::empty() const {
-  for (auto& x : val_)
-    if (x != T(0)) return false;
-  return true;
+  double sum = 0;
+  for (auto& x : val_) sum += std::abs(x);
+  return sum < TOL;
 }
 
 // ---------------------------------------------------------------- qtensor<T>
@@ -315,6 +329,64 @@ qtensor<T> qtensor<T>::left() const { return take(0, dim_[0][0] - 1); }
 
 template <typename T>
 qtensor<T> qtensor<T>::right() const { return take(2, 0).exchange(0, 1); }
+
+// identity blocks of a 4-leg operator with legs (0, 2) and (1, 3) paired:
+// one block per (s0, s1) sector pair, sym (s0, s1, s0, s1)
+template <typename T>
+static qtensor<T> identity_blocks(qtensor<T>& r) {
+  vector<T> h;
+  for (size_t s0 = 0; s0 < r.dim_[0].size(); ++s0)
+    for (size_t s1 = 0; s1 < r.dim_[1].size(); ++s1) {
+      r.sym_.push_back({(int)s0, (int)s1, (int)s0, (int)s1});
+      const int d0 = r.dim_[0][s0], d1 = r.dim_[1][s1];
+      for (int i0 = 0; i0 < d0; ++i0)
+        for (int i1 = 0; i1 < d1; ++i1)
+          for (int j0 = 0; j0 < d0; ++j0)
+            for (int j1 = 0; j1 < d1; ++j1) h.push_back(i0 == j0 && i1 == j1 ? T(1) : T(0));
+    }
+  r.layout();
+  DMRGX_CHECK(r.total_ == (int64_t)h.size(), "id: block sizes");
+  r.allocate();
+  if (r.total_) upload(r.data(), h.data(), h.size() * sizeof(T), rt().stream);
+  return r;
+}
+
+// src/qtensor.cpp:157-182: the bulk MPO's legs, the bond legs 0 and 2 with
+// this tensor's leg-1 sectors
+template <typename T>
+qtensor<T> qtensor<T>::id(const qtensor<T>& bulk) const {
+  DMRGX_CHECK(dim_.size() >= 2 && bulk.dim_.size() == 4 && bulk.dir_.size() == 4, "id: wrong number of legs");
+  qtensor<T> r;
+  r.dim_ = bulk.dim_;
+  r.dim_[0] = dim_[1];
+  r.dim_[2] = dim_[1];
+  r.dir_ = bulk.dir_;
+  r.dir_[0] = dir_.empty() ? 0 : dir_[1];
+  r.dir_[2] = r.dir_[0];
+  return identity_blocks(r);
+}
+
+// src/qtensor.cpp:128-154
+template <typename T>
+qtensor<T> qtensor<T>::id() const {
+  DMRGX_CHECK(dim_.size() == 4 && dim_[0] == dim_[2] && dim_[1] == dim_[3], "id: wrong dim");
+  qtensor<T> r;
+  r.dim_ = dim_;
+  r.dir_ = dir_;
+  return identity_blocks(r);
+}
+
+template <typename T>
+void qtensor<T>::u_dim(int i0, int i1, int i2, int i3, int i4, int i5, int i6, int i7) {
+  vector<int> idx = pack_args(i0, i1, i2, i3, i4, i5, i6, i7, 1);
+  DMRGX_CHECK(idx.size() == dim_.size(), "u_dim: the number of index doesn't match");
+  for (size_t i = 0; i < idx.size(); ++i)
+    for (int& d : dim_[i]) {
+      DMRGX_CHECK(d == 0 || d == idx[i], "u_dim: a sector already has another dimension");
+      d = idx[i];
+    }
+  layout();
+}
 
 // src/qtensor.cpp:466-585
 template <typename T>

@@ -22,11 +22,18 @@ constexpr int NEV = 6;         // global.h:21
 constexpr int NSI = 20;        // global.h:22
 constexpr double PI_REF = 3.14159265359;  // global.h:25
 
-// Z_N charge group (the reference's compile-time SYMMETRY, src/useful.cpp:29-31)
+// Charge group: Z_N (the reference's compile-time SYMMETRY, src/useful.cpp:29-31)
+// or, for SYM_S4E, the Z2 x Z2 x Z2 group of its S4E build (src/useful.cpp:4-27):
+// eight sectors 0..7 (three bits), fusion a XOR |b|, fermion parity by table
+constexpr int SYM_S4E = -8;
 void set_symmetry(int n);
-int symmetry();
-inline int ss(int a, int b) { return (a + b + symmetry()) % symmetry(); }  // src/useful.cpp:30
-inline int ff(int a) { return a % 2; }                                     // src/useful.cpp:32
+int symmetry();  // number of sectors
+bool s4e();
+int s4e_parity(int a);  // src/useful.cpp:22-26: {0,0,1,1,1,1,0,0}
+inline int ss(int a, int b) {  // src/useful.cpp:6-20, 30
+  return s4e() ? (a ^ (b < 0 ? -b : b)) : (a + b + symmetry()) % symmetry();
+}
+inline int ff(int a) { return s4e() ? s4e_parity(a) : a % 2; }  // src/useful.cpp:22-26, 32
 
 void generate_map(vector<vector<int>>& map, const vector<int>& index);  // src/useful.cpp:119-134
 int64_t get_dimension(const vector<vector<int>>& dim, const vector<vector<int>>& sym);  // :137-148
@@ -91,6 +98,15 @@ class qtensor {
   qtensor take(int n, int k) const;
   qtensor left() const;
   qtensor right() const;
+  // qtensor::id(bulk_mpo), src/qtensor.cpp:157-182: the identity operator
+  // with the legs of the bulk MPO, its bond legs 0 and 2 given this tensor's
+  // leg-1 sectors (carries a periodic edge term through a site)
+  qtensor id(const qtensor& bulk_mpo) const;
+  // qtensor::id(), src/qtensor.cpp:128-154: the identity on a 4-leg operator's
+  // own legs (legs 0, 2 and 1, 3 paired)
+  qtensor id() const;
+  // qtensor::u_dim, src/qtensor.cpp:90-124: every sector of leg i gets dimension d_i
+  void u_dim(int i0, int i1 = 0, int i2 = 0, int i3 = 0, int i4 = 0, int i5 = 0, int i6 = 0, int i7 = 0);
   qtensor combine(int min = 0, int max = 1) const;
   qtensor split(int index, const vector<vector<int>>& dim) const;
   qtensor remove_symmetry() const;
@@ -148,15 +164,40 @@ class mps {
  public:
   int cut_ = 0;
   vector<qtensor<T>> state_, store_;
+  // the periodic edge environments (the reference's PBC build,
+  // include/mps.h:18-50): edge_[0] / edge_[size-1] start as the two halves
+  // of the boundary term and are carried through the chain like store_
+  vector<qtensor<T>> edge_;
+  bool pbc_ = false;
   qtensor<double> singular_;
   mps() = default;
   explicit mps(const mpo<T>& m) { state_.resize(m.size()); store_.resize(m.size()); }
   int size() const { return (int)state_.size(); }
-  void clear(int i) { state_[i].clear(); store_[i].clear(); }
+  void clear(int i) {
+    state_[i].clear();
+    store_[i].clear();
+    if (pbc_) edge_[i].clear();
+  }
+  // mps::add_edge (include/mps.h:47-50)
+  void add_edge(const qtensor<T>& left, const qtensor<T>& right) {
+    edge_.assign(state_.size(), qtensor<T>());
+    edge_.front() = left;
+    edge_.back() = right;
+    pbc_ = true;
+  }
+  bool pbc() const { return pbc_; }
+  qtensor<T>& edge(int n) { return edge_[n]; }
   qtensor<T>& operator[](int n) { return state_[n]; }
   qtensor<T>& operator()(int n) { return store_[n]; }
   qtensor<double>& center() { return singular_; }
   qtensor<double>& center(int n) { cut_ = n; return singular_; }
+  int position() const { return cut_; }
+  // src/mps.cpp:56-128: the chain reversed (site i -> size-1-i, bond legs
+  // exchanged; no environments), and the centre moved by one bond through a
+  // full (untruncated) SVD of the two sites beside it
+  mps inverse() const;
+  void move_cut_left();
+  void move_cut_right();
 };
 
 // Model MPOs (src/operators.cpp)

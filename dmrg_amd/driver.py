@@ -7,21 +7,24 @@ from dataclasses import dataclass
 
 import numpy as np
 
-MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3}
+MODELS = {"xyz": 0, "kitaev": 1, "potts3": 2, "potts2": 3, "4e": 4}
+COMPLEX_MODELS = ("xyz", "potts3", "4e")
 STEP_LOG_WIDTH = 19
+NPARAM = 8  # DMRGX_NPARAM
 DRIVER_EXPORTS = ["dmrgx_run_dmrg", "dmrgx_synth_heff_create", "dmrgx_get_map", "dmrgx_generate_dim_sym",
                   "dmrgx_run_dmrg_state", "dmrgx_state_norm", "dmrgx_state_overlap", "dmrgx_state_expect",
                   "dmrgx_state_destroy", "dmrgx_state_derivative", "dmrgx_run_batch", "dmrgx_parse_args",
-                  "dmrgx_state_heff_create"]
+                  "dmrgx_state_heff_create", "dmrgx_state_position", "dmrgx_state_move_cut",
+                  "dmrgx_state_inverse", "dmrgx_state_correlations_4e"]
 BATCH_LOG_WIDTH = 11
 
 
 class RunConfig(C.Structure):
-    _fields_ = [("model", C.c_int), ("p", C.c_double * 4), ("L", C.c_int), ("m", C.c_int), ("sweeps", C.c_int),
+    _fields_ = [("model", C.c_int), ("p", C.c_double * NPARAM), ("L", C.c_int), ("m", C.c_int), ("sweeps", C.c_int),
                 ("sector", C.c_int), ("schedule", C.c_int), ("prec", C.c_int), ("sign_rule", C.c_int),
                 ("ladder", C.c_int), ("seed", C.c_uint64), ("tol", C.c_double), ("tol_lo", C.c_double),
                 ("refine_k", C.c_int), ("verbose", C.c_int), ("nev_lo", C.c_int), ("output", C.c_char_p),
-                ("warm_start", C.c_int)]
+                ("warm_start", C.c_int), ("pbc", C.c_int)]
 
 
 class RunStats(C.Structure):
@@ -66,6 +69,14 @@ def declare(lib):
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
                                      dp, C.c_char_p, C.c_int]
     lib.dmrgx_parse_args.restype = C.c_int
+    lib.dmrgx_state_position.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    lib.dmrgx_state_position.restype = C.c_int
+    lib.dmrgx_state_move_cut.argtypes = [C.c_void_p, C.c_int]
+    lib.dmrgx_state_move_cut.restype = C.c_int
+    lib.dmrgx_state_inverse.argtypes = [C.c_void_p, C.POINTER(C.c_void_p)]
+    lib.dmrgx_state_inverse.restype = C.c_int
+    lib.dmrgx_state_correlations_4e.argtypes = [C.c_void_p, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int)]
+    lib.dmrgx_state_correlations_4e.restype = C.c_int
     lib.dmrgx_state_destroy.argtypes = [C.c_void_p]
     lib.dmrgx_state_destroy.restype = C.c_int
     return lib
@@ -98,14 +109,14 @@ class State:
     def expect(self, params, model=None) -> complex:
         """calc(mps, mpo) = <a|H(model, params)|a>"""
         from . import lib
-        p = (C.c_double * 4)(*([float(x) for x in params] + [0.0] * (4 - len(params))))
+        p = (C.c_double * NPARAM)(*([float(x) for x in params] + [0.0] * (NPARAM - len(params))))
         return self._call(lib().dmrgx_state_expect, self.h, MODELS[model or self.model], p)
 
     def derivative(self, params, dparams, prec="fp64"):
         """(dE0/dt along dparams at params, E0 of the centre) from the adjoint
         H_eff kernels with recomputed environment tangents."""
         from . import PREC_NAMES, check, lib
-        pad = lambda v: (C.c_double * 4)(*([float(x) for x in v] + [0.0] * (4 - len(v))))
+        pad = lambda v: (C.c_double * NPARAM)(*([float(x) for x in v] + [0.0] * (NPARAM - len(v))))
         out = np.zeros(2)
         check(lib().dmrgx_state_derivative(self.h, pad(params), pad(dparams), PREC_NAMES[prec],
                                            out.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_state_derivative")
@@ -118,8 +129,36 @@ class State:
         h, n, psi = C.c_void_p(), C.c_int64(), C.c_void_p()
         check(lib().dmrgx_state_heff_create(self.h, PREC_NAMES[prec], C.byref(h), C.byref(n), C.byref(psi)),
               "dmrgx_state_heff_create")
-        dt = C64 if self.model in ("xyz", "potts3") else R64
+        dt = C64 if self.model in COMPLEX_MODELS else R64
         return SynthHeff.from_handle(h, n.value, dt, psi.value)
+
+    def position(self) -> int:
+        """mps::position: the bond that holds the singular values"""
+        from . import check, lib
+        c = C.c_int()
+        check(lib().dmrgx_state_position(self.h, C.byref(c)), "dmrgx_state_position")
+        return c.value
+
+    def move_cut(self, steps: int):
+        """mps::move_cut_right (steps > 0) or move_cut_left, |steps| times"""
+        from . import check, lib
+        check(lib().dmrgx_state_move_cut(self.h, int(steps)), "dmrgx_state_move_cut")
+
+    def inverse(self) -> "State":
+        """mps::inverse: the chain read from the other end, as a new state"""
+        from . import check, lib
+        h = C.c_void_p()
+        check(lib().dmrgx_state_inverse(self.h, C.byref(h)), "dmrgx_state_inverse")
+        return State(h, self.model)
+
+    def correlations_4e(self) -> np.ndarray:
+        """The 4e main's data-file lines (x, re, im), dmrgx_state_correlations_4e"""
+        from . import check, lib
+        rows = np.zeros((4096, 3))
+        n = C.c_int()
+        check(lib().dmrgx_state_correlations_4e(self.h, rows.ctypes.data_as(C.POINTER(C.c_double)), len(rows),
+                                                C.byref(n)), "dmrgx_state_correlations_4e")
+        return rows[:n.value].copy()
 
     def fidelity(self, other: "State") -> float:
         """|<b|a>| / sqrt(<a|a> <b|b>)"""
@@ -147,11 +186,11 @@ class RunResult:
 
 
 def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo, refine_k,
-            verbose, nev_lo=0, output=None, warm_start=False):
+            verbose, nev_lo=0, output=None, warm_start=False, pbc=False):
     from . import PREC_NAMES
     cfg = RunConfig()
     cfg.model = MODELS[model]
-    for i, v in enumerate(list(params)[:4]):
+    for i, v in enumerate(list(params)[:NPARAM]):
         cfg.p[i] = float(v)
     cfg.L, cfg.m, cfg.sweeps, cfg.sector = int(L), int(m), int(sweeps), int(sector)
     cfg.schedule = 1 if schedule == "dmrg2" else 0
@@ -164,16 +203,17 @@ def _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladd
     cfg.nev_lo = int(nev_lo)
     cfg.output = output.encode() if output else None
     cfg.warm_start = 1 if warm_start else 0
+    cfg.pbc = 1 if pbc else 0
     return cfg
 
 
 def run_dmrg(model: str, L: int, m: int, sweeps: int, params=(), sector=0, schedule="dmrg2", prec="fp64",
              sign_rule="physical", ladder=True, seed=1234, tol=1e-10, tol_lo=1e-6, refine_k=8,
-             verbose=False, keep_state=False, nev_lo=0, output=None, warm_start=False) -> RunResult:
+             verbose=False, keep_state=False, nev_lo=0, output=None, warm_start=False, pbc=False) -> RunResult:
     from . import check, init, lib
     init()
     cfg = _config(model, L, m, sweeps, params, sector, schedule, prec, sign_rule, ladder, seed, tol, tol_lo,
-                  refine_k, verbose, nev_lo, output, warm_start)
+                  refine_k, verbose, nev_lo, output, warm_start, pbc)
     st = RunStats()
     cap = 20000
     log = np.zeros((cap, STEP_LOG_WIDTH))
@@ -394,6 +434,16 @@ class SynthHeff:
             self._genv = (DeviceArray(rl, npd), DeviceArray(ll, npd))
         gr, gl = self._genv
         check(lib().dmrgx_heff_adjoint(self.h, psi.ptr, g_out.ptr, g_psi.ptr, gr.ptr, gl.ptr), "dmrgx_heff_adjoint")
+
+    def time_adjoint(self, psi, g_out, g_psi, iters):
+        """HIP-event milliseconds per adjoint_synth call (dmrgx_heff_time_adjoint)."""
+        from . import check, lib
+        self.adjoint_synth(psi, g_out, g_psi)  # allocates the environment cotangents
+        gr, gl = self._genv
+        ms = np.zeros(1)
+        check(lib().dmrgx_heff_time_adjoint(self.h, psi.ptr, g_out.ptr, g_psi.ptr, gr.ptr, gl.ptr, int(iters),
+                                            ms.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_heff_time_adjoint")
+        return float(ms[0])
 
     def close(self):
         from . import lib

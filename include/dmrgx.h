@@ -122,6 +122,11 @@ int dmrgx_heff_destroy(dmrgx_heff* plan);
  * GEMM kernel, ms[3] / ms[4] = algorithmic flops of the R / L pass, ms[5] /
  * ms[6] = flops the R / L pass executes (7 doubles). */
 int dmrgx_heff_time(dmrgx_heff* plan, const void* in, void* out, int iters, double* ms);
+/* dmrgx_heff_adjoint run `iters` times between two HIP events on the library's
+ * stream: ms[0] = event time per call (no counterpart in the reference, which
+ * has no derivatives) */
+int dmrgx_heff_time_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out, void* g_psi, void* g_renv,
+                            void* g_lenv, int iters, double* ms);
 
 /* ---- eigensolver -------------------------------------------------------------
  * Replaces znaupd(n, nev, Evals, Evecs, lanczos<T>&) (src/dsaupd.cpp:65-155,

@@ -16,14 +16,19 @@
 extern "C" {
 #endif
 
-enum { DMRGX_MODEL_XYZ = 0, DMRGX_MODEL_KITAEV = 1, DMRGX_MODEL_POTTS3 = 2, DMRGX_MODEL_POTTS2 = 3 };
+enum { DMRGX_MODEL_XYZ = 0, DMRGX_MODEL_KITAEV = 1, DMRGX_MODEL_POTTS3 = 2, DMRGX_MODEL_POTTS2 = 3, DMRGX_MODEL_4E = 4 };
+/* model parameters carried by a configuration and by a finished state */
+#define DMRGX_NPARAM 8
 enum { DMRGX_SCHEDULE_DMRG = 0, DMRGX_SCHEDULE_DMRG2 = 1 };
 enum { DMRGX_SIGNS_PHYSICAL = 0, DMRGX_SIGNS_REFERENCE = 1 };
 
 typedef struct dmrgx_run_config {
   int model;        /* DMRGX_MODEL_* */
-  double p[4];      /* xyz: jx jy jz; kitaev: t p u; potts3: f j p t; potts2: h j */
-  int L, m, sweeps, sector;   /* the reference's l= m= n= s= (src/functions.cpp:46-66) */
+  double p[DMRGX_NPARAM];  /* xyz: jx jy jz; kitaev: t p u; potts3: f j p t; potts2: h j;
+                              4e: t1 t3 f0 f1 frac (4e.cpp:39); unused: 0 */
+  int L, m, sweeps, sector;   /* the reference's l= m= n= s= (src/functions.cpp:46-66); for
+                                 4e, L physical sites (a multiple of 4) in L/2 MPO tensors
+                                 (4e.cpp:46-47) and a Z2 x Z2 x Z2 sector 0..7 */
   int schedule;     /* DMRGX_SCHEDULE_DMRG (the mains' dmrg) or _DMRG2 (two-site) */
   int prec;         /* matvec arithmetic: DMRGX_FP64 / DMRGX_FP32 / DMRGX_BF16X3 */
   int sign_rule;    /* DMRGX_SIGNS_PHYSICAL (alpha=1 for bosonic models) or _REFERENCE */
@@ -42,6 +47,10 @@ typedef struct dmrgx_run_config {
                        reference's six pairs; 1: a sweep step starts from the state the previous
                        step left (White's prediction) and converges the lowest pair only
                        (E1..E5 read 0).  dmrgx_run_batch accepts 0 only. */
+  int pbc;          /* 1: periodic chain, the reference's PBC build: the mains' add_edge terms
+                       (xyz.cpp:39-43, kitaev.cpp:43-46, potts.cpp:45-48) carried by edge
+                       environments, H_eff = bulk + L_edge W_id W_id R_edge (src/core.cpp:27-30);
+                       not for potts2, dmrgx_run_batch or the derivative / centre-operator calls */
 } dmrgx_run_config;
 
 typedef struct dmrgx_run_stats {
@@ -100,7 +109,8 @@ int dmrgx_run_batch(const dmrgx_run_config* cfg, int n_chains, const double* p, 
  *   dmrgx_state_norm:    calc(mps)        = <a|a>
  *   dmrgx_state_overlap: calc(mps, mps)   = <b|a>  (same model, chain length)
  *   dmrgx_state_expect:  calc(mps, mpo)   = <a|H(model, p)|a>, H built by the
- *                        model builder with couplings p (same symmetry as a);
+ *                        model builder with couplings p[DMRGX_NPARAM] (same
+ *                        symmetry as a; open-chain MPO, no edge terms);
  *                        linear builders give Hellmann-Feynman derivatives,
  *                        e.g. XYZ with p = {0, 0, 1} is dH/dJz. */
 typedef struct dmrgx_state dmrgx_state;
@@ -115,7 +125,8 @@ int dmrgx_state_expect(const dmrgx_state* a, int model, const double* p, double*
  * H(p + dp) - H(p) are recomputed from the MPS site by site, and
  * dE0/dt = Re(<g_L, dL> + <g_R, dR>) with g_L, g_R the environment
  * cotangents for the output cotangent psi (Hellmann-Feynman at fixed MPS
- * basis).  out = {dE0/dt, E0 of the centre}; prec DMRGX_FP64 or DMRGX_FP32. */
+ * basis).  out = {dE0/dt, E0 of the centre}; prec DMRGX_FP64 or DMRGX_FP32.
+ * p and dp hold DMRGX_NPARAM values.  States of periodic runs are refused. */
 int dmrgx_state_derivative(const dmrgx_state* a, const double* p, const double* dp, int prec, double* out);
 /* The two-site H_eff at the centre bond of a finished run's state (the
  * operator its last step solved, rebuilt): both environments recomputed from
@@ -125,6 +136,23 @@ int dmrgx_state_derivative(const dmrgx_state* a, const double* p, const double* 
  * state's two-site vector in the plan's psi layout (n_psi elements, owned by
  * the plan).  No counterpart in the reference beyond av() itself. */
 int dmrgx_state_heff_create(const dmrgx_state* a, int prec, dmrgx_heff** plan, int64_t* n_psi, const void** psi);
+/* mps::position, mps::move_cut_left / move_cut_right (repeated |steps| times;
+ * steps > 0 moves right) and mps::inverse (src/mps.cpp:56-128, include/mps.h:55-74)
+ * on the state as the run left it; the observables above then use the moved
+ * or reversed state.  The centre stays on bonds 1..L-1 (an error otherwise;
+ * the reference prints one and returns).  dmrgx_state_inverse returns a new
+ * state, released by dmrgx_state_destroy. */
+int dmrgx_state_position(const dmrgx_state* a, int* cut);
+int dmrgx_state_move_cut(dmrgx_state* a, int steps);
+int dmrgx_state_inverse(const dmrgx_state* a, dmrgx_state** out);
+/* The correlation functions the 4e main writes to its data file
+ * (4e.cpp:71-196), for a state of the 4e model: the end-to-end pair, c_up and
+ * c_down correlations (L1 R1 .. L2 R2 for each), the normalization calc(mps),
+ * then the Cooper-pair expectation on every interior tensor (positions 2q+1,
+ * 2q+2) and the pair-pair ones (2q+1.5, 2q+2.5).  Row i of rows = {x, re, im}
+ * (x: the number of tensors for the first 13 lines, else the position), as
+ * the main prints them; *n rows are written (at most cap). */
+int dmrgx_state_correlations_4e(const dmrgx_state* a, double* rows, int cap, int* n);
 int dmrgx_state_destroy(dmrgx_state* a);
 
 /* Synthetic two-site operator with the block structure of a Z_S model at bond
@@ -133,6 +161,12 @@ int dmrgx_state_destroy(dmrgx_state* a);
  * plan owns its environments. */
 int dmrgx_synth_heff_create(int S, int D, const int* bond, int n_sec, int dtype, int prec, uint64_t seed,
                             int fermion, dmrgx_heff** plan, int64_t* n_psi);
+
+/* `symmetry` below: N >= 1 for the Z_N group of the reference's SYMMETRY N
+ * build, or DMRGX_SYM_S4E for the Z2 x Z2 x Z2 group of its S4E build
+ * (eight sectors, fusion by XOR and the fermion-parity table of
+ * src/useful.cpp:4-27). */
+#define DMRGX_SYM_S4E (-8)
 
 /* qtensor<T>::get_map on structure only (src/qtensor.cpp:997-1119).
  * Tensor / array descriptions: desc = [nleg, nsec[nleg], dims (per leg, per

@@ -41,9 +41,12 @@ def kitaev_bdg_e0(L, T, P, U):
     return float(e0), float(pos[0])
 
 
-def kitaev_ed(L, T, P, U, sector=0, k=1):
+def kitaev_ed(L, T, P, U, sector=0, k=1, pbc=False):
     """Brute-force Jordan-Wigner diagonalisation of the same Hamiltonian in a
-    fixed fermion-parity sector (local state 1 = occupied)."""
+    fixed fermion-parity sector (local state 1 = occupied).  pbc adds the bond
+    (L-1, 0) written in the fermion operators themselves (the Jordan-Wigner
+    string of c_0 c_{L-1} comes out of the matrices): the periodic ring with
+    no twist."""
     Z = sp.csr_matrix(np.diag([1.0, -1.0]))
     cm = sp.csr_matrix(np.array([[0.0, 1.0], [0.0, 0.0]]))  # <0|c|1> = 1
     I2 = sp.identity(2, format="csr")
@@ -57,9 +60,10 @@ def kitaev_ed(L, T, P, U, sector=0, k=1):
 
     cs = [c(i) for i in range(L)]
     H = sp.csr_matrix((2 ** L, 2 ** L))
-    for i in range(L - 1):
-        hop = T * (cs[i].T @ cs[i + 1])
-        pair = -P * (cs[i] @ cs[i + 1])
+    for i in range(L if pbc else L - 1):
+        j = (i + 1) % L
+        hop = T * (cs[i].T @ cs[j])
+        pair = -P * (cs[i] @ cs[j])
         H = H + hop + hop.T + pair + pair.T
     for i in range(L):
         H = H - 2 * U * (cs[i].T @ cs[i])
@@ -81,16 +85,16 @@ def _site_op(op, i, L, d):
     return sp.kron(sp.kron(left, op, format="csr"), right, format="csr")
 
 
-def xyz_hamiltonian(L, Jx, Jy, Jz):
+def xyz_hamiltonian(L, Jx, Jy, Jz, pbc=False):
     """Open XYZ chain in Pauli units (H_XYZ, src/operators.cpp:355-391):
     H = sum_i Jx sx sx + Jy sy sy + Jz sz sz.  Basis: site 0 most significant,
-    local state 0 = spin up (sz=+1)."""
+    local state 0 = spin up (sz=+1).  pbc adds the bond (L-1, 0)."""
     sx, sy, sz = _pauli()
     H = sp.csr_matrix((2 ** L, 2 ** L), dtype=complex)
-    for i in range(L - 1):
+    for i in range(L if pbc else L - 1):
         for J, o in ((Jx, sx), (Jy, sy), (Jz, sz)):
             if J != 0:
-                H = H + J * (_site_op(o, i, L, 2) @ _site_op(o, i + 1, L, 2))
+                H = H + J * (_site_op(o, i, L, 2) @ _site_op(o, (i + 1) % L, L, 2))
     return H
 
 
@@ -101,8 +105,8 @@ def sector_projector_z2(L):
     return idx[par == 0], idx[par == 1]
 
 
-def xyz_e0(L, Jx, Jy, Jz, sector=0, k=1):
-    H = xyz_hamiltonian(L, Jx, Jy, Jz).tocsr()
+def xyz_e0(L, Jx, Jy, Jz, sector=0, k=1, pbc=False):
+    H = xyz_hamiltonian(L, Jx, Jy, Jz, pbc).tocsr()
     ev, od = sector_projector_z2(L)
     sel = ev if sector == 0 else od
     Hs = H[sel][:, sel]
@@ -125,17 +129,21 @@ def xy_free_fermion_e0(L, J):
     return float(np.sum(e[e < 0])), int(np.sum(e < 0))
 
 
-def potts3_hamiltonian(L, f, j, p, t):
+def potts3_hamiltonian(L, f, j, p, t, pbc=False):
     """Chiral 3-state Potts chain in the reference convention
     (H_Potts3, src/operators.cpp:293-350):
        H = -f sum_i (e^{ip} T_i + h.c.) - J sum_i (e^{it} S_i S+_{i+1} + h.c.)
     with T = diag(1, w, w^2), w = e^{2 pi i/3} (truncated PI), S the cyclic shift
     S|q> = |q-1>.  The MPO entries fix the exact operator ordering; this
     function builds the same operator from the MPO tensors directly, so it is
-    independent of the DMRG machinery but not of the MPO."""
+    independent of the DMRG machinery but not of the MPO.  pbc adds the edge
+    term of H_Potts3(0, j, p, t) (see mpo_edge_dense)."""
     from .models import H_Potts3
     W = H_Potts3(f, j, p, t).remove_symmetry().val[0]  # (4,3,4,3)
-    return mpo_dense(W, L, 3)
+    H = mpo_dense(W, L, 3)
+    if pbc:
+        H = H + mpo_edge_dense(H_Potts3(0.0, j, p, t).remove_symmetry().val[0], L, 3)
+    return H.tocsr()
 
 
 def mpo_dense(W, L, d):
@@ -161,6 +169,26 @@ def mpo_dense(W, L, d):
             nxt.append(acc)
         cur = nxt
     return cur[0].tocsr()
+
+
+def mpo_edge_dense(W, L, d):
+    """The boundary term that a periodic chain's two edge tensors W.left()
+    (on site 0) and W.right() (on site L-1) carry between them through
+    identity operators: sum_w W[D-1, :, w, :] (site 0) x 1 x ... x 1 x
+    W[w, :, 0, :] (site L-1).  Each term of W starts on site 0 and ends on
+    site L-1: the bond (0, L-1) with the orientation of an open bond (i, i+1),
+    not of the ring bond (L-1, 0); the two agree for bonds symmetric under
+    exchanging the sites (XYZ) and differ for the chiral Potts bond."""
+    D = W.shape[0]
+    mid = sp.identity(d ** (L - 2), format="csr")
+    H = None
+    for w in range(D):
+        a, b = W[D - 1, :, w, :], W[w, :, 0, :]
+        if not a.any() or not b.any():
+            continue
+        t = sp.kron(sp.kron(sp.csr_matrix(a), mid, format="csr"), sp.csr_matrix(b), format="csr")
+        H = t if H is None else H + t
+    return H if H is not None else sp.csr_matrix((d ** L, d ** L), dtype=W.dtype)
 
 
 def z_n_sector_indices(L, d, n, sector):

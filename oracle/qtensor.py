@@ -19,12 +19,21 @@ NSI = 20             # global.h:22
 PI = 3.14159265359   # global.h:25 (truncated pi, used by H_Potts3)
 
 # Z_N group order; the reference fixes it at compile time (global.h:16).
+# S4E = -8 selects the Z2 x Z2 x Z2 group of the reference's S4E build.
+S4E = -8
 _SYMMETRY = 2
+_S4E = False
+# src/useful.cpp:9-16 and 24: the fusion table and the fermion parities of
+# the eight Z2^3 sectors (bits 000 .. 111)
+_S4E_TABLE = [[0, 1, 2, 3, 4, 5, 6, 7], [1, 0, 3, 2, 5, 4, 7, 6], [2, 3, 0, 1, 6, 7, 4, 5], [3, 2, 1, 0, 7, 6, 5, 4],
+              [4, 5, 6, 7, 0, 1, 2, 3], [5, 4, 7, 6, 1, 0, 3, 2], [6, 7, 4, 5, 2, 3, 0, 1], [7, 6, 5, 4, 3, 2, 1, 0]]
+_S4E_PARITY = [0, 0, 1, 1, 1, 1, 0, 0]
 
 
 def set_symmetry(n: int) -> None:
-    global _SYMMETRY
-    _SYMMETRY = int(n)
+    global _SYMMETRY, _S4E
+    _S4E = int(n) == S4E
+    _SYMMETRY = 8 if _S4E else int(n)
 
 
 def symmetry() -> int:
@@ -32,12 +41,18 @@ def symmetry() -> int:
 
 
 def ss(a: int, b: int) -> int:
-    """Charge fusion, src/useful.cpp:30: (a+b+N)%N."""
+    """Charge fusion, src/useful.cpp:30: (a+b+N)%N; the S4E table lookup
+    p[a][|b|] (src/useful.cpp:6-20) for Z2^3."""
+    if _S4E:
+        return _S4E_TABLE[a][-b if b < 0 else b]
     return (a + b + _SYMMETRY) % _SYMMETRY
 
 
 def ff(a: int) -> int:
-    """Fermion parity of a charge, src/useful.cpp:32: a%2."""
+    """Fermion parity of a charge, src/useful.cpp:32: a%2; the S4E table
+    (src/useful.cpp:22-26) for Z2^3."""
+    if _S4E:
+        return _S4E_PARITY[a]
     return a % 2
 
 

@@ -107,3 +107,19 @@ def test_batched_warm_start_refused():
     rc = _lib().dmrgx_run_batch(C.byref(cfg), 2, dp(p), dp(e), dp(tr), C.byref(st), None, 0, C.byref(n), None)
     assert rc != 0
     assert "warm" in _err()
+
+
+def test_periodic_refusals():
+    """potts2 has no edge terms in the reference; periodic chains are not batched."""
+    cfg = driver._config("potts2", 8, 16, 1, (1.0, 1.0), 0, "dmrg2", "fp64", "physical", True, 1, 1e-10, 1e-6, 8,
+                         False, pbc=True)
+    st = driver.RunStats()
+    n = C.c_int()
+    assert _lib().dmrgx_run_dmrg_state(C.byref(cfg), C.byref(st), None, 0, C.byref(n), None) != 0
+    assert "periodic" in _err()
+    cfg = driver._config("xyz", 8, 16, 1, (), 0, "dmrg2", "fp64", "physical", True, 1, 1e-10, 1e-6, 8, False,
+                         pbc=True)
+    p = np.ones((2, 4))
+    assert _lib().dmrgx_run_batch(C.byref(cfg), 2, p.ctypes.data_as(C.POINTER(C.c_double)), None, None,
+                                  C.byref(st), None, 0, C.byref(n), None) != 0
+    assert "periodic" in _err()

@@ -89,3 +89,64 @@ def test_overlap_fidelity_between_runs(gpu):
     assert abs(ab - np.conj(ba)) < 1e-10
     for r in (a, b, c):
         r.state.close()
+
+
+def test_move_cut_and_inverse(gpu):
+    """mps::move_cut_left / move_cut_right and mps::inverse
+    (src/mps.cpp:56-128).  Moving the centre splits two sites again with every
+    singular value kept, so the state is unchanged: norm 1, <psi|H|psi> = E0,
+    overlap 1 with an unmoved copy, and the derivative through the
+    environments at the new centre still equals Hellmann-Feynman.  The
+    reversed XYZ chain has the same energy (H is reflection symmetric), and
+    reversing twice gives the state back."""
+    L, m = 12, 32
+    print(f"case xyz L={L} m={m} move_cut / inverse", flush=True)
+    res = driver.run_dmrg("xyz", L, m, 2, XYZ, keep_state=True)
+    st = res.state
+    c0 = st.position()
+    assert 1 <= c0 <= L - 1
+    copy = st.inverse().inverse()
+    assert copy.position() == c0
+    assert abs(copy.fidelity(st) - 1) < 1e-12
+    hf = st.expect((0.0, 0.0, 1.0)).real
+    for steps in (-3, 5, -2):
+        st.move_cut(steps)
+        c0 += steps
+        assert st.position() == c0
+        assert abs(st.norm2() - 1) < 1e-10
+        e = st.expect(XYZ)
+        assert abs(e.real - res.energy) < 1e-9 and abs(e.imag) < 1e-10, (steps, e, res.energy)
+        assert abs(abs(st.overlap(copy)) - 1) < 1e-10
+        der, ec = st.derivative(XYZ, (0.0, 0.0, 1.0))
+        assert abs(ec - res.energy) < 1e-9 and abs(der - hf) < 1e-9, (steps, der, hf)
+    # the centre stays on bonds 1..L-1
+    with pytest.raises(RuntimeError):
+        st.move_cut(L)
+    with pytest.raises(RuntimeError):
+        st.move_cut(-c0)
+    assert st.position() == c0
+    inv = st.inverse()
+    assert inv.position() == L - c0
+    assert abs(inv.norm2() - 1) < 1e-10
+    assert abs(inv.expect(XYZ).real - res.energy) < 1e-9
+    back = inv.inverse()
+    assert abs(back.fidelity(st) - 1) < 1e-12
+    for s in (st, copy, inv, back):
+        s.close()
+
+
+def test_potts3_inverse_is_the_reflected_chain(gpu):
+    """Reflection maps the chiral Potts chain H(theta) to H(-theta)
+    (sigma+_i sigma_{i+1} e^{i theta} becomes sigma+_{i+1} sigma_i e^{i theta}),
+    so the reversed ground state has energy E0 under H(-theta) and, being no
+    eigenstate of H(theta) in general, a higher one under H(theta)."""
+    L, m = 8, 81
+    print(f"case potts3 L={L} m={m} inverse", flush=True)
+    res = driver.run_dmrg("potts3", L, m, 2, POTTS, keep_state=True)
+    inv = res.state.inverse()
+    assert abs(inv.norm2() - 1) < 1e-10
+    mirror = POTTS[:3] + (-POTTS[3],)
+    assert abs(inv.expect(mirror).real - res.energy) < 1e-9
+    assert inv.expect(POTTS).real > res.energy + 1e-6
+    inv.close()
+    res.state.close()

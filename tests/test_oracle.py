@@ -26,6 +26,30 @@ def test_kitaev_bdg_matches_jordan_wigner_ed(L):
     assert abs(abs(ev - od) - gap) < 1e-12
 
 
+@pytest.mark.parametrize("sector", [0, 1])
+def test_kitaev_ring_hopping_is_the_periodic_free_fermion_ring(sector):
+    """With the pairing off, the periodic ED of kitaev_ed is free fermions on a
+    ring with periodic boundary conditions in both parity sectors:
+    e_k = 2T cos(2 pi n / L) - 2U, E0 = the lowest N levels with N of the
+    sector's parity (the ring bond is written in fermion operators, so no
+    parity-dependent twist appears)."""
+    L, T, U = 7, -1.0, 0.3
+    e = np.sort(2 * T * np.cos(2 * np.pi * np.arange(L) / L) - 2 * U)
+    best = min(np.sum(e[:n]) for n in range(L + 1) if n % 2 == sector)
+    assert abs(exact.kitaev_ed(L, T, 0.0, U, sector, pbc=True)[0] - best) < 1e-12
+    # the open chain differs
+    assert abs(exact.kitaev_ed(L, T, 0.0, U, sector)[0] - best) > 1e-3
+
+
+def test_edge_term_of_the_xyz_mpo_is_the_ring_bond():
+    """mpo_edge_dense(W) for the XYZ MPO equals the bond (L-1, 0) of the ring:
+    the XYZ bond is symmetric under exchanging its two sites."""
+    L = 6
+    W = models.H_XYZ(1.0, 0.8, 0.5).remove_symmetry().val[0]
+    ring = exact.xyz_hamiltonian(L, 1.0, 0.8, 0.5, pbc=True)
+    assert abs(ring - (exact.mpo_dense(W, L, 2) + exact.mpo_edge_dense(W, L, 2))).max() < 1e-14
+
+
 def test_xy_free_fermions_match_ed():
     L = 8
     e, nf = exact.xy_free_fermion_e0(L, 1.0)
