@@ -384,3 +384,40 @@ def calc_expect(st, mpo, fermion=True):
     tmp.conjugate()
     end = QTensor().contract_gemm(tmp, store, "N", "N", 1).simplify()
     return left.trace(end, fermion)
+
+
+def exact_chain_spectrum(mpo, sector, k=1, edges=None, alpha=True, tol=1e-12):
+    """The k lowest eigenvalues of a four-tensor chain from its exact two-site
+    operator on tensors (1, 2): the stores of tensors 0 and 3 are the MPO edge
+    tensors themselves (no basis rotation, nothing truncated), so H_eff is the
+    whole Hamiltonian (update_two's environments, src/core.cpp:188-192, and
+    av, src/core.cpp:22-31).  `edges` = (edge of tensor 0, edge of tensor 3)
+    adds the periodic edge term as update_two does under PBC
+    (src/core.cpp:193-199, 241-247): both edge environments through the
+    identity operator built from the left edge, the edge R pass over its own
+    store layout and the edge L pass accumulated onto the result."""
+    assert mpo.size() == 4
+    lenv = QTensor().contract_index(mpo[0], 1, mpo[1], 0).exchange(3, 4)
+    renv = QTensor().contract_index(mpo[3], 1, mpo[2].exchange(0, 2), 0, True).exchange(0, 1)
+    dim, sym = generate_dim_sym(lenv.dimension_all(), 2, renv.dimension_all(), 2, sector)
+    n = get_dimension(dim, sym)
+    ctx = Lanczos(lenv=lenv, renv=renv, r_num=2, l_num=3)
+    ctx.rmap, dim_ret, sym_ret = renv.get_map(dim, sym, "N", "T", 2, 0, False, fermion=alpha)
+    ctx.d_store = get_dimension(dim_ret, sym_ret)
+    ctx.lmap, _, _ = lenv.get_map(dim_ret, sym_ret, "N", "T", 3, 0, True, sym_ret=sym, fermion=alpha)
+    ops = [ctx]
+    if edges is not None:
+        e0, e3 = edges
+        ope = e0.id(mpo[1])
+        ledge = QTensor().contract_index(e0, 1, ope, 0).exchange(3, 4)
+        redge = QTensor().contract_index(e3, 1, ope, 0, True).exchange(0, 1)
+        ectx = Lanczos(lenv=ledge, renv=redge, r_num=2, l_num=3)
+        ectx.rmap, edim_ret, esym_ret = redge.get_map(dim, sym, "N", "T", 2, 0, False, fermion=alpha)
+        ectx.d_store = get_dimension(edim_ret, esym_ret)
+        ectx.lmap, _, _ = ledge.get_map(edim_ret, esym_ret, "N", "T", 3, 0, True, sym_ret=sym, fermion=alpha)
+        ops.append(ectx)
+    dtype = np.result_type(lenv.dtype, renv.dtype)
+    op = spla.LinearOperator((n, n), matvec=lambda x: sum(av(np.asarray(x, dtype=dtype).ravel(), c) for c in ops),
+                             dtype=dtype)
+    w = spla.eigsh(op, k=k, which="SA", tol=tol, v0=np.ones(n, dtype=dtype))[0]
+    return np.sort(w), n

@@ -154,9 +154,10 @@ class QTensor:
 
     def update(self, block, *loc):
         """src/qtensor.cpp:51-85 (empty blocks are skipped by the reference:
-        ``tensor::empty`` is true when every value is 0)."""
+        ``tensor::empty`` is true when the sum of |values| is below TOL,
+        src/tensor.cpp:176-182)."""
         block = np.asarray(block, dtype=self.dtype)
-        if not np.any(block != 0):
+        if np.sum(np.abs(block)) < TOL:
             return
         assert len(loc) == len(self.dim)
         self.sym.append(list(loc))
@@ -167,6 +168,38 @@ class QTensor:
         """src/qtensor.cpp:184-202."""
         self.dir = list(dirs)
         assert len(self.dir) == len(self.dim)
+
+    def u_dim(self, *dims):
+        """src/qtensor.cpp:90-124: every sector of leg i gets dimension dims[i]."""
+        assert len(dims) == len(self.dim)
+        for i, d in enumerate(dims):
+            for j in range(len(self.dim[i])):
+                assert self.dim[i][j] in (0, d)
+                self.dim[i][j] = d
+
+    def _identity(self):
+        for s0 in range(len(self.dim[0])):
+            for s1 in range(len(self.dim[1])):
+                d0, d1 = self.dim[0][s0], self.dim[1][s1]
+                self.sym.append([s0, s1, s0, s1])
+                self.val.append(np.einsum("ac,bd->abcd", np.eye(d0), np.eye(d1)).astype(self.dtype))
+        return self
+
+    def id(self, bulk=None):
+        """qtensor::id() (src/qtensor.cpp:128-154) and id(bulk_mpo)
+        (src/qtensor.cpp:157-182): identity blocks with legs (0, 2) and (1, 3)
+        paired; with a bulk MPO, its legs with bond legs 0 and 2 given this
+        tensor's leg-1 sectors."""
+        r = QTensor(dtype=self.dtype)
+        if bulk is None:
+            assert self.dim[0] == self.dim[2] and self.dim[1] == self.dim[3]
+            r.dim, r.dir = copy.deepcopy(self.dim), list(self.dir)
+        else:
+            r.dim = copy.deepcopy(bulk.dim)
+            r.dim[0], r.dim[2] = list(self.dim[1]), list(self.dim[1])
+            r.dir = list(bulk.dir)
+            r.dir[0] = r.dir[2] = self.dir[1]
+        return r._identity()
 
     def dimension_all(self):
         return copy.deepcopy(self.dim)

@@ -2,11 +2,15 @@
 //
 // Built by oracle/Makefile from /root/reference/src/{qtensor,tensor,useful,
 // operators}.cpp with the reference's shipped global.h (SYMMETRY 2, FERMION)
-// into oracle/_ref/ref_harness; see the Makefile for why the numerical
-// wrappers are left unlinked.  It reads one case on stdin and prints the
-// reference's results on stdout, for tests/golden/make_ref_vectors.py:
+// into oracle/_ref/ref_harness, and once more with the build switches of
+// its 4e main (SYMMETRY 8, S4E, PBC; 4e.cpp:28-32) into
+// oracle/_ref/ref_harness_s4e, which adds 4e.cpp's builders; see the
+// Makefile for why the numerical wrappers are left unlinked.  It reads one
+// case on stdin and prints the reference's results on stdout, for
+// tests/golden/make_ref_vectors.py:
 //
 //   input:   model <kitaev|xyz|ising|potts2> p0 p1 p2 p3
+//            (S4E build: model <4e1|4e2> t1 t3 f0 f1 p1 frac)
 //            lstore <tensor>      the left environment store(l-1) [a, w, a']
 //            rstore <tensor>      the right environment store(r+1) [b, w, b']
 //            sector <s>
@@ -21,6 +25,14 @@
 //            rmap, dim_ret, sym_ret   renv.get_map(..., r_num=2, back=false)
 //            lmap, ldim            lenv.get_map(..., l_num=3, back=true)
 //                                  (src/core.cpp:236-240, src/qtensor.cpp:997-1119)
+//            ledge / redge         (PBC build) the edge environments of
+//                                  update_two with lstore / rstore as the
+//                                  stored edges: the identity operator
+//                                  id(W) (src/qtensor.cpp:157-182) and the
+//                                  contractions of src/core.cpp:193-199
+//            ops                   (S4E build) the bond-1 operators of the
+//                                  4e main, c_up1 .. c_pair_dagger2 and
+//                                  c_pair1().id() (4e.cpp:587-779, 93)
 #include <cstdio>
 #include <cstdlib>
 #include <sstream>
@@ -31,6 +43,21 @@
 #include "operators.h"
 #include "qtensor.h"
 #include "useful.h"
+
+#ifdef S4E
+// 4e.cpp:10-25 (compiled from 4e.cpp with its main renamed)
+qtensor<complex<double> > H_4e1(double t1, double t3, double f0, double f1, double p1, double frac);
+qtensor<complex<double> > H_4e2(double t1, double t3, double f0, double f1, double p1, double frac);
+qtensor<complex<double> > c_up1();
+qtensor<complex<double> > c_down1();
+qtensor<complex<double> > c_up2();
+qtensor<complex<double> > c_down2();
+qtensor<complex<double> > c_pair1();
+qtensor<complex<double> > c_pair2();
+qtensor<complex<double> > c_pairpair();
+qtensor<complex<double> > c_pair_dagger1();
+qtensor<complex<double> > c_pair_dagger2();
+#endif
 
 namespace {
 
@@ -190,6 +217,28 @@ int run(const qtensor<T>& W) {
   lenv.get_map(lmap, ldim, lsym, dim_ret, sym_ret, 'N', 'T', 3, 0, true);
   print_map("lmap", lmap);
   print_dimsym("ldim", ldim);
+#ifdef PBC
+  // src/core.cpp:193-199: both edges through the identity built from the left one
+  qtensor<T> id = ls.id(W), ledge, redge;
+  ledge.contract(ls, 1, id, 0);
+  ledge = ledge.exchange(3, 4);
+  redge.contract(rs, 1, id, 0, true);
+  redge = redge.exchange(0, 1);
+  print_tensor("ledge", ledge);
+  print_tensor("redge", redge);
+#endif
+#ifdef S4E
+  print_tensor("op_up1", c_up1());
+  print_tensor("op_down1", c_down1());
+  print_tensor("op_up2", c_up2());
+  print_tensor("op_down2", c_down2());
+  print_tensor("op_pair1", c_pair1());
+  print_tensor("op_pair2", c_pair2());
+  print_tensor("op_pairpair", c_pairpair());
+  print_tensor("op_pair_dagger1", c_pair_dagger1());
+  print_tensor("op_pair_dagger2", c_pair_dagger2());
+  print_tensor("op_id", c_pair1().id());
+#endif
   return 0;
 }
 
@@ -199,6 +248,14 @@ int main() {
   try {
     if (read_word() != "model") throw std::string("expected model");
     const std::string m = read_word();
+#ifdef S4E
+    double q[6];
+    for (int i = 0; i < 6; ++i)
+      if (std::scanf("%lf", &q[i]) != 1) throw std::string("bad parameter");
+    if (m == "4e1") return run<complex<double> >(H_4e1(q[0], q[1], q[2], q[3], q[4], q[5]));
+    if (m == "4e2") return run<complex<double> >(H_4e2(q[0], q[1], q[2], q[3], q[4], q[5]));
+    throw std::string("the S4E build knows the models 4e1 and 4e2 only");
+#endif
     double p[4];
     for (int i = 0; i < 4; ++i)
       if (std::scanf("%lf", &p[i]) != 1) throw std::string("bad parameter");
