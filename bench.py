@@ -61,9 +61,12 @@ FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
 
 def workload_key(real, a):
+    """The traffic table's workload: the synthetic shapes, or the centre
+    operator of a converged chain, whose block structure (sectors, MPO bond,
+    the zero slices the plan skips) depends on the model and m, not on L."""
     if not real:
         return "synthetic"
-    return f"real_{a.sweep_model}_L{a.sweep_L}_m{a.sweep_m}"
+    return f"real_{a.sweep_model}_m{a.sweep_m}"
 
 
 def traffic_bytes(workload, r_items, l_items):
@@ -315,17 +318,30 @@ def cpu_sweep_leg(L=40, m=64, sweeps=5):
     t0 = time.perf_counter()
     g = driver.run_dmrg("kitaev", L, m, sweeps, p, schedule="dmrg2")
     gpu_s = time.perf_counter() - t0
-    mpo, n, ferm = make_model("kitaev", L, t=p[0], p=p[1], u=p[2])
-    R = Runner(mpo, n, ferm)
-    t0 = time.perf_counter()
-    e_cpu = R.dmrg2(m, sweeps)
-    cpu_s = time.perf_counter() - t0
+    from threadpoolctl import threadpool_limits
+
     threads = max([i.get("num_threads", 1) for i in threadpool_info()] or [1])
-    print(f"[bench] kitaev L={L} m={m} {sweeps} sweeps: GPU {gpu_s:.2f} s, CPU oracle {cpu_s:.1f} s", file=sys.stderr,
-          flush=True)
-    return {"workload": f"Kitaev L={L} m={m}, {sweeps} two-site sweeps (Z2, real FP64)", "steps": len(R.log),
-            "gpu_fp64_s": round(gpu_s, 3), "cpu_s": round(cpu_s, 3), "cpu_threads": int(threads), "kind": "port",
-            "gpu_E0": g.energy, "cpu_E0": e_cpu, "bdg_E0": e_bdg,
+    model, ncpu = cpu_info()
+
+    def host_run(limit):
+        mpo, n, ferm = make_model("kitaev", L, t=p[0], p=p[1], u=p[2])
+        R = Runner(mpo, n, ferm)
+        t0 = time.perf_counter()
+        if limit:
+            with threadpool_limits(limits=limit):
+                e = R.dmrg2(m, sweeps)
+        else:
+            e = R.dmrg2(m, sweeps)
+        return e, time.perf_counter() - t0, len(R.log)
+
+    e_cpu, cpu_s, nsteps = host_run(None)
+    e_one, one_s, _ = host_run(1)
+    print(f"[bench] kitaev L={L} m={m} {sweeps} sweeps: GPU {gpu_s:.2f} s, CPU oracle {cpu_s:.1f} s on {threads} "
+          f"threads, {one_s:.1f} s on 1 ({model})", file=sys.stderr, flush=True)
+    return {"workload": f"Kitaev L={L} m={m}, {sweeps} two-site sweeps (Z2, real FP64)", "steps": nsteps,
+            "gpu_fp64_s": round(gpu_s, 3), "cpu_s": round(cpu_s, 3), "cpu_threads": int(threads),
+            "cpu_one_thread_s": round(one_s, 3), "cpu_model": model, "host_cpus": ncpu, "kind": "port",
+            "gpu_E0": g.energy, "cpu_E0": e_cpu, "cpu_one_thread_E0": e_one, "bdg_E0": e_bdg,
             "gpu_err": abs(g.energy - e_bdg), "cpu_err": abs(e_cpu - e_bdg)}
 
 
@@ -513,6 +529,35 @@ def copy_leg(reps):
     return {"copy_bytes": 8 * n, "ms": round(ms, 4), "GB/s (read+write)": round(16 * n / ms / 1e6, 1)}
 
 
+MFMA_KINDS = {0: ("v_mfma_f32_32x32x16_bf16", "bf16x3"), 1: ("v_mfma_f32_16x16x32_bf16", "bf16x3"),
+              2: ("v_mfma_f64_16x16x4f64", "fp64"), 3: ("v_mfma_f32_16x16x4f32", "fp32")}
+
+
+def mfma_leg(rounds=20000):
+    """The matrix-core ceilings: each kind of MFMA instruction the kernels use,
+    issued back to back on four independent register accumulators per
+    wavefront (no memory traffic), at 1 and 2 wavefronts per SIMD of every CU
+    (dmrgx_calibrate_mfma), against the published dense peak of its type."""
+    import ctypes as C
+    import dmrg_amd
+
+    dmrg_amd.init(0)
+    lib = dmrg_amd.lib()
+    out = {}
+    for kind, (name, prec) in MFMA_KINDS.items():
+        row = {"peak": PEAK[prec]}
+        for w in (1, 2):
+            v = C.c_double(0)
+            n = rounds if kind != 2 else rounds // 4
+            dmrg_amd.check(lib.dmrgx_calibrate_mfma(kind, w, n, C.byref(v)), "calibrate_mfma")
+            row[f"tflops_{w}wave"] = round(v.value, 1)
+        row["frac_of_peak"] = round(max(row["tflops_1wave"], row["tflops_2wave"]) / PEAK[prec], 3)
+        out[name] = row
+        print(f"[bench] {name}: {row['tflops_1wave']:.1f} TFLOP/s (1 wave/SIMD), {row['tflops_2wave']:.1f} (2), "
+              f"published peak {PEAK[prec]}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -540,9 +585,10 @@ def main():
     ap.add_argument("--deriv-m", type=int, default=1024, help="--part deriv: bond dimension")
     ap.add_argument("--deriv-sweeps", type=int, default=3, help="--part deriv: full sweeps per run")
     ap.add_argument("--no-fd", action="store_true", help="--part deriv: skip the two finite-difference runs")
-    ap.add_argument("--part", default="", choices=["", "svd", "copy", "sweep", "cpusweep", "deriv"],
+    ap.add_argument("--part", default="", choices=["", "svd", "copy", "sweep", "cpusweep", "deriv", "mfma"],
                     help="run one part alone: the step SVD, the copy kernel used to calibrate PMC counters, the "
-                         "sweep comparison or the host Kitaev sweep (JSON of that part only)")
+                         "sweep comparison, the host Kitaev sweep, the derivative at scale or the matrix-core "
+                         "ceilings (JSON of that part only)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {a.gpus}); the multi-GPU layout is design-only.",
@@ -553,6 +599,9 @@ def main():
         return
     if a.part == "copy":
         print(json.dumps(copy_leg(a.steps)), flush=True)
+        return
+    if a.part == "mfma":
+        print(json.dumps(mfma_leg()), flush=True)
         return
     if a.part == "sweep":
         print(json.dumps(sweep_leg(a.sweep_model, a.sweep_L, a.sweep_m, a.sweep, a.ladder,

@@ -29,7 +29,7 @@ EXPORTS = [
     "dmrgx_stream", "dmrgx_task_gemm", "dmrgx_heff_create", "dmrgx_heff_apply", "dmrgx_heff_flops",
     "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_eigsh_batched", "dmrgx_set_lanczos_graphs", "dmrgx_set_autotune", "dmrgx_autotune_keys", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time", "dmrgx_heff_time_adjoint", "dmrgx_heff_set_split", "dmrgx_heff_get_split",
-    "dmrgx_task_gemm_noalloc", "dmrgx_set_skip_zero_slices", "dmrgx_heff_flops_exec",
+    "dmrgx_task_gemm_noalloc", "dmrgx_set_skip_zero_slices", "dmrgx_heff_flops_exec", "dmrgx_calibrate_mfma",
 ]
 
 _lib = None
@@ -103,6 +103,7 @@ def _declare(lib):
         "dmrgx_leg_contract": ([C.c_int, vp, vp, vp, i64p, C.c_int], C.c_int),
         "dmrgx_permute": ([C.c_int, vp, vp, i64p, C.c_int], C.c_int),
         "dmrgx_copy_boxes": ([C.c_int, vp, vp, i64p, C.c_int], C.c_int),
+        "dmrgx_calibrate_mfma": ([C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)

@@ -1,0 +1,54 @@
+"""Per-kernel statistics of the last dispatches in a rocprofv3 CSV file.
+
+A whole bench run dispatches the pass kernels thousands of times (the sweep
+comparison) before the timed matvecs of the real operator come last; this
+summarises only the last N dispatches of the kernels whose names match a
+pattern, grouped by kernel and grid size, so that the averages describe the
+timed launches (and the autotuner's candidates dispatched just before them).
+
+    python3 -m dmrg_amd.profile_tail <kernel_trace.csv | counter_collection.csv> N [name-regex]
+
+Kernel traces give durations (us); counter collections give each counter's
+mean value per dispatch.
+"""
+from __future__ import annotations
+
+import collections
+import csv
+import re
+import sys
+
+
+def summarise(path: str, n: int, pattern: str = ".") -> str:
+    rx = re.compile(pattern)
+    rows = [r for r in csv.DictReader(open(path)) if rx.search(r["Kernel_Name"])]
+    counters = "Counter_Name" in (rows[0] if rows else {})
+    key_id = "Dispatch_Id" if counters else "Start_Timestamp"
+    ids = sorted({int(r[key_id]) for r in rows})[-n:]
+    keep = set(ids)
+    rows = [r for r in rows if int(r[key_id]) in keep]
+    out = [f"{path}: last {len(ids)} dispatches matching {pattern!r}"]
+    if counters:
+        agg = collections.defaultdict(lambda: collections.defaultdict(list))
+        for r in rows:
+            k = (r["Kernel_Name"].split("(")[0][-40:], r["Grid_Size"])
+            agg[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+        for (name, grid), cs in sorted(agg.items()):
+            vals = ", ".join(f"{c} {sum(v) / len(v):.1f}" for c, v in sorted(cs.items()))
+            out.append(f"{name:40s} grid {grid:>9s} dispatches {len(next(iter(cs.values()))):4d}: {vals}")
+    else:
+        agg = collections.defaultdict(list)
+        for r in rows:
+            gx = r.get("Grid_Size_X") or r.get("Grid_Size", "")
+            agg[(r["Kernel_Name"].split("(")[0][-40:], gx)].append(
+                (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000)
+        for (name, grid), v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+            out.append(f"{name:40s} grid {grid:>9s} calls {len(v):4d} avg {sum(v) / len(v):8.1f} us "
+                       f"min {min(v):8.1f} total {sum(v):9.1f}")
+    return "\n".join(out)
+
+
+if __name__ == "__main__":
+    if len(sys.argv) < 3:
+        sys.exit(__doc__)
+    print(summarise(sys.argv[1], int(sys.argv[2]), sys.argv[3] if len(sys.argv) > 3 else "."))

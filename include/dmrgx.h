@@ -255,6 +255,15 @@ int dmrgx_permute(int dtype, const void* in, void* out, const int64_t* tasks, in
  * (times -1 when scale_sign < 0; conjugated when scale_sign == 2). */
 int dmrgx_copy_boxes(int dtype, const void* src, void* dst, const int64_t* tasks, int n_task);
 
+/* ---- measurement ---------------------------------------------------------
+ * No counterpart in the reference (which has no GPU path): calibration for
+ * the roofline figures.  dmrgx_calibrate_mfma times `rounds` x 4 independent
+ * instructions of one matrix-core kind per wavefront, register operands only,
+ * on waves_per_simd wavefronts per SIMD of every CU, and returns the rate in
+ * TFLOP/s: kind 0 v_mfma_f32_32x32x16_bf16, 1 v_mfma_f32_16x16x32_bf16,
+ * 2 v_mfma_f64_16x16x4f64, 3 v_mfma_f32_16x16x4f32. */
+int dmrgx_calibrate_mfma(int kind, int waves_per_simd, int rounds, double* tflops);
+
 #ifdef __cplusplus
 }
 #endif
