@@ -56,6 +56,9 @@ TRAFFIC_KIB = {
     ("synthetic", "R", 640): (61582.8, 81920.0), ("synthetic", "L", 64): (82023.6, 8192.0),
     ("synthetic", "L", 256): (82220.4, 32768.0), ("synthetic", "L", 512): (82258.0, 65536.0),
     ("synthetic", "L", 640): (82051.6, 81920.0),
+    # the centre operator of a converged XYZ m=512 chain (zero MPO slices
+    # skipped), measured on the L=20 chain's centre (the same block structure)
+    ("real_xyz_m512", "R", 192): (15532.0, 27819.2), ("real_xyz_m512", "L", 192): (24694.1, 24576.0),
 }
 FETCH_SCALE, WRITE_SCALE = 2.0, 1.0
 
@@ -94,14 +97,20 @@ def time_plan(h, prec, x, steps, warmup):
     if warmup:
         h.time(xd, yd, warmup)
     ms = h.time(xd, yd, steps)
+    # each pass's GEMM kernel alone, launched back to back: the per-launch
+    # duration of the roofline (the events around single launches inside an
+    # apply add the event packets' own latency)
+    kr, kl = h.time_kernels(xd, yd, steps)
     flops = h.flops
     _, _, r_items, l_items = h.split()
-    out = dict(ms=ms[0], ms_r=ms[1], ms_l=ms[2], flops=flops, flops_r=ms[3], flops_l=ms[4],
+    out = dict(ms=ms[0], ms_r=kr, ms_l=kl, ms_r_bracketed=ms[1], ms_l_bracketed=ms[2], flops=flops,
+               flops_r=ms[3], flops_l=ms[4],
                flops_exec=h.flops_exec, exec_r=ms[5], exec_l=ms[6],
                tflops=flops / (ms[0] * 1e-3) / 1e12, exec_tflops=h.flops_exec / (ms[0] * 1e-3) / 1e12,
                n_psi=h.n_psi, r_items=r_items, l_items=l_items)
     print(f"[bench] {prec}: {ms[0]:.3f} ms/matvec, {out['tflops']:.1f} TFLOP/s algorithmic, "
-          f"{out['exec_tflops']:.1f} executed (R {ms[1]:.3f} ms, L {ms[2]:.3f} ms)", file=sys.stderr, flush=True)
+          f"{out['exec_tflops']:.1f} executed (kernels back to back: R {kr:.4f} ms, L {kl:.4f} ms; inside the "
+          f"apply: R {ms[1]:.4f} ms, L {ms[2]:.4f} ms)", file=sys.stderr, flush=True)
     xd.free()
     yd.free()
     return out

@@ -30,6 +30,7 @@ EXPORTS = [
     "dmrgx_heff_destroy", "dmrgx_heff_adjoint", "dmrgx_heff_sizes", "dmrgx_eigsh", "dmrgx_eigsh_batched", "dmrgx_set_lanczos_graphs", "dmrgx_set_autotune", "dmrgx_autotune_keys", "dmrgx_block_svd", "dmrgx_block_svd_sweeps", "dmrgx_block_gemm", "dmrgx_leg_contract",
     "dmrgx_permute", "dmrgx_copy_boxes", "dmrgx_heff_time", "dmrgx_heff_time_adjoint", "dmrgx_heff_set_split", "dmrgx_heff_get_split",
     "dmrgx_task_gemm_noalloc", "dmrgx_set_skip_zero_slices", "dmrgx_heff_flops_exec", "dmrgx_calibrate_mfma",
+    "dmrgx_heff_time_kernels",
 ]
 
 _lib = None
@@ -84,6 +85,7 @@ def _declare(lib):
         "dmrgx_heff_flops_exec": ([vp], C.c_double),
         "dmrgx_set_skip_zero_slices": ([C.c_int], C.c_int),
         "dmrgx_heff_time": ([vp, vp, vp, C.c_int, dp], C.c_int),
+        "dmrgx_heff_time_kernels": ([vp, vp, vp, C.c_int, dp], C.c_int),
         "dmrgx_heff_time_adjoint": ([vp, vp, vp, vp, vp, vp, C.c_int, dp], C.c_int),
         "dmrgx_heff_destroy": ([vp], C.c_int),
         "dmrgx_heff_adjoint": ([vp, vp, vp, vp, vp, vp], C.c_int),
@@ -228,6 +230,14 @@ class Heff:
         ms = np.zeros(7)
         check(lib().dmrgx_heff_time(self.h, x.ptr, y.ptr, int(iters), ms.ctypes.data_as(C.POINTER(C.c_double))),
               "dmrgx_heff_time")
+        return ms
+
+    def time_kernels(self, x: DeviceArray, y: DeviceArray, iters: int):
+        """(ms per launch of the R-pass GEMM kernel, of the L-pass kernel),
+        each launched `iters` times back to back (dmrgx_heff_time_kernels)"""
+        ms = np.zeros(2)
+        check(lib().dmrgx_heff_time_kernels(self.h, x.ptr, y.ptr, int(iters),
+                                            ms.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_heff_time_kernels")
         return ms
 
     def eigsh(self, nev=6, ncv=None, tol=1e-10, max_restart=1000, seed=1234, v0=None, refine=None,

@@ -1069,6 +1069,29 @@ void reduce_partials(const GemmPlan& p, void* C, hipStream_t s) {
 
 void reduce_partials(const GemmPlan& p, void* C, hipStream_t s);
 
+void launch_planes_kernel(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B,
+                          int64_t b_stride, void* C, int out, int64_t c_ps, hipStream_t s) {
+  if (!p.n_items) return;
+  float* wsp = (float*)p.ws;
+  const int per = p.grid / 8;
+  auto it = p.items;
+  auto sg = p.segs;
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, dim3(p.grid), dim3(PNT), 0, s, it, p.n_items, per, sg, (const unsigned short*)A,
+                       a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp, p.row_maps);
+  };
+  if (p.dtype == DT_C64) {
+    if (out == OUT_F32) go(k_gemm_bf3p<true, OUT_F32>);
+    else if (out == OUT_PLANES) go(k_gemm_bf3p<true, OUT_PLANES>);
+    else go(k_gemm_bf3p<true, OUT_F64>);
+  } else {
+    if (out == OUT_F32) go(k_gemm_bf3p<false, OUT_F32>);
+    else if (out == OUT_PLANES) go(k_gemm_bf3p<false, OUT_PLANES>);
+    else go(k_gemm_bf3p<false, OUT_F64>);
+  }
+  DMRGX_HIP(hipGetLastError());
+}
+
 void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
                      void* C, int out, int64_t c_ps, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
   DMRGX_CHECK(p.planes, "run_gemm_planes needs a pre-split plan");
@@ -1081,26 +1104,7 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
     else fill_zero((char*)C + z.first * (out == OUT_F64 ? 8 : 4) * cw, (size_t)z.second * (out == OUT_F64 ? 8 : 4) * cw, s);
   }
   if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
-  float* wsp = (float*)p.ws;
-  if (p.n_items) {
-    const int per = p.grid / 8;
-    auto it = p.items;
-    auto sg = p.segs;
-    auto go = [&](auto kern) {
-      hipLaunchKernelGGL(kern, dim3(p.grid), dim3(PNT), 0, s, it, p.n_items, per, sg, (const unsigned short*)A,
-                         a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp, p.row_maps);
-    };
-    if (p.dtype == DT_C64) {
-      if (out == OUT_F32) go(k_gemm_bf3p<true, OUT_F32>);
-      else if (out == OUT_PLANES) go(k_gemm_bf3p<true, OUT_PLANES>);
-      else go(k_gemm_bf3p<true, OUT_F64>);
-    } else {
-      if (out == OUT_F32) go(k_gemm_bf3p<false, OUT_F32>);
-      else if (out == OUT_PLANES) go(k_gemm_bf3p<false, OUT_PLANES>);
-      else go(k_gemm_bf3p<false, OUT_F64>);
-    }
-    DMRGX_HIP(hipGetLastError());
-  }
+  launch_planes_kernel(p, A, a_stride, B, b_stride, C, out, c_ps, s);
   if (ev_end) DMRGX_HIP(hipEventRecord(ev_end, s));
   if (out == OUT_F32) {
     reduce_partials(p, C, s);
@@ -1181,12 +1185,7 @@ void split_padded_f32(const float* src, uint16_t* dst, int64_t ps, const PadLayo
   split_padded(src, dst, ps, L, cplx, s);
 }
 
-void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, hipEvent_t ev_begin,
-                   hipEvent_t ev_end) {
-  const size_t eb = (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);
-  for (auto& z : p.zero_ranges) fill_zero((char*)C + z.first * eb, (size_t)z.second * eb, s);
-  // events bracket the GEMM kernel only; the split-K reduction follows ev_end
-  if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
+void launch_plan_kernel(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s) {
   if (p.n_items) {
     const int per = p.grid / 8;
     if (p.prec == PREC_FP64) {
@@ -1201,6 +1200,15 @@ void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hip
     }
     DMRGX_HIP(hipGetLastError());
   }
+}
+
+void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s, hipEvent_t ev_begin,
+                   hipEvent_t ev_end) {
+  const size_t eb = (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);
+  for (auto& z : p.zero_ranges) fill_zero((char*)C + z.first * eb, (size_t)z.second * eb, s);
+  // events bracket the GEMM kernel only; the split-K reduction follows ev_end
+  if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
+  launch_plan_kernel(p, A, B, C, s);
   if (ev_end) DMRGX_HIP(hipEventRecord(ev_end, s));
   reduce_partials(p, C, s);
 }

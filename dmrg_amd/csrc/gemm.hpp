@@ -151,6 +151,12 @@ std::unique_ptr<PadLayout> make_pad_layout(std::vector<PadBlk> blocks);
 void split_padded_f64(const double* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s);
 void split_padded_f32(const float* src, uint16_t* dst, int64_t ps, const PadLayout& L, bool cplx, hipStream_t s);
 
+// The GEMM kernel of a plan alone (no zero fills, no split-K reduction): what
+// the per-launch timing of a pass repeats back to back.
+void launch_plan_kernel(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s);
+void launch_planes_kernel(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B,
+                          int64_t b_stride, void* C, int out, int64_t c_ps, hipStream_t s);
+
 // Precision conversion helpers (n counts real scalars).
 void convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);
 void convert_f32_to_f64(const float* src, double* dst, int64_t n, hipStream_t s);

@@ -758,6 +758,43 @@ void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* e
   }
 }
 
+void HeffPlan::time_kernels(const double* in, double* out, int iters, hipStream_t s, double* ms_r, double* ms_l) {
+  DMRGX_CHECK(iters > 0, "time_kernels: iters must be positive");
+  apply(in, out, s);
+  // each pass's kernel with the operands apply() uses
+  auto r_pass = [&]() {
+    if (prec == PREC_FP64) launch_plan_kernel(*rplan, renv, in, store->ptr, s);
+    else if (prec == PREC_BF16X3)
+      launch_planes_kernel(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
+                           store_direct ? store_pl->ptr : store->ptr, store_direct ? OUT_PLANES : OUT_F32,
+                           store_direct ? store_ps : 0, s);
+    else launch_plan_kernel(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s);
+  };
+  auto l_pass = [&]() {
+    if (prec == PREC_FP64) launch_plan_kernel(*lplan, lenv, store->ptr, out, s);
+    else if (prec == PREC_BF16X3)
+      launch_planes_kernel(*lplan, (const uint16_t*)lenv_pl->ptr, lenv_ps, (const uint16_t*)store_pl->ptr,
+                           store_ps, out, OUT_F64, 0, s);
+    else launch_plan_kernel(*lplan, lenv_lo->ptr, store->ptr, out_lo->ptr, s);
+  };
+  hipEvent_t ev[3];
+  for (auto& e : ev) DMRGX_HIP(hipEventCreate(&e));
+  DMRGX_HIP(hipEventRecord(ev[0], s));
+  for (int i = 0; i < iters; ++i) r_pass();
+  DMRGX_HIP(hipEventRecord(ev[1], s));
+  for (int i = 0; i < iters; ++i) l_pass();
+  DMRGX_HIP(hipEventRecord(ev[2], s));
+  DMRGX_HIP(hipEventSynchronize(ev[2]));
+  float a = 0, b = 0;
+  DMRGX_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+  DMRGX_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *ms_r = a / iters;
+  *ms_l = b / iters;
+  // the L-pass kernel alone leaves split-K partials unreduced: restore out
+  apply(in, out, s);
+}
+
 }  // namespace dmrgx
 
 using namespace dmrgx;
@@ -915,6 +952,15 @@ int dmrgx_heff_time(dmrgx_heff* plan, const void* in, void* out, int iters, doub
   ms[4] = al;
   ms[5] = h.rplan->flops;
   ms[6] = h.lplan->flops;
+  CAPI_END
+}
+
+int dmrgx_heff_time_kernels(dmrgx_heff* plan, const void* in, void* out, int iters, double* ms) {
+  CAPI_BEGIN
+  DMRGX_CHECK(plan && plan->p && iters > 0 && ms, "heff_time_kernels: bad arguments");
+  hipStream_t s = rt().stream;
+  plan->p->time_kernels((const double*)in, (double*)out, iters, s, &ms[0], &ms[1]);
+  DMRGX_HIP(hipStreamSynchronize(s));
   CAPI_END
 }
 

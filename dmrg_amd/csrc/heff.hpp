@@ -59,6 +59,12 @@ struct HeffPlan {
   std::unique_ptr<DevBuf> edge_out;
   // ev: optional {R begin, R end, L begin, L end} events around the two GEMM launches
   void apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev = nullptr);
+  // Per-launch time (ms) of each pass's GEMM kernel alone: one apply of `in`
+  // (so that every intermediate holds its data), then `iters` back-to-back
+  // launches of the R-pass kernel and of the L-pass kernel between events
+  // (the launches rewrite the same outputs; out gets the apply's result again
+  // by a final apply)
+  void time_kernels(const double* in, double* out, int iters, hipStream_t s, double* ms_r, double* ms_l);
   // Adjoint of out = H_eff(psi; renv, lenv): for a cotangent g of out, the
   // cotangents of psi, renv and lenv (each may be null): for every task
   // C += alpha A B^T, gA += alpha g conj(B) and gB += alpha g^T conj(A), so

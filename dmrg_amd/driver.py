@@ -401,6 +401,15 @@ class SynthHeff:
               "dmrgx_heff_time")
         return ms
 
+    def time_kernels(self, x, y, iters):
+        """(ms per launch of the R-pass GEMM kernel, of the L-pass kernel),
+        each launched `iters` times back to back (dmrgx_heff_time_kernels)"""
+        from . import check, lib
+        ms = np.zeros(2)
+        check(lib().dmrgx_heff_time_kernels(self.h, x.ptr, y.ptr, int(iters),
+                                            ms.ctypes.data_as(C.POINTER(C.c_double))), "dmrgx_heff_time_kernels")
+        return ms
+
     def apply(self, x, y):
         from . import check, lib
         check(lib().dmrgx_heff_apply(self.h, x.ptr, y.ptr), "dmrgx_heff_apply")

@@ -19,6 +19,15 @@ import re
 import sys
 
 
+def kernel_name(full: str) -> str:
+    """The kernel's name with its template arguments, without the namespace
+    and the parameter list ("k_gemm_bf3p<true, 1>")."""
+    m = re.search(r"\b(k_\w+(?:<[^<>]*>)?)\(", full)
+    if m:
+        return m.group(1)
+    return full.split("(")[0][-40:] if not full.startswith("void ") else full[5:45]
+
+
 def summarise(path: str, n: int, pattern: str = ".") -> str:
     rx = re.compile(pattern)
     rows = [r for r in csv.DictReader(open(path)) if rx.search(r["Kernel_Name"])]
@@ -31,7 +40,7 @@ def summarise(path: str, n: int, pattern: str = ".") -> str:
     if counters:
         agg = collections.defaultdict(lambda: collections.defaultdict(list))
         for r in rows:
-            k = (r["Kernel_Name"].split("(")[0][-40:], r["Grid_Size"])
+            k = (kernel_name(r["Kernel_Name"]), r["Grid_Size"])
             agg[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
         for (name, grid), cs in sorted(agg.items()):
             vals = ", ".join(f"{c} {sum(v) / len(v):.1f}" for c, v in sorted(cs.items()))
@@ -40,7 +49,7 @@ def summarise(path: str, n: int, pattern: str = ".") -> str:
         agg = collections.defaultdict(list)
         for r in rows:
             gx = r.get("Grid_Size_X") or r.get("Grid_Size", "")
-            agg[(r["Kernel_Name"].split("(")[0][-40:], gx)].append(
+            agg[(kernel_name(r["Kernel_Name"]), gx)].append(
                 (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1000)
         for (name, grid), v in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
             out.append(f"{name:40s} grid {grid:>9s} calls {len(v):4d} avg {sum(v) / len(v):8.1f} us "

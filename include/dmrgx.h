@@ -122,6 +122,12 @@ int dmrgx_heff_destroy(dmrgx_heff* plan);
  * GEMM kernel, ms[3] / ms[4] = algorithmic flops of the R / L pass, ms[5] /
  * ms[6] = flops the R / L pass executes (7 doubles). */
 int dmrgx_heff_time(dmrgx_heff* plan, const void* in, void* out, int iters, double* ms);
+/* One apply of `in`, then `iters` back-to-back launches of the R-pass GEMM
+ * kernel alone and of the L-pass GEMM kernel alone between HIP events (no
+ * zero fills or split-K reductions): ms[0] / ms[1] = ms per launch of the R /
+ * L kernel (2 doubles); `out` holds H_eff in on return.  A measurement of
+ * this build (the reference has no GPU kernels). */
+int dmrgx_heff_time_kernels(dmrgx_heff* plan, const void* in, void* out, int iters, double* ms);
 /* dmrgx_heff_adjoint run `iters` times between two HIP events on the library's
  * stream: ms[0] = event time per call (no counterpart in the reference, which
  * has no derivatives) */

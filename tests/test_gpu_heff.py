@@ -146,3 +146,24 @@ def test_heff_skips_zero_slices(gpu, model, kw, L, m, ratio, prec):
         err = np.max(np.abs(yd.numpy() - ref)) / np.max(np.abs(ref))
         assert err < TOL[prec], (rt, lt, err)
     assert len(items) > 1
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3"])
+def test_time_kernels_leaves_the_apply_result(gpu, prec):
+    """dmrgx_heff_time_kernels launches each pass kernel alone, back to back
+    (split-K partials left unreduced in between), and returns with `out`
+    holding H_eff in again: the oracle's av() to the precision's tolerance,
+    and positive per-launch times."""
+    case = synthetic_case(3, 4, 200, dtype=np.complex128, uneven=[81, 67, 52])
+    err, h = _run(gpu, case, prec)
+    assert err < TOL[prec]
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(case.n_psi) + 1j * rng.standard_normal(case.n_psi)
+    xd = gpu.DeviceArray.from_numpy(x)
+    yd = gpu.DeviceArray(case.n_psi, np.complex128)
+    ms = h.time_kernels(xd, yd, 5)
+    assert ms[0] > 0 and ms[1] > 0
+    ref = case.oracle_apply(x)
+    err = np.max(np.abs(yd.numpy() - ref)) / np.max(np.abs(ref))
+    assert err < TOL[prec], err
+    h.close()
