@@ -198,7 +198,11 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
   const double tol = cfg.prec == PREC_FP64 ? cfg.tol : cfg.tol_lo;
   uint64_t seed = cfg.seed * 0x9E3779B97F4A7C15ull + (uint64_t)log.size() + 1;
   auto tp1 = clk::now();
-  EigshResult r = eigsh(op, d, cx, nev, ncv, tol, cfg.max_restart, seed, v0, (double*)evec.ptr, pv, lg.warm);
+  // a reduced-precision solve keeps its Krylov basis in FP32 (its 1e-4
+  // tolerance is far above the basis' orthogonality; the FP64 refinement
+  // follows)
+  EigshResult r = eigsh(op, d, cx, nev, ncv, tol, cfg.max_restart, seed, v0, (double*)evec.ptr, pv, lg.warm,
+                        plan64 != nullptr);
   auto tp2 = clk::now();
   lg.t_plan = secs(tp0, tp1);
   lg.t_lanczos = secs(tp1, tp2);

@@ -1033,7 +1033,9 @@ int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
     vecs = std::make_unique<DevBuf>((size_t)pv * h->n_psi * (cx ? 16 : 8));
     ev = (double*)vecs->ptr;
   }
-  EigshResult r = eigsh(op, h->n_psi, cx, nev, ncv, tol, max_restart, seed, (const double*)v0, ev, pv);
+  // a refined low-precision solve keeps its Krylov basis in FP32
+  EigshResult r = eigsh(op, h->n_psi, cx, nev, ncv, tol, max_restart, seed, (const double*)v0, ev, pv, false,
+                        use_ref && h->prec != PREC_FP64);
   for (int i = 0; i < (int)r.evals.size(); ++i) evals[i] = r.evals[i];
   Refined rf;
   if (use_ref) {

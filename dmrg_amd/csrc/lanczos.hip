@@ -9,8 +9,10 @@
 #include <algorithm>
 #include <cmath>
 #include <limits>
+#include <type_traits>
 
 #include "common.hpp"
+#include "gemm.hpp"
 #include "lanczos.hpp"
 
 namespace dmrgx {
@@ -22,9 +24,11 @@ constexpr int PARTIAL_DOUBLES = (DOT_BLOCKS_MAX + 64) * 2 * G;
 
 // Partial <X_c, w> (conjugate-linear in X) for columns c of group blockIdx.y:
 // X_c = V_c for c < k, and X_k = w itself when with_self (the squared norm).
-template <bool CPLX>
-__global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int64_t ld, int k, int with_self,
-                                               const double* __restrict__ w, int64_t n,
+// V holds TV elements (double, or float for the basis of a low-precision
+// solve), w TW elements; the sums are FP64.
+template <bool CPLX, typename TV = double, typename TW = double>
+__global__ __launch_bounds__(256) void k_vdots(const TV* __restrict__ V, int64_t ld, int k, int with_self,
+                                               const TW* __restrict__ w, int64_t n,
                                                double* __restrict__ partial) {
   const int j0 = blockIdx.y * G;
   const int ktot = k + with_self;
@@ -36,20 +40,27 @@ __global__ __launch_bounds__(256) void k_vdots(const double* __restrict__ V, int
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     if (CPLX) {
-      double wr = w[2 * i], wi = w[2 * i + 1];
+      const double wr = w[2 * i], wi = w[2 * i + 1];
 #pragma unroll
       for (int c = 0; c < G; ++c)
         if (c < g) {
-          const double* v = (j0 + c < k) ? V + (int64_t)(j0 + c) * ld : w;
-          double vr = v[2 * i], vi = v[2 * i + 1];
+          double vr, vi;
+          if (j0 + c < k) {
+            const TV* v = V + (int64_t)(j0 + c) * ld;
+            vr = v[2 * i];
+            vi = v[2 * i + 1];
+          } else {
+            vr = wr;
+            vi = wi;
+          }
           re[c] += vr * wr + vi * wi;
           im[c] += vr * wi - vi * wr;
         }
     } else {
-      double wr = w[i];
+      const double wr = w[i];
 #pragma unroll
       for (int c = 0; c < G; ++c)
-        if (c < g) re[c] += ((j0 + c < k) ? V[(int64_t)(j0 + c) * ld + i] : w[i]) * wr;
+        if (c < g) re[c] += ((j0 + c < k) ? (double)V[(int64_t)(j0 + c) * ld + i] : wr) * wr;
     }
   }
   __shared__ double red[2 * G][256];
@@ -81,9 +92,9 @@ __global__ void k_reduce_partials(const double* __restrict__ partial, int nblk, 
 }
 
 // w -= sum_j V_j h_j
-template <bool CPLX>
-__global__ __launch_bounds__(256) void k_gemv_sub(const double* __restrict__ V, int64_t ld, int k,
-                                                  const double* __restrict__ h, double* __restrict__ w,
+template <bool CPLX, typename TV = double, typename TW = double>
+__global__ __launch_bounds__(256) void k_gemv_sub(const TV* __restrict__ V, int64_t ld, int k,
+                                                  const double* __restrict__ h, TW* __restrict__ w,
                                                   int64_t n) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -91,51 +102,52 @@ __global__ __launch_bounds__(256) void k_gemv_sub(const double* __restrict__ V, 
     if (CPLX) {
       double ar = 0, ai = 0;
       for (int j = 0; j < k; ++j) {
-        const double* v = V + (int64_t)j * ld;
-        double vr = v[2 * i], vi = v[2 * i + 1], hr = h[2 * j], hi = h[2 * j + 1];
+        const TV* v = V + (int64_t)j * ld;
+        const double vr = v[2 * i], vi = v[2 * i + 1], hr = h[2 * j], hi = h[2 * j + 1];
         ar += vr * hr - vi * hi;
         ai += vr * hi + vi * hr;
       }
-      w[2 * i] -= ar;
-      w[2 * i + 1] -= ai;
+      w[2 * i] = (TW)((double)w[2 * i] - ar);
+      w[2 * i + 1] = (TW)((double)w[2 * i + 1] - ai);
     } else {
       double a = 0;
-      for (int j = 0; j < k; ++j) a += V[(int64_t)j * ld + i] * h[2 * j];
-      w[i] -= a;
+      for (int j = 0; j < k; ++j) a += (double)V[(int64_t)j * ld + i] * h[2 * j];
+      w[i] = (TW)((double)w[i] - a);
     }
   }
 }
 
 // out = (w - sum_j V_j h_j) * a
-template <bool CPLX>
-__global__ __launch_bounds__(256) void k_gemv_sub_scale(const double* __restrict__ V, int64_t ld, int k,
+template <bool CPLX, typename TV = double>
+__global__ __launch_bounds__(256) void k_gemv_sub_scale(const TV* __restrict__ V, int64_t ld, int k,
                                                         const double* __restrict__ h, const double* __restrict__ w,
-                                                        double* __restrict__ out, int64_t n, double a) {
+                                                        TV* __restrict__ out, int64_t n, double a) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) {
     if (CPLX) {
       double ar = 0, ai = 0;
       for (int j = 0; j < k; ++j) {
-        const double* v = V + (int64_t)j * ld;
-        double vr = v[2 * i], vi = v[2 * i + 1], hr = h[2 * j], hi = h[2 * j + 1];
+        const TV* v = V + (int64_t)j * ld;
+        const double vr = v[2 * i], vi = v[2 * i + 1], hr = h[2 * j], hi = h[2 * j + 1];
         ar += vr * hr - vi * hi;
         ai += vr * hi + vi * hr;
       }
-      out[2 * i] = (w[2 * i] - ar) * a;
-      out[2 * i + 1] = (w[2 * i + 1] - ai) * a;
+      out[2 * i] = (TV)((w[2 * i] - ar) * a);
+      out[2 * i + 1] = (TV)((w[2 * i + 1] - ai) * a);
     } else {
       double s = 0;
-      for (int j = 0; j < k; ++j) s += V[(int64_t)j * ld + i] * h[2 * j];
-      out[i] = (w[i] - s) * a;
+      for (int j = 0; j < k; ++j) s += (double)V[(int64_t)j * ld + i] * h[2 * j];
+      out[i] = (TV)((w[i] - s) * a);
     }
   }
 }
 
 // out_c = sum_j V_j Y[j][c] for c in [c0, c0+g), real coefficients Y (m x ky row-major)
-__global__ __launch_bounds__(256) void k_combine(const double* __restrict__ V, int64_t ld, int m,
+template <typename TV = double, typename TO = double>
+__global__ __launch_bounds__(256) void k_combine(const TV* __restrict__ V, int64_t ld, int m,
                                                  const double* __restrict__ Y, int ky, int c0, int g,
-                                                 double* __restrict__ out, int64_t nr) {
+                                                 TO* __restrict__ out, int64_t ldo, int64_t nr) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < nr; i += stride) {
@@ -143,27 +155,29 @@ __global__ __launch_bounds__(256) void k_combine(const double* __restrict__ V, i
 #pragma unroll
     for (int c = 0; c < G; ++c) acc[c] = 0;
     for (int j = 0; j < m; ++j) {
-      double v = V[(int64_t)j * ld + i];
+      const double v = V[(int64_t)j * ld + i];
 #pragma unroll
       for (int c = 0; c < G; ++c)
         if (c < g) acc[c] += v * Y[j * ky + c0 + c];
     }
 #pragma unroll
     for (int c = 0; c < G; ++c)
-      if (c < g) out[(int64_t)(c0 + c) * ld + i] = acc[c];
+      if (c < g) out[(int64_t)(c0 + c) * ldo + i] = (TO)acc[c];
   }
 }
 
-__global__ void k_scale(double* __restrict__ v, int64_t nr, double a) {
+template <typename T>
+__global__ void k_scale(T* __restrict__ v, int64_t nr, double a) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < nr; i += stride) v[i] *= a;
+  for (; i < nr; i += stride) v[i] = (T)((double)v[i] * a);
 }
 
-__global__ void k_scale_copy(const double* __restrict__ s, double* __restrict__ d, int64_t nr, double a) {
+template <typename TS, typename TD>
+__global__ void k_scale_copy(const TS* __restrict__ s, TD* __restrict__ d, int64_t nr, double a) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (; i < nr; i += stride) d[i] = s[i] * a;
+  for (; i < nr; i += stride) d[i] = (TD)((double)s[i] * a);
 }
 
 __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
@@ -173,12 +187,13 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
-__global__ void k_random(double* __restrict__ v, int64_t nr, uint64_t seed) {
+template <typename T>
+__global__ void k_random(T* __restrict__ v, int64_t nr, uint64_t seed) {
   int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (; i < nr; i += stride) {
     uint64_t x = splitmix64(seed ^ splitmix64((uint64_t)i));
-    v[i] = (double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    v[i] = (T)((double)(x >> 11) * (2.0 / 9007199254740992.0) - 1.0);
   }
 }
 
@@ -194,37 +209,57 @@ struct Work {
 };
 
 // h[0..k) = V[0..k)^H w, plus h[k] = <w, w> when with_self (device, complex pairs)
-void vdots(const double* V, int64_t ld, int k, const double* w, int64_t n, bool cplx, double* h,
-           double* partial, hipStream_t s, bool with_self = false) {
+template <typename TV, typename TW>
+void vdots(const TV* V, int64_t ld, int k, const TW* w, int64_t n, bool cplx, double* h, double* partial,
+           hipStream_t s, bool with_self = false) {
   const int ktot = k + (with_self ? 1 : 0);
   if (ktot <= 0) return;
   const int ng = (ktot + G - 1) / G;
   int nblk = (int)std::min<int64_t>(DOT_BLOCKS_MAX / ng + 1, std::max<int64_t>(1, (n + 1023) / 1024));
   nblk = std::max(1, std::min(nblk, DOT_BLOCKS_MAX));
   dim3 grid(nblk, ng);
-  if (cplx) hipLaunchKernelGGL(k_vdots<true>, grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
-  else hipLaunchKernelGGL(k_vdots<false>, grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
+  if (cplx)
+    hipLaunchKernelGGL((k_vdots<true, TV, TW>), grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
+  else
+    hipLaunchKernelGGL((k_vdots<false, TV, TW>), grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(ng), dim3(64 * 2 * G), 0, s, partial, nblk, ktot, h);
 }
 
-void gemv_sub(const double* V, int64_t ld, int k, const double* h, double* w, int64_t n, bool cplx,
-              hipStream_t s) {
+template <typename TV, typename TW>
+void gemv_sub(const TV* V, int64_t ld, int k, const double* h, TW* w, int64_t n, bool cplx, hipStream_t s) {
   if (k <= 0) return;
-  if (cplx) hipLaunchKernelGGL(k_gemv_sub<true>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, k, h, w, n);
-  else hipLaunchKernelGGL(k_gemv_sub<false>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, k, h, w, n);
+  if (cplx) hipLaunchKernelGGL((k_gemv_sub<true, TV, TW>), dim3(grid_for(n)), dim3(256), 0, s, V, ld, k, h, w, n);
+  else hipLaunchKernelGGL((k_gemv_sub<false, TV, TW>), dim3(grid_for(n)), dim3(256), 0, s, V, ld, k, h, w, n);
+}
+
+template <typename T>
+void dev_random_t(T* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s) {
+  int64_t nr = n * (cplx ? 2 : 1);
+  hipLaunchKernelGGL(k_random<T>, dim3(grid_for(nr)), dim3(256), 0, s, v, nr, seed);
+}
+
+template <typename T>
+void dev_scale_t(T* v, int64_t n, bool cplx, double a, hipStream_t s) {
+  int64_t nr = n * (cplx ? 2 : 1);
+  hipLaunchKernelGGL(k_scale<T>, dim3(grid_for(nr)), dim3(256), 0, s, v, nr, a);
+}
+
+template <typename T>
+double dev_norm2_t(const T* v, int64_t n, bool cplx, hipStream_t s) {
+  DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
+  DevBuf h(2 * G * sizeof(double));
+  vdots(v, 0, 1, v, n, cplx, (double*)h.ptr, (double*)partial.ptr, s);
+  double o[2];
+  DMRGX_HIP(hipMemcpyAsync(o, h.ptr, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+  DMRGX_HIP(hipStreamSynchronize(s));
+  return std::sqrt(std::max(o[0], 0.0));
 }
 
 }  // namespace
 
-void dev_random(double* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s) {
-  int64_t nr = n * (cplx ? 2 : 1);
-  hipLaunchKernelGGL(k_random, dim3(grid_for(nr)), dim3(256), 0, s, v, nr, seed);
-}
+void dev_random(double* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s) { dev_random_t(v, n, cplx, seed, s); }
 
-void dev_scale(double* v, int64_t n, bool cplx, double a, hipStream_t s) {
-  int64_t nr = n * (cplx ? 2 : 1);
-  hipLaunchKernelGGL(k_scale, dim3(grid_for(nr)), dim3(256), 0, s, v, nr, a);
-}
+void dev_scale(double* v, int64_t n, bool cplx, double a, hipStream_t s) { dev_scale_t(v, n, cplx, a, s); }
 
 void dev_dot(const double* a, const double* b, int64_t n, bool cplx, double out[2], hipStream_t s) {
   DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
@@ -250,67 +285,44 @@ void dev_orthogonalise(const double* V, int64_t ld, int k, double* v, int64_t n,
   }
 }
 
-EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
-                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec, bool early) {
-  ensure_init();
+namespace {
+
+// The Krylov iteration of eigsh with basis vectors stored as TV (double, or
+// float for a low-precision solve); the operator's input and output, the
+// Gram-Schmidt sums and the projected problem stay FP64.
+template <typename TV>
+EigshResult eigsh_krylov(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol, int max_restart,
+                         uint64_t seed, const double* v0, double* evec, int n_vec, bool early) {
   hipStream_t s = rt().stream;
   EigshResult res;
-  DMRGX_CHECK(n > 0, "eigsh: empty operator");
+  constexpr bool F64 = std::is_same<TV, double>::value;
   const int64_t nr = n * (cplx ? 2 : 1);
   const int64_t ld = nr;
-  nev = (int)std::min<int64_t>(nev, n);
-  ncv = (int)std::min<int64_t>(std::max(ncv, nev + 2), n);
   const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
 
-  if (n <= 64 || ncv <= nev + 1) {
-    // materialise the operator column by column and diagonalise on the host
-    DevBuf e(nr * 8), y(nr * 8);
-    std::vector<double> ycol(nr);
-    std::vector<dmrgx::cplx> H((size_t)n * n);
-    for (int64_t j = 0; j < n; ++j) {
-      fill_zero(e.ptr, nr * 8, s);
-      double one = 1.0;
-      DMRGX_HIP(hipMemcpyAsync((double*)e.ptr + (cplx ? 2 * j : j), &one, 8, hipMemcpyHostToDevice, s));
-      op((const double*)e.ptr, (double*)y.ptr);
-      res.n_matvec++;
-      DMRGX_HIP(hipMemcpyAsync(ycol.data(), y.ptr, nr * 8, hipMemcpyDeviceToHost, s));
-      DMRGX_HIP(hipStreamSynchronize(s));
-      for (int64_t i = 0; i < n; ++i)
-        H[(size_t)i * n + j] = cplx ? dmrgx::cplx(ycol[2 * i], ycol[2 * i + 1]) : dmrgx::cplx(ycol[i], 0);
-    }
-    std::vector<double> w;
-    std::vector<dmrgx::cplx> V;
-    host_hermitian_eig((int)n, H, !cplx, w, V);
-    res.evals.assign(w.begin(), w.begin() + nev);
-    const int nv = (int)std::min<int64_t>(std::max(n_vec, 1), n);
-    std::vector<double> x((size_t)std::max(n_vec, 1) * nr, 0.0);
-    for (int c = 0; c < nv; ++c)
-      for (int64_t i = 0; i < n; ++i) {
-        double* xc = x.data() + (size_t)c * nr;
-        if (cplx) { xc[2 * i] = V[(size_t)i * n + c].real(); xc[2 * i + 1] = V[(size_t)i * n + c].imag(); }
-        else xc[i] = V[(size_t)i * n + c].real();
-      }
-    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), x.size() * 8, hipMemcpyHostToDevice, s));
-    DMRGX_HIP(hipStreamSynchronize(s));
-    res.converged = true;
-    return res;
-  }
-
-  DevBuf Vb((size_t)(ncv + 1) * nr * 8), Tmp((size_t)ncv * nr * 8);
+  DevBuf Vb((size_t)(ncv + 1) * nr * sizeof(TV)), Tmp((size_t)ncv * nr * sizeof(TV));
+  // FP64 operator input (a float basis column widened) and output
+  DevBuf win(F64 ? 0 : (size_t)nr * 8), wout(F64 ? 0 : (size_t)nr * 8);
   DevBuf partial(PARTIAL_DOUBLES * sizeof(double));
   DevBuf hbuf(2 * (ncv + 2) * sizeof(double)), h2buf(2 * (ncv + 2) * sizeof(double));
   DevBuf Yb((size_t)ncv * ncv * sizeof(double));
-  double* V = (double*)Vb.ptr;
-  double* w = (double*)Tmp.ptr;  // scratch vector (first column of Tmp) between restarts
+  TV* V = (TV*)Vb.ptr;
+  // scratch vector w (the operator's output): the first column of Tmp between
+  // restarts for an FP64 basis, its own buffer otherwise
+  double* w = F64 ? (double*)Tmp.ptr : (double*)wout.ptr;
   double* h = (double*)hbuf.ptr;
   double* h2 = (double*)h2buf.ptr;
 
   // start vector
-  if (v0) DMRGX_HIP(hipMemcpyAsync(V, v0, nr * 8, hipMemcpyDeviceToDevice, s));
-  else dev_random(V, n, cplx, seed, s);
-  double nv = dev_norm2(V, n, cplx, s);
+  if (v0) {
+    if (F64) DMRGX_HIP(hipMemcpyAsync(V, v0, nr * 8, hipMemcpyDeviceToDevice, s));
+    else convert_f64_to_f32(v0, (float*)V, nr, s);
+  } else {
+    dev_random_t(V, n, cplx, seed, s);
+  }
+  double nv = dev_norm2_t(V, n, cplx, s);
   DMRGX_CHECK(nv > 0, "eigsh: zero start vector");
-  dev_scale(V, n, cplx, 1.0 / nv, s);
+  dev_scale_t(V, n, cplx, 1.0 / nv, s);
 
   const int m = ncv;
   std::vector<double> T((size_t)m * m, 0.0);
@@ -325,8 +337,13 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
   for (int restart = 0; restart <= max_restart; ++restart) {
     res.n_restart = restart;
     for (int j = j0; j < m; ++j) {
-      double* vj = V + (int64_t)j * ld;
-      op(vj, w);
+      TV* vj = V + (int64_t)j * ld;
+      if (F64) {
+        op((const double*)vj, w);
+      } else {
+        convert_f32_to_f64((const float*)vj, (double*)win.ptr, nr, s);
+        op((const double*)win.ptr, w);
+      }
       res.n_matvec++;
       // classical Gram-Schmidt against v_0..v_j, always two passes ("twice
       // is enough", Giraud, Langou and Rozloznik 2005).  One pass amplifies
@@ -357,21 +374,21 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
         T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = beta;
       }
       beta_last = beta;
-      double* vn = V + (int64_t)(j + 1) * ld;
+      TV* vn = V + (int64_t)(j + 1) * ld;
       if (beta > 1e-13 * std::max(std::fabs(alpha), std::sqrt(w2)) && beta > 1e-300) {
-        if (cplx) hipLaunchKernelGGL(k_gemv_sub_scale<true>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,
+        if (cplx) hipLaunchKernelGGL((k_gemv_sub_scale<true, TV>), dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,
                                      hfin, (const double*)w, vn, n, 1.0 / beta);
-        else hipLaunchKernelGGL(k_gemv_sub_scale<false>, dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,
+        else hipLaunchKernelGGL((k_gemv_sub_scale<false, TV>), dim3(grid_for(n)), dim3(256), 0, s, V, ld, j + 1,
                                 hfin, (const double*)w, vn, n, 1.0 / beta);
       } else {
         // invariant subspace: continue with a fresh random direction
-        dev_random(vn, n, cplx, seed + 7919ull * (j + 1) + restart, s);
+        dev_random_t(vn, n, cplx, seed + 7919ull * (j + 1) + restart, s);
         for (int pass = 0; pass < 2; ++pass) {
-          vdots(V, ld, j + 1, vn, n, cplx, h, (double*)partial.ptr, s);
+          vdots(V, ld, j + 1, (const TV*)vn, n, cplx, h, (double*)partial.ptr, s);
           gemv_sub(V, ld, j + 1, h, vn, n, cplx, s);
         }
-        double z = dev_norm2(vn, n, cplx, s);
-        dev_scale(vn, n, cplx, 1.0 / z, s);
+        double z = dev_norm2_t(vn, n, cplx, s);
+        dev_scale_t(vn, n, cplx, 1.0 / z, s);
         if (j + 1 < m) T[(size_t)j * m + j + 1] = T[(size_t)(j + 1) * m + j] = 0;
         beta_last = 0;
       }
@@ -421,10 +438,11 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
       for (int c = 0; c < keep; ++c) Yk[(size_t)r * keep + c] = Y[(size_t)r * m + c];
     DMRGX_HIP(hipMemcpyAsync(Yb.ptr, Yk.data(), Yk.size() * 8, hipMemcpyHostToDevice, s));
     for (int c0 = 0; c0 < keep; c0 += G)
-      hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, m, (const double*)Yb.ptr,
-                         keep, c0, std::min(G, keep - c0), (double*)Tmp.ptr, nr);
-    DMRGX_HIP(hipMemcpyAsync(V, Tmp.ptr, (size_t)keep * nr * 8, hipMemcpyDeviceToDevice, s));
-    DMRGX_HIP(hipMemcpyAsync(V + (int64_t)keep * ld, V + (int64_t)m * ld, nr * 8, hipMemcpyDeviceToDevice, s));
+      hipLaunchKernelGGL((k_combine<TV, TV>), dim3(grid_for(nr)), dim3(256), 0, s, V, ld, m, (const double*)Yb.ptr,
+                         keep, c0, std::min(G, keep - c0), (TV*)Tmp.ptr, ld, nr);
+    DMRGX_HIP(hipMemcpyAsync(V, Tmp.ptr, (size_t)keep * nr * sizeof(TV), hipMemcpyDeviceToDevice, s));
+    DMRGX_HIP(hipMemcpyAsync(V + (int64_t)keep * ld, V + (int64_t)m * ld, nr * sizeof(TV), hipMemcpyDeviceToDevice,
+                             s));
     std::fill(T.begin(), T.end(), 0.0);
     for (int i = 0; i < keep; ++i) {
       T[(size_t)i * m + i] = theta[i];
@@ -436,21 +454,74 @@ EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, doub
     // alpha comes from the Gram-Schmidt coefficients like any other step
   }
   res.evals.assign(theta.begin(), theta.begin() + nev);
-  // the n_vec lowest Ritz vectors, the lowest normalised
+  // the n_vec lowest Ritz vectors (FP64), the lowest normalised
   const int nvec = std::max(1, std::min(n_vec, mm));
   std::vector<double> y0((size_t)mm * nvec);
   for (int r = 0; r < mm; ++r)
     for (int c = 0; c < nvec; ++c) y0[(size_t)r * nvec + c] = Y[(size_t)r * mm + c];
   DMRGX_HIP(hipMemcpyAsync(Yb.ptr, y0.data(), y0.size() * 8, hipMemcpyHostToDevice, s));
   for (int c0 = 0; c0 < nvec; c0 += G)
-    hipLaunchKernelGGL(k_combine, dim3(grid_for(nr)), dim3(256), 0, s, V, ld, mm, (const double*)Yb.ptr, nvec, c0,
-                       std::min(G, nvec - c0), (double*)Tmp.ptr, nr);
-  double z = dev_norm2((double*)Tmp.ptr, n, cplx, s);
-  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(nr)), dim3(256), 0, s, (const double*)Tmp.ptr, evec, nr, 1.0 / z);
-  if (nvec > 1) copy_d2d(evec + nr, (const double*)Tmp.ptr + nr, (size_t)(nvec - 1) * nr * 8, s);
+    hipLaunchKernelGGL((k_combine<TV, double>), dim3(grid_for(nr)), dim3(256), 0, s, V, ld, mm,
+                       (const double*)Yb.ptr, nvec, c0, std::min(G, nvec - c0), evec, nr, nr);
+  double z = dev_norm2_t((const double*)evec, n, cplx, s);
+  dev_scale_t(evec, n, cplx, 1.0 / z, s);
   DMRGX_HIP(hipStreamSynchronize(s));
   DMRGX_HIP(hipGetLastError());
   return res;
+}
+
+}  // namespace
+
+EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
+                  int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec, bool early,
+                  bool fp32_basis) {
+  ensure_init();
+  hipStream_t s = rt().stream;
+  EigshResult res;
+  DMRGX_CHECK(n > 0, "eigsh: empty operator");
+  const int64_t nr = n * (cplx ? 2 : 1);
+  nev = (int)std::min<int64_t>(nev, n);
+  ncv = (int)std::min<int64_t>(std::max(ncv, nev + 2), n);
+
+  if (n <= 64 || ncv <= nev + 1) {
+    // materialise the operator column by column and diagonalise on the host
+    DevBuf e(nr * 8), y(nr * 8);
+    std::vector<double> ycol(nr);
+    std::vector<dmrgx::cplx> H((size_t)n * n);
+    for (int64_t j = 0; j < n; ++j) {
+      fill_zero(e.ptr, nr * 8, s);
+      double one = 1.0;
+      DMRGX_HIP(hipMemcpyAsync((double*)e.ptr + (cplx ? 2 * j : j), &one, 8, hipMemcpyHostToDevice, s));
+      op((const double*)e.ptr, (double*)y.ptr);
+      res.n_matvec++;
+      DMRGX_HIP(hipMemcpyAsync(ycol.data(), y.ptr, nr * 8, hipMemcpyDeviceToHost, s));
+      DMRGX_HIP(hipStreamSynchronize(s));
+      for (int64_t i = 0; i < n; ++i)
+        H[(size_t)i * n + j] = cplx ? dmrgx::cplx(ycol[2 * i], ycol[2 * i + 1]) : dmrgx::cplx(ycol[i], 0);
+    }
+    std::vector<double> w;
+    std::vector<dmrgx::cplx> V;
+    host_hermitian_eig((int)n, H, !cplx, w, V);
+    res.evals.assign(w.begin(), w.begin() + nev);
+    const int nv = (int)std::min<int64_t>(std::max(n_vec, 1), n);
+    std::vector<double> x((size_t)std::max(n_vec, 1) * nr, 0.0);
+    for (int c = 0; c < nv; ++c)
+      for (int64_t i = 0; i < n; ++i) {
+        double* xc = x.data() + (size_t)c * nr;
+        if (cplx) { xc[2 * i] = V[(size_t)i * n + c].real(); xc[2 * i + 1] = V[(size_t)i * n + c].imag(); }
+        else xc[i] = V[(size_t)i * n + c].real();
+      }
+    DMRGX_HIP(hipMemcpyAsync(evec, x.data(), x.size() * 8, hipMemcpyHostToDevice, s));
+    DMRGX_HIP(hipStreamSynchronize(s));
+    res.converged = true;
+    return res;
+  }
+
+  // a float basis holds orthogonality to about 1e-7: only for solves whose
+  // tolerance it does not limit
+  if (fp32_basis && tol >= 1e-6)
+    return eigsh_krylov<float>(op, n, cplx, nev, ncv, tol, max_restart, seed, v0, evec, n_vec, early);
+  return eigsh_krylov<double>(op, n, cplx, nev, ncv, tol, max_restart, seed, v0, evec, n_vec, early);
 }
 
 }  // namespace dmrgx

@@ -29,10 +29,13 @@ struct EigshResult {
 // (n_vec <= ncv), the lowest normalised.  `early`: the test also runs after
 // every step, not only when the basis is full (ARPACK tests at the end of a
 // cycle only), so a start vector close to an eigenvector stops within a few
-// matvecs.
+// matvecs.  `fp32_basis`: keep the Krylov basis in FP32 (operator input and
+// output, Gram-Schmidt sums and Ritz vectors stay FP64), which halves the
+// Gram-Schmidt traffic of a low-precision solve; used only when tol >= 1e-6,
+// above the basis' orthogonality level.
 EigshResult eigsh(const MatVec& op, int64_t n, bool cplx, int nev, int ncv, double tol,
                   int max_restart, uint64_t seed, const double* v0, double* evec, int n_vec = 1,
-                  bool early = false);
+                  bool early = false, bool fp32_basis = false);
 
 // device vector helpers (n = number of elements, complex if cplx)
 void dev_random(double* v, int64_t n, bool cplx, uint64_t seed, hipStream_t s);
