@@ -425,17 +425,22 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
     "<prec>+warm" starts every sweep step from the previous step's state
     (White's prediction) and converges the lowest pair only; the plain runs
     start from random vectors and converge six pairs, as the reference's
-    ARPACK calls do."""
+    ARPACK calls do.  "+nevK" makes a reduced-precision solve test the
+    residuals of its K lowest pairs, "+tolX" sets its tolerance (default
+    1e-4)."""
     from dmrg_amd import driver
     params = {"xyz": JXYZ, "potts3": POTTS}[model]
     res = {"model": model, "L": L, "m": m, "sweeps": sweeps, "ladder": bool(ladder), "params": list(params),
            "timed": "last full sweep"}
     states = {}
     for spec in precs:
-        prec, warm = spec.split("+")[0], spec.endswith("+warm")
+        prec, *mods = spec.split("+")
+        warm = "warm" in mods
+        nev_lo = next((int(x[3:]) for x in mods if x.startswith("nev")), 0)
+        tol_lo = next((float(x[3:]) for x in mods if x.startswith("tol")), 1e-4)
         t0 = time.perf_counter()
-        r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=1e-4, refine_k=8,
-                            ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm)
+        r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=tol_lo, refine_k=8,
+                            ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm, nev_lo=nev_lo)
         states[spec] = r.state
         st = r.stats
         per = sweep_seconds(r.steps, L, sweeps)

@@ -121,10 +121,12 @@ void BatchEngine<T>::solve(std::vector<qtensor<T>>& lenv, std::vector<qtensor<T>
   // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64 polish
   int64_t dmin = d[0];
   for (auto x : d) dmin = std::min(dmin, x);
+  // (it tests the residuals of its nev_lo lowest pairs and sizes its basis for four)
   const int nev = plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
-  // ARPACK's basis size for the nev this solve asks for (src/dsaupd.cpp:76, src/znaupd.cpp:94)
-  const int ncv = (cx ? 6 : 4) * nev;
-  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, dmin})) : 1;
+  const int nb = plan64 ? std::max(nev, 4) : nev;
+  // ARPACK's basis size for nb pairs (src/dsaupd.cpp:76, src/znaupd.cpp:94)
+  const int ncv = (cx ? 6 : 4) * nb;
+  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nb, dmin})) : 1;
   DevBuf evec(P * sizeof(T) * pv);
   // converged chains leave the operator at the end of their restart cycle
   RetireFn retire = [&](const std::vector<char>& active) {

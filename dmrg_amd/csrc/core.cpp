@@ -187,13 +187,15 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
     if (matched > 0 && dev_norm2((const double*)start.data(), d, cx, s) > 0.1) v0 = (const double*)start.data();
   }
   lg.warm = v0 != nullptr;
-  // a reduced-precision solve hands its pv lowest Ritz vectors to the FP64
-  // polish; a warm start converges the lowest pair only
+  // a reduced-precision solve tests the residuals of its nev_lo lowest pairs,
+  // sizes its basis for four and hands its pv lowest Ritz vectors to the
+  // FP64 polish; a warm start converges the lowest pair only
   const int nev = lg.warm ? 1 : plan64 ? std::min(cfg.nev_lo, cfg.nev) : cfg.nev;
-  // ARPACK's basis size for the nev this solve asks for (src/dsaupd.cpp:76,
-  // src/znaupd.cpp:94); a warm start keeps a 20-vector basis for its one pair
-  const int ncv = lg.warm ? 20 : (cx ? 6 : 4) * nev;
-  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, d})) : 1;
+  const int nb = lg.warm ? 1 : plan64 ? std::max(nev, 4) : nev;
+  // ARPACK's basis size for nb pairs (src/dsaupd.cpp:76, src/znaupd.cpp:94);
+  // a warm start keeps a 20-vector basis for its one pair
+  const int ncv = lg.warm ? 20 : (cx ? 6 : 4) * nb;
+  const int pv = plan64 ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nb, d})) : 1;
   DevBuf evec(std::max<int64_t>(d, 1) * sizeof(T) * pv);
   const double tol = cfg.prec == PREC_FP64 ? cfg.tol : cfg.tol_lo;
   uint64_t seed = cfg.seed * 0x9E3779B97F4A7C15ull + (uint64_t)log.size() + 1;

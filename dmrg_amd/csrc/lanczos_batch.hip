@@ -23,6 +23,7 @@ namespace dmrgx {
 namespace {
 
 constexpr int G = 8;      // Krylov columns per pass of the dot / combine kernels
+constexpr int COLS = G;   // the same, for code where G names a projected matrix
 // elements of one chain handled by one work-group: enough work-groups to
 // fill the chip (about 1024), between 256 and 4096 elements each
 inline int64_t seg_elems(int64_t total) {
@@ -190,6 +191,190 @@ __global__ __launch_bounds__(256) void k_combine_z_seg(const double* __restrict_
         }
       }
   }
+}
+
+// The conjugate-gradient inner solve of the Jacobi-Davidson refinement keeps
+// its scalars on the device, one set per chain (cg[field * nc + chain]), so
+// that its steps queue without host synchronisations.
+enum { CG_THETA, CG_RHO, CG_RR, CG_LIVE, CG_ALPHA, CG_BETA, CG_FIELDS };
+
+// the work-group sums of (re, im) into partial[blk][0..1] (the layout
+// k_reduce_seg reads for one column)
+__device__ __forceinline__ void seg_store2(double re, double im, double* __restrict__ partial) {
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = re;
+  red[1][threadIdx.x] = im;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + s];
+      red[1][threadIdx.x] += red[1][threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 2) partial[(int64_t)blockIdx.x * 2 * G + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// k_vdots_seg of x = w - theta_chain d, formed on the fly: partials of
+// <V_j, x> for j < k and of <x, x>
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_vdots_shift_seg(const double* __restrict__ V, int64_t ld, int k,
+                                                         const double* __restrict__ w, const double* __restrict__ d,
+                                                         const double* __restrict__ cg, int nc,
+                                                         const SegBlk* __restrict__ blks,
+                                                         double* __restrict__ partial) {
+  const int j0 = blockIdx.y * G;
+  const int g = min(G, k + 1 - j0);
+  const SegBlk B = blks[blockIdx.x];
+  const double th = cg[CG_THETA * nc + B.chain];
+  double re[G], im[G];
+#pragma unroll
+  for (int c = 0; c < G; ++c) re[c] = im[c] = 0;
+  for (int64_t i = B.b + threadIdx.x; i < B.e; i += 256) {
+    if (CPLX) {
+      const double xr = w[2 * i] - th * d[2 * i], xi = w[2 * i + 1] - th * d[2 * i + 1];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) {
+          double vr = xr, vi = xi;
+          if (j0 + c < k) {
+            const double* v = V + (int64_t)(j0 + c) * ld;
+            vr = v[2 * i];
+            vi = v[2 * i + 1];
+          }
+          re[c] += vr * xr + vi * xi;
+          im[c] += vr * xi - vi * xr;
+        }
+    } else {
+      const double xr = w[i] - th * d[i];
+#pragma unroll
+      for (int c = 0; c < G; ++c)
+        if (c < g) re[c] += ((j0 + c < k) ? V[(int64_t)(j0 + c) * ld + i] : xr) * xr;
+    }
+  }
+  __shared__ double red[2 * G][256];
+#pragma unroll
+  for (int c = 0; c < G; ++c) {
+    red[2 * c][threadIdx.x] = re[c];
+    red[2 * c + 1][threadIdx.x] = im[c];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int q = 0; q < 2 * G; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 2 * G)
+    partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 2 * G + threadIdx.x] = red[threadIdx.x][0];
+}
+
+// w = P (w - theta d) = (w - theta d) - sum_{j<k} V_j h_j (h = V^H (w - theta d)),
+// with the partials of <d, w> after it
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_cg_project(const double* __restrict__ V, int64_t ld, int k,
+                                                    const double* __restrict__ h, int hs, double* __restrict__ w,
+                                                    const double* __restrict__ d, const double* __restrict__ cg,
+                                                    int nc, const SegBlk* __restrict__ blks,
+                                                    double* __restrict__ partial) {
+  const SegBlk B = blks[blockIdx.x];
+  const double* hc = h + (int64_t)B.chain * hs;
+  const double th = cg[CG_THETA * nc + B.chain];
+  double re = 0, im = 0;
+  for (int64_t i = B.b + threadIdx.x; i < B.e; i += 256) {
+    if (CPLX) {
+      double ar = 0, ai = 0;
+      for (int j = 0; j < k; ++j) {
+        const double* v = V + (int64_t)j * ld;
+        const double vr = v[2 * i], vi = v[2 * i + 1], hr = hc[2 * j], hi = hc[2 * j + 1];
+        ar += vr * hr - vi * hi;
+        ai += vr * hi + vi * hr;
+      }
+      const double dr = d[2 * i], di = d[2 * i + 1];
+      const double xr = (w[2 * i] - th * dr) - ar, xi = (w[2 * i + 1] - th * di) - ai;
+      w[2 * i] = xr;
+      w[2 * i + 1] = xi;
+      re += dr * xr + di * xi;
+      im += dr * xi - di * xr;
+    } else {
+      double acc = 0;
+      for (int j = 0; j < k; ++j) acc += V[(int64_t)j * ld + i] * hc[2 * j];
+      const double x = (w[i] - th * d[i]) - acc;
+      w[i] = x;
+      re += d[i] * x;
+    }
+  }
+  seg_store2(re, im, partial);
+}
+
+// per chain: alpha = rho / <d, w> while the chain is live and <d, w> > 0,
+// else alpha = 0 and the chain stops
+__global__ __launch_bounds__(256) void k_cg_alpha(int nc, const double* __restrict__ dw, int hs,
+                                                  double* __restrict__ cg) {
+  for (int c = threadIdx.x; c < nc; c += 256) {
+    const double x = dw[(int64_t)c * hs];
+    const bool live = cg[CG_LIVE * nc + c] != 0;
+    if (live && x > 0) {
+      cg[CG_ALPHA * nc + c] = cg[CG_RHO * nc + c] / x;
+    } else {
+      cg[CG_ALPHA * nc + c] = 0;
+      cg[CG_LIVE * nc + c] = 0;
+    }
+  }
+}
+
+// s += alpha d, res -= alpha w, with the partials of |res|^2 after it
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_cg_update(const double* __restrict__ d, double* __restrict__ sv,
+                                                   const double* __restrict__ w, double* __restrict__ res,
+                                                   const double* __restrict__ cg, int nc,
+                                                   const SegBlk* __restrict__ blks, double* __restrict__ partial) {
+  constexpr int C = CPLX ? 2 : 1;
+  const SegBlk B = blks[blockIdx.x];
+  const double a = cg[CG_ALPHA * nc + B.chain];
+  double acc = 0;
+  for (int64_t i = C * B.b + threadIdx.x; i < C * B.e; i += 256) {
+    sv[i] += a * d[i];
+    const double r = res[i] - a * w[i];
+    res[i] = r;
+    acc += r * r;
+  }
+  seg_store2(acc, 0.0, partial);
+}
+
+// per chain: beta = |res|^2 / rho (0 once the chain stopped), rho = |res|^2,
+// and the chain stops once |res|^2 <= 1e-6 |r|^2; live[0] = the chains still live
+__global__ __launch_bounds__(256) void k_cg_beta(int nc, const double* __restrict__ rn, int hs,
+                                                 double* __restrict__ cg, int* __restrict__ live_out) {
+  __shared__ int cnt[256];
+  int n = 0;
+  for (int c = threadIdx.x; c < nc; c += 256) {
+    const double x = fmax(rn[(int64_t)c * hs], 0.0);
+    const double rho = cg[CG_RHO * nc + c];
+    bool live = cg[CG_LIVE * nc + c] != 0;
+    cg[CG_BETA * nc + c] = (live && rho > 0) ? x / rho : 0.0;
+    cg[CG_RHO * nc + c] = x;
+    if (live && x <= 1e-6 * cg[CG_RR * nc + c]) live = false;
+    cg[CG_LIVE * nc + c] = live ? 1.0 : 0.0;
+    n += live ? 1 : 0;
+  }
+  cnt[threadIdx.x] = n;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) cnt[threadIdx.x] += cnt[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) live_out[0] = cnt[0];
+}
+
+// dn = res + beta d
+template <bool CPLX>
+__global__ __launch_bounds__(256) void k_cg_dir(const double* __restrict__ res, const double* __restrict__ d,
+                                                double* __restrict__ dn, const double* __restrict__ cg, int nc,
+                                                const SegBlk* __restrict__ blks) {
+  constexpr int C = CPLX ? 2 : 1;
+  const SegBlk B = blks[blockIdx.x];
+  const double b = cg[CG_BETA * nc + B.chain];
+  for (int64_t i = C * B.b + threadIdx.x; i < C * B.e; i += 256) dn[i] = res[i] + b * d[i];
 }
 
 // x holds e_j (and no longer e_{j-1}) in every listed chain with j < len;
@@ -714,6 +899,27 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
   const size_t vb = (size_t)nr * 8;
   DevBuf Vb(qmax * vb), HVb(qmax * vb), wk(7 * vb), hb((size_t)2 * nc * hs * 8), scl((size_t)nc * 8),
       Zb((size_t)nc * qmax * 16);
+  // the CG scalars (device), their pinned staging and the per-step live counts
+  DevBuf cgb((size_t)CG_FIELDS * nc * 8), liveb((size_t)std::max(inner, 1) * sizeof(int));
+  double* cgd = (double*)cgb.ptr;
+  int* live_dev = (int*)liveb.ptr;
+  struct CgHost {
+    double* pin = nullptr;
+    int* live = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~CgHost() {
+      if (pin) (void)hipHostFree(pin);
+      if (live) (void)hipHostFree(live);
+      for (auto e : ev)
+        if (e) (void)hipEventDestroy(e);
+    }
+  } cgh;
+  DMRGX_HIP(hipHostMalloc((void**)&cgh.pin, (size_t)CG_FIELDS * nc * 8, hipHostMallocDefault));
+  DMRGX_HIP(hipHostMalloc((void**)&cgh.live, (size_t)std::max(inner, 1) * sizeof(int), hipHostMallocDefault));
+  for (auto& e : cgh.ev) DMRGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  double* cg_pin = cgh.pin;
+  int* live_pin = cgh.live;
+  hipEvent_t* cg_ev = cgh.ev;
   double* V = (double*)Vb.ptr;
   double* HV = (double*)HVb.ptr;
   double* u = (double*)wk.ptr;
@@ -811,8 +1017,6 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
     // converged chains keep s = 0
     fill_zero(sv, vb, s);
     copy_d2d(dv, r, vb, s);
-    std::vector<double> rho(nc);
-    for (int c = 0; c < nc; ++c) rho[c] = done[c] ? 0.0 : rr[c];
     // zero the residual of converged chains so their s stays zero
     {
       std::vector<double> keep(nc);
@@ -821,44 +1025,56 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
       ops.sub(nullptr, 0, h, dv, dv, (const double*)scl.ptr);
       copy_d2d(t1, dv, vb, s);  // t1: the CG residual
     }
-    std::vector<char> live(nc);
-    for (int c = 0; c < nc; ++c) live[c] = !done[c] && rho[c] > 0;
+    // the CG scalars start on the device; the steps queue without waiting on
+    // the host, which reads each step's count of live chains one step later
+    for (int c = 0; c < nc; ++c) {
+      const double rho = done[c] ? 0.0 : rr[c];
+      cg_pin[CG_THETA * nc + c] = th0[c];
+      cg_pin[CG_RHO * nc + c] = rho;
+      cg_pin[CG_RR * nc + c] = rr[c];
+      cg_pin[CG_LIVE * nc + c] = (!done[c] && rho > 0) ? 1.0 : 0.0;
+      cg_pin[CG_ALPHA * nc + c] = cg_pin[CG_BETA * nc + c] = 0.0;
+    }
+    DMRGX_HIP(hipMemcpyAsync(cgd, cg_pin, (size_t)CG_FIELDS * nc * 8, hipMemcpyHostToDevice, s));
+    const int ng = (q + 1 + COLS - 1) / COLS;
+    double* part = (double*)ops.partial.ptr;
+    const int* cb = (const int*)ops.sg.cb->ptr;
+    const int* ce = (const int*)ops.sg.ce->ptr;
+    const int nblk = ops.sg.nblk;
     for (int it = 0; it < inner; ++it) {
       op_lo(dv, wv);
       if (nlo) (*nlo)++;
-      put_coef(th0, h);
-      ops.sub(dv, 1, h, wv, wv, nullptr);   // w = (H_lo - theta) d
-      ops.vdots(V, q, wv, h);
-      ops.sub(V, q, h, wv, wv, nullptr);    // w = P w
-      ops.vdots(dv, 1, wv, h);              // <d, w>
-      const double* dw = ops.fetch(h, 1);
-      std::vector<double> alpha(nc, 0.0);
-      for (int c = 0; c < nc; ++c) {
-        const double x = dw[(size_t)c * hs];
-        if (live[c] && x > 0) alpha[c] = rho[c] / x;
-        else live[c] = 0;
+      // w = P (H_lo - theta) d and <d, w>
+      if (cplx) hipLaunchKernelGGL(k_vdots_shift_seg<true>, dim3(nblk, ng), dim3(256), 0, s, V, nr, q, wv, dv, cgd,
+                                   nc, ops.sg.B(), part);
+      else hipLaunchKernelGGL(k_vdots_shift_seg<false>, dim3(nblk, ng), dim3(256), 0, s, V, nr, q, wv, dv, cgd, nc,
+                              ops.sg.B(), part);
+      hipLaunchKernelGGL(k_reduce_seg, dim3(nc, ng), dim3(64 * 2 * COLS), 0, s, part, nblk, q + 1, cb, ce, h, hs);
+      if (cplx) hipLaunchKernelGGL(k_cg_project<true>, dim3(nblk), dim3(256), 0, s, V, nr, q, h, hs, wv, dv, cgd, nc,
+                                   ops.sg.B(), part);
+      else hipLaunchKernelGGL(k_cg_project<false>, dim3(nblk), dim3(256), 0, s, V, nr, q, h, hs, wv, dv, cgd, nc,
+                              ops.sg.B(), part);
+      hipLaunchKernelGGL(k_reduce_seg, dim3(nc, 1), dim3(64 * 2 * COLS), 0, s, part, nblk, 1, cb, ce, h2, hs);
+      hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nc, h2, hs, cgd);
+      // s += alpha d, res -= alpha w, |res|^2
+      if (cplx) hipLaunchKernelGGL(k_cg_update<true>, dim3(nblk), dim3(256), 0, s, dv, sv, wv, t1, cgd, nc,
+                                   ops.sg.B(), part);
+      else hipLaunchKernelGGL(k_cg_update<false>, dim3(nblk), dim3(256), 0, s, dv, sv, wv, t1, cgd, nc, ops.sg.B(),
+                              part);
+      hipLaunchKernelGGL(k_reduce_seg, dim3(nc, 1), dim3(64 * 2 * COLS), 0, s, part, nblk, 1, cb, ce, h2, hs);
+      hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(256), 0, s, nc, h2, hs, cgd, live_dev + it);
+      DMRGX_HIP(hipMemcpyAsync(live_pin + it, live_dev + it, sizeof(int), hipMemcpyDeviceToHost, s));
+      DMRGX_HIP(hipEventRecord(cg_ev[it & 1], s));
+      if (it + 1 == inner) break;
+      // an inner solve a few digits deep is enough for the outer step: stop
+      // once no chain was live after the previous step (this step then ran
+      // with every alpha zero and changed nothing)
+      if (it > 0) {
+        DMRGX_HIP(hipEventSynchronize(cg_ev[(it - 1) & 1]));
+        if (live_pin[it - 1] == 0) break;
       }
-      std::vector<double> na(nc);
-      for (int c = 0; c < nc; ++c) na[c] = -alpha[c];
-      put_coef(na, h);
-      ops.sub(dv, 1, h, sv, sv, nullptr);   // s += alpha d
-      put_coef(alpha, h);
-      ops.sub(wv, 1, h, t1, t1, nullptr);   // res -= alpha w
-      ops.vdots(nullptr, 0, t1, h2);
-      const double* rn = ops.fetch(h2, 1);
-      std::vector<double> nb(nc, 0.0);
-      bool any = false;
-      for (int c = 0; c < nc; ++c) {
-        const double x = std::max(rn[(size_t)c * hs], 0.0);
-        if (live[c] && rho[c] > 0) nb[c] = -x / rho[c];
-        rho[c] = x;
-        // an inner solve a few digits deep is enough for the outer step
-        if (live[c] && x <= 1e-6 * rr[c]) live[c] = 0;
-        any = any || live[c];
-      }
-      if (!any || it + 1 == inner) break;
-      put_coef(nb, h);
-      ops.sub(dv, 1, h, t1, dn, nullptr);   // d = res + beta d
+      if (cplx) hipLaunchKernelGGL(k_cg_dir<true>, dim3(nblk), dim3(256), 0, s, t1, dv, dn, cgd, nc, ops.sg.B());
+      else hipLaunchKernelGGL(k_cg_dir<false>, dim3(nblk), dim3(256), 0, s, t1, dv, dn, cgd, nc, ops.sg.B());
       std::swap(dv, dn);
     }
     add_vector(q++, sv);

@@ -155,3 +155,26 @@ def test_batch_64_chains_L32_m128(gpu):
         print(f"case batch chain {k} single run", flush=True)
         one = driver.run_dmrg("xyz", L, m, sweeps, tuple(J[k]))
         assert abs(b64.energies[k] - one.energy) < 1e-9 * abs(one.energy), (k, b64.energies[k], one.energy)
+
+
+def test_xyz_m512_mixed_sweep_energy(gpu):
+    """Config 2's bond dimension with full-dimension steps: the open XYZ chain
+    (Jx, Jy, Jz = 1, 0.8, 0.5) at L=24, m=512 (the centre bonds reach m;
+    two-site vectors of 524288 elements), growth and two sweeps (no cutoff
+    ladder) in FP64 and in BF16x3 with the FP64 refinement: |dE0| < 1e-8 and
+    the final states agree (fidelity 1 to 1e-8).  (After one sweep the two
+    runs are still on different paths toward the ground state: at L=40 they
+    differ by 9e-8, the BF16x3 run lower.)"""
+    L, m = 24, 512
+    print(f"case xyz L={L} m={m} fp64", flush=True)
+    a = driver.run_dmrg("xyz", L, m, 2, (1.0, 0.8, 0.5), ladder=False, keep_state=True)
+    print(f"case xyz L={L} m={m} bf16x3", flush=True)
+    b = driver.run_dmrg("xyz", L, m, 2, (1.0, 0.8, 0.5), prec="bf16x3", tol_lo=1e-4, ladder=False,
+                        keep_state=True)
+    dE = abs(a.energy - b.energy)
+    fid = b.state.fidelity(a.state)
+    print(f"  E0 fp64 {a.energy:.12f} bf16x3 {b.energy:.12f} |dE0| {dE:.2e} fidelity {fid:.15f}", flush=True)
+    assert dE < 1e-8, dE
+    assert abs(fid - 1) < 1e-8, fid
+    a.state.close()
+    b.state.close()
