@@ -87,6 +87,10 @@ S, D, M = 2, 5, 512
 L_CHAIN = 100
 JXYZ = (1.0, 0.8, 0.5)
 POTTS = (1.0, 1.0, 0.0, 3.14159265359 / 8)  # f, j, p, theta of H_Potts3 (the reference's truncated pi)
+# tolerance of the BF16x3 Lanczos solve ahead of the FP64 refinement (its
+# lowest pair; measured fastest of 1e-3 .. 3e-5 on the L=100 m=512 XYZ and
+# L=64 m=1024 Potts sweeps)
+TOL_LO = 3e-5
 
 
 def time_plan(h, prec, x, steps, warmup):
@@ -161,7 +165,7 @@ def quick_state(model, L, m):
     full sweep comparison: BF16x3 growth and one sweep at m."""
     from dmrg_amd import driver
     params = {"xyz": JXYZ, "potts3": POTTS}[model]
-    r = driver.run_dmrg(model, L, m, 1, params, schedule="dmrg2", prec="bf16x3", tol_lo=1e-4, refine_k=8,
+    r = driver.run_dmrg(model, L, m, 1, params, schedule="dmrg2", prec="bf16x3", tol_lo=TOL_LO, refine_k=8,
                         ladder=False, keep_state=True)
     return r.state
 
@@ -424,10 +428,11 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
     fidelity is |<psi|psi_fp64>| of the final states.  A run named
     "<prec>+warm" starts every sweep step from the previous step's state
     (White's prediction) and converges the lowest pair only; the plain runs
-    start from random vectors and converge six pairs, as the reference's
-    ARPACK calls do.  "+nevK" makes a reduced-precision solve test the
-    residuals of its K lowest pairs, "+tolX" sets its tolerance (default
-    1e-4)."""
+    start from random vectors, and the FP64 ones converge six pairs, as the
+    reference's ARPACK calls do.  "+nevK" makes a reduced-precision solve test
+    the residuals of its K lowest pairs (default 1), "+tolX" sets its tolerance
+    (default TOL_LO), "+rkN" gives the refinement's inner solves 2N
+    conjugate-gradient steps (default 8)."""
     from dmrg_amd import driver
     params = {"xyz": JXYZ, "potts3": POTTS}[model]
     res = {"model": model, "L": L, "m": m, "sweeps": sweeps, "ladder": bool(ladder), "params": list(params),
@@ -437,10 +442,11 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
         prec, *mods = spec.split("+")
         warm = "warm" in mods
         nev_lo = next((int(x[3:]) for x in mods if x.startswith("nev")), 0)
-        tol_lo = next((float(x[3:]) for x in mods if x.startswith("tol")), 1e-4)
+        tol_lo = next((float(x[3:]) for x in mods if x.startswith("tol")), TOL_LO)
+        refine_k = next((int(x[2:]) for x in mods if x.startswith("rk")), 8)
         t0 = time.perf_counter()
-        r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=tol_lo, refine_k=8,
-                            ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm, nev_lo=nev_lo)
+        r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=tol_lo,
+                            refine_k=refine_k, ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm, nev_lo=nev_lo)
         states[spec] = r.state
         st = r.stats
         per = sweep_seconds(r.steps, L, sweeps)
@@ -496,7 +502,7 @@ def batch_leg(n_chains=64, L=32, m=128, sweeps=1):
     runs = {}
     for prec in ("fp64", "bf16x3"):
         t0 = time.perf_counter()
-        b = driver.run_batch("xyz", L, m, sweeps, J, prec=prec, tol_lo=1e-4, refine_k=8)
+        b = driver.run_batch("xyz", L, m, sweeps, J, prec=prec, tol_lo=TOL_LO, refine_k=8)
         el = time.perf_counter() - t0
         st = b.stats
         runs[prec] = b

@@ -27,7 +27,7 @@ struct RunConfig {
   // (White's prediction) and converge the lowest pair only (E1..E5 read 0)
   bool warm_start = false;
   int nev = NEV;
-  int nev_lo = 4;              // eigenpairs converged by a reduced-precision solve
+  int nev_lo = 1;              // eigenpairs whose residual a reduced-precision solve tests
   int max_restart = 5000;
   int verbose = 0;             // 1: a line per step on stdout, 2: on stderr
   // when set, every step appends to "energy" + output_suffix and "singular" +

@@ -286,7 +286,7 @@ RunConfig make_cfg(const dmrgx_run_config* c, bool fermion) {
   r.tol = c->tol > 0 ? c->tol : 1e-10;
   r.tol_lo = c->tol_lo > 0 ? c->tol_lo : 1e-6;
   r.refine_k = c->refine_k > 0 ? c->refine_k : 8;
-  r.nev_lo = c->nev_lo > 0 ? c->nev_lo : 4;
+  r.nev_lo = c->nev_lo > 0 ? c->nev_lo : 1;
   if (c->output) r.output_suffix = c->output;
   r.warm_start = c->warm_start != 0;
   r.verbose = c->verbose == 2 ? 2 : (c->verbose != 0 ? 1 : 0);

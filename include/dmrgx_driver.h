@@ -38,7 +38,8 @@ typedef struct dmrgx_run_config {
   double tol_lo;    /* reduced-precision Lanczos tolerance */
   int refine_k;     /* FP64 Krylov polish steps after a reduced-precision solve */
   int verbose;      /* 0: quiet; 1: one line per step on stdout; 2: the same on stderr */
-  int nev_lo;       /* eigenpairs converged by a reduced-precision solve (0: 4); FP64 solves use 6 */
+  int nev_lo;       /* eigenpairs whose residual a reduced-precision solve tests (0: 1; its basis is sized
+                       for four and its four lowest Ritz vectors start the FP64 refinement); FP64 solves use 6 */
   const char* output;  /* NULL: no files; else every step appends to "energy" + output and
                           "singular" + output in the working directory in the reference's
                           formats (print_energy / print_singular, src/useful.cpp:54-102) */
