@@ -712,6 +712,34 @@ __global__ void k_split_padded(const SRC* __restrict__ src, unsigned short* __re
     const int col0 = (int)(loc - row * b.Kp);
     const bool rok = row < b.rows;
     const int64_t si = b.src_off + row * b.K + col0;
+    if constexpr (std::is_same<SRC, float>::value) {
+      if (cplx) {
+        // complex FP32 source: one 8-byte load per element, both parts split
+        // in FP32 (x - hi is exact there, so the planes equal the FP64 route's)
+        const float2* s2 = reinterpret_cast<const float2*>(src);
+        unsigned short h[2][8], l[2][8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const bool ok = rok && col0 + j < b.K;
+          const float2 x = ok ? s2[si + j] : make_float2(0.f, 0.f);
+          h[0][j] = f2bf_rne(x.x);
+          l[0][j] = f2bf_rne(x.x - bf2f(h[0][j]));
+          h[1][j] = f2bf_rne(x.y);
+          l[1][j] = f2bf_rne(x.y - bf2f(h[1][j]));
+        }
+#pragma unroll
+        for (int part = 0; part < 2; ++part) {
+          uint4 vh, vl;
+          vh.x = h[part][0] | ((unsigned)h[part][1] << 16); vh.y = h[part][2] | ((unsigned)h[part][3] << 16);
+          vh.z = h[part][4] | ((unsigned)h[part][5] << 16); vh.w = h[part][6] | ((unsigned)h[part][7] << 16);
+          vl.x = l[part][0] | ((unsigned)l[part][1] << 16); vl.y = l[part][2] | ((unsigned)l[part][3] << 16);
+          vl.z = l[part][4] | ((unsigned)l[part][5] << 16); vl.w = l[part][6] | ((unsigned)l[part][7] << 16);
+          *reinterpret_cast<uint4*>(dst + (2 * part) * ps + e) = vh;
+          *reinterpret_cast<uint4*>(dst + (2 * part + 1) * ps + e) = vl;
+        }
+        continue;
+      }
+    }
     for (int part = 0; part <= cplx; ++part) {
       unsigned short h[8], l[8];
 #pragma unroll

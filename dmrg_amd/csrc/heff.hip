@@ -1025,8 +1025,10 @@ int dmrgx_eigsh(dmrgx_heff* plan, int nev, int ncv, double tol, int max_restart,
   const bool cx = h->dtype == DT_C64;
   hipStream_t s = rt().stream;
   MatVec op = [h, s](const double* x, double* y) { h->apply(x, y, s); };
-  // a refined solve hands its pv lowest Ritz vectors to the FP64 refinement
-  const int pv = use_ref ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, h->n_psi})) : 1;
+  // a refined solve hands its pv lowest Ritz vectors to the FP64 refinement:
+  // four whenever the basis holds them, also when only the lowest pair's
+  // residual is tested (nev = 1)
+  const int pv = use_ref ? (int)std::max<int64_t>(1, std::min<int64_t>({4, std::max(nev, ncv), h->n_psi})) : 1;
   std::unique_ptr<DevBuf> vecs;
   double* ev = (double*)evec;
   if (pv > 1) {
@@ -1078,7 +1080,7 @@ int dmrgx_eigsh_batched(dmrgx_heff* plan, int n_chains, const int64_t* len, int 
   const bool cx = h->dtype == DT_C64;
   hipStream_t s = rt().stream;
   MatVec op = [h, s](const double* x, double* y) { h->apply(x, y, s); };
-  const int pv = use_ref ? (int)std::max<int64_t>(1, std::min<int64_t>({4, nev, nmin})) : 1;
+  const int pv = use_ref ? (int)std::max<int64_t>(1, std::min<int64_t>({4, std::max(nev, ncv), nmin})) : 1;
   std::unique_ptr<DevBuf> vecs;
   double* ev = (double*)evec;
   if (pv > 1) {

@@ -558,14 +558,16 @@ __global__ __launch_bounds__(256) void k_binit(const double* __restrict__ Q, dou
 
 // Partial Gram matrices of one round: work-group = (pair, row chunk).  256
 // threads: thread t owns the 4x4 tile (t & 63) of the 32x32 result over the
-// chunk rows [16 * (t >> 6), +16); the four row quarters are summed in LDS.
+// chunk rows [16 * (t >> 6), +16); the four row quarters are summed by the
+// first wavefront through one LDS tile that reuses the column stage, so a
+// work-group holds 33 KB of LDS (complex) and four fit on a CU.
 template <bool CPLX>
 __global__ __launch_bounds__(256) void k_bgram(const double* __restrict__ W, const JBlk* __restrict__ blks, int nb,
                                                int64_t total, int round, double* __restrict__ GP) {
   constexpr int C = CPLX ? 2 : 1;
   constexpr int XS = (RCG + 1) * C;  // LDS row stride of one column (padded)
   __shared__ double sX[PJ * XS];
-  __shared__ double sR[3][PJ * PJ * C];
+  static_assert(PJ * PJ * C <= PJ * XS, "the reduction tile must fit in the column stage");
   const int64_t g = blockIdx.x;
   if (g >= total) return;
   const int b = find_blk(blks, nb, g, BY_GRAM);
@@ -606,17 +608,33 @@ __global__ __launch_bounds__(256) void k_bgram(const double* __restrict__ W, con
         if (CPLX) ai[i][j] += pr_[i] * qim[j] - pim[i] * qr_[j];
       }
   }
-  if (ph > 0) {
+  // quarters 1, 2, 3 in turn through sX (now free): quarter 0's sum is
+  // ((q0 + q1) + q2) + q3
+  double* sR = sX;
+  for (int q = 1; q < 4; ++q) {
+    __syncthreads();
+    if (ph == q) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int e = (pi0 + i) * PJ + qj0 + j;
-        sR[ph - 1][e * C] = ar[i][j];
-        if (CPLX) sR[ph - 1][e * C + 1] = ai[i][j];
-      }
+        for (int j = 0; j < 4; ++j) {
+          const int e = (pi0 + i) * PJ + qj0 + j;
+          sR[e * C] = ar[i][j];
+          if (CPLX) sR[e * C + 1] = ai[i][j];
+        }
+    }
+    __syncthreads();
+    if (ph == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int e = (pi0 + i) * PJ + qj0 + j;
+          ar[i][j] += sR[e * C];
+          if (CPLX) ai[i][j] += sR[e * C + 1];
+        }
+    }
   }
-  __syncthreads();
   if (ph == 0) {
     double* out = GP + C * (k.gp_off + ((int64_t)pair * k.nchg + chunk) * PJ * PJ);
 #pragma unroll
@@ -624,8 +642,8 @@ __global__ __launch_bounds__(256) void k_bgram(const double* __restrict__ W, con
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int e = (pi0 + i) * PJ + qj0 + j;
-        out[e * C] = ar[i][j] + sR[0][e * C] + sR[1][e * C] + sR[2][e * C];
-        if (CPLX) out[e * C + 1] = ai[i][j] + sR[0][e * C + 1] + sR[1][e * C + 1] + sR[2][e * C + 1];
+        out[e * C] = ar[i][j];
+        if (CPLX) out[e * C + 1] = ai[i][j];
       }
   }
 }

@@ -144,8 +144,9 @@ int dmrgx_heff_time_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out
  * vector seeded by `seed`.
  * Mixed precision: when `plan` is an FP32 / BF16X3 plan and `refine` is an FP64
  * plan of the same operator (same psi length and element type, else an error),
- * the Lanczos solve runs on `plan` to `tol` (e.g. 1e-4) and its lowest
- * min(4, nev) Ritz vectors start an FP64 Jacobi-Davidson refinement: the
+ * the Lanczos solve runs on `plan` until the residuals of its nev lowest pairs
+ * pass `tol` (e.g. nev 1, ncv 24, tol 3e-5, as the sweep engine does) and its
+ * lowest min(4, ncv) Ritz vectors start an FP64 Jacobi-Davidson refinement: the
  * Rayleigh-Ritz basis grows by one vector per step (one FP64 matvec), the
  * correction from at most 16 conjugate-gradient steps on the projected
  * correction equation with the low-precision operator, at most 12 steps, until

@@ -85,14 +85,17 @@ def test_eigsh_spectrum_far_from_zero(gpu):
         assert np.linalg.norm(H @ xb[k] - evb[k, 0] * xb[k]) < 1e-7 * abs(w[0])
 
 
-def test_eigsh_mixed_refine_near_degenerate_pair(gpu):
+@pytest.mark.parametrize("nev,ncv", [(6, None), (1, 24)])
+def test_eigsh_mixed_refine_near_degenerate_pair(gpu, nev, ncv):
     """The mixed-precision C ABI path on a step whose two lowest states lie
     2.2e-5 apart: the warm-up step 14 of an open XYZ chain L=32 with Jz
     dominant (chain 63 of the batched workload's couplings).  The BF16x3
-    Lanczos stops at 1e-4, below that gap; the FP64 refinement starts from the
-    four lowest Ritz vectors, so it must return the true ground state (energy to
-    1e-10, overlap with the exact vector to 1e-8), not its partner."""
-    print("case eigsh bf16x3 refine near-degenerate", flush=True)
+    Lanczos stops at 1e-4, below that gap (on six pairs, or on the lowest
+    pair only with the sweep engine's 24-vector basis); the FP64 refinement
+    starts from the four lowest Ritz vectors either way, so it must return the
+    true ground state (energy to 1e-10, overlap with the exact vector to
+    1e-8), not its partner."""
+    print(f"case eigsh bf16x3 refine near-degenerate nev={nev}", flush=True)
     J = (0.54790153, 0.57295701, 1.46374402)
     case = dmrg_step_case("xyz", 32, 16, 14, jx=J[0], jy=J[1], jz=J[2])
     H = dense_heff(case.ctx, case.n_psi, case.dtype)
@@ -100,8 +103,8 @@ def test_eigsh_mixed_refine_near_degenerate_pair(gpu):
     assert 1e-6 < w[1] - w[0] < 1e-4, w[:2]
     lo = _plan(gpu, case, "bf16x3")
     hi = _plan(gpu, case, "fp64")
-    ev, vec, info, res = lo.eigsh(nev=6, tol=1e-4, refine=hi, refine_tol=1e-10)
-    assert info[4] == 1 and info[3] > 0
+    ev, vec, info, res = lo.eigsh(nev=nev, ncv=ncv, tol=1e-4, refine=hi, refine_tol=1e-10)
+    assert info[4] == 1 and info[3] >= 4
     assert abs(ev[0] - w[0]) < 1e-10 * abs(w[0]), (ev[0], w[0], w[1])
     x = vec.numpy()
     assert abs(abs(np.vdot(v[:, 0], x)) - 1) < 1e-8
