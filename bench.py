@@ -446,14 +446,18 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
         refine_k = next((int(x[2:]) for x in mods if x.startswith("rk")), 8)
         t0 = time.perf_counter()
         r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=tol_lo,
-                            refine_k=refine_k, ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm, nev_lo=nev_lo)
+                            refine_k=refine_k, ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm,
+                            nev_lo=nev_lo)
         states[spec] = r.state
         st = r.stats
         per = sweep_seconds(r.steps, L, sweeps)
+        steps = np.asarray(r.steps)
+        full = steps[steps[:, 16] == steps[:, 16].max()]  # the steps at full two-site dimension
         res[spec] = dict(energy=r.energy, sweep_s=round(st["last_sweep_seconds"], 4),
                          sweeps_s=[round(x, 3) for x in per], total_s=round(time.perf_counter() - t0, 3),
                          matvecs=st["matvecs"], polish=st["polish_matvecs"], trunc_err=st["trunc_err"],
-                         t_eig=round(st["t_eig"], 3), t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3))
+                         t_eig=round(st["t_eig"], 3), t_svd=round(st["t_svd"], 3), t_env=round(st["t_env"], 3),
+                         full_dim_steps=int(len(full)), svd_jacobi_sweeps_full=round(float(full[:, 17].mean()), 2))
         print(f"[bench] sweep {spec} {model} L={L} m={m}: E0={r.energy:.12f} last sweep {st['last_sweep_seconds']:.2f} s "
               f"(eig {st['t_eig']:.2f} svd {st['t_svd']:.2f} env {st['t_env']:.2f} s in all steps)", file=sys.stderr,
               flush=True)

@@ -19,7 +19,7 @@ namespace dmrgx {
 namespace {
 
 constexpr int G = 8;  // Krylov columns handled per pass of the dot / combine kernels
-constexpr int DOT_BLOCKS_MAX = 1024;
+constexpr int DOT_BLOCKS_MAX = 4096;  // work-groups of one dot-product launch (all column groups)
 constexpr int PARTIAL_DOUBLES = (DOT_BLOCKS_MAX + 64) * 2 * G;
 
 // Partial <X_c, w> (conjugate-linear in X) for columns c of group blockIdx.y:
