@@ -60,6 +60,16 @@ __device__ __forceinline__ int64_t out_index(const RedTask& r, const int32_t* __
   return r.c_off + (int64_t)RM[r.row_map + row] * r.ncol + (e - row * r.ncol);
 }
 
+// The same for plane output, which may land in a padded layout (OutPad).
+__device__ __forceinline__ int64_t out_index_planes(const RedTask& r, const int32_t* __restrict__ RM, int64_t e) {
+  if (r.op.kp == 0) return out_index(r, RM, e);
+  int64_t row = e / r.ncol;
+  const int64_t col = e - row * r.ncol;
+  if (r.row_map >= 0) row = RM[r.row_map + row];
+  const int64_t rr = r.op.r0 + row;
+  return r.op.base + (rr / r.op.q) * r.op.kp + (rr % r.op.q) * r.ncol + col;
+}
+
 template <typename S, bool CPLX>
 __global__ __launch_bounds__(256) void k_reduce_ws(const S* __restrict__ W, S* __restrict__ C,
                                                    const RedTask* __restrict__ t, int n,
@@ -477,7 +487,7 @@ __global__ __launch_bounds__(256) void k_reduce_ws_out(const float* __restrict__
     for (int k = 0; k < r.nch; ++k) a += W[r.w_off + k * r.len + e];
     acc[0] = a;
   }
-  const int64_t o = out_index(r, RM, e);
+  const int64_t o = OUT == OUT_PLANES ? out_index_planes(r, RM, e) : out_index(r, RM, e);
 #pragma unroll
   for (int q = 0; q < c; ++q) {
     if (OUT == OUT_F64) {
@@ -651,17 +661,34 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
 
   const bool partial = it.flags & ITEM_PARTIAL;
   const bool beta = !partial && (it.flags & ITEM_BETA);
+  // where each of the lane's eight output rows starts (-1: past the block):
+  // through the row map, and into the padded layout for plane output
+  int64_t rstart[2][4];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int row = it.m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+      int64_t st = -1;
+      if (row < it.M) {
+        if (!partial && it.row_map >= 0) row = RM[it.row_map + row];
+        st = it.c_off + (int64_t)row * it.N;
+        if (OUT == OUT_PLANES && !partial && it.op.kp > 0) {
+          const int rr = it.op.r0 + row;
+          st = it.op.base + (int64_t)(rr / it.op.q) * it.op.kp + (int64_t)(rr % it.op.q) * it.N;
+        }
+      }
+      rstart[i][r] = st;
+    }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        int row = it.m0 + wm + i * 16 + (lane >> 4) * 4 + r;
-        int col = it.n0 + wn + j * 16 + (lane & 15);
-        if (row < it.M && col < it.N) {
-          if (!partial && it.row_map >= 0) row = RM[it.row_map + row];
-          int64_t e = it.c_off + (int64_t)row * it.N + col;
+        const int col = it.n0 + wn + j * 16 + (lane & 15);
+        if (rstart[i][r] >= 0 && col < it.N) {
+          const int64_t e = rstart[i][r] + col;
           const float vr = acc[0][i][j][r], vi = CPLX ? acc[NP - 1][i][j][r] : 0.0f;
           if (partial || OUT == OUT_F32) {
             float* D = partial ? W : (float*)Cv;
@@ -690,72 +717,64 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
       }
 }
 
-// Eight consecutive padded elements per thread: every block starts on a
-// multiple of 8 and Kp is one, so the eight lie in one row of one block; the
-// hi / lo parts of each go out as one 16-byte store per plane.
+// hi / lo bf16 parts of x: hi = x rounded to bf16 (through FP32), lo = the
+// rest rounded likewise (for an FP32 x the rest is exact in FP32)
+__device__ __forceinline__ void split_bf(float x, unsigned short& h, unsigned short& l) {
+  h = f2bf_rne(x);
+  l = f2bf_rne(x - bf2f(h));
+}
+__device__ __forceinline__ void split_bf(double x, unsigned short& h, unsigned short& l) {
+  h = f2bf_rne((float)x);
+  l = f2bf_rne((float)(x - (double)bf2f(h)));
+}
+
+// Two consecutive padded elements per thread (Kp and every block start are
+// multiples of 8, so the pair lies in one row), consecutive threads on
+// consecutive pairs: coalesced loads of the source and 4-byte stores to each
+// plane.  A work-group walks SPLIT_PER chunks of 512 elements, finding the
+// block of its first pair by binary search and the later ones by stepping
+// forward.  Padding columns (K <= col < Kp) are written as 0.
+constexpr int SPLIT_PER = 8;
 template <typename SRC>
-__global__ void k_split_padded(const SRC* __restrict__ src, unsigned short* __restrict__ dst, int64_t ps,
-                               const PadBlk* __restrict__ blk, int nb, int64_t total, int cplx) {
-  int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t st = (int64_t)gridDim.x * blockDim.x;
-  for (; g < total / 8; g += st) {
-    const int64_t e = g * 8;
-    int lo = 0, hi = nb - 1;
-    while (lo < hi) {
-      int mid = (lo + hi + 1) >> 1;
-      if (blk[mid].dst_off <= e) lo = mid;
+__global__ __launch_bounds__(256) void k_split_padded(const SRC* __restrict__ src, unsigned short* __restrict__ dst,
+                                                      int64_t ps, const PadBlk* __restrict__ blk, int nb,
+                                                      int64_t total, int cplx) {
+  using SRC2 = typename std::conditional<std::is_same<SRC, float>::value, float2, double2>::type;
+  const int64_t step = (int64_t)gridDim.x * 512 * SPLIT_PER;
+  for (int64_t c0 = (int64_t)blockIdx.x * 512 * SPLIT_PER; c0 < total; c0 += step) {
+    int64_t e = c0 + 2 * threadIdx.x;
+    int bi = 0, hi = nb - 1;
+    while (bi < hi) {
+      const int mid = (bi + hi + 1) >> 1;
+      if (blk[mid].dst_off <= e) bi = mid;
       else hi = mid - 1;
     }
-    const PadBlk b = blk[lo];
-    const int64_t loc = e - b.dst_off;
-    const int64_t row = loc / b.Kp;
-    const int col0 = (int)(loc - row * b.Kp);
-    const bool rok = row < b.rows;
-    const int64_t si = b.src_off + row * b.K + col0;
-    if constexpr (std::is_same<SRC, float>::value) {
+    for (int i = 0; i < SPLIT_PER && e < total; ++i, e += 512) {
+      while (bi + 1 < nb && blk[bi + 1].dst_off <= e) ++bi;
+      const PadBlk b = blk[bi];
+      const int loc = (int)(e - b.dst_off);  // blocks hold < 2^31 elements
+      const int row = loc / b.Kp, col = loc - row * b.Kp;
+      const bool ok0 = row < b.rows && col < b.K, ok1 = row < b.rows && col + 1 < b.K;
+      const int64_t si = b.src_off + (int64_t)row * b.K + col;
+      unsigned short h[2][2], l[2][2];  // [part][element]
       if (cplx) {
-        // complex FP32 source: one 8-byte load per element, both parts split
-        // in FP32 (x - hi is exact there, so the planes equal the FP64 route's)
-        const float2* s2 = reinterpret_cast<const float2*>(src);
-        unsigned short h[2][8], l[2][8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const bool ok = rok && col0 + j < b.K;
-          const float2 x = ok ? s2[si + j] : make_float2(0.f, 0.f);
-          h[0][j] = f2bf_rne(x.x);
-          l[0][j] = f2bf_rne(x.x - bf2f(h[0][j]));
-          h[1][j] = f2bf_rne(x.y);
-          l[1][j] = f2bf_rne(x.y - bf2f(h[1][j]));
-        }
-#pragma unroll
-        for (int part = 0; part < 2; ++part) {
-          uint4 vh, vl;
-          vh.x = h[part][0] | ((unsigned)h[part][1] << 16); vh.y = h[part][2] | ((unsigned)h[part][3] << 16);
-          vh.z = h[part][4] | ((unsigned)h[part][5] << 16); vh.w = h[part][6] | ((unsigned)h[part][7] << 16);
-          vl.x = l[part][0] | ((unsigned)l[part][1] << 16); vl.y = l[part][2] | ((unsigned)l[part][3] << 16);
-          vl.z = l[part][4] | ((unsigned)l[part][5] << 16); vl.w = l[part][6] | ((unsigned)l[part][7] << 16);
-          *reinterpret_cast<uint4*>(dst + (2 * part) * ps + e) = vh;
-          *reinterpret_cast<uint4*>(dst + (2 * part + 1) * ps + e) = vl;
-        }
-        continue;
+        const SRC2* s2 = reinterpret_cast<const SRC2*>(src);
+        SRC2 x0, x1;
+        x0.x = x0.y = x1.x = x1.y = 0;
+        if (ok0) x0 = s2[si];
+        if (ok1) x1 = s2[si + 1];
+        split_bf(x0.x, h[0][0], l[0][0]);
+        split_bf(x0.y, h[1][0], l[1][0]);
+        split_bf(x1.x, h[0][1], l[0][1]);
+        split_bf(x1.y, h[1][1], l[1][1]);
+      } else {
+        split_bf(ok0 ? src[si] : (SRC)0, h[0][0], l[0][0]);
+        split_bf(ok1 ? src[si + 1] : (SRC)0, h[0][1], l[0][1]);
       }
-    }
-    for (int part = 0; part <= cplx; ++part) {
-      unsigned short h[8], l[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const bool ok = rok && col0 + j < b.K;
-        const double x = ok ? (double)src[cplx ? 2 * (si + j) + part : si + j] : 0.0;
-        h[j] = f2bf_rne((float)x);
-        l[j] = f2bf_rne((float)(x - (double)bf2f(h[j])));
+      for (int part = 0; part <= cplx; ++part) {
+        *reinterpret_cast<unsigned*>(dst + (2 * part) * ps + e) = h[part][0] | ((unsigned)h[part][1] << 16);
+        *reinterpret_cast<unsigned*>(dst + (2 * part + 1) * ps + e) = l[part][0] | ((unsigned)l[part][1] << 16);
       }
-      uint4 vh, vl;
-      vh.x = h[0] | ((unsigned)h[1] << 16); vh.y = h[2] | ((unsigned)h[3] << 16);
-      vh.z = h[4] | ((unsigned)h[5] << 16); vh.w = h[6] | ((unsigned)h[7] << 16);
-      vl.x = l[0] | ((unsigned)l[1] << 16); vl.y = l[2] | ((unsigned)l[3] << 16);
-      vl.z = l[4] | ((unsigned)l[5] << 16); vl.w = l[6] | ((unsigned)l[7] << 16);
-      *reinterpret_cast<uint4*>(dst + (2 * part) * ps + e) = vh;
-      *reinterpret_cast<uint4*>(dst + (2 * part + 1) * ps + e) = vl;
     }
   }
 }
@@ -808,7 +827,7 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
                                          int split_target, bool planes, GemmTables* host_only,
-                                         const std::vector<int32_t>* row_maps) {
+                                         const std::vector<int32_t>* row_maps, const PadLayout* out_pad) {
   if (!host_only) ensure_init();
   auto plan = std::make_unique<GemmPlan>();
   plan->planes = planes;
@@ -932,6 +951,31 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     gnch = best_nc;
   }
 
+  // plane output into a padded layout: each output block must be a run of
+  // whole N-element rows of one padded block whose K is a multiple of N
+  std::vector<OutPad> gop(G, OutPad{0, 0, 0, 0, 0});
+  if (out_pad && planes && !out_pad->identity && !host_only) {
+    bool ok = true;
+    const auto& PB = out_pad->blk;
+    for (size_t gi = 0; gi < G && ok; ++gi) {
+      const Group& g = groups[gi];
+      int64_t rows = g.M;  // rows of the block the group writes (its row map's largest index + 1)
+      if (g.row_map >= 0) {
+        rows = 0;
+        for (int i = 0; i < g.M; ++i) rows = std::max<int64_t>(rows, (*row_maps)[g.row_map + i] + 1);
+      }
+      auto it = std::upper_bound(PB.begin(), PB.end(), g.c_off,
+                                 [](int64_t v, const PadBlk& b) { return v < b.src_off; });
+      if (it == PB.begin()) { ok = false; break; }
+      const PadBlk& b = *(it - 1);
+      const int64_t rel = g.c_off - b.src_off;
+      if (b.K % g.N != 0 || rel % g.N != 0 || rel + rows * g.N > (int64_t)b.rows * b.K) { ok = false; break; }
+      gop[gi] = OutPad{b.dst_off, (int)(rel / g.N), b.K / g.N, b.Kp, 0};
+    }
+    if (ok) plan->out_padded = true;
+    else std::fill(gop.begin(), gop.end(), OutPad{0, 0, 0, 0, 0});
+  }
+
   std::vector<GemmItem> items, half;  // balanced mode: whole tiles, then half tiles
   std::vector<GemmSeg> segs;
   std::vector<RedTask> red;
@@ -985,7 +1029,7 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
     if (shared) {
       w_off = plan->ws_elems;
       plan->ws_elems += len * nch;
-      red.push_back(RedTask{g.c_off, w_off, len, g.row_map, nch, g.beta, g.N, 0});
+      red.push_back(RedTask{g.c_off, w_off, len, g.row_map, nch, g.beta, g.N, 0, gop[gi]});
     }
     for (int c = 0; c < nch; ++c)
       for (int n0 = 0; n0 < g.N; n0 += BN)
@@ -1002,6 +1046,7 @@ std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks_in, 
           itm.seg_begin = chunk_rng[c].first;
           itm.seg_end = chunk_rng[c].second;
           itm.flags = shared ? ITEM_PARTIAL : (g.beta ? ITEM_BETA : 0);
+          itm.op = gop[gi];
           (balanced && shared ? half : items).push_back(itm);
         }
   }
@@ -1127,9 +1172,14 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
   DMRGX_CHECK(out != OUT_PLANES || !p.any_beta, "run_gemm_planes: plane output cannot accumulate");
   const int cw = p.dtype == DT_C64 ? 2 : 1;
   for (auto& z : p.zero_ranges) {
-    if (out == OUT_PLANES)
-      for (int q = 0; q < 2 * cw; ++q) fill_zero_u16((uint16_t*)C + q * c_ps + z.first, (size_t)z.second, s);
-    else fill_zero((char*)C + z.first * (out == OUT_F64 ? 8 : 4) * cw, (size_t)z.second * (out == OUT_F64 ? 8 : 4) * cw, s);
+    if (out == OUT_PLANES) {
+      // a padded output layout was zeroed once by its owner (the plan never
+      // writes these ranges, and their flat offsets are not padded ones)
+      if (!p.out_padded)
+        for (int q = 0; q < 2 * cw; ++q) fill_zero_u16((uint16_t*)C + q * c_ps + z.first, (size_t)z.second, s);
+    } else {
+      fill_zero((char*)C + z.first * (out == OUT_F64 ? 8 : 4) * cw, (size_t)z.second * (out == OUT_F64 ? 8 : 4) * cw, s);
+    }
   }
   if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
   launch_planes_kernel(p, A, a_stride, B, b_stride, C, out, c_ps, s);
@@ -1180,6 +1230,7 @@ std::unique_ptr<PadLayout> make_pad_layout(std::vector<PadBlk> blocks) {
     }
     PadBlk q = b;
     q.Kp = (b.K + 7) / 8 * 8;
+    DMRGX_CHECK((int64_t)q.rows * q.Kp < (1ll << 31), "padded layout: a block of 2^31 elements or more");
     q.dst_off = pos;
     q.pad = 0;
     if (q.dst_off != q.src_off || q.Kp != q.K) L->identity = false;
@@ -1199,7 +1250,7 @@ static void split_padded(const SRC* src, uint16_t* dst, int64_t ps, const PadLay
   if (L.total <= 0) return;
   DMRGX_CHECK(ps >= L.total, "split_padded: plane stride below the padded length");
   DMRGX_CHECK(ps % 8 == 0 && L.total % 8 == 0, "split_padded: plane stride and length must be multiples of 8");
-  const int64_t blocks = std::min<int64_t>((L.total / 8 + 255) / 256, 8192);
+  const int64_t blocks = std::min<int64_t>((L.total + 512 * SPLIT_PER - 1) / (512 * SPLIT_PER), 8192);
   hipLaunchKernelGGL(k_split_padded<SRC>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, ps,
                      (const PadBlk*)L.d_blk->ptr, (int)L.blk.size(), L.total, cplx ? 1 : 0);
   DMRGX_HIP(hipGetLastError());

@@ -26,6 +26,16 @@ struct GemmTask {
   int64_t row_map = -1;
 };
 
+// Where a plane-output block's rows land in a padded layout (PadLayout): the
+// block (M x N, row-major at c_off) is rows [r0, r0 + M) of a run of rows of
+// N elements, q of which make one row of a padded block of row stride kp at
+// base.  Row `row` (after any row map) starts at
+// base + ((r0 + row) / q) * kp + ((r0 + row) % q) * N.
+struct OutPad {
+  int64_t base;
+  int r0, q, kp, pad;
+};
+
 // Device work decomposition (shared with the kernels).
 struct GemmSeg {
   int64_t a_off, b_off;
@@ -44,6 +54,7 @@ struct GemmItem {
   int seg_begin, seg_end;
   int flags;  // bit0: shared tile (atomic accumulate), bit1: beta (exclusive accumulate)
   int pad;
+  OutPad op;  // plane output into a padded layout (op.kp == 0: C row-major at c_off)
 };
 
 // ITEM_PARTIAL: the item writes a K-chunk partial tile into the plan's
@@ -56,6 +67,7 @@ struct RedTask {
   int64_t row_map;            // row-map offset of the output rows (-1: rows 0..M)
   int nch, beta;
   int ncol, pad;              // N: output elements per row
+  OutPad op;                  // as GemmItem::op
 };
 
 // Output format of the pre-split BF16x3 kernel: FP32, bf16 hi/lo planes
@@ -93,6 +105,10 @@ struct GemmPlan {
   std::unique_ptr<DevBuf> own_items, own_segs, own_ws, own_red, own_red_start, own_row_maps;
   double flops = 0;        // algorithmic real flops (c*M*N*K summed)
   bool any_beta = false;   // some output block accumulates onto existing C
+  // built with an output layout: every output block maps into it (plane
+  // output then lands in that padded layout, which the caller zeroes once:
+  // the plan writes neither the padding nor the structurally zero ranges)
+  bool out_padded = false;
 };
 
 // Build a plan; `extra_zero` lists C ranges with no task that must read as 0.
@@ -103,12 +119,17 @@ struct GemmPlan {
 // is returned in *host_only and the caller sets the device addresses.
 // `row_maps` holds the row indices that row-mapped tasks point into (tasks of
 // one output block with different row maps form separate groups).
+// `out_pad` (plane output only): the padded layout C's planes are stored in;
+// when some output block does not map into it as OutPad describes, the plan
+// is built without it (out_padded false).
+struct PadLayout;
 std::unique_ptr<GemmPlan> make_gemm_plan(const std::vector<GemmTask>& tasks, int dtype, int prec,
                                          char ta, char tb,
                                          const std::vector<std::pair<int64_t, int64_t>>& extra_zero,
                                          int split_target = 0, bool planes = false,
                                          GemmTables* host_only = nullptr,
-                                         const std::vector<int32_t>* row_maps = nullptr);
+                                         const std::vector<int32_t>* row_maps = nullptr,
+                                         const PadLayout* out_pad = nullptr);
 // Bytes of the workspace of a plan (split-K partial tiles).
 inline size_t gemm_ws_bytes(const GemmPlan& p) {
   return (size_t)p.ws_elems * (p.prec == PREC_FP64 ? 8 : 4) * (p.dtype == DT_C64 ? 2 : 1);

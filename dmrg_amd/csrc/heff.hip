@@ -408,7 +408,8 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
   h->lexec = lx;
   h->rzero = rzero;
   h->lzero = lzero;
-  h->rplan = make_gemm_plan(rx, dtype, prec, 'N', 'T', rzero, 0, planes, nullptr, h->maps());
+  h->rplan = make_gemm_plan(rx, dtype, prec, 'N', 'T', rzero, 0, planes, nullptr, h->maps(),
+                            planes ? h->store_lay.get() : nullptr);
   h->lplan = make_gemm_plan(lx, dtype, prec, 'N', 'T', lzero, 0, planes, nullptr, h->maps());
   h->flops_exec = h->rplan->flops + h->lplan->flops;
   const int c = dtype == DT_C64 ? 2 : 1;
@@ -416,14 +417,15 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
     h->store = std::make_unique<DevBuf>(std::max<int64_t>(n_store, 1) * 8 * c);
   } else if (planes) {
     const int npl = 2 * c;
-    // the R pass can write `store` as planes when its padded layout is the
-    // plain one and no output accumulates; otherwise it writes FP32 and the
-    // planes are split from it
-    h->store_direct = h->store_lay->identity && !h->rplan->any_beta && h->store_lay->total == n_store;
+    // the R pass writes `store` as planes when no output accumulates and its
+    // blocks map into the L pass's padded layout (row by row, zeroed once
+    // below); otherwise it writes FP32 and the planes are split from it
+    h->store_direct = !h->rplan->any_beta &&
+                      (h->store_lay->identity ? h->store_lay->total == n_store : h->rplan->out_padded);
     h->renv_ps = plane_stride(h->renv_lay->total);
     h->lenv_ps = plane_stride(h->lenv_lay->total);
     h->psi_ps = plane_stride(h->psi_lay->total);
-    h->store_ps = plane_stride(h->store_direct ? n_store : h->store_lay->total);
+    h->store_ps = plane_stride(h->store_lay->total);  // n_store when the layout is the plain one
     h->renv_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->renv_ps, 8) * npl * 2);
     h->lenv_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->lenv_ps, 8) * npl * 2);
     h->in_pl = std::make_unique<DevBuf>(std::max<int64_t>(h->psi_ps, 8) * npl * 2);
@@ -446,9 +448,15 @@ std::unique_ptr<HeffPlan> make_heff(int dtype, int prec, const void* renv, const
   return h;
 }
 
+// the padded layout the R pass writes its plane output into, if any
+const PadLayout* HeffPlan::r_out_layout() const {
+  return prec == PREC_BF16X3 && store_direct && store_lay && !store_lay->identity ? store_lay.get() : nullptr;
+}
+
 void HeffPlan::replan(int r_target, int l_target) {
   const bool planes = prec == PREC_BF16X3;
-  rplan = make_gemm_plan(rexec, dtype, prec, 'N', 'T', rzero, r_target, planes, nullptr, maps());
+  rplan = make_gemm_plan(rexec, dtype, prec, 'N', 'T', rzero, r_target, planes, nullptr, maps(), r_out_layout());
+  DMRGX_CHECK(!r_out_layout() || rplan->out_padded, "replan: the R pass lost its padded output layout");
   lplan = make_gemm_plan(lexec, dtype, prec, 'N', 'T', lzero, l_target, planes, nullptr, maps());
   r_split = r_target;
   l_split = l_target;
@@ -472,7 +480,10 @@ void HeffPlan::restrict_tasks(const std::vector<char>& keep_r, const std::vector
   // dropped outputs read as zero: the ranges that no kept original R task
   // writes (rows skipped inside kept blocks stay zero) and that no kept
   // executed L task writes
-  rplan = make_gemm_plan(r, dtype, prec, 'N', 'T', uncovered(ro, n_store), r_split, planes, nullptr, maps());
+  rplan = make_gemm_plan(r, dtype, prec, 'N', 'T', uncovered(ro, n_store), r_split, planes, nullptr, maps(),
+                         r_out_layout());
+  DMRGX_CHECK(!r_out_layout() || rplan->out_padded || rplan->n_items == 0,
+              "restrict_tasks: the R pass lost its padded output layout");
   lplan = make_gemm_plan(l, dtype, prec, 'N', 'T', uncovered(l, n_psi), l_split, planes, nullptr, maps());
 }
 

@@ -36,6 +36,7 @@ struct HeffPlan {
   std::vector<int> rexec_src, lexec_src;
   std::vector<int32_t> row_maps;
   const std::vector<int32_t>* maps() const { return row_maps.empty() ? nullptr : &row_maps; }
+  const PadLayout* r_out_layout() const;
   std::vector<std::pair<int64_t, int64_t>> rzero, lzero;
   int r_split = 0, l_split = 0;
   // rebuild the two pass plans with the given split-K targets (work items

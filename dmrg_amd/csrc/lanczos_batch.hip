@@ -16,6 +16,7 @@
 #include <memory>
 
 #include "common.hpp"
+#include "gemm.hpp"
 #include "host_pool.hpp"
 #include "lanczos.hpp"
 
@@ -217,8 +218,8 @@ __device__ __forceinline__ void seg_store2(double re, double im, double* __restr
 
 // k_vdots_seg of x = w - theta_chain d, formed on the fly: partials of
 // <V_j, x> for j < k and of <x, x>
-template <bool CPLX>
-__global__ __launch_bounds__(256) void k_vdots_shift_seg(const double* __restrict__ V, int64_t ld, int k,
+template <bool CPLX, typename TV>
+__global__ __launch_bounds__(256) void k_vdots_shift_seg(const TV* __restrict__ V, int64_t ld, int k,
                                                          const double* __restrict__ w, const double* __restrict__ d,
                                                          const double* __restrict__ cg, int nc,
                                                          const SegBlk* __restrict__ blks,
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(256) void k_vdots_shift_seg(const double* __restric
         if (c < g) {
           double vr = xr, vi = xi;
           if (j0 + c < k) {
-            const double* v = V + (int64_t)(j0 + c) * ld;
+            const TV* v = V + (int64_t)(j0 + c) * ld;
             vr = v[2 * i];
             vi = v[2 * i + 1];
           }
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(256) void k_vdots_shift_seg(const double* __restric
       const double xr = w[i] - th * d[i];
 #pragma unroll
       for (int c = 0; c < G; ++c)
-        if (c < g) re[c] += ((j0 + c < k) ? V[(int64_t)(j0 + c) * ld + i] : xr) * xr;
+        if (c < g) re[c] += ((j0 + c < k) ? (double)V[(int64_t)(j0 + c) * ld + i] : xr) * xr;
     }
   }
   __shared__ double red[2 * G][256];
@@ -270,8 +271,8 @@ __global__ __launch_bounds__(256) void k_vdots_shift_seg(const double* __restric
 
 // w = P (w - theta d) = (w - theta d) - sum_{j<k} V_j h_j (h = V^H (w - theta d)),
 // with the partials of <d, w> after it
-template <bool CPLX>
-__global__ __launch_bounds__(256) void k_cg_project(const double* __restrict__ V, int64_t ld, int k,
+template <bool CPLX, typename TV>
+__global__ __launch_bounds__(256) void k_cg_project(const TV* __restrict__ V, int64_t ld, int k,
                                                     const double* __restrict__ h, int hs, double* __restrict__ w,
                                                     const double* __restrict__ d, const double* __restrict__ cg,
                                                     int nc, const SegBlk* __restrict__ blks,
@@ -284,7 +285,7 @@ __global__ __launch_bounds__(256) void k_cg_project(const double* __restrict__ V
     if (CPLX) {
       double ar = 0, ai = 0;
       for (int j = 0; j < k; ++j) {
-        const double* v = V + (int64_t)j * ld;
+        const TV* v = V + (int64_t)j * ld;
         const double vr = v[2 * i], vi = v[2 * i + 1], hr = hc[2 * j], hi = hc[2 * j + 1];
         ar += vr * hr - vi * hi;
         ai += vr * hi + vi * hr;
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(256) void k_cg_project(const double* __restrict__ V
       im += dr * xi - di * xr;
     } else {
       double acc = 0;
-      for (int j = 0; j < k; ++j) acc += V[(int64_t)j * ld + i] * hc[2 * j];
+      for (int j = 0; j < k; ++j) acc += (double)V[(int64_t)j * ld + i] * hc[2 * j];
       const double x = (w[i] - th * d[i]) - acc;
       w[i] = x;
       re += d[i] * x;
@@ -899,6 +900,10 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
   const size_t vb = (size_t)nr * 8;
   DevBuf Vb(qmax * vb), HVb(qmax * vb), wk(7 * vb), hb((size_t)2 * nc * hs * 8), scl((size_t)nc * 8),
       Zb((size_t)nc * qmax * 16);
+  // an FP32 copy of the basis for the CG projections (the inner solve is a
+  // few digits deep; the correction is orthogonalised against V in FP64)
+  DevBuf Vfb(qmax * (size_t)nr * 4);
+  float* Vf = (float*)Vfb.ptr;
   // the CG scalars (device), their pinned staging and the per-step live counts
   DevBuf cgb((size_t)CG_FIELDS * nc * 8), liveb((size_t)std::max(inner, 1) * sizeof(int));
   double* cgd = (double*)cgb.ptr;
@@ -961,6 +966,7 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
     }
     ops.put(sc, (double*)scl.ptr);
     ops.sub(V, j, h2, t1, V + (size_t)j * nr, (const double*)scl.ptr);
+    convert_f64_to_f32(V + (size_t)j * nr, Vf + (size_t)j * nr, nr, s);
     op64(V + (size_t)j * nr, HV + (size_t)j * nr);
     if (n64) (*n64)++;
     ops.vdots(V, j + 1, HV + (size_t)j * nr, h);
@@ -1045,15 +1051,15 @@ std::vector<double> jd_polish_batched(const MatVec& op_lo, const MatVec& op64, c
       op_lo(dv, wv);
       if (nlo) (*nlo)++;
       // w = P (H_lo - theta) d and <d, w>
-      if (cplx) hipLaunchKernelGGL(k_vdots_shift_seg<true>, dim3(nblk, ng), dim3(256), 0, s, V, nr, q, wv, dv, cgd,
-                                   nc, ops.sg.B(), part);
-      else hipLaunchKernelGGL(k_vdots_shift_seg<false>, dim3(nblk, ng), dim3(256), 0, s, V, nr, q, wv, dv, cgd, nc,
-                              ops.sg.B(), part);
+      if (cplx) hipLaunchKernelGGL((k_vdots_shift_seg<true, float>), dim3(nblk, ng), dim3(256), 0, s, Vf, nr, q, wv,
+                                   dv, cgd, nc, ops.sg.B(), part);
+      else hipLaunchKernelGGL((k_vdots_shift_seg<false, float>), dim3(nblk, ng), dim3(256), 0, s, Vf, nr, q, wv, dv,
+                              cgd, nc, ops.sg.B(), part);
       hipLaunchKernelGGL(k_reduce_seg, dim3(nc, ng), dim3(64 * 2 * COLS), 0, s, part, nblk, q + 1, cb, ce, h, hs);
-      if (cplx) hipLaunchKernelGGL(k_cg_project<true>, dim3(nblk), dim3(256), 0, s, V, nr, q, h, hs, wv, dv, cgd, nc,
-                                   ops.sg.B(), part);
-      else hipLaunchKernelGGL(k_cg_project<false>, dim3(nblk), dim3(256), 0, s, V, nr, q, h, hs, wv, dv, cgd, nc,
-                              ops.sg.B(), part);
+      if (cplx) hipLaunchKernelGGL((k_cg_project<true, float>), dim3(nblk), dim3(256), 0, s, Vf, nr, q, h, hs, wv, dv,
+                                   cgd, nc, ops.sg.B(), part);
+      else hipLaunchKernelGGL((k_cg_project<false, float>), dim3(nblk), dim3(256), 0, s, Vf, nr, q, h, hs, wv, dv,
+                              cgd, nc, ops.sg.B(), part);
       hipLaunchKernelGGL(k_reduce_seg, dim3(nc, 1), dim3(64 * 2 * COLS), 0, s, part, nblk, 1, cb, ce, h2, hs);
       hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(256), 0, s, nc, h2, hs, cgd);
       // s += alpha d, res -= alpha w, |res|^2
