@@ -22,36 +22,41 @@ constexpr int G = 8;  // Krylov columns handled per pass of the dot / combine ke
 constexpr int DOT_BLOCKS_MAX = 4096;  // work-groups of one dot-product launch (all column groups)
 constexpr int PARTIAL_DOUBLES = (DOT_BLOCKS_MAX + 64) * 2 * G;
 
-// Partial <X_c, w> (conjugate-linear in X) for columns c of group blockIdx.y:
+// element type of one complex entry of a T array, loaded as one vector
+template <typename T>
+using cplx_of = typename std::conditional<std::is_same<T, float>::value, float2, double2>::type;
+
+// Partial <X_c, w> (conjugate-linear in X) for columns c of one group of G:
 // X_c = V_c for c < k, and X_k = w itself when with_self (the squared norm).
 // V holds TV elements (double, or float for the basis of a low-precision
-// solve), w TW elements; the sums are FP64.
+// solve), w TW elements; the sums are FP64.  The launch is one-dimensional
+// with the column group fastest, so the groups that read one range of w run
+// side by side; complex entries load as one vector each.
 template <bool CPLX, typename TV = double, typename TW = double>
 __global__ __launch_bounds__(256) void k_vdots(const TV* __restrict__ V, int64_t ld, int k, int with_self,
-                                               const TW* __restrict__ w, int64_t n,
+                                               const TW* __restrict__ w, int64_t n, int ng,
                                                double* __restrict__ partial) {
-  const int j0 = blockIdx.y * G;
+  const int gy = blockIdx.x % ng, bx = blockIdx.x / ng, nbx = gridDim.x / ng;
+  const int j0 = gy * G;
   const int ktot = k + with_self;
   const int g = min(G, ktot - j0);
   double re[G], im[G];
 #pragma unroll
   for (int c = 0; c < G; ++c) re[c] = im[c] = 0;
-  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = bx * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)nbx * blockDim.x;
   for (; i < n; i += stride) {
     if (CPLX) {
-      const double wr = w[2 * i], wi = w[2 * i + 1];
+      const cplx_of<TW> wv = reinterpret_cast<const cplx_of<TW>*>(w)[i];
+      const double wr = wv.x, wi = wv.y;
 #pragma unroll
       for (int c = 0; c < G; ++c)
         if (c < g) {
-          double vr, vi;
+          double vr = wr, vi = wi;
           if (j0 + c < k) {
-            const TV* v = V + (int64_t)(j0 + c) * ld;
-            vr = v[2 * i];
-            vi = v[2 * i + 1];
-          } else {
-            vr = wr;
-            vi = wi;
+            const cplx_of<TV> v = reinterpret_cast<const cplx_of<TV>*>(V + (int64_t)(j0 + c) * ld)[i];
+            vr = v.x;
+            vi = v.y;
           }
           re[c] += vr * wr + vi * wi;
           im[c] += vr * wi - vi * wr;
@@ -75,8 +80,7 @@ __global__ __launch_bounds__(256) void k_vdots(const TV* __restrict__ V, int64_t
       for (int q = 0; q < 2 * G; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x < 2 * G)
-    partial[((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 2 * G + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 2 * G) partial[((int64_t)gy * nbx + bx) * 2 * G + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // one workgroup per column group: sum the per-block partials
@@ -217,11 +221,13 @@ void vdots(const TV* V, int64_t ld, int k, const TW* w, int64_t n, bool cplx, do
   const int ng = (ktot + G - 1) / G;
   int nblk = (int)std::min<int64_t>(DOT_BLOCKS_MAX / ng + 1, std::max<int64_t>(1, (n + 1023) / 1024));
   nblk = std::max(1, std::min(nblk, DOT_BLOCKS_MAX));
-  dim3 grid(nblk, ng);
+  dim3 grid(nblk * ng);
   if (cplx)
-    hipLaunchKernelGGL((k_vdots<true, TV, TW>), grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
+    hipLaunchKernelGGL((k_vdots<true, TV, TW>), grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, ng,
+                       partial);
   else
-    hipLaunchKernelGGL((k_vdots<false, TV, TW>), grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, partial);
+    hipLaunchKernelGGL((k_vdots<false, TV, TW>), grid, dim3(256), 0, s, V, ld, k, with_self ? 1 : 0, w, n, ng,
+                       partial);
   hipLaunchKernelGGL(k_reduce_partials, dim3(ng), dim3(64 * 2 * G), 0, s, partial, nblk, ktot, h);
 }
 

@@ -687,7 +687,7 @@ __global__ __launch_bounds__(256) void k_beig(const JBlk* __restrict__ blks, int
     if (CPLX) sZ[r * GS + c * C + 1] = 0.0;
   }
   __syncthreads();
-  // 2. largest |G_pq| / sqrt(G_pp G_qq), p < q
+  // 2. largest |G_pq| / sqrt(G_pp G_qq), p < q (as its square, one root at the end)
   double mx = 0;
   for (int e = t; e < PJ * PJ; e += 256) {
     const int r = e / PJ, c = e % PJ;
@@ -695,14 +695,14 @@ __global__ __launch_bounds__(256) void k_beig(const JBlk* __restrict__ blks, int
       const double a = sG[r * GS + r * C], d = sG[c * GS + c * C];
       if (a > k.negl2 && d > k.negl2) {
         const double gr = sG[r * GS + c * C], gi = CPLX ? sG[r * GS + c * C + 1] : 0.0;
-        mx = fmax(mx, (CPLX ? hypot(gr, gi) : fabs(gr)) / sqrt(a * d));
+        mx = fmax(mx, (gr * gr + gi * gi) / (a * d));
       }
     }
   }
   for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off, 64));
   if (lane == 0) sRed[wave] = mx;
   __syncthreads();
-  mx = fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3]));
+  mx = sqrt(fmax(fmax(sRed[0], sRed[1]), fmax(sRed[2], sRed[3])));
   if (t == 0) {
     if (mx > 0) atomicMax(&flag[b], (unsigned long long)__double_as_longlong(mx));
     ZF[k.pair_base + pair] = mx > k.tol ? 1 : 0;
@@ -726,15 +726,20 @@ __global__ __launch_bounds__(256) void k_beig(const JBlk* __restrict__ blks, int
         rr_pair(PJ, rd, t, p, q);
         const double a = Gc[p * GS + p * C], d = Gc[q * GS + q * C];
         const double gr = Gc[p * GS + q * C], gi = CPLX ? Gc[p * GS + q * C + 1] : 0.0;
-        const double ag = CPLX ? hypot(gr, gi) : fabs(gr);
+        const double ag2 = gr * gr + gi * gi;
         double c = 1, sn = 0, er = 1, ei = 0;
-        if (a > k.negl2 && d > k.negl2 && ag > k.tol * sqrt(a * d)) {
-          const double zeta = (d - a) / (2 * ag);
-          const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1 + zeta * zeta));
+        if (a > k.negl2 && d > k.negl2 && ag2 > k.tol * k.tol * (a * d)) {
+          // t = tan(theta) = sign(d - a) 2|g| / (|d - a| + sqrt((d - a)^2 + 4|g|^2)),
+          // the smaller root of t^2 + 2 zeta t - 1 = 0 with zeta = (d - a) / (2|g|);
+          // the two square roots are independent, so the chain is sqrt, div,
+          // sqrt, div
+          const double ag = sqrt(ag2), dd = d - a;
+          const double tt = (dd >= 0 ? 2.0 : -2.0) * ag / (fabs(dd) + sqrt(dd * dd + 4 * ag2));
           c = 1 / sqrt(1 + tt * tt);
           sn = c * tt;
-          er = CPLX ? gr / ag : (gr >= 0 ? 1.0 : -1.0);
-          ei = CPLX ? gi / ag : 0.0;
+          const double rag = 1 / ag;
+          er = CPLX ? gr * rag : (gr >= 0 ? 1.0 : -1.0);
+          ei = CPLX ? gi * rag : 0.0;
           sChanged = 1;
         }
         sPQ[t][0] = p;
