@@ -92,7 +92,7 @@ int main(int argc, char** argv) {
   cfg.ladder = 1;
   cfg.seed = 1234;
   cfg.tol = 1e-10;
-  cfg.tol_lo = 1e-4;
+  cfg.tol_lo = 3e-5;  // the reduced-precision Lanczos solve ahead of the FP64 refinement
   cfg.refine_k = 8;
   cfg.output = suffix;
   cfg.warm_start = word(argc, argv, "warm", "0") == "1" ? 1 : 0;

@@ -731,17 +731,17 @@ __device__ __forceinline__ void split_bf(double x, unsigned short& h, unsigned s
 // Two consecutive padded elements per thread (Kp and every block start are
 // multiples of 8, so the pair lies in one row), consecutive threads on
 // consecutive pairs: coalesced loads of the source and 4-byte stores to each
-// plane.  A work-group walks SPLIT_PER chunks of 512 elements, finding the
-// block of its first pair by binary search and the later ones by stepping
-// forward.  Padding columns (K <= col < Kp) are written as 0.
-constexpr int SPLIT_PER = 8;
+// plane.  A work-group walks `per` chunks of 512 elements (per is chosen so
+// that the launch has about a thousand work-groups), finding the block of
+// its first pair by binary search and the later ones by stepping forward.
+// Padding columns (K <= col < Kp) are written as 0.
 template <typename SRC>
 __global__ __launch_bounds__(256) void k_split_padded(const SRC* __restrict__ src, unsigned short* __restrict__ dst,
                                                       int64_t ps, const PadBlk* __restrict__ blk, int nb,
-                                                      int64_t total, int cplx) {
+                                                      int64_t total, int cplx, int per) {
   using SRC2 = typename std::conditional<std::is_same<SRC, float>::value, float2, double2>::type;
-  const int64_t step = (int64_t)gridDim.x * 512 * SPLIT_PER;
-  for (int64_t c0 = (int64_t)blockIdx.x * 512 * SPLIT_PER; c0 < total; c0 += step) {
+  const int64_t step = (int64_t)gridDim.x * 512 * per;
+  for (int64_t c0 = (int64_t)blockIdx.x * 512 * per; c0 < total; c0 += step) {
     int64_t e = c0 + 2 * threadIdx.x;
     int bi = 0, hi = nb - 1;
     while (bi < hi) {
@@ -749,7 +749,7 @@ __global__ __launch_bounds__(256) void k_split_padded(const SRC* __restrict__ sr
       if (blk[mid].dst_off <= e) bi = mid;
       else hi = mid - 1;
     }
-    for (int i = 0; i < SPLIT_PER && e < total; ++i, e += 512) {
+    for (int i = 0; i < per && e < total; ++i, e += 512) {
       while (bi + 1 < nb && blk[bi + 1].dst_off <= e) ++bi;
       const PadBlk b = blk[bi];
       const int loc = (int)(e - b.dst_off);  // blocks hold < 2^31 elements
@@ -1250,9 +1250,11 @@ static void split_padded(const SRC* src, uint16_t* dst, int64_t ps, const PadLay
   if (L.total <= 0) return;
   DMRGX_CHECK(ps >= L.total, "split_padded: plane stride below the padded length");
   DMRGX_CHECK(ps % 8 == 0 && L.total % 8 == 0, "split_padded: plane stride and length must be multiples of 8");
-  const int64_t blocks = std::min<int64_t>((L.total + 512 * SPLIT_PER - 1) / (512 * SPLIT_PER), 8192);
+  // about 1024 work-groups, each walking at most 8 chunks of 512 elements
+  const int per = (int)std::max<int64_t>(1, std::min<int64_t>(8, L.total / (512 * 1024)));
+  const int64_t blocks = std::min<int64_t>((L.total + 512 * per - 1) / (512 * per), 8192);
   hipLaunchKernelGGL(k_split_padded<SRC>, dim3((unsigned)blocks), dim3(256), 0, s, src, (unsigned short*)dst, ps,
-                     (const PadBlk*)L.d_blk->ptr, (int)L.blk.size(), L.total, cplx ? 1 : 0);
+                     (const PadBlk*)L.d_blk->ptr, (int)L.blk.size(), L.total, cplx ? 1 : 0, per);
   DMRGX_HIP(hipGetLastError());
 }
 
