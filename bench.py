@@ -91,6 +91,12 @@ POTTS = (1.0, 1.0, 0.0, 3.14159265359 / 8)  # f, j, p, theta of H_Potts3 (the re
 # lowest pair; measured fastest of 1e-3 .. 3e-5 on the L=100 m=512 XYZ and
 # L=64 m=1024 Potts sweeps)
 TOL_LO = 3e-5
+# FP64 residual tolerance of the refinement that follows it: half a digit
+# past the FP64 solves' 1e-10, which leaves the refined vector's error in the
+# high-energy directions small enough that the block SVD needs no more
+# Jacobi sweeps than on the FP64 run's states (measured fastest of 1e-10 ..
+# 1e-12 on the two sweeps above)
+TOL_REFINE = 5e-11
 
 
 def time_plan(h, prec, x, steps, warmup):
@@ -432,7 +438,9 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
     reference's ARPACK calls do.  "+nevK" makes a reduced-precision solve test
     the residuals of its K lowest pairs (default 1), "+tolX" sets its tolerance
     (default TOL_LO), "+rkN" gives the refinement's inner solves 2N
-    conjugate-gradient steps (default 8)."""
+    conjugate-gradient steps (default 8), "+rtolX" sets the FP64 tolerance
+    (the refinement's, for a reduced-precision run; default 1e-10 for FP64,
+    TOL_REFINE for the others)."""
     from dmrg_amd import driver
     params = {"xyz": JXYZ, "potts3": POTTS}[model]
     res = {"model": model, "L": L, "m": m, "sweeps": sweeps, "ladder": bool(ladder), "params": list(params),
@@ -444,8 +452,9 @@ def sweep_leg(model, L, m, sweeps, ladder, keep=None, precs=("fp64", "bf16x3")):
         nev_lo = next((int(x[3:]) for x in mods if x.startswith("nev")), 0)
         tol_lo = next((float(x[3:]) for x in mods if x.startswith("tol")), TOL_LO)
         refine_k = next((int(x[2:]) for x in mods if x.startswith("rk")), 8)
+        tol = next((float(x[4:]) for x in mods if x.startswith("rtol")), 1e-10 if prec == "fp64" else TOL_REFINE)
         t0 = time.perf_counter()
-        r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol_lo=tol_lo,
+        r = driver.run_dmrg(model, L, m, sweeps, params, schedule="dmrg2", prec=prec, tol=tol, tol_lo=tol_lo,
                             refine_k=refine_k, ladder=ladder, keep_state=True, verbose="stderr", warm_start=warm,
                             nev_lo=nev_lo)
         states[spec] = r.state
