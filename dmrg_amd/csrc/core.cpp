@@ -215,7 +215,11 @@ void Engine<T>::solve(const qtensor<T>& lenv, const qtensor<T>& renv, int lnum, 
     HeffPlan* hp64 = plan64.get();
     MatVec op64 = [hp64, s](const double* x, double* y) { hp64->apply(x, y, s); };
     int pm = 0, plo = 0;
-    vals[0] = jd_polish_batched(op, op64, {d}, cx, pv, (double*)evec.ptr, cfg.tol, cfg.polish_cycles,
+    // the inner solves (a few digits deep) on the coarse product of a BF16x3 plan
+    MatVec op_in = cfg.prec == PREC_BF16X3 ? MatVec([hp, s](const double* x, double* y) {
+      hp->apply(x, y, s, nullptr, true);
+    }) : op;
+    vals[0] = jd_polish_batched(op_in, op64, {d}, cx, pv, (double*)evec.ptr, cfg.tol, cfg.polish_cycles,
                                 2 * std::max(1, cfg.refine_k), &pm, &plo)[0];
     lg.n_polish = pm;
     lg.n_matvec += plo;

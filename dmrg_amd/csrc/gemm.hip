@@ -520,7 +520,10 @@ __device__ __forceinline__ uint4 xor_sign(uint4 v) {
   return v;
 }
 
-template <bool CPLX, int OUT>
+// HI: the coarse product of the hi planes alone (one bf16 product per real
+// product, the lo planes neither loaded nor multiplied), for operators that
+// need only a few digits
+template <bool CPLX, int OUT, bool HI = false>
 __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ items, int nitems, int per_xcd,
                                                   const GemmSeg* __restrict__ segs,
                                                   const unsigned short* __restrict__ A, int64_t as,
@@ -580,6 +583,7 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
     const bool la = okA && in, lb = okB && in;
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
+      if (HI && (j & 1)) continue;  // lo planes (re_lo, im_lo)
       uint4 v = make_uint4(0, 0, 0, 0);
       if (la) v = *reinterpret_cast<const uint4*>(A + j * as + offA + c.k);
       ra[j] = neg ? xor_sign(v) : v;
@@ -591,6 +595,7 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
   auto stage = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < CPT; ++j) {
+      if (HI && (j & 1)) continue;
       *reinterpret_cast<uint4*>(&sA[buf][j][srow * PLDK + skc]) = ra[j];
       *reinterpret_cast<uint4*>(&sB[buf][j][srow * PLDK + skc]) = rb[j];
     }
@@ -631,6 +636,19 @@ __global__ __launch_bounds__(PNT) void k_gemm_bf3p(const GemmItem* __restrict__ 
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
+        if (HI) {
+          f4v r = acc[0][i][j];
+          if (CPLX) r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], nbh, r, 0, 0, 0);
+          r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[0], r, 0, 0, 0);
+          acc[0][i][j] = r;
+          if (CPLX) {
+            f4v q = acc[NP - 1][i][j];
+            q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[2], q, 0, 0, 0);
+            q = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[2][i], b[0], q, 0, 0, 0);
+            acc[NP - 1][i][j] = q;
+          }
+          continue;
+        }
         f4v r = acc[0][i][j];
         r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[1][i], b[0], r, 0, 0, 0);
         r = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0][i], b[1], r, 0, 0, 0);
@@ -1143,7 +1161,7 @@ void reduce_partials(const GemmPlan& p, void* C, hipStream_t s) {
 void reduce_partials(const GemmPlan& p, void* C, hipStream_t s);
 
 void launch_planes_kernel(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B,
-                          int64_t b_stride, void* C, int out, int64_t c_ps, hipStream_t s) {
+                          int64_t b_stride, void* C, int out, int64_t c_ps, hipStream_t s, bool hi) {
   if (!p.n_items) return;
   float* wsp = (float*)p.ws;
   const int per = p.grid / 8;
@@ -1154,19 +1172,32 @@ void launch_planes_kernel(const GemmPlan& p, const uint16_t* A, int64_t a_stride
                        a_stride, (const unsigned short*)B, b_stride, C, c_ps, wsp, p.row_maps);
   };
   if (p.dtype == DT_C64) {
-    if (out == OUT_F32) go(k_gemm_bf3p<true, OUT_F32>);
-    else if (out == OUT_PLANES) go(k_gemm_bf3p<true, OUT_PLANES>);
-    else go(k_gemm_bf3p<true, OUT_F64>);
+    if (hi) {
+      if (out == OUT_F32) go(k_gemm_bf3p<true, OUT_F32, true>);
+      else if (out == OUT_PLANES) go(k_gemm_bf3p<true, OUT_PLANES, true>);
+      else go(k_gemm_bf3p<true, OUT_F64, true>);
+    } else {
+      if (out == OUT_F32) go(k_gemm_bf3p<true, OUT_F32>);
+      else if (out == OUT_PLANES) go(k_gemm_bf3p<true, OUT_PLANES>);
+      else go(k_gemm_bf3p<true, OUT_F64>);
+    }
   } else {
-    if (out == OUT_F32) go(k_gemm_bf3p<false, OUT_F32>);
-    else if (out == OUT_PLANES) go(k_gemm_bf3p<false, OUT_PLANES>);
-    else go(k_gemm_bf3p<false, OUT_F64>);
+    if (hi) {
+      if (out == OUT_F32) go(k_gemm_bf3p<false, OUT_F32, true>);
+      else if (out == OUT_PLANES) go(k_gemm_bf3p<false, OUT_PLANES, true>);
+      else go(k_gemm_bf3p<false, OUT_F64, true>);
+    } else {
+      if (out == OUT_F32) go(k_gemm_bf3p<false, OUT_F32>);
+      else if (out == OUT_PLANES) go(k_gemm_bf3p<false, OUT_PLANES>);
+      else go(k_gemm_bf3p<false, OUT_F64>);
+    }
   }
   DMRGX_HIP(hipGetLastError());
 }
 
 void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
-                     void* C, int out, int64_t c_ps, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end) {
+                     void* C, int out, int64_t c_ps, hipStream_t s, hipEvent_t ev_begin, hipEvent_t ev_end,
+                     bool hi) {
   DMRGX_CHECK(p.planes, "run_gemm_planes needs a pre-split plan");
   DMRGX_CHECK(a_stride % 8 == 0 && b_stride % 8 == 0, "plane strides must be multiples of 8");
   DMRGX_CHECK(out != OUT_PLANES || !p.any_beta, "run_gemm_planes: plane output cannot accumulate");
@@ -1182,7 +1213,7 @@ void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, con
     }
   }
   if (ev_begin) DMRGX_HIP(hipEventRecord(ev_begin, s));
-  launch_planes_kernel(p, A, a_stride, B, b_stride, C, out, c_ps, s);
+  launch_planes_kernel(p, A, a_stride, B, b_stride, C, out, c_ps, s, hi);
   if (ev_end) DMRGX_HIP(hipEventRecord(ev_end, s));
   if (out == OUT_F32) {
     reduce_partials(p, C, s);

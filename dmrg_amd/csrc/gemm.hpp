@@ -145,9 +145,10 @@ void run_gemm_plan(const GemmPlan& p, const void* A, const void* B, void* C, hip
 // (re_hi, re_lo[, im_hi, im_lo]) of a_stride / b_stride elements each (both
 // multiples of 8); C is FP32 or FP64 (complex interleaved when dtype ==
 // DT_C64) or, for OUT_PLANES, bf16 planes of stride c_ps.
+// hi: the coarse product of the hi planes alone (about 3 significant digits)
 void run_gemm_planes(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B, int64_t b_stride,
                      void* C, int out, int64_t c_ps, hipStream_t s, hipEvent_t ev_begin = nullptr,
-                     hipEvent_t ev_end = nullptr);
+                     hipEvent_t ev_end = nullptr, bool hi = false);
 // Stride of a bf16 plane of n elements (a multiple of 8: 16-byte rows).
 inline int64_t plane_stride(int64_t n) { return (n + 7) / 8 * 8; }
 
@@ -176,7 +177,7 @@ void split_padded_f32(const float* src, uint16_t* dst, int64_t ps, const PadLayo
 // the per-launch timing of a pass repeats back to back.
 void launch_plan_kernel(const GemmPlan& p, const void* A, const void* B, void* C, hipStream_t s);
 void launch_planes_kernel(const GemmPlan& p, const uint16_t* A, int64_t a_stride, const uint16_t* B,
-                          int64_t b_stride, void* C, int out, int64_t c_ps, hipStream_t s);
+                          int64_t b_stride, void* C, int out, int64_t c_ps, hipStream_t s, bool hi = false);
 
 // Precision conversion helpers (n counts real scalars).
 void convert_f64_to_f32(const double* src, float* dst, int64_t n, hipStream_t s);

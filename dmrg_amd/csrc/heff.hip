@@ -730,7 +730,7 @@ void attach_edge(HeffPlan& h, std::unique_ptr<HeffPlan> e) {
   h.edge = std::move(e);
 }
 
-void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev) {
+void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev, bool coarse) {
   const int c = dtype == DT_C64 ? 2 : 1;
   hipEvent_t e0 = ev ? ev[0] : nullptr, e1 = ev ? ev[1] : nullptr, e2 = ev ? ev[2] : nullptr,
              e3 = ev ? ev[3] : nullptr;
@@ -743,14 +743,14 @@ void HeffPlan::apply(const double* in, double* out, hipStream_t s, hipEvent_t* e
     split_padded_f64(in, (uint16_t*)in_pl->ptr, psi_ps, *psi_lay, c == 2, s);
     if (store_direct) {
       run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
-                      store_pl->ptr, OUT_PLANES, store_ps, s, e0, e1);
+                      store_pl->ptr, OUT_PLANES, store_ps, s, e0, e1, coarse);
     } else {
       run_gemm_planes(*rplan, (const uint16_t*)renv_pl->ptr, renv_ps, (const uint16_t*)in_pl->ptr, psi_ps,
-                      store->ptr, OUT_F32, 0, s, e0, e1);
+                      store->ptr, OUT_F32, 0, s, e0, e1, coarse);
       split_padded_f32((const float*)store->ptr, (uint16_t*)store_pl->ptr, store_ps, *store_lay, c == 2, s);
     }
     run_gemm_planes(*lplan, (const uint16_t*)lenv_pl->ptr, lenv_ps, (const uint16_t*)store_pl->ptr, store_ps, out,
-                    OUT_F64, 0, s, e2, e3);
+                    OUT_F64, 0, s, e2, e3, coarse);
   } else {
     convert_f64_to_f32(in, (float*)in_lo->ptr, n_psi * c, s);
     run_gemm_plan(*rplan, renv_lo->ptr, in_lo->ptr, store->ptr, s, e0, e1);

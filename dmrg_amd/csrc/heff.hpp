@@ -58,8 +58,10 @@ struct HeffPlan {
   // to the bulk one; edge_out holds its result
   std::unique_ptr<HeffPlan> edge;
   std::unique_ptr<DevBuf> edge_out;
-  // ev: optional {R begin, R end, L begin, L end} events around the two GEMM launches
-  void apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev = nullptr);
+  // ev: optional {R begin, R end, L begin, L end} events around the two GEMM
+  // launches; coarse (BF16x3 plans): the hi planes' product alone, about
+  // three significant digits, for the refinement's inner solves
+  void apply(const double* in, double* out, hipStream_t s, hipEvent_t* ev = nullptr, bool coarse = false);
   // Per-launch time (ms) of each pass's GEMM kernel alone: one apply of `in`
   // (so that every intermediate holds its data), then `iters` back-to-back
   // launches of the R-pass kernel and of the L-pass kernel between events
