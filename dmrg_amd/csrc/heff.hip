@@ -1007,7 +1007,10 @@ struct Refined {
 };
 Refined refine_lowest(HeffPlan* lo, HeffPlan* f, const std::vector<int64_t>& len, int pv, double* vecs, double tol) {
   hipStream_t s = rt().stream;
-  MatVec op = [lo, s](const double* x, double* y) { lo->apply(x, y, s); };
+  // the inner solves on the coarse hi-plane product of a BF16x3 plan, as the
+  // sweep engine does
+  const bool coarse = lo->prec == PREC_BF16X3;
+  MatVec op = [lo, s, coarse](const double* x, double* y) { lo->apply(x, y, s, nullptr, coarse); };
   MatVec op64 = [f, s](const double* x, double* y) { f->apply(x, y, s); };
   Refined r;
   // the sweep engine's settings: at most 12 outer steps of at most 16 CG steps

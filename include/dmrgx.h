@@ -149,7 +149,8 @@ int dmrgx_heff_time_adjoint(dmrgx_heff* plan, const void* psi, const void* g_out
  * lowest min(4, ncv) Ritz vectors start an FP64 Jacobi-Davidson refinement: the
  * Rayleigh-Ritz basis grows by one vector per step (one FP64 matvec), the
  * correction from at most 16 conjugate-gradient steps on the projected
- * correction equation with the low-precision operator, at most 12 steps, until
+ * correction equation with the low-precision operator (for a BF16X3 plan its
+ * coarse product of the hi planes alone), at most 12 steps, until
  * the FP64 residual passes refine_tol*max(|theta|, eps^(2/3)) (refine_tol <= 0:
  * 1e-10).  Starting from several Ritz vectors keeps near-degenerate lowest
  * pairs apart.  evals[0] and evec then come from the refinement.
